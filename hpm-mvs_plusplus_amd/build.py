@@ -1,0 +1,45 @@
+"""Builds libhpm_mi355x.so (the HIP kernels + C ABI) for gfx950, in-tree.
+
+    python build.py            # -> lib/libhpm_mi355x.so
+
+Flags: -ffp-contract=off keeps every fused multiply-add explicit (the exact
+FP32 path must perform the operations its canonical order names); packed FP32
+instructions are disabled (-packed-fp32-ops).
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "lib", "libhpm_mi355x.so")
+SOURCES = [os.path.join(HERE, "csrc", "hpm_capi.hip")]
+HEADERS = ["hpm_math.h", "hpm_device.h", "hpm_kernels.h", "hpm_jbu.h", "hpm_fast.h"]
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off", "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-value", "-Wno-unused-result",
+]
+
+
+def hipcc():
+    for cand in ("/opt/rocm/bin/hipcc", "hipcc"):
+        if os.path.exists(cand) or cand == "hipcc":
+            return cand
+
+
+def build(verbose=False, extra=()):
+    os.makedirs(os.path.dirname(LIB), exist_ok=True)
+    tmp = LIB + ".tmp"
+    cmd = [hipcc()] + HIPCC_FLAGS + list(extra) + SOURCES + ["-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=HERE)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True, extra=sys.argv[1:])
+    print(LIB)

@@ -1,0 +1,526 @@
+// hpm_capi.hip -- the C ABI of include/hpm_mi355x.h: context, device buffers,
+// per-run constant block and kernel launches on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hpm_mi355x.h"
+#include "hpm_fast.h"
+#include "hpm_jbu.h"
+#include "hpm_kernels.h"
+
+using namespace hpm;
+
+struct hpm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  hpm_params params{};
+  int nimg = 0, W = 0, H = 0;
+  std::vector<hpm_camera> cams;
+  std::vector<float*> d_img;
+  std::vector<float*> d_dimg;
+  std::vector<void*> d_img_lp;     // FP16 / BF16 copies (fast paths)
+  int lp_kind = -1;                // precision of d_img_lp
+  float4* d_planes = nullptr;
+  float* d_costs = nullptr;
+  float* d_pre_costs = nullptr;
+  float* d_conf = nullptr;
+  float* d_texture = nullptr;
+  uint32_t* d_sel = nullptr;
+  uint32_t* d_masks = nullptr;
+  uint32_t* d_canny = nullptr;
+  float4* d_prior = nullptr;
+  float4* d_scaled = nullptr;
+  int scaled_w = 0, scaled_h = 0;
+  bool upsample = false;
+  RunConsts* d_consts = nullptr;
+  FastConsts* d_fconsts = nullptr;
+  unsigned long long* d_counters = nullptr;
+  bool timing = false;
+  hipEvent_t ev[8] = {};
+  hpm_stats stats{};
+};
+
+namespace {
+
+int fail(hpm_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIP_OK(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess) return fail(ctx, HPM_E_DEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T> int dalloc(hpm_ctx* ctx, T** p, size_t n) {
+  if (*p) { hipFree(*p); *p = nullptr; }
+  if (n == 0) return HPM_OK;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) { *p = nullptr; return fail(ctx, HPM_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)); }
+  return HPM_OK;
+}
+
+void free_views(hpm_ctx* c) {
+  for (float* p : c->d_img) hipFree(p);
+  for (float* p : c->d_dimg) hipFree(p);
+  for (void* p : c->d_img_lp) hipFree(p);
+  c->d_img.clear(); c->d_dimg.clear(); c->d_img_lp.clear(); c->lp_kind = -1;
+  void* bufs[] = {c->d_planes, c->d_costs, c->d_pre_costs, c->d_conf, c->d_texture, c->d_sel,
+                  c->d_masks,  c->d_canny, c->d_prior,     c->d_scaled};
+  for (void* p : bufs) if (p) hipFree(p);
+  c->d_planes = nullptr; c->d_costs = nullptr; c->d_pre_costs = nullptr; c->d_conf = nullptr;
+  c->d_texture = nullptr; c->d_sel = nullptr; c->d_masks = nullptr; c->d_canny = nullptr;
+  c->d_prior = nullptr; c->d_scaled = nullptr;
+  c->nimg = 0; c->W = c->H = 0;
+}
+
+// The reference's FP32 view-pair basis (HPM.cu:253-279).
+void pair_basis(const hpm_camera& rc, const hpm_camera& sc, PairBasis& B) {
+  float ref_C[3], src_C[3];
+  for (int r = 0; r < 3; ++r) {
+    ref_C[r] = -(rc.R[0 + r] * rc.t[0] + rc.R[3 + r] * rc.t[1] + rc.R[6 + r] * rc.t[2]);
+    src_C[r] = -(sc.R[0 + r] * sc.t[0] + sc.R[3 + r] * sc.t[1] + sc.R[6 + r] * sc.t[2]);
+  }
+  float C_rel[3];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      B.R_rel[3 * a + b] = sc.R[3 * a + 0] * rc.R[3 * b + 0] + sc.R[3 * a + 1] * rc.R[3 * b + 1] + sc.R[3 * a + 2] * rc.R[3 * b + 2];
+  for (int r = 0; r < 3; ++r) C_rel[r] = ref_C[r] - src_C[r];
+  for (int a = 0; a < 3; ++a) B.t_rel[a] = sc.R[3 * a + 0] * C_rel[0] + sc.R[3 * a + 1] * C_rel[1] + sc.R[3 * a + 2] * C_rel[2];
+}
+
+int build_consts(hpm_ctx* ctx, RunConsts& C) {
+  std::memset(&C, 0, sizeof(C));
+  const hpm_params& P = ctx->params;
+  C.W = ctx->W; C.H = ctx->H; C.V = ctx->nimg - 1;
+  C.geom = P.geom_consistency; C.prior = P.prior_consistency; C.hierarchy = P.hierarchy;
+  C.mand = P.mand_consistency; C.upsample = (P.hierarchy && ctx->upsample) ? 1 : 0;
+  C.num_iterations = P.num_iterations;
+  C.depth_min = ctx->cams[0].depth_min * 0.6f;  // HPM.cpp:685
+  C.depth_max = ctx->cams[0].depth_max * 1.2f;  // HPM.cpp:686
+  C.sigma_spatial = P.sigma_spatial; C.sigma_color = P.sigma_color; C.top_k = P.top_k;
+  C.scaled_cols = (float)ctx->scaled_w; C.scaled_rows = (float)ctx->scaled_h;
+  for (int it = 0; it < 3; ++it) {
+    for (int e = 0; e < 3; ++e) C.judge_thr[it][e] = (float)(0.8 * exp_hpm(-it * it * (3 - e) / 90.0));
+    const float thr = (float)(0.8 * (double)expf_hpm((float)(it * it) / (-90.0f)));
+    C.vs_thr[it] = thr;
+    C.vs_fb[it] = expf_hpm(thr * thr / (-0.32f));
+  }
+  C.depth_sigma = (C.depth_max - C.depth_min) / 64.0f;
+  C.two_dss = 2 * C.depth_sigma * C.depth_sigma;
+  C.angle_sigma = (float)(3.14159265358979323846 * (5.0f / 180.0f));
+  C.two_ass = 2 * C.angle_sigma * C.angle_sigma;
+  C.pert_refine = (float)(0.02f * 3.14159265358979323846);
+  C.pert_prior_init = (float)(3 * 0.02f * 3.14159265358979323846);
+  for (int i = 0; i < ctx->nimg; ++i) {
+    std::memcpy(&C.cams[i], &ctx->cams[i], sizeof(DevCam));
+    C.img[i] = ctx->d_img[i];
+    C.dimg[i] = i < (int)ctx->d_dimg.size() ? ctx->d_dimg[i] : nullptr;
+  }
+  for (int v = 1; v < ctx->nimg; ++v) pair_basis(ctx->cams[0], ctx->cams[v], C.pair[v - 1]);
+  return HPM_OK;
+}
+
+DevState make_state(hpm_ctx* c) {
+  DevState S;
+  S.planes = c->d_planes; S.costs = c->d_costs; S.pre_costs = c->d_pre_costs; S.sel = c->d_sel;
+  S.prior = c->d_prior; S.masks = c->d_masks; S.conf = c->d_conf; S.canny = c->d_canny;
+  S.texture = c->d_texture; S.scaled = c->d_scaled; S.counters = c->d_counters;
+  return S;
+}
+
+int check_params(hpm_ctx* ctx, const hpm_params& p) {
+  if (p.patch_size != 11 || p.radius_increment != 2)
+    return fail(ctx, HPM_E_INVALID, "only patch_size 11 with radius_increment 2 (36 taps) is supported");
+  if (p.num_iterations < 0 || p.num_iterations > 3)
+    return fail(ctx, HPM_E_INVALID, "num_iterations must be in [0, 3]");
+  if (p.top_k < 1) return fail(ctx, HPM_E_INVALID, "top_k must be >= 1");
+  if (p.precision < 0 || p.precision > 3) return fail(ctx, HPM_E_INVALID, "unknown precision");
+  if (p.prior_consistency && p.mand_consistency)
+    return fail(ctx, HPM_E_INVALID, "prior and mandatory consistency are separate passes");
+  return HPM_OK;
+}
+
+size_t npix(const hpm_ctx* c) { return (size_t)c->W * c->H; }
+
+}  // namespace
+
+extern "C" {
+
+void hpm_default_params(hpm_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->patch_size = 11;
+  p->radius_increment = 2;
+  p->sigma_spatial = 5.0f;
+  p->sigma_color = 3.0f;
+  p->top_k = 4;
+  p->num_iterations = 3;
+  p->precision = HPM_PREC_FP32_EXACT;
+}
+
+int hpm_create(hpm_ctx** out, int device) {
+  if (!out) return HPM_E_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return HPM_E_DEVICE;
+  hpm_ctx* ctx = new hpm_ctx();
+  ctx->device = device;
+  hpm_default_params(&ctx->params);
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
+    delete ctx;
+    return HPM_E_DEVICE;
+  }
+  if (hipMalloc((void**)&ctx->d_consts, sizeof(RunConsts)) != hipSuccess ||
+      hipMalloc((void**)&ctx->d_fconsts, sizeof(FastConsts)) != hipSuccess ||
+      hipMalloc((void**)&ctx->d_counters, 8 * sizeof(unsigned long long)) != hipSuccess) {
+    hipFree(ctx->d_consts); hipFree(ctx->d_fconsts);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return HPM_E_NOMEM;
+  }
+  for (auto& e : ctx->ev) hipEventCreate(&e);
+  *out = ctx;
+  return HPM_OK;
+}
+
+void hpm_destroy(hpm_ctx* ctx) {
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  free_views(ctx);
+  hipFree(ctx->d_consts);
+  hipFree(ctx->d_fconsts);
+  hipFree(ctx->d_counters);
+  for (auto& e : ctx->ev) hipEventDestroy(e);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* hpm_last_error(const hpm_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int hpm_set_params(hpm_ctx* ctx, const hpm_params* params) {
+  if (!ctx || !params) return HPM_E_INVALID;
+  int rc = check_params(ctx, *params);
+  if (rc) return rc;
+  ctx->params = *params;
+  return HPM_OK;
+}
+
+int hpm_set_views(hpm_ctx* ctx, int num_images, const hpm_camera* cameras, const float* const* images) {
+  if (!ctx) return HPM_E_INVALID;
+  if (!cameras || !images) return fail(ctx, HPM_E_INVALID, "null cameras or images");
+  if (num_images < 2 || num_images > HPM_MAX_IMAGES)
+    return fail(ctx, HPM_E_INVALID, "num_images must be in [2, 33] (view masks are 32-bit)");
+  for (int i = 0; i < num_images; ++i) {
+    if (!images[i]) return fail(ctx, HPM_E_INVALID, "null image pointer");
+    if (cameras[i].width <= 0 || cameras[i].height <= 0 || (size_t)cameras[i].width * cameras[i].height > (1u << 28))
+      return fail(ctx, HPM_E_INVALID, "bad image size");
+  }
+  if (cameras[0].width < 16 || cameras[0].height < 16)
+    return fail(ctx, HPM_E_INVALID, "reference image must be at least 16x16");
+  HIP_OK(hipSetDevice(ctx->device));
+  HIP_OK(hipStreamSynchronize(ctx->stream));
+  free_views(ctx);
+  ctx->nimg = num_images;
+  ctx->cams.assign(cameras, cameras + num_images);
+  ctx->W = cameras[0].width;
+  ctx->H = cameras[0].height;
+  int rc;
+  ctx->d_img.assign(num_images, nullptr);
+  for (int i = 0; i < num_images; ++i) {
+    const size_t n = (size_t)cameras[i].width * cameras[i].height;
+    if ((rc = dalloc(ctx, &ctx->d_img[i], n))) return rc;
+    HIP_OK(hipMemcpyAsync(ctx->d_img[i], images[i], n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  }
+  const size_t P = npix(ctx);
+  if ((rc = dalloc(ctx, &ctx->d_planes, P)) || (rc = dalloc(ctx, &ctx->d_costs, P)) ||
+      (rc = dalloc(ctx, &ctx->d_pre_costs, P)) || (rc = dalloc(ctx, &ctx->d_conf, P)) ||
+      (rc = dalloc(ctx, &ctx->d_texture, P)) || (rc = dalloc(ctx, &ctx->d_sel, P)) ||
+      (rc = dalloc(ctx, &ctx->d_canny, P)))
+    return rc;
+  HIP_OK(hipMemsetAsync(ctx->d_planes, 0, P * sizeof(float4), ctx->stream));
+  HIP_OK(hipMemsetAsync(ctx->d_costs, 0, P * sizeof(float), ctx->stream));
+  HIP_OK(hipMemsetAsync(ctx->d_pre_costs, 0, P * sizeof(float), ctx->stream));
+  HIP_OK(hipMemsetAsync(ctx->d_conf, 0, P * sizeof(float), ctx->stream));
+  HIP_OK(hipMemsetAsync(ctx->d_texture, 0, P * sizeof(float), ctx->stream));
+  HIP_OK(hipMemsetAsync(ctx->d_sel, 0, P * sizeof(uint32_t), ctx->stream));
+  HIP_OK(hipMemsetAsync(ctx->d_canny, 0, P * sizeof(uint32_t), ctx->stream));
+  HIP_OK(hipStreamSynchronize(ctx->stream));
+  return HPM_OK;
+}
+
+int hpm_set_src_depths(hpm_ctx* ctx, const float* const* depths) {
+  if (!ctx || !depths) return HPM_E_INVALID;
+  if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
+  HIP_OK(hipSetDevice(ctx->device));
+  for (float* p : ctx->d_dimg) hipFree(p);
+  ctx->d_dimg.assign(ctx->nimg, nullptr);
+  for (int i = 0; i < ctx->nimg; ++i) {
+    if (!depths[i]) { if (i == 0) continue; return fail(ctx, HPM_E_INVALID, "null source depth map"); }
+    const size_t n = (size_t)ctx->cams[i].width * ctx->cams[i].height;
+    int rc = dalloc(ctx, &ctx->d_dimg[i], n);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(ctx->d_dimg[i], depths[i], n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  }
+  HIP_OK(hipStreamSynchronize(ctx->stream));
+  return HPM_OK;
+}
+
+static int upload(hpm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  HIP_OK(hipSetDevice(ctx->device));
+  HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  HIP_OK(hipStreamSynchronize(ctx->stream));
+  return HPM_OK;
+}
+
+int hpm_set_canny(hpm_ctx* ctx, const uint32_t* canny) {
+  if (!ctx || !canny) return HPM_E_INVALID;
+  if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
+  return upload(ctx, ctx->d_canny, canny, npix(ctx) * sizeof(uint32_t));
+}
+
+int hpm_set_confidence(hpm_ctx* ctx, const float* conf) {
+  if (!ctx || !conf) return HPM_E_INVALID;
+  if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
+  return upload(ctx, ctx->d_conf, conf, npix(ctx) * sizeof(float));
+}
+
+int hpm_reload_hypotheses(hpm_ctx* ctx, const float* planes, const float* costs) {
+  if (!ctx || !planes || !costs) return HPM_E_INVALID;
+  if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
+  int rc = upload(ctx, ctx->d_planes, planes, npix(ctx) * sizeof(float4));
+  if (rc) return rc;
+  return upload(ctx, ctx->d_costs, costs, npix(ctx) * sizeof(float));
+}
+
+int hpm_set_prior(hpm_ctx* ctx, const float* prior_planes, const uint32_t* masks) {
+  if (!ctx || !prior_planes || !masks) return HPM_E_INVALID;
+  if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
+  int rc;
+  if ((rc = dalloc(ctx, &ctx->d_prior, npix(ctx))) || (rc = dalloc(ctx, &ctx->d_masks, npix(ctx)))) return rc;
+  if ((rc = upload(ctx, ctx->d_prior, prior_planes, npix(ctx) * sizeof(float4)))) return rc;
+  return upload(ctx, ctx->d_masks, masks, npix(ctx) * sizeof(uint32_t));
+}
+
+int hpm_set_coarse(hpm_ctx* ctx, const float* coarse, int cw, int ch, const float* fine_depth) {
+  if (!ctx || !coarse || !fine_depth || cw <= 0 || ch <= 0) return HPM_E_INVALID;
+  if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
+  int rc = dalloc(ctx, &ctx->d_scaled, (size_t)cw * ch);
+  if (rc) return rc;
+  if ((rc = upload(ctx, ctx->d_scaled, coarse, (size_t)cw * ch * sizeof(float4)))) return rc;
+  ctx->scaled_w = cw;
+  ctx->scaled_h = ch;
+  // HPM.cpp:869 compares the coarse width with the fine rows (kept as is)
+  ctx->upsample = (cw != ctx->H || ch != ctx->W);
+  if (!ctx->upsample && (cw != ctx->W || ch != ctx->H))
+    return fail(ctx, HPM_E_INVALID, "coarse map must match the reference size when not upsampling");
+  std::vector<float4> fine(npix(ctx));
+  for (size_t i = 0; i < fine.size(); ++i) fine[i] = make_float4(0.0f, 0.0f, 0.0f, fine_depth[i]);
+  return upload(ctx, ctx->d_planes, fine.data(), fine.size() * sizeof(float4));
+}
+
+int hpm_set_timing(hpm_ctx* ctx, int enable) {
+  if (!ctx) return HPM_E_INVALID;
+  ctx->timing = enable != 0;
+  return HPM_OK;
+}
+
+int hpm_run_patchmatch_partial(hpm_ctx* ctx, uint64_t seed, int n_halfpasses, int do_output) {
+  if (!ctx) return HPM_E_INVALID;
+  if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
+  const hpm_params& P = ctx->params;
+  if (P.geom_consistency && (int)ctx->d_dimg.size() != ctx->nimg)
+    return fail(ctx, HPM_E_STATE, "geometric consistency needs hpm_set_src_depths");
+  if (P.prior_consistency && (!ctx->d_prior || !ctx->d_masks))
+    return fail(ctx, HPM_E_STATE, "prior consistency needs hpm_set_prior");
+  if (P.hierarchy && !ctx->d_scaled) return fail(ctx, HPM_E_STATE, "hierarchy needs hpm_set_coarse");
+  if (P.precision != HPM_PREC_FP32_EXACT) return hpm_fast_run(ctx, seed, n_halfpasses, do_output);
+  HIP_OK(hipSetDevice(ctx->device));
+  RunConsts C;
+  build_consts(ctx, C);
+  C.seed = seed;
+  HIP_OK(hipMemcpyAsync(ctx->d_consts, &C, sizeof(C), hipMemcpyHostToDevice, ctx->stream));
+  HIP_OK(hipMemsetAsync(ctx->d_counters, 0, 8 * sizeof(unsigned long long), ctx->stream));
+  const DevState S = make_state(ctx);
+  const dim3 b16(16, 16), g16((ctx->W + 15) / 16, (ctx->H + 15) / 16);
+  const int H2 = (ctx->H / 2) * 2;
+  const dim3 bp(32, 8), gp(((ctx->W + 1) / 2 + 31) / 32, (H2 + 7) / 8);
+  hipStream_t st = ctx->stream;
+  if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[0], st));
+  hipLaunchKernelGGL(k_init, g16, b16, 0, st, ctx->d_consts, S);
+  HIP_OK(hipGetLastError());
+  if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[1], st));
+  int done = 0;
+  for (int it = 0; it < P.num_iterations; ++it) {
+    for (int colour = 0; colour < 2; ++colour) {
+      if (n_halfpasses >= 0 && done >= n_halfpasses) break;
+      hipLaunchKernelGGL(k_propagate, gp, bp, 0, st, ctx->d_consts, S, colour, it, H2);
+      HIP_OK(hipGetLastError());
+      ++done;
+    }
+  }
+  if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[2], st));
+  if (do_output) {
+    hipLaunchKernelGGL(k_output, g16, b16, 0, st, ctx->d_consts, ctx->d_planes);
+    HIP_OK(hipGetLastError());
+  }
+  if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[3], st));
+  ctx->stats.propagate_launches = done;
+  return HPM_OK;
+}
+
+int hpm_run_patchmatch(hpm_ctx* ctx, uint64_t seed) { return hpm_run_patchmatch_partial(ctx, seed, -1, 1); }
+
+int hpm_synchronize(hpm_ctx* ctx) {
+  if (!ctx) return HPM_E_INVALID;
+  HIP_OK(hipSetDevice(ctx->device));
+  HIP_OK(hipStreamSynchronize(ctx->stream));
+  return HPM_OK;
+}
+
+int hpm_get_stats(hpm_ctx* ctx, hpm_stats* out) {
+  if (!ctx || !out) return HPM_E_INVALID;
+  HIP_OK(hipSetDevice(ctx->device));
+  HIP_OK(hipStreamSynchronize(ctx->stream));
+  unsigned long long cnt[8];
+  HIP_OK(hipMemcpy(cnt, ctx->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
+  hpm_stats s = ctx->stats;
+  s.ncc_init = cnt[0];
+  s.ncc_propagate = cnt[1];
+  s.ncc_reference = cnt[2] + (uint64_t)ctx->W * ctx->H * (ctx->nimg - 1);
+  if (ctx->timing) {
+    float a = 0, b = 0, c = 0;
+    HIP_OK(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]));
+    HIP_OK(hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]));
+    HIP_OK(hipEventElapsedTime(&c, ctx->ev[2], ctx->ev[3]));
+    s.ms_init = a; s.ms_propagate = b; s.ms_output = c;
+  }
+  *out = s;
+  return HPM_OK;
+}
+
+static int download(hpm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || !dst) return HPM_E_INVALID;
+  if (ctx->nimg == 0 || !src) return fail(ctx, HPM_E_STATE, "nothing to read yet");
+  HIP_OK(hipSetDevice(ctx->device));
+  HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_OK(hipStreamSynchronize(ctx->stream));
+  return HPM_OK;
+}
+
+int hpm_get_planes(hpm_ctx* ctx, float* out) { return download(ctx, out, ctx ? ctx->d_planes : nullptr, ctx ? npix(ctx) * 16 : 0); }
+int hpm_get_costs(hpm_ctx* ctx, float* out) { return download(ctx, out, ctx ? ctx->d_costs : nullptr, ctx ? npix(ctx) * 4 : 0); }
+int hpm_get_texture(hpm_ctx* ctx, float* out) { return download(ctx, out, ctx ? ctx->d_texture : nullptr, ctx ? npix(ctx) * 4 : 0); }
+int hpm_get_selected_views(hpm_ctx* ctx, uint32_t* out) { return download(ctx, out, ctx ? ctx->d_sel : nullptr, ctx ? npix(ctx) * 4 : 0); }
+int hpm_get_confidence(hpm_ctx* ctx, float* out) { return download(ctx, out, ctx ? ctx->d_conf : nullptr, ctx ? npix(ctx) * 4 : 0); }
+
+int hpm_get_size(const hpm_ctx* ctx, int* w, int* h, int* n) {
+  if (!ctx) return HPM_E_INVALID;
+  if (w) *w = ctx->W;
+  if (h) *h = ctx->H;
+  if (n) *n = ctx->nimg;
+  return HPM_OK;
+}
+
+int hpm_cost_fixed_device(hpm_ctx* ctx, const void* planes_dev, void* out_dev, int precision) {
+  if (!ctx || !planes_dev || !out_dev) return HPM_E_INVALID;
+  if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
+  if (precision != HPM_PREC_FP32_EXACT) return hpm_fast_cost_fixed(ctx, planes_dev, out_dev, precision);
+  HIP_OK(hipSetDevice(ctx->device));
+  RunConsts C;
+  build_consts(ctx, C);
+  HIP_OK(hipMemcpyAsync(ctx->d_consts, &C, sizeof(C), hipMemcpyHostToDevice, ctx->stream));
+  const dim3 b16(16, 16), g16((ctx->W + 15) / 16, (ctx->H + 15) / 16);
+  hipLaunchKernelGGL(k_cost_fixed, g16, b16, 0, ctx->stream, ctx->d_consts, (const float4*)planes_dev, (float*)out_dev);
+  HIP_OK(hipGetLastError());
+  return HPM_OK;
+}
+
+int hpm_cost_fixed(hpm_ctx* ctx, const float* planes, float* out, int precision) {
+  if (!ctx || !planes || !out) return HPM_E_INVALID;
+  if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
+  HIP_OK(hipSetDevice(ctx->device));
+  const size_t P = npix(ctx), V = ctx->nimg - 1;
+  float4* dp = nullptr;
+  float* dout = nullptr;
+  int rc;
+  if ((rc = dalloc(ctx, &dp, P)) || (rc = dalloc(ctx, &dout, P * V))) { hipFree(dp); return rc; }
+  rc = upload(ctx, dp, planes, P * sizeof(float4));
+  if (!rc) rc = hpm_cost_fixed_device(ctx, dp, dout, precision);
+  if (!rc) rc = download(ctx, out, dout, P * V * sizeof(float));
+  hipFree(dp);
+  hipFree(dout);
+  return rc;
+}
+
+int hpm_init_cost(hpm_ctx* ctx, const float* planes, float* out_cost, uint32_t* out_masks) {
+  if (!ctx || !planes || !out_cost || !out_masks) return HPM_E_INVALID;
+  if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
+  HIP_OK(hipSetDevice(ctx->device));
+  const size_t P = npix(ctx);
+  float4* dp = nullptr;
+  float* dc = nullptr;
+  uint32_t* dm = nullptr;
+  int rc;
+  if ((rc = dalloc(ctx, &dp, P)) || (rc = dalloc(ctx, &dc, P)) || (rc = dalloc(ctx, &dm, P))) {
+    hipFree(dp); hipFree(dc); return rc;
+  }
+  RunConsts C;
+  build_consts(ctx, C);
+  rc = upload(ctx, ctx->d_consts, &C, sizeof(C));
+  if (!rc) rc = upload(ctx, dp, planes, P * sizeof(float4));
+  if (!rc) {
+    const dim3 b16(16, 16), g16((ctx->W + 15) / 16, (ctx->H + 15) / 16);
+    hipLaunchKernelGGL(k_init_cost, g16, b16, 0, ctx->stream, ctx->d_consts, dp, dc, dm);
+    if (hipGetLastError() != hipSuccess) rc = fail(ctx, HPM_E_DEVICE, "k_init_cost launch failed");
+  }
+  if (!rc) rc = download(ctx, out_cost, dc, P * sizeof(float));
+  if (!rc) rc = download(ctx, out_masks, dm, P * sizeof(uint32_t));
+  hipFree(dp); hipFree(dc); hipFree(dm);
+  return rc;
+}
+
+int hpm_geom_cost(hpm_ctx* ctx, const float* planes, float* out) {
+  if (!ctx || !planes || !out) return HPM_E_INVALID;
+  if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
+  if ((int)ctx->d_dimg.size() != ctx->nimg) return fail(ctx, HPM_E_STATE, "hpm_set_src_depths first");
+  HIP_OK(hipSetDevice(ctx->device));
+  const size_t P = npix(ctx), V = ctx->nimg - 1;
+  float4* dp = nullptr;
+  float* dout = nullptr;
+  int rc;
+  if ((rc = dalloc(ctx, &dp, P)) || (rc = dalloc(ctx, &dout, P * V))) { hipFree(dp); return rc; }
+  RunConsts C;
+  build_consts(ctx, C);
+  rc = upload(ctx, ctx->d_consts, &C, sizeof(C));
+  if (!rc) rc = upload(ctx, dp, planes, P * sizeof(float4));
+  if (!rc) {
+    const dim3 b16(16, 16), g16((ctx->W + 15) / 16, (ctx->H + 15) / 16);
+    hipLaunchKernelGGL(k_geom_cost, g16, b16, 0, ctx->stream, ctx->d_consts, dp, dout);
+    if (hipGetLastError() != hipSuccess) rc = fail(ctx, HPM_E_DEVICE, "k_geom_cost launch failed");
+  }
+  if (!rc) rc = download(ctx, out, dout, P * V * sizeof(float));
+  hipFree(dp); hipFree(dout);
+  return rc;
+}
+
+}  // extern "C"
+
+// The mixed-precision path is compiled in hpm_fast.hip when present; this
+// library build carries the FP32-exact path only.
+#ifndef HPM_HAVE_FAST
+int hpm_fast_run(hpm_ctx* ctx, uint64_t, int, int) {
+  return fail(ctx, HPM_E_INVALID, "mixed-precision path is not part of this build");
+}
+int hpm_fast_cost_fixed(hpm_ctx* ctx, const void*, void*, int) {
+  return fail(ctx, HPM_E_INVALID, "mixed-precision path is not part of this build");
+}
+#endif

@@ -1,0 +1,707 @@
+// hpm_kernels.h -- the PatchMatch kernels of the exact FP32 path.
+//
+//   k_init        RandomInitialization (HPM.cu:567-763), one thread per pixel
+//   k_propagate   BlackPixelUpdate / RedPixelUpdate (HPM.cu:2793-2827) with
+//                 CheckerboardPropagation (HPM.cu:1347-1873) and its
+//                 mandatory-consistency variant (HPM.cu:1875-2791)
+//   k_output      GetDepthandNormal (HPM.cu:2829-2842)
+//
+// Red/black scheme: a colour pass updates the pixels with (x + y) even
+// (black, colour 0) or odd (red, colour 1).  Every neighbour a pixel reads
+// (far/diagonal samples HPM.cu:1367-1607, the extended left-up samples
+// HPM.cu:1070-1100, the view-selection neighbours HPM.cu:1690) lies at an odd
+// offset, i.e. on the other colour, so a pass has no read/write races and its
+// result does not depend on the order pixels are visited in.
+#pragma once
+#include "hpm_device.h"
+
+namespace hpm {
+
+struct Sample {
+  float ca[8 * kMaxSrc];   // cost rows, direction-major: ca[d * V + v]
+  int pos[8];
+  bool flag[8];
+};
+
+__device__ __forceinline__ void wave_count_add(unsigned long long* counter, unsigned long long x) {
+  // one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  if ((threadIdx.x + threadIdx.y * blockDim.x) % 64 == 0) atomicAdd(counter, x);
+}
+
+// ComputeMultiViewInitialCostandSelectedViews, HPM.cu:450-488
+__device__ float init_cost(const RunConsts& C, int px, int py, float4 pl, const RefWin& rw, uint32_t& sel, unsigned& nncc) {
+  const int V = C.V;
+  float cv[kMaxSrc], cvc[kMaxSrc];
+  int valid = 0;
+  for (int i = 0; i < V; ++i) {
+    const float x = ncc_exact(C, i + 1, px, py, pl, rw);
+    cv[i] = x;
+    cvc[i] = x;
+    if (x < 2.0f) ++valid;
+  }
+  nncc += V;
+  for (int i = 1; i < V; i++) {  // insertion sort, HPM.cu:15-24
+    const float tmp = cv[i];
+    int j;
+    for (j = i; j >= 1 && tmp < cv[j - 1]; j--) cv[j] = cv[j - 1];
+    cv[j] = tmp;
+  }
+  sel = 0u;
+  const int top_k = valid < C.top_k ? valid : C.top_k;
+  if (top_k > 0) {
+    float cost = 0.0f;
+    for (int i = 0; i < top_k; ++i) cost += cv[i];
+    const float thr = cv[top_k - 1];
+    for (int i = 0; i < V; ++i)
+      if (cvc[i] <= thr) sel |= (1u << i);
+    return cost / (float)top_k;
+  }
+  return 2.0f;
+}
+
+__device__ __forceinline__ void reload_plane(const RunConsts& C, const DevState& S, int px, int py, const RefWin& rw, unsigned& nncc) {  // HPM.cu:749-760
+  const int center = py * C.W + px;
+  float4 pl = C.hierarchy ? S.scaled[center] : S.planes[center];
+  pl = to_ref(C.cams[0], pl);
+  const float depth = pl.w;
+  pl.w = dist_to_origin(C.cams[0], px, py, depth, pl);
+  S.planes[center] = pl;
+  uint32_t sel;
+  S.costs[center] = init_cost(C, px, py, pl, rw, sel, nncc);
+  S.sel[center] = sel;
+}
+
+// JBU of the coarse normal/cost map inside RandomInitialization, HPM.cu:633-698.
+__device__ void upsample_init(const RunConsts& C, const DevState& S, int px, int py, const RefWin& rw, unsigned& nncc) {
+  const int W = C.W, H = C.H;
+  const int center = py * W + px;
+  const float sc_cols = C.scaled_cols, sc_rows = C.scaled_rows;
+  const float scale = (float)(1.0 * sc_cols / W);
+  const float sigmad = 0.50f, sigmar = 25.5f;
+  const float fa = (float)W / sc_cols, fb = (float)H / sc_rows;
+  const int Imagescale = (int)(fa > fb ? fa : fb);
+  const int WinWidth = Imagescale * Imagescale + 1;
+  const int nn = WinWidth / 2;
+  const float o_y = (float)py * scale, o_x = (float)px * scale;
+  const float* ref = C.img[0];
+  const float refPix = tex_at(ref, W, H, px, py);
+  float nf = 0.0f, ctot = 0.0f;
+  float4 nt = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  for (int j = -nn; j <= nn; ++j) {
+    int r_y = (int)(o_y + (float)j);
+    r_y = (r_y > 0 ? ((float)r_y < sc_rows ? r_y : (int)(sc_rows - 1)) : 0);
+    const int r_ys = py + j;
+    for (int i = -nn; i <= nn; ++i) {
+      int r_x = (int)(o_x + (float)i);
+      r_x = (r_x > 0 ? ((float)r_x < sc_cols ? r_x : (int)(sc_cols - 1)) : 0);
+      const int s_center = (int)((float)r_y * sc_cols + (float)r_x);
+      const float4 sn = S.scaled[s_center];
+      const float nb = tex_at(ref, W, H, px + i, r_ys);
+      const float dxs = o_x - (float)r_x, dys = o_y - (float)r_y;
+      const float dis = dxs * dxs + dys * dys - 0.0f;
+      const float sg = (float)exp_hpm(-1.0 * (double)dis / (double)(2 * sigmad * sigmad));
+      const float xr = fabsf(refPix - nb);
+      const float rg = (float)exp_hpm(-1.0 * (double)(xr * xr) / (double)(2 * sigmar * sigmar));
+      const float tg = sg * rg;
+      nf += tg;
+      ctot += sn.w * tg;
+      nt.x = nt.x + sn.x * tg;
+      nt.y = nt.y + sn.y * tg;
+      nt.z = nt.z + sn.z * tg;
+    }
+  }
+  S.costs[center] = ctot / nf;
+  nt.x = nt.x / nf; nt.y = nt.y / nf; nt.z = nt.z / nf;
+  normalize3(nt);
+  uint32_t sel;
+  float c0 = init_cost(C, px, py, S.planes[center], rw, sel, nncc);
+  S.pre_costs[center] = c0;
+  float4 pl = to_ref(C.cams[0], nt);
+  const float depth = S.planes[center].w;
+  pl.w = dist_to_origin(C.cams[0], px, py, depth, pl);
+  S.planes[center] = pl;
+  S.costs[center] = init_cost(C, px, py, pl, rw, sel, nncc);
+  S.sel[center] = sel;
+}
+
+__global__ __launch_bounds__(256) void k_init(const RunConsts* __restrict__ Cp, DevState S) {
+  const RunConsts& C = *Cp;
+  const int px = blockIdx.x * blockDim.x + threadIdx.x;
+  const int py = blockIdx.y * blockDim.y + threadIdx.y;
+  unsigned nncc = 0;
+  if (px < C.W && py < C.H) {
+    const int center = py * C.W + px;
+    Philox rng(C.seed, (uint32_t)center, 0u);
+    const DevCam& cam = C.cams[0];
+    if (!C.prior && !C.mand) S.texture[center] = texture_factor(C, S.canny, px, py);
+    bool need_rw = !(C.prior && !(S.masks[center] > 0u && !(S.conf[center] > 0.3f)));
+    RefWin rw;
+    if (need_rw) build_refwin(C, px, py, rw);
+    if (!C.geom && !C.hierarchy && !C.prior) {
+      // GenerateRandomPlaneHypothesis, HPM.cu:224-230
+      const float depth = rng.uniform() * (C.depth_max - C.depth_min) + C.depth_min;
+      float4 pl = random_normal(cam, px, py, rng, depth);
+      pl.w = dist_to_origin(cam, px, py, depth, pl);
+      S.planes[center] = pl;
+      uint32_t sel;
+      S.costs[center] = init_cost(C, px, py, pl, rw, sel, nncc);
+      S.sel[center] = sel;
+    } else if (C.prior) {
+      if (S.masks[center] > 0u) {
+        if (S.conf[center] > 0.3f) {
+          float4 pl = S.planes[center];
+          const float depth = pl.w;
+          pl.w = dist_to_origin(cam, px, py, depth, pl);
+          S.planes[center] = pl;
+          S.costs[center] = 0.1f;
+        } else if (S.costs[center] > 0.1f) {
+          const float perturbation = 0.02f;
+          const float4 pl = S.prior[center];
+          float dp = pl.w;
+          const float dminp = (1 - 3 * perturbation) * dp;
+          const float dmaxp = (1 + 3 * perturbation) * dp;
+          dp = rng.uniform() * (dmaxp - dminp) + dminp;
+          float4 pp = perturbed_normal(cam, px, py, pl, rng, C.pert_prior_init);
+          pp.w = dp;  // the depth goes into w here (HPM.cu:609)
+          S.planes[center] = pp;
+          uint32_t sel;
+          S.costs[center] = init_cost(C, px, py, pp, rw, sel, nncc);
+          S.sel[center] = sel;
+        } else {
+          float4 pl = S.planes[center];
+          const float depth = pl.w;
+          pl.w = dist_to_origin(cam, px, py, depth, pl);
+          S.planes[center] = pl;
+          uint32_t sel;
+          S.costs[center] = init_cost(C, px, py, pl, rw, sel, nncc);
+          S.sel[center] = sel;
+        }
+      } else {
+        float4 pl = S.planes[center];
+        const float depth = pl.w;
+        pl.w = dist_to_origin(cam, px, py, depth, pl);
+        S.planes[center] = pl;
+        if (S.conf[center] > 0.3f) S.costs[center] = 0.1f;
+      }
+    } else {
+      if (C.upsample) {
+        upsample_init(C, S, px, py, rw, nncc);
+      } else if (C.mand) {
+        if (!(S.conf[center] >= 0.3f)) {
+          const float tex = texture_factor(C, S.canny, px, py);
+          if (tex > 0.5f) S.conf[center] = 0.1f;
+        }
+        reload_plane(C, S, px, py, rw, nncc);
+      } else {
+        reload_plane(C, S, px, py, rw, nncc);
+      }
+    }
+  }
+  wave_count_add(&S.counters[0], nncc);
+}
+
+// ------------------------------------------------------------------ propagation
+// Neighbour search (HPM.cu:1385-1608) and left-up extension (HPM.cu:1609-1680,
+// 1063-1126, 1322-1345).  Only direction 0 extends: the reference's
+// `bool symbol_eight_orientations[10] = { true }` enables index 0 only.
+__device__ void sample_neighbours(const RunConsts& C, const DevState& S, int px, int py, int iter, const RefWin& rw,
+                                  Sample& s, unsigned& nncc, unsigned& n_ext) {
+  const int W = C.W, H = C.H, V = C.V;
+  const int center = py * W + px;
+  const float* __restrict__ costs = S.costs;
+  for (int d = 0; d < 8; ++d) s.flag[d] = false;
+  for (int k = 0; k < 8 * V; ++k) s.ca[k] = 0.0f;
+  s.ca[0] = 2.0f;  // `float cost_array[8][32] = { 2.0f }`
+  int pos[8] = {center - 5 - 6 * W, center - W * 5, center + 6 - 5 * W, center + W * 5,
+                center + 5 + 6 * W, center - 5,     center - 6 + 5 * W, center + 5};
+  // axis directions: 1 up, 3 down, 5 left, 7 right (7 keeps the maximum)
+  if (py > 4) {
+    s.flag[1] = true;
+    float m = costs[pos[1]]; int mp = pos[1];
+    for (int i = 1; i < 5; ++i) {
+      if (py > 4 + 2 * i) { const int pt = pos[1] - 2 * i * W; const float c = costs[pt]; if (c < m) { m = c; mp = pt; } } else break;
+    }
+    pos[1] = mp;
+  }
+  if (py < H - 5) {
+    s.flag[3] = true;
+    float m = costs[pos[3]]; int mp = pos[3];
+    for (int i = 1; i < 5; ++i) {
+      if (py < H - 5 - 2 * i) { const int pt = pos[3] + 2 * i * W; const float c = costs[pt]; if (c < m) { m = c; mp = pt; } } else break;
+    }
+    pos[3] = mp;
+  }
+  if (px > 4) {
+    s.flag[5] = true;
+    float m = costs[pos[5]]; int mp = pos[5];
+    for (int i = 1; i < 5; ++i) {
+      if (px > 4 + 2 * i) { const int pt = pos[5] - 2 * i; const float c = costs[pt]; if (c < m) { m = c; mp = pt; } } else break;
+    }
+    pos[5] = mp;
+  }
+  if (px < W - 5) {
+    s.flag[7] = true;
+    float m = costs[pos[7]]; int mp = pos[7];
+    for (int i = 1; i < 5; ++i) {
+      if (px < W - 5 - 2 * i) { const int pt = pos[7] + 2 * i; const float c = costs[pt]; if (m < c) { m = c; mp = pt; } } else break;
+    }
+    pos[7] = mp;
+  }
+  // diagonal zig-zags (HPM.cu:1480-1608)
+  const bool dok[4] = {px > 4 && py > 5, px < W - 6 && py > 4, px < W - 5 && py < H - 6, px > 5 && py < H - 5};
+  const int sx[4] = {px - 5, px + 6, px + 5, px - 6};
+  const int sy[4] = {py - 6, py - 5, py + 6, py + 5};
+  const bool even_x[4] = {true, false, true, false};
+  const int ddx[4] = {-2, 2, 2, -2};
+  const int ddy[4] = {-2, -2, 2, 2};
+  for (int k = 0; k < 4; ++k) {
+    if (!dok[k]) continue;
+    const int d = 2 * k;
+    s.flag[d] = true;
+    float m = costs[pos[d]]; int mp = pos[d];
+    int x = sx[k], y = sy[k];
+    for (int i = 0; i < 7; i++) {
+      if (((i & 1) == 0) == even_x[k]) x += ddx[k]; else y += ddy[k];
+      if (x >= 0 && x < W && y < H && y >= 0) {
+        const int pt = y * W + x;
+        const float c = costs[pt];
+        if (c < m) { m = c; mp = pt; }
+      } else break;
+    }
+    pos[d] = mp;
+  }
+  // cost rows of the valid directions
+  for (int d = 0; d < 8; ++d) {
+    if (!s.flag[d]) continue;
+    const float4 pl = S.planes[pos[d]];
+    for (int v = 0; v < V; ++v) s.ca[d * V + v] = ncc_exact(C, v + 1, px, py, pl, rw);
+    nncc += V;
+  }
+  // extension of direction 0
+  for (int it = 0; it < 3; ++it) {
+    if (!s.flag[0]) break;
+    const float good = C.judge_thr[iter][it];
+    int gs = 0, bs = 0;
+    for (int v = 0; v < V; v++) {
+      const float x = s.ca[v];
+      if (x < good) gs++;
+      if (x > 1.2f) bs++;
+    }
+    if (gs >= 1 && bs <= 2) break;
+    const int lu = center - 5 - 6 * W - 8 * it - 8 * W * it;
+    if (px > 4 + 8 * it && py > 5 + 8 * it) {
+      float m = costs[lu]; int mp = lu;
+      int x = px - 5 - 8 * it, y = py - 6 - 8 * it;
+      for (int i = 0; i < 7; i++) {
+        if ((i & 1) == 0) x -= 2; else y -= 2;
+        if (x >= 0 && x < W && y < H && y >= 0) {
+          const int pt = y * W + x;
+          const float c = costs[pt];
+          if (c < m) { m = c; mp = pt; }
+        } else break;
+      }
+      if (m > costs[pos[0]]) {
+        // keeps the previous sample and its costs
+      } else {
+        pos[0] = mp;
+        const float4 pl = S.planes[mp];
+        for (int v = 0; v < V; ++v) s.ca[v] = ncc_exact(C, v + 1, px, py, pl, rw);
+        nncc += V;
+        n_ext += 1;
+      }
+    }
+  }
+  for (int d = 0; d < 8; ++d) s.pos[d] = pos[d];
+}
+
+// Joint view selection, HPM.cu:1686-1751
+__device__ void view_selection(const RunConsts& C, const DevState& S, int px, int py, int iter, const Sample& s, Philox& rng,
+                               float* vw, uint32_t& tsel, float& wn, int& nsel) {
+  const int W = C.W, V = C.V;
+  const int center = py * W + px;
+  float prior[kMaxSrc];
+  for (int v = 0; v < V; ++v) { vw[v] = 0.0f; prior[v] = 0.0f; }
+  const int nbr[4] = {center - W * 5, center + W * 5, center - 5, center + 5};
+  for (int i = 0; i < 4; ++i) {
+    if (!s.flag[2 * i + 1]) continue;
+    const uint32_t m = S.sel[nbr[i]];
+    for (int j = 0; j < V; ++j) prior[j] += ((m >> j) & 1u) ? 0.9f : 0.1f;
+  }
+  float probs[kMaxSrc];
+  const float thr = C.vs_thr[iter];
+  for (int i = 0; i < V; i++) {
+    float count = 0.0f, tmpw = 0.0f;
+    int count_false = 0;
+    for (int j = 0; j < 8; j++) {
+      const float x = s.ca[j * V + i];
+      if (x < thr) { tmpw += expf_hpm(x * x / (-0.18f)); count++; }
+      if (x > 1.2f) count_false++;
+    }
+    float p = 0.0f;
+    if (count > 2 && count_false < 3) p = tmpw / count;
+    else if (count_false < 3) p = C.vs_fb[iter];
+    probs[i] = p * prior[i];
+  }
+  float sum = 0.0f;
+  for (int i = 0; i < V; ++i) sum += probs[i];
+  const float inv = 1.0f / sum;
+  float cum = 0.0f;
+  for (int i = 0; i < V; ++i) { cum += probs[i] * inv; probs[i] = cum; }
+  for (int k = 0; k < 15; ++k) {
+    const float r = rng.uniform() - 1.19209290e-07f;
+    for (int v = 0; v < V; ++v) {
+      if (probs[v] > r) { vw[v] += 1.0f; break; }
+    }
+  }
+  tsel = 0u; wn = 0.0f; nsel = 0;
+  for (int i = 0; i < V; ++i)
+    if (vw[i] > 0.0f) { tsel |= (1u << i); wn += vw[i]; nsel++; }
+}
+
+__device__ __forceinline__ void final_costs(const RunConsts& C, const DevState& S, int px, int py, const Sample& s,
+                                            const float* vw, float wn, float* fc) {  // HPM.cu:1753-1771
+  const int V = C.V;
+  for (int i = 0; i < 8; ++i) {
+    float f = 0.0f;
+    for (int j = 0; j < V; ++j) {
+      if (vw[j] > 0.0f) {
+        const float c = s.ca[i * V + j];
+        if (C.geom) {
+          if (s.flag[i]) f += vw[j] * (c + 0.2f * geom_cost(C, j + 1, S.planes[s.pos[i]], px, py));
+          else f += vw[j] * (c + 0.1f * 3.0f);
+        } else {
+          f += vw[j] * c;
+        }
+      }
+    }
+    fc[i] = f / wn;
+  }
+}
+
+__device__ __forceinline__ float current_cost(const RunConsts& C, int px, int py, float4 pl, const float* vw, float wn,
+                                              const RefWin& rw, unsigned& nncc) {  // HPM.cu:1775-1786
+  const int V = C.V;
+  float cn = 0.0f;
+  for (int i = 0; i < V; ++i) {
+    if (!(vw[i] > 0.0f)) continue;
+    const float ci = ncc_exact(C, i + 1, px, py, pl, rw);
+    ++nncc;
+    if (C.geom) cn += vw[i] * (ci + 0.2f * geom_cost(C, i + 1, pl, px, py));
+    else cn += vw[i] * ci;
+  }
+  return cn / wn;
+}
+
+__device__ __forceinline__ int find_min_idx(const float* x) {  // HPM.cu:41-52
+  float m = x[0]; int idx = 0;
+  for (int i = 1; i < 8; ++i) if (x[i] <= m) { m = x[i]; idx = i; }
+  return idx;
+}
+__device__ __forceinline__ int find_max_idx(const float* x) {  // HPM.cu:54-65
+  float m = x[0]; int idx = 0;
+  for (int i = 1; i < 8; ++i) if (x[i] >= m) { m = x[i]; idx = i; }
+  return idx;
+}
+
+// PlaneHypothesisRefinement, HPM.cu:765-841
+__device__ void refine(const RunConsts& C, const DevState& S, float4& plane, float& depth, float& cost, Philox& rng,
+                       const float* vw, float wn, float& rcost, int px, int py, float texture, const RefWin& rw,
+                       unsigned& nncc) {
+  const DevCam& cam = C.cams[0];
+  const int center = py * C.W + px;
+  const float perturbation = 0.02f;
+  float depth_prior = 0.0f, depth_rand;
+  float4 plane_rand;
+  const bool use_prior = C.prior && S.masks[center] > 0u;
+  if (use_prior) {
+    const float4 pp = S.prior[center];
+    depth_prior = depth_from_plane(cam, pp, px, py);
+    depth_rand = rng.uniform() * 6 * C.depth_sigma + (depth_prior - 3 * C.depth_sigma);
+    plane_rand = perturbed_normal(cam, px, py, pp, rng, C.angle_sigma);
+  } else {
+    depth_rand = rng.uniform() * (C.depth_max - C.depth_min) + C.depth_min;
+    plane_rand = random_normal(cam, px, py, rng, depth);
+  }
+  float depth_pert = depth;
+  const float dminp = (1 - perturbation) * depth_pert;
+  const float dmaxp = (1 + perturbation) * depth_pert;
+  depth_pert = rng.uniform() * (dmaxp - dminp) + dminp;
+  const float4 plane_pert = perturbed_normal(cam, px, py, plane, rng, C.pert_refine);
+  const float depths[5] = {depth_rand, depth, depth_rand, depth, depth_pert};
+  const float4 normals[5] = {plane, plane_rand, plane_rand, plane_pert, plane};
+  const int V = C.V;
+  for (int i = 0; i < 5; ++i) {
+    float4 tp = normals[i];
+    tp.w = dist_to_origin(cam, px, py, depths[i], tp);
+    float tc = 0.0f;
+    for (int j = 0; j < V; ++j) {
+      if (!(vw[j] > 0.0f)) continue;
+      const float cj = ncc_exact(C, j + 1, px, py, tp, rw);
+      ++nncc;
+      if (C.geom) tc += vw[j] * (cj + 0.2f * geom_cost(C, j + 1, tp, px, py));
+      else tc += vw[j] * cj;
+    }
+    tc /= wn;
+    const float db = depth_from_plane(cam, tp, px, py);
+    if (use_prior) {
+      const float dd = depths[i] - depth_prior;
+      const float nd = norm_diff(S.prior[center], tp);
+      const float pr = prior_term(dd, nd, texture, C.two_dss, C.two_ass);
+      const float rtc = restricted_cost(tc, 0.18f, texture, pr);
+      if (db >= C.depth_min && db <= C.depth_max && rtc > rcost) { depth = db; plane = tp; cost = tc; rcost = rtc; }
+    } else {
+      if (db >= C.depth_min && db <= C.depth_max && tc < cost) { depth = db; plane = tp; cost = tc; }
+    }
+  }
+}
+
+// CheckerboardPropagation, HPM.cu:1347-1873
+__device__ void propagate_pixel(const RunConsts& C, const DevState& S, int px, int py, int iter, const RefWin& rw,
+                                Sample& s, unsigned& nncc, unsigned& n_ext) {
+  const DevCam& cam = C.cams[0];
+  const int center = py * C.W + px;
+  Philox rng(C.seed, (uint32_t)center, (uint32_t)(1 + iter));
+  sample_neighbours(C, S, px, py, iter, rw, s, nncc, n_ext);
+  float vw[kMaxSrc]; uint32_t tsel; float wn; int nsel;
+  view_selection(C, S, px, py, iter, s, rng, vw, tsel, wn, nsel);
+  float fc[8];
+  final_costs(C, S, px, py, s, vw, wn, fc);
+  const int min_idx = find_min_idx(fc);
+  float cost_now = current_cost(C, px, py, S.planes[center], vw, wn, rw, nncc);
+  S.costs[center] = cost_now;
+  float depth_now = depth_from_plane(cam, S.planes[center], px, py);
+  float rcost = 0.0f, texture = 0.0f;
+  if (C.prior) {
+    const float4 pc = S.prior[center];
+    const float depth_prior = depth_from_plane(cam, pc, px, py);
+    texture = texture_factor(C, S.canny, px, py);
+    if (S.masks[center] > 0u) {
+      float rfc[8];
+      for (int i = 0; i < 8; i++) {
+        rfc[i] = 0.0f;
+        if (s.flag[i]) {
+          const float4 q = S.planes[s.pos[i]];
+          const float dn = depth_from_plane(cam, q, px, py);
+          const float pr = prior_term(dn - depth_prior, norm_diff(pc, q), texture, C.two_dss, C.two_ass);
+          rfc[i] = restricted_cost(fc[i], 0.18f, texture, pr);
+        }
+      }
+      const int max_idx = find_max_idx(rfc);
+      const float4 cur = S.planes[center];
+      const float dn = depth_from_plane(cam, cur, px, py);
+      const float pr = prior_term(dn - depth_prior, norm_diff(pc, cur), texture, C.two_dss, C.two_ass);
+      const float rc_now = restricted_cost(cost_now, 0.18f, texture, pr);
+      if (s.flag[max_idx]) {
+        const float4 q = S.planes[s.pos[max_idx]];
+        const float db = depth_from_plane(cam, q, px, py);
+        if (db >= C.depth_min && db <= C.depth_max && rfc[max_idx] > rc_now) {
+          depth_now = db;
+          S.planes[center] = q;
+          S.costs[center] = fc[max_idx];
+          rcost = rfc[max_idx];
+          S.sel[center] = tsel;
+        }
+      }
+    } else if (s.flag[min_idx]) {
+      const float4 q = S.planes[s.pos[min_idx]];
+      const float db = depth_from_plane(cam, q, px, py);
+      if (db >= C.depth_min && db <= C.depth_max && fc[min_idx] < cost_now) {
+        depth_now = db;
+        S.planes[center] = q;
+        S.costs[center] = fc[min_idx];
+      }
+    }
+  }
+  float4 plane_now = S.planes[center];  // the reference leaves this uninitialised
+  if (!C.prior && s.flag[min_idx]) {
+    const float4 q = S.planes[s.pos[min_idx]];
+    const float db = depth_from_plane(cam, q, px, py);
+    if (db >= C.depth_min && db <= C.depth_max && fc[min_idx] < cost_now) {
+      depth_now = db; plane_now = q; cost_now = fc[min_idx]; S.sel[center] = tsel;
+    }
+    const float cc = S.costs[center];
+    if (cc != cc) {
+      depth_now = db; plane_now = q; cost_now = fc[min_idx]; S.sel[center] = tsel;
+    }
+  }
+  refine(C, S, plane_now, depth_now, cost_now, rng, vw, wn, rcost, px, py, texture, rw, nncc);
+  if (C.hierarchy) {
+    if (cost_now < S.pre_costs[center] - 0.1f) {
+      S.costs[center] = cost_now;
+      S.planes[center] = plane_now;
+    }
+  } else {
+    S.costs[center] = cost_now;
+    S.planes[center] = plane_now;
+  }
+}
+
+// CheckerboardPropagation_MandatoryConsistency, HPM.cu:1875-2791
+__device__ void propagate_pixel_mand(const RunConsts& C, const DevState& S, int px, int py, int iter, const RefWin& rw,
+                                     Sample& s, unsigned& nncc, unsigned& n_ext) {
+  const DevCam& cam = C.cams[0];
+  const int V = C.V;
+  const int center = py * C.W + px;
+  Philox rng(C.seed, (uint32_t)center, (uint32_t)(1 + iter));
+  sample_neighbours(C, S, px, py, iter, rw, s, nncc, n_ext);
+  float vw[kMaxSrc]; uint32_t tsel; float wn; int nsel;
+  view_selection(C, S, px, py, iter, s, rng, vw, tsel, wn, nsel);
+  if (S.conf[center] >= 0.3f) {
+    float fc[8];
+    final_costs(C, S, px, py, s, vw, wn, fc);
+    const int min_idx = find_min_idx(fc);
+    float cost_now = current_cost(C, px, py, S.planes[center], vw, wn, rw, nncc);
+    S.costs[center] = cost_now;
+    float depth_now = depth_from_plane(cam, S.planes[center], px, py);
+    float4 plane_now = S.planes[center];
+    if (s.flag[min_idx]) {
+      const float4 q = S.planes[s.pos[min_idx]];
+      const float db = depth_from_plane(cam, q, px, py);
+      if (db >= C.depth_min && db <= C.depth_max && fc[min_idx] < cost_now) {
+        depth_now = db; plane_now = q; cost_now = fc[min_idx]; S.sel[center] = tsel;
+      }
+      const float cc = S.costs[center];
+      if (cc != cc) { plane_now = q; cost_now = fc[min_idx]; S.sel[center] = tsel; }
+    }
+    float unused = 0.0f;
+    refine(C, S, plane_now, depth_now, cost_now, rng, vw, wn, unused, px, py, 0.0f, rw, nncc);
+    S.costs[center] = cost_now;
+    S.planes[center] = plane_now;
+    return;
+  }
+  float fc[8], rfc[8];
+  for (int i = 0; i < 8; i++) {
+    float f = 0.0f;
+    for (int j = 0; j < V; ++j) {
+      if (vw[j] > 0.0f) {
+        const float c = s.ca[i * V + j];
+        if (C.geom) {
+          if (s.flag[i]) f += vw[j] * (c + 0.2f * geom_cost(C, j + 1, S.planes[s.pos[i]], px, py));
+          else f += vw[j] * (c + 0.1f * 3.0f);
+        } else {
+          f += vw[j] * c;
+        }
+      }
+    }
+    f /= wn;
+    rfc[i] = 0.0f;
+    if (s.flag[i]) {
+      const float gce = expf_hpm(-f * f / 0.18f);
+      const float mmc = (float)nsel * 0.5f;
+      const float cp = S.conf[s.pos[i]];
+      const float dlt = fmaxf(0.0f, mmc - cp);
+      const float mce = (float)(0.5 + exp_hpm((double)(-dlt * dlt) / ((double)(mmc * mmc) * 1.0)));
+      rfc[i] = (cp > 0.0f) ? fmaxf(3.0f, mce) * gce : mce * gce;
+      if (cp > 0.0f) f = (float)((double)f * fmin((double)expf_hpm(-cp * 2.0f), 0.35));
+    }
+    fc[i] = f;
+  }
+  const int max_idx = find_max_idx(rfc);
+  float cost_now = current_cost(C, px, py, S.planes[center], vw, wn, rw, nncc);
+  S.costs[center] = cost_now;
+  const float gce = expf_hpm(-cost_now * cost_now / 0.18f);
+  const float mmc = (float)nsel * 0.5f;
+  const float cc = S.conf[center];
+  const float dlt = fmaxf(0.0f, mmc - cc);
+  const float mce = (float)(0.5 + exp_hpm((double)(-dlt * dlt) / ((double)(mmc * mmc) * 1.0)));
+  const float rc_now = (cc > 0.0f) ? fmaxf(3.0f, mce) * gce : mce * gce;
+  if (cc < 0.3f) cost_now = (float)((double)cost_now * exp_hpm(0.2 + (double)(cc * 2.0f)));
+  else cost_now = cost_now * expf_hpm(-cc * 2.0f);
+  float4 plane_now = S.planes[center];
+  if (s.flag[max_idx]) {
+    const int q = s.pos[max_idx];
+    const float4 qp = S.planes[q];
+    const float db = depth_from_plane(cam, qp, px, py);
+    if (db >= C.depth_min && db <= C.depth_max && rfc[max_idx] > rc_now) {
+      plane_now = qp; cost_now = fc[max_idx]; S.sel[center] = tsel;
+      S.conf[center] = (float)(0.5 * (double)S.conf[q]);
+    }
+    const float c2 = S.costs[center];
+    if (c2 != c2) {
+      plane_now = qp; cost_now = fc[max_idx]; S.sel[center] = tsel;
+      S.conf[center] = (float)(0.5 * (double)S.conf[q]);
+    }
+  }
+  S.costs[center] = cost_now;
+  S.planes[center] = plane_now;
+}
+
+// One colour pass.  Thread (tx, y) of a 32 x 8 block handles the tx-th pixel of
+// the pass's colour in row y: x = 2 tx + ((y + colour) & 1).
+__global__ __launch_bounds__(256) void k_propagate(const RunConsts* __restrict__ Cp, DevState S, int colour, int iter, int H2) {
+  const RunConsts& C = *Cp;
+  const int tx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int py = blockIdx.y * blockDim.y + threadIdx.y;
+  const int px = 2 * tx + ((py + colour) & 1);
+  unsigned nncc = 0, n_ext = 0;
+  bool active = px < C.W && py < H2;
+  if (active && C.prior) {
+    const int center = py * C.W + px;
+    if (S.conf[center] > 0.3f || S.masks[center] == 0u) active = false;
+  }
+  if (active) {
+    RefWin rw;
+    build_refwin(C, px, py, rw);
+    Sample s;
+    if (C.mand) propagate_pixel_mand(C, S, px, py, iter, rw, s, nncc, n_ext);
+    else propagate_pixel(C, S, px, py, iter, rw, s, nncc, n_ext);
+  }
+  wave_count_add(&S.counters[1], nncc);
+  // evaluations the reference performs for this pixel: (8 + E + 1 + 5) * V
+  const unsigned ref_count = active ? (unsigned)((14 + n_ext) * C.V) : 0u;
+  wave_count_add(&S.counters[2], ref_count);
+}
+
+__global__ __launch_bounds__(256) void k_output(const RunConsts* __restrict__ Cp, float4* planes) {  // HPM.cu:2829-2842
+  const RunConsts& C = *Cp;
+  const int px = blockIdx.x * blockDim.x + threadIdx.x;
+  const int py = blockIdx.y * blockDim.y + threadIdx.y;
+  if (px >= C.W || py >= C.H) return;
+  const int center = py * C.W + px;
+  float4 pl = planes[center];
+  pl.w = depth_from_plane(C.cams[0], pl, px, py);
+  planes[center] = to_world(C.cams[0], pl);
+}
+
+// ComputeMultiViewCostVector over all pixels at fixed planes (cost microbench).
+__global__ __launch_bounds__(256) void k_cost_fixed(const RunConsts* __restrict__ Cp, const float4* __restrict__ planes,
+                                                    float* __restrict__ out) {
+  const RunConsts& C = *Cp;
+  const int px = blockIdx.x * blockDim.x + threadIdx.x;
+  const int py = blockIdx.y * blockDim.y + threadIdx.y;
+  if (px >= C.W || py >= C.H) return;
+  const int p = py * C.W + px;
+  RefWin rw;
+  build_refwin(C, px, py, rw);
+  const float4 pl = planes[p];
+  for (int v = 0; v < C.V; ++v) out[(size_t)p * C.V + v] = ncc_exact(C, v + 1, px, py, pl, rw);
+}
+
+__global__ __launch_bounds__(256) void k_init_cost(const RunConsts* __restrict__ Cp, const float4* __restrict__ planes,
+                                                   float* __restrict__ out, uint32_t* __restrict__ sel) {
+  const RunConsts& C = *Cp;
+  const int px = blockIdx.x * blockDim.x + threadIdx.x;
+  const int py = blockIdx.y * blockDim.y + threadIdx.y;
+  if (px >= C.W || py >= C.H) return;
+  const int p = py * C.W + px;
+  RefWin rw;
+  build_refwin(C, px, py, rw);
+  unsigned n = 0;
+  uint32_t m;
+  out[p] = init_cost(C, px, py, planes[p], rw, m, n);
+  sel[p] = m;
+}
+
+__global__ __launch_bounds__(256) void k_geom_cost(const RunConsts* __restrict__ Cp, const float4* __restrict__ planes,
+                                                   float* __restrict__ out) {
+  const RunConsts& C = *Cp;
+  const int px = blockIdx.x * blockDim.x + threadIdx.x;
+  const int py = blockIdx.y * blockDim.y + threadIdx.y;
+  if (px >= C.W || py >= C.H) return;
+  const int p = py * C.W + px;
+  const float4 pl = planes[p];
+  for (int v = 0; v < C.V; ++v) out[(size_t)p * C.V + v] = geom_cost(C, v + 1, pl, px, py);
+}
+
+}  // namespace hpm

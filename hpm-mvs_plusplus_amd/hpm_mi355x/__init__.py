@@ -1,0 +1,313 @@
+"""Python binding of libhpm_mi355x.so (include/hpm_mi355x.h).
+
+`HPM` mirrors the reference's host class (HPM.h:57-140): the same method names
+(InuputInitialization, CudaSpaceInitialization, RunPatchMatch,
+GetPlaneHypothesis, GetCost, GetTexture, Set*Params, the prior/JBU entry
+points) over the C ABI, with bulk NumPy getters added.  The HIP library is the
+only compute path: if it is missing, importing works but every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libhpm_mi355x.so")
+
+HPM_OK = 0
+PREC_FP32_EXACT = 0
+PREC_FP16 = 1
+PREC_BF16 = 2
+PREC_FP32_FAST = 3
+PRECISIONS = {"fp32": PREC_FP32_EXACT, "fp16": PREC_FP16, "bf16": PREC_BF16, "fp32fast": PREC_FP32_FAST}
+
+
+class HpmError(RuntimeError):
+    pass
+
+
+class hpm_camera(ctypes.Structure):
+    _fields_ = [("K", ctypes.c_float * 9), ("R", ctypes.c_float * 9), ("t", ctypes.c_float * 3),
+                ("height", ctypes.c_int32), ("width", ctypes.c_int32),
+                ("depth_min", ctypes.c_float), ("depth_max", ctypes.c_float)]
+
+
+class hpm_params(ctypes.Structure):
+    _fields_ = [("patch_size", ctypes.c_int32), ("radius_increment", ctypes.c_int32),
+                ("sigma_spatial", ctypes.c_float), ("sigma_color", ctypes.c_float),
+                ("top_k", ctypes.c_int32), ("geom_consistency", ctypes.c_int32),
+                ("prior_consistency", ctypes.c_int32), ("hierarchy", ctypes.c_int32),
+                ("mand_consistency", ctypes.c_int32), ("num_iterations", ctypes.c_int32),
+                ("precision", ctypes.c_int32)]
+
+
+class hpm_stats(ctypes.Structure):
+    _fields_ = [("ncc_init", ctypes.c_uint64), ("ncc_propagate", ctypes.c_uint64),
+                ("ncc_reference", ctypes.c_uint64), ("ms_init", ctypes.c_double),
+                ("ms_propagate", ctypes.c_double), ("ms_output", ctypes.c_double),
+                ("propagate_launches", ctypes.c_int32)]
+
+
+EXPORTS = [
+    "hpm_default_params", "hpm_create", "hpm_destroy", "hpm_last_error", "hpm_set_params", "hpm_set_views",
+    "hpm_set_src_depths", "hpm_set_canny", "hpm_set_confidence", "hpm_reload_hypotheses", "hpm_set_prior",
+    "hpm_set_coarse", "hpm_run_patchmatch", "hpm_run_patchmatch_partial", "hpm_synchronize", "hpm_get_planes",
+    "hpm_get_costs", "hpm_get_texture", "hpm_get_selected_views", "hpm_get_confidence", "hpm_get_size",
+    "hpm_get_stats", "hpm_set_timing", "hpm_cost_fixed", "hpm_cost_fixed_device", "hpm_init_cost",
+    "hpm_geom_cost", "hpm_jbu_depth", "hpm_jbu_prior",
+]
+
+_lib = None
+
+
+def lib():
+    """Loads the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HpmError(f"{LIB_PATH} is missing: run `python __graft_entry__.py build` (or build.py)")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    fp = ctypes.POINTER(ctypes.c_float)
+    up = ctypes.POINTER(ctypes.c_uint32)
+    L.hpm_default_params.argtypes = [ctypes.POINTER(hpm_params)]
+    L.hpm_default_params.restype = None
+    L.hpm_create.argtypes = [ctypes.POINTER(P), ctypes.c_int]
+    L.hpm_destroy.argtypes = [P]
+    L.hpm_destroy.restype = None
+    L.hpm_last_error.argtypes = [P]
+    L.hpm_last_error.restype = ctypes.c_char_p
+    L.hpm_set_params.argtypes = [P, ctypes.POINTER(hpm_params)]
+    L.hpm_set_views.argtypes = [P, ctypes.c_int, ctypes.POINTER(hpm_camera), ctypes.POINTER(fp)]
+    L.hpm_set_src_depths.argtypes = [P, ctypes.POINTER(fp)]
+    L.hpm_set_canny.argtypes = [P, up]
+    L.hpm_set_confidence.argtypes = [P, fp]
+    L.hpm_reload_hypotheses.argtypes = [P, fp, fp]
+    L.hpm_set_prior.argtypes = [P, fp, up]
+    L.hpm_set_coarse.argtypes = [P, fp, ctypes.c_int, ctypes.c_int, fp]
+    L.hpm_run_patchmatch.argtypes = [P, ctypes.c_uint64]
+    L.hpm_run_patchmatch_partial.argtypes = [P, ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
+    L.hpm_synchronize.argtypes = [P]
+    for n in ("hpm_get_planes", "hpm_get_costs", "hpm_get_texture", "hpm_get_confidence"):
+        getattr(L, n).argtypes = [P, fp]
+    L.hpm_get_selected_views.argtypes = [P, up]
+    L.hpm_get_size.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.hpm_get_stats.argtypes = [P, ctypes.POINTER(hpm_stats)]
+    L.hpm_set_timing.argtypes = [P, ctypes.c_int]
+    L.hpm_cost_fixed.argtypes = [P, fp, fp, ctypes.c_int]
+    L.hpm_cost_fixed_device.argtypes = [P, P, P, ctypes.c_int]
+    L.hpm_init_cost.argtypes = [P, fp, fp, up]
+    L.hpm_geom_cost.argtypes = [P, fp, fp]
+    L.hpm_jbu_depth.argtypes = [fp, ctypes.c_int, ctypes.c_int, fp, ctypes.c_int, ctypes.c_int, fp]
+    L.hpm_jbu_prior.argtypes = [fp, ctypes.c_int, ctypes.c_int, fp, fp, ctypes.c_int, ctypes.c_int, fp, fp]
+    _lib = L
+    return L
+
+
+def _fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _uptr(a: np.ndarray):
+    assert a.dtype == np.uint32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+def make_camera(K, R, t, width, height, depth_min=1.0, depth_max=5.0) -> hpm_camera:
+    c = hpm_camera()
+    K = np.asarray(K, np.float32).reshape(9)
+    R = np.asarray(R, np.float32).reshape(9)
+    t = np.asarray(t, np.float32).reshape(3)
+    for i in range(9):
+        c.K[i] = float(K[i])
+        c.R[i] = float(R[i])
+    for i in range(3):
+        c.t[i] = float(t[i])
+    c.width, c.height = int(width), int(height)
+    c.depth_min, c.depth_max = float(depth_min), float(depth_max)
+    return c
+
+
+def camera_from_scene(cam) -> hpm_camera:
+    return make_camera(cam.K, cam.R, cam.t, cam.width, cam.height, cam.depth_min, cam.depth_max)
+
+
+def default_params(**kw) -> hpm_params:
+    p = hpm_params()
+    lib().hpm_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        if k == "precision" and isinstance(v, str):
+            v = PRECISIONS[v]
+        setattr(p, k, v)
+    return p
+
+
+class Context:
+    """Thin RAII wrapper of one hpm_ctx (one device, one stream)."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        h = ctypes.c_void_p()
+        rc = L.hpm_create(ctypes.byref(h), device)
+        if rc != HPM_OK:
+            raise HpmError(f"hpm_create failed ({rc}): no usable HIP device {device}")
+        self._h = h
+        self._L = L
+        self.width = self.height = self.num_images = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.hpm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != HPM_OK:
+            msg = self._L.hpm_last_error(self._h)
+            raise HpmError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_params(self, params: hpm_params):
+        self._check(self._L.hpm_set_params(self._h, ctypes.byref(params)), "hpm_set_params")
+
+    def set_views(self, cameras, images):
+        n = len(cameras)
+        cams = (hpm_camera * n)(*cameras)
+        imgs = [np.ascontiguousarray(im, dtype=np.float32) for im in images]
+        self._keep = imgs
+        arr = (ctypes.POINTER(ctypes.c_float) * n)(*[_fptr(a) for a in imgs])
+        self._check(self._L.hpm_set_views(self._h, n, cams, arr), "hpm_set_views")
+        self.width, self.height, self.num_images = cameras[0].width, cameras[0].height, n
+
+    def set_src_depths(self, depths):
+        n = self.num_images
+        ds = [None if d is None else np.ascontiguousarray(d, dtype=np.float32) for d in depths]
+        self._keep_d = ds
+        arr = (ctypes.POINTER(ctypes.c_float) * n)(*[(_fptr(a) if a is not None else ctypes.POINTER(ctypes.c_float)()) for a in ds])
+        self._check(self._L.hpm_set_src_depths(self._h, arr), "hpm_set_src_depths")
+
+    def set_canny(self, canny):
+        a = np.ascontiguousarray(canny, dtype=np.uint32)
+        self._check(self._L.hpm_set_canny(self._h, _uptr(a)), "hpm_set_canny")
+
+    def set_confidence(self, conf):
+        a = np.ascontiguousarray(conf, dtype=np.float32)
+        self._check(self._L.hpm_set_confidence(self._h, _fptr(a)), "hpm_set_confidence")
+
+    def reload_hypotheses(self, planes, costs):
+        p = np.ascontiguousarray(planes, dtype=np.float32)
+        c = np.ascontiguousarray(costs, dtype=np.float32)
+        self._check(self._L.hpm_reload_hypotheses(self._h, _fptr(p), _fptr(c)), "hpm_reload_hypotheses")
+
+    def set_prior(self, prior_planes, masks):
+        p = np.ascontiguousarray(prior_planes, dtype=np.float32)
+        m = np.ascontiguousarray(masks, dtype=np.uint32)
+        self._check(self._L.hpm_set_prior(self._h, _fptr(p), _uptr(m)), "hpm_set_prior")
+
+    def set_coarse(self, coarse_planes, fine_depth):
+        cp = np.ascontiguousarray(coarse_planes, dtype=np.float32)
+        fd = np.ascontiguousarray(fine_depth, dtype=np.float32)
+        ch, cw = cp.shape[0], cp.shape[1]
+        self._check(self._L.hpm_set_coarse(self._h, _fptr(cp), cw, ch, _fptr(fd)), "hpm_set_coarse")
+
+    def run(self, seed: int, n_halfpasses: int = -1, do_output: bool = True, sync: bool = True):
+        self._check(self._L.hpm_run_patchmatch_partial(self._h, ctypes.c_uint64(seed), n_halfpasses, int(do_output)),
+                    "hpm_run_patchmatch")
+        if sync:
+            self.synchronize()
+
+    def synchronize(self):
+        self._check(self._L.hpm_synchronize(self._h), "hpm_synchronize")
+
+    def set_timing(self, on=True):
+        self._check(self._L.hpm_set_timing(self._h, int(on)), "hpm_set_timing")
+
+    def stats(self) -> hpm_stats:
+        s = hpm_stats()
+        self._check(self._L.hpm_get_stats(self._h, ctypes.byref(s)), "hpm_get_stats")
+        return s
+
+    def planes(self):
+        out = np.empty((self.height, self.width, 4), np.float32)
+        self._check(self._L.hpm_get_planes(self._h, _fptr(out)), "hpm_get_planes")
+        return out
+
+    def costs(self):
+        out = np.empty((self.height, self.width), np.float32)
+        self._check(self._L.hpm_get_costs(self._h, _fptr(out)), "hpm_get_costs")
+        return out
+
+    def texture(self):
+        out = np.empty((self.height, self.width), np.float32)
+        self._check(self._L.hpm_get_texture(self._h, _fptr(out)), "hpm_get_texture")
+        return out
+
+    def confidence(self):
+        out = np.empty((self.height, self.width), np.float32)
+        self._check(self._L.hpm_get_confidence(self._h, _fptr(out)), "hpm_get_confidence")
+        return out
+
+    def selected_views(self):
+        out = np.empty((self.height, self.width), np.uint32)
+        self._check(self._L.hpm_get_selected_views(self._h, _uptr(out)), "hpm_get_selected_views")
+        return out
+
+    def cost_fixed(self, planes, precision=PREC_FP32_EXACT):
+        p = np.ascontiguousarray(planes, dtype=np.float32)
+        out = np.empty((self.height, self.width, self.num_images - 1), np.float32)
+        self._check(self._L.hpm_cost_fixed(self._h, _fptr(p), _fptr(out), int(precision)), "hpm_cost_fixed")
+        return out
+
+    def cost_fixed_device(self, planes_ptr: int, out_ptr: int, precision=PREC_FP32_EXACT):
+        self._check(self._L.hpm_cost_fixed_device(self._h, ctypes.c_void_p(planes_ptr), ctypes.c_void_p(out_ptr),
+                                                  int(precision)), "hpm_cost_fixed_device")
+
+    def init_cost(self, planes):
+        p = np.ascontiguousarray(planes, dtype=np.float32)
+        c = np.empty((self.height, self.width), np.float32)
+        m = np.empty((self.height, self.width), np.uint32)
+        self._check(self._L.hpm_init_cost(self._h, _fptr(p), _fptr(c), _uptr(m)), "hpm_init_cost")
+        return c, m
+
+    def geom_cost(self, planes):
+        p = np.ascontiguousarray(planes, dtype=np.float32)
+        out = np.empty((self.height, self.width, self.num_images - 1), np.float32)
+        self._check(self._L.hpm_geom_cost(self._h, _fptr(p), _fptr(out)), "hpm_geom_cost")
+        return out
+
+
+def jbu_depth(fine_gray, coarse_depth):
+    f = np.ascontiguousarray(fine_gray, np.float32)
+    c = np.ascontiguousarray(coarse_depth, np.float32)
+    out = np.empty_like(f)
+    rc = lib().hpm_jbu_depth(_fptr(f), f.shape[1], f.shape[0], _fptr(c), c.shape[1], c.shape[0], _fptr(out))
+    if rc != HPM_OK:
+        raise HpmError(f"hpm_jbu_depth failed ({rc})")
+    return out
+
+
+def jbu_prior(fine_gray, coarse_depth, coarse_normals):
+    f = np.ascontiguousarray(fine_gray, np.float32)
+    c = np.ascontiguousarray(coarse_depth, np.float32)
+    cn = np.zeros(c.shape + (4,), np.float32)
+    cn[..., :3] = coarse_normals[..., :3]
+    od = np.empty_like(f)
+    on = np.empty(f.shape + (4,), np.float32)
+    rc = lib().hpm_jbu_prior(_fptr(f), f.shape[1], f.shape[0], _fptr(c), _fptr(cn), c.shape[1], c.shape[0],
+                             _fptr(od), _fptr(on))
+    if rc != HPM_OK:
+        raise HpmError(f"hpm_jbu_prior failed ({rc})")
+    return od, on
+

@@ -1,0 +1,294 @@
+"""Seeded synthetic multi-view scenes: textured planes viewed by pinhole cameras.
+
+The reference ships no data (it reads COLMAP-converted `images/`, `cams/`,
+`pair.txt` folders, main.cpp:123-124,173 and HPM.cpp:159-184).  The workloads
+here are ray-cast procedurally so that ground-truth depth and normals are known:
+
+* planes with offsets U[lo, hi] m along the scene axis and normals within a
+  maximum angle of it, each textured with seeded 4-octave value noise;
+* a fronto-parallel background plane at 5 m so that every ray hits a surface;
+* cameras on a horizontal arc (or two rings) aimed at the origin;
+* 8-bit grey images with additive Gaussian noise.
+
+Camera convention is the reference's: x_cam = R X + t, K the intrinsics
+(main.h:39-47); world y points down, the scene axis is +z.
+"""
+from __future__ import annotations
+
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class Camera:
+    K: np.ndarray  # 3x3
+    R: np.ndarray  # 3x3 world->camera
+    t: np.ndarray  # 3
+    width: int
+    height: int
+    depth_min: float = 1.0
+    depth_max: float = 5.0
+
+    @property
+    def center(self) -> np.ndarray:
+        return -self.R.T @ self.t
+
+
+@dataclass
+class Plane:
+    normal: np.ndarray  # unit, world frame, facing the cameras (n_z < 0)
+    point: np.ndarray   # a point on the plane
+    tex_seed: int
+    e1: np.ndarray = field(default=None)
+    e2: np.ndarray = field(default=None)
+
+    def __post_init__(self):
+        a = np.array([1.0, 0.0, 0.0]) if abs(self.normal[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+        e1 = a - self.normal * (a @ self.normal)
+        self.e1 = e1 / np.linalg.norm(e1)
+        self.e2 = np.cross(self.normal, self.e1)
+
+
+@dataclass
+class Scene:
+    planes: list
+    cameras: list
+    images: list          # uint8 HxW per camera
+    depths: list          # float32 ground-truth depth per camera
+    normals: list         # float32 HxWx3 world normals per camera
+    pairs: list           # per camera: list of (src_id, score)
+
+
+def look_at_camera(center, target, f, cx, cy, width, height, down=(0.0, 1.0, 0.0)) -> Camera:
+    center = np.asarray(center, dtype=np.float64)
+    fwd = np.asarray(target, dtype=np.float64) - center
+    fwd /= np.linalg.norm(fwd)
+    d = np.asarray(down, dtype=np.float64)
+    right = np.cross(d, fwd)
+    right /= np.linalg.norm(right)
+    down_v = np.cross(fwd, right)
+    R = np.stack([right, down_v, fwd])
+    t = -R @ center
+    K = np.array([[f, 0.0, cx], [0.0, f, cy], [0.0, 0.0, 1.0]])
+    return Camera(K=K, R=R, t=t, width=width, height=height)
+
+
+def arc_cameras(n, radius, arc_deg, f, cx, cy, width, height):
+    cams = []
+    for k in range(n):
+        th = math.radians(-arc_deg / 2 + arc_deg * k / max(n - 1, 1))
+        c = radius * np.array([math.sin(th), 0.0, -math.cos(th)])
+        cams.append(look_at_camera(c, (0.0, 0.0, 0.0), f, cx, cy, width, height))
+    return cams
+
+
+def ring_cameras(n, radius, elevations_deg, f, cx, cy, width, height, span_deg=60.0):
+    cams = []
+    per = [n // len(elevations_deg) + (1 if i < n % len(elevations_deg) else 0) for i in range(len(elevations_deg))]
+    for el, m in zip(elevations_deg, per):
+        e = math.radians(el)
+        for k in range(m):
+            az = math.radians(-span_deg / 2 + span_deg * k / max(m - 1, 1))
+            c = radius * np.array([math.cos(e) * math.sin(az), -math.sin(e), -math.cos(e) * math.cos(az)])
+            cams.append(look_at_camera(c, (0.0, 0.0, 0.0), f, cx, cy, width, height))
+    return cams
+
+
+def random_planes(rng, n, axis_origin, offset_lo, offset_hi, max_tilt_deg):
+    planes = []
+    for _ in range(n):
+        off = rng.uniform(offset_lo, offset_hi)
+        # normal within max_tilt of -z (facing the cameras), uniform in solid angle
+        cos_max = math.cos(math.radians(max_tilt_deg))
+        cz = rng.uniform(cos_max, 1.0)
+        phi = rng.uniform(0, 2 * math.pi)
+        sz = math.sqrt(max(0.0, 1 - cz * cz))
+        n_ = np.array([sz * math.cos(phi), sz * math.sin(phi), -cz])
+        p = np.asarray(axis_origin, dtype=np.float64) + np.array([0.0, 0.0, off])
+        planes.append(Plane(normal=n_, point=p, tex_seed=int(rng.integers(1 << 30))))
+    return planes
+
+
+def value_noise(u, v, seed, octaves=4, base_freq=6.0):
+    """4-octave value noise in [0, 1]: smoothstep-interpolated lattice values."""
+    tab_rng = np.random.default_rng(seed)
+    out = np.zeros_like(u, dtype=np.float64)
+    amp, norm = 1.0, 0.0
+    for o in range(octaves):
+        table = tab_rng.random((256, 256))
+        f = base_freq * (2 ** o)
+        x = u * f
+        y = v * f
+        x0 = np.floor(x)
+        y0 = np.floor(y)
+        fx = x - x0
+        fy = y - y0
+        sx = fx * fx * (3 - 2 * fx)
+        sy = fy * fy * (3 - 2 * fy)
+        ix = x0.astype(np.int64) & 255
+        iy = y0.astype(np.int64) & 255
+        ix1 = (ix + 1) & 255
+        iy1 = (iy + 1) & 255
+        a = table[iy, ix]
+        b = table[iy, ix1]
+        c = table[iy1, ix]
+        d = table[iy1, ix1]
+        val = (a * (1 - sx) + b * sx) * (1 - sy) + (c * (1 - sx) + d * sx) * sy
+        out += amp * val
+        norm += amp
+        amp *= 0.5
+    return out / norm
+
+
+def render(cam: Camera, planes, noise_sigma, rng, rows_per_chunk=256):
+    """Ray-cast the nearest plane per pixel: uint8 image, GT depth, GT world normal."""
+    H, W = cam.height, cam.width
+    img = np.zeros((H, W), np.float64)
+    depth = np.zeros((H, W), np.float32)
+    normal = np.zeros((H, W, 3), np.float32)
+    Kinv = np.linalg.inv(cam.K)
+    C = cam.center
+    all_planes = list(planes)
+    xs = np.arange(W, dtype=np.float64)
+    for r0 in range(0, H, rows_per_chunk):
+        r1 = min(H, r0 + rows_per_chunk)
+        ys = np.arange(r0, r1, dtype=np.float64)
+        u, v = np.meshgrid(xs, ys)
+        pix = np.stack([u, v, np.ones_like(u)], axis=-1)
+        rays_cam = pix @ Kinv.T                 # z = 1 in camera frame
+        rays_w = rays_cam @ cam.R               # R^T applied to each row vector
+        best_t = np.full(u.shape, np.inf)
+        best_k = np.full(u.shape, -1, np.int64)
+        for k, pl in enumerate(all_planes):
+            denom = rays_w @ pl.normal
+            num = (pl.point - C) @ pl.normal
+            with np.errstate(divide="ignore", invalid="ignore"):
+                t = num / denom
+            ok = (t > 0.5) & (t < 8.0) & np.isfinite(t)
+            better = ok & (t < best_t)
+            best_t = np.where(better, t, best_t)
+            best_k = np.where(better, k, best_k)
+        # rays_cam has z=1, so t is the depth along the optical axis
+        depth[r0 - 0:r1 - 0] = np.where(np.isfinite(best_t), best_t, 0).astype(np.float32)
+        X = C[None, None, :] + best_t[..., None] * rays_w
+        shade = np.zeros(u.shape)
+        for k, pl in enumerate(all_planes):
+            m = best_k == k
+            if not np.any(m):
+                continue
+            d = X[m] - pl.point
+            pu = d @ pl.e1
+            pv = d @ pl.e2
+            shade[m] = value_noise(pu, pv, pl.tex_seed)
+            normal[r0:r1][m] = pl.normal.astype(np.float32)
+        img[r0:r1] = 20.0 + 215.0 * shade
+    img += rng.normal(0.0, noise_sigma, img.shape)
+    img8 = np.clip(np.rint(img), 0, 255).astype(np.uint8)
+    return img8, depth, normal
+
+
+def pair_scores(cams, k):
+    """Top-k neighbours by a baseline-angle score (peaks at 5 deg, like colmap2mvsnet)."""
+    n = len(cams)
+    pairs = []
+    for i in range(n):
+        sc = []
+        for j in range(n):
+            if j == i:
+                continue
+            a = cams[i].center
+            b = cams[j].center
+            ang = math.degrees(math.acos(np.clip((a @ b) / (np.linalg.norm(a) * np.linalg.norm(b)), -1, 1)))
+            s = math.exp(-((ang - 5.0) ** 2) / (2 * (1.0 if ang <= 5.0 else 10.0) ** 2))
+            sc.append((j, s))
+        sc.sort(key=lambda x: -x[1])
+        pairs.append(sc[:k])
+    return pairs
+
+
+def make_plane_scene(seed=1, n_planes=8, n_cams=10, width=640, height=480, f=525.0, cx=None, cy=None,
+                     arc_deg=30.0, radius=3.0, offset_lo=2.0, offset_hi=4.0, max_tilt_deg=50.0,
+                     noise_sigma=2.0, n_src=9, render_views=None) -> Scene:
+    cx = width / 2 if cx is None else cx
+    cy = height / 2 if cy is None else cy
+    rng = np.random.default_rng(seed)
+    cams = arc_cameras(n_cams, radius, arc_deg, f, cx, cy, width, height)
+    mid = cams[len(cams) // 2].center
+    planes = random_planes(rng, n_planes, (0.0, 0.0, mid[2]), offset_lo, offset_hi, max_tilt_deg)
+    # fronto-parallel background 5 m in front of the middle camera
+    planes.append(Plane(normal=np.array([0.0, 0.0, -1.0]), point=np.array([0.0, 0.0, mid[2] + 5.0]),
+                        tex_seed=int(rng.integers(1 << 30))))
+    images, depths, normals = [], [], []
+    noise_rng = np.random.default_rng(seed + 2)
+    for idx, c in enumerate(cams):
+        if render_views is not None and idx not in render_views:
+            images.append(None); depths.append(None); normals.append(None)
+            continue
+        im, d, nm = render(c, planes, noise_sigma, noise_rng)
+        images.append(im); depths.append(d); normals.append(nm)
+    return Scene(planes=planes, cameras=cams, images=images, depths=depths, normals=normals,
+                 pairs=pair_scores(cams, n_src))
+
+
+def reference_view_problem(scene: Scene, ref: int, n_src: int):
+    """(ref camera, source cameras, images as float32) for one RunPatchMatch."""
+    ids = [ref] + [j for j, s in scene.pairs[ref][:n_src] if s > 0]
+    cams = [scene.cameras[i] for i in ids]
+    imgs = [scene.images[i].astype(np.float32) for i in ids]
+    return ids, cams, imgs
+
+
+def random_planes_per_pixel(cam: Camera, seed, depth_lo=1.0, depth_hi=5.0, max_angle_deg=60.0):
+    """Per-pixel random planes in the reference-camera frame (normal within
+    max_angle of the reversed view ray), stored as (nx, ny, nz, d) with
+    n . X + d = 0 (HPM.cu:135-140)."""
+    rng = np.random.default_rng(seed)
+    H, W = cam.height, cam.width
+    u, v = np.meshgrid(np.arange(W, dtype=np.float64), np.arange(H, dtype=np.float64))
+    K = cam.K
+    ray = np.stack([(u - K[0, 2]) / K[0, 0], (v - K[1, 2]) / K[1, 1], np.ones_like(u)], -1)
+    ray /= np.linalg.norm(ray, axis=-1, keepdims=True)
+    depth = rng.uniform(depth_lo, depth_hi, (H, W))
+    cos_max = math.cos(math.radians(max_angle_deg))
+    cz = rng.uniform(cos_max, 1.0, (H, W))
+    phi = rng.uniform(0, 2 * math.pi, (H, W))
+    sz = np.sqrt(1 - cz * cz)
+    # build a basis around -ray
+    a = -ray
+    tmp = np.where(np.abs(a[..., :1]) < 0.9, np.array([1.0, 0, 0]), np.array([0, 1.0, 0]))
+    e1 = tmp - a * np.sum(tmp * a, -1, keepdims=True)
+    e1 /= np.linalg.norm(e1, axis=-1, keepdims=True)
+    e2 = np.cross(a, e1)
+    n = cz[..., None] * a + sz[..., None] * (np.cos(phi)[..., None] * e1 + np.sin(phi)[..., None] * e2)
+    X = np.stack([depth * (u - K[0, 2]) / K[0, 0], depth * (v - K[1, 2]) / K[1, 1], depth], -1)
+    d = -np.sum(n * X, -1)
+    out = np.concatenate([n, d[..., None]], -1).astype(np.float32)
+    return out
+
+
+# ----------------------------------------------------------------- file formats
+def write_dense_folder(scene: Scene, folder: str, depth_lo=1.0, depth_hi=5.0, n_depth=192):
+    """images/%08d.pgm, cams/%08d_cam.txt and pair.txt in the layout the
+    reference reads (ReadCamera HPM.cpp:159-184, GenerateSampleList
+    main.cpp:171-200).  Images are binary PGM (no JPEG codec here)."""
+    os.makedirs(os.path.join(folder, "images"), exist_ok=True)
+    os.makedirs(os.path.join(folder, "cams"), exist_ok=True)
+    for i, (cam, im) in enumerate(zip(scene.cameras, scene.images)):
+        with open(os.path.join(folder, "images", f"{i:08d}.pgm"), "wb") as fh:
+            fh.write(f"P5\n{cam.width} {cam.height}\n255\n".encode())
+            fh.write(np.ascontiguousarray(im).tobytes())
+        with open(os.path.join(folder, "cams", f"{i:08d}_cam.txt"), "w") as fh:
+            fh.write("extrinsic\n")
+            for r in range(3):
+                fh.write(" ".join(f"{x:.9g}" for x in cam.R[r]) + f" {cam.t[r]:.9g}\n")
+            fh.write("0 0 0 1\n\nintrinsic\n")
+            for r in range(3):
+                fh.write(" ".join(f"{x:.9g}" for x in cam.K[r]) + "\n")
+            fh.write(f"\n{depth_lo} {(depth_hi - depth_lo) / n_depth} {n_depth} {depth_hi}\n")
+    with open(os.path.join(folder, "pair.txt"), "w") as fh:
+        fh.write(f"{len(scene.cameras)}\n")
+        for i, pr in enumerate(scene.pairs):
+            fh.write(f"{i}\n{len(pr)} " + " ".join(f"{j} {s:.6f}" for j, s in pr) + "\n")

@@ -1,0 +1,187 @@
+/* hpm_mi355x.h -- C ABI of the MI355X PatchMatch depth/normal estimator.
+ *
+ * Drop-in boundary for HPM-MVS++'s `class HPM` (HPM.h:57-140) and the JBU
+ * helpers (HPM.h:158-202, RunJBU HPM.h:20).  The reference is a C++ class used
+ * in-process by ProcessProblem (main.cpp:237-634); here the same operations are
+ * plain C functions on an opaque context so that the C++ facade
+ * (hpm-mvs_plusplus_amd/host/HPM.h), Python (ctypes) or any FFI can call them.
+ *
+ * Conventions
+ *   - Every function returns HPM_OK (0) or a negative HPM_E* status; it never
+ *     exits the process (the reference's CudaSafeCall exits, HPM.cpp:68-76).
+ *     hpm_last_error() gives the message of the last failure on a context.
+ *   - Images and per-pixel maps are row-major, width = cameras[i].width.
+ *   - "planes" are float4 per pixel (x, y, z, w).  What x,y,z,w hold is stated
+ *     per function, following the reference.
+ *   - One context owns one HIP stream on one device; calls on a context are not
+ *     thread-safe (the reference is single-threaded too, main.cpp:250).
+ */
+#ifndef HPM_MI355X_H_
+#define HPM_MI355X_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HPM_OK 0
+#define HPM_E_INVALID (-1)      /* bad argument / unsupported configuration */
+#define HPM_E_STATE (-2)        /* call out of order (e.g. run before set_views) */
+#define HPM_E_DEVICE (-3)       /* HIP runtime error */
+#define HPM_E_NOMEM (-4)        /* device allocation failed */
+
+#define HPM_MAX_IMAGES 33       /* reference + 32 sources: view masks are uint32 (HPM.cu:67-75) */
+
+/* Precision of the image storage used by the cost.  All accumulation is FP32.
+ * FP32_EXACT reproduces the FP32 oracle bit for bit. */
+#define HPM_PREC_FP32_EXACT 0
+#define HPM_PREC_FP16 1
+#define HPM_PREC_BF16 2
+#define HPM_PREC_FP32_FAST 3
+
+/* Mirrors struct Camera (main.h:39-47): x_cam = R * X + t. 92 bytes. */
+typedef struct hpm_camera {
+  float K[9];
+  float R[9];
+  float t[3];
+  int32_t height;
+  int32_t width;
+  float depth_min;
+  float depth_max;
+} hpm_camera;
+
+/* The live fields of PatchMatchParams (HPM.h:32-55) plus the pass mode that
+ * SetGeomConsistencyParams / SetPlanarPriorParams / SetHierarchyParams /
+ * SetMandConsistencyParams select (HPM.cpp:565-587). */
+typedef struct hpm_params {
+  int32_t patch_size;        /* 11 (only 11 with radius_increment 2 is supported) */
+  int32_t radius_increment;  /* 2 */
+  float sigma_spatial;       /* 5 */
+  float sigma_color;         /* 3 */
+  int32_t top_k;             /* 4 */
+  int32_t geom_consistency;
+  int32_t prior_consistency;
+  int32_t hierarchy;
+  int32_t mand_consistency;
+  int32_t num_iterations;    /* 3 (HPM.cu:3013 hard-codes 3) */
+  int32_t precision;         /* HPM_PREC_* */
+} hpm_params;
+
+typedef struct hpm_ctx hpm_ctx;
+
+/* Fills the reference defaults (HPM.h:32-55, 3 iterations, FP32 exact). */
+void hpm_default_params(hpm_params* p);
+
+/* HPM::HPM (HPM.cpp:103-113): creates a context on HIP device `device`. */
+int hpm_create(hpm_ctx** out, int device);
+/* HPM::~HPM + CudaSpaceRelease (HPM.cpp:115-157, 599-633). NULL is a no-op. */
+void hpm_destroy(hpm_ctx* ctx);
+const char* hpm_last_error(const hpm_ctx* ctx);
+
+/* Set* mode calls (HPM.cpp:565-587) in one struct. */
+int hpm_set_params(hpm_ctx* ctx, const hpm_params* params);
+
+/* InuputInitialization (HPM.cpp:644-733) + the image/camera half of
+ * CudaSpaceInitialization (HPM.cpp:741-772): num_images = 1 reference + sources,
+ * grey float images (HPM.cpp:660-661), cameras already scaled to the image size
+ * (HPM.cpp:668-676).  The depth range is cameras[0].depth_min*0.6 ..
+ * depth_max*1.2 (HPM.cpp:685-686).  Copies everything to the device and
+ * allocates the per-pixel state. */
+int hpm_set_views(hpm_ctx* ctx, int num_images, const hpm_camera* cameras, const float* const* images);
+
+/* Geometric-consistency depth textures (HPM.cpp:693-732, 786-811): one depth
+ * map per image, index 0 (the reference's own) is not read by the cost. */
+int hpm_set_src_depths(hpm_ctx* ctx, const float* const* depths);
+
+/* CudaCannyInitialization (HPM.cpp:908-920): edge map of the reference image,
+ * nonzero = edge (cv::Canny output, main.cpp:287). */
+int hpm_set_canny(hpm_ctx* ctx, const uint32_t* canny);
+
+/* CudaConfidenceInitialization (HPM.cpp:922-940). */
+int hpm_set_confidence(hpm_ctx* ctx, const float* confidences);
+
+/* CudaHypothesesReload (HPM.cpp:942-962) and the geometric-pass reload
+ * (HPM.cpp:813-846): planes = (world normal xyz, depth w) per pixel, costs. */
+int hpm_reload_hypotheses(hpm_ctx* ctx, const float* planes, const float* costs);
+
+/* CudaPlanarPriorInitialization / ReloadPlanarPriorInitialization
+ * (HPM.cpp:964-984, 1433-1453): prior plane per pixel and triangle mask
+ * (0 = no prior, else triangle index + 1). */
+int hpm_set_prior(hpm_ctx* ctx, const float* prior_planes, const uint32_t* masks);
+
+/* Hierarchy reload (HPM.cpp:849-905): the coarse level's maps (cw x ch) and the
+ * JBU-upsampled fine depth.  coarse = (world normal xyz, w) where w is the
+ * coarse cost when the sizes differ (`upsample`) else the coarse depth.  The
+ * reference's size test compares width with rows (HPM.cpp:869); that test is
+ * kept.  The fine planes get xyz = 0 and w = fine depth (HPM.cpp:894-901). */
+int hpm_set_coarse(hpm_ctx* ctx, const float* coarse_planes, int cw, int ch, const float* fine_depth);
+
+/* RunPatchMatch (HPM.cu:2987-3035): initialisation, num_iterations red/black
+ * sweeps and the plane -> (world normal, depth) conversion.  `seed` keys the
+ * per-pixel random streams.  Asynchronous on the context stream. */
+int hpm_run_patchmatch(hpm_ctx* ctx, uint64_t seed);
+
+/* Partial schedule for tests: run initialisation plus `n_halfpasses` colour
+ * passes (-1 = all) and the output conversion if do_output != 0. */
+int hpm_run_patchmatch_partial(hpm_ctx* ctx, uint64_t seed, int n_halfpasses, int do_output);
+
+/* Waits for the context stream. */
+int hpm_synchronize(hpm_ctx* ctx);
+
+/* Bulk getters replacing GetPlaneHypothesis / GetCost / GetTexture
+ * (HPM.cpp:1001-1014) and the D2H copies of HPM.cu:3029-3033.  Each copies
+ * width*height elements (planes: 4 floats per pixel). */
+int hpm_get_planes(hpm_ctx* ctx, float* out_planes);
+int hpm_get_costs(hpm_ctx* ctx, float* out_costs);
+int hpm_get_texture(hpm_ctx* ctx, float* out_texture);
+int hpm_get_selected_views(hpm_ctx* ctx, uint32_t* out_masks);
+int hpm_get_confidence(hpm_ctx* ctx, float* out_confidences);
+int hpm_get_size(const hpm_ctx* ctx, int* width, int* height, int* num_images);
+
+/* Work counters of the last run: NCC evaluations per kernel class. */
+typedef struct hpm_stats {
+  uint64_t ncc_init;        /* NCC evaluations in the initialisation kernel */
+  uint64_t ncc_propagate;   /* NCC evaluations in the red/black kernels */
+  uint64_t ncc_reference;   /* evaluations the reference would perform: V at init,
+                               (14 + E) * V per updated pixel and iteration */
+  double ms_init;           /* device time of the init kernel (HIP events) */
+  double ms_propagate;      /* summed device time of the red/black kernels */
+  double ms_output;
+  int32_t propagate_launches;
+} hpm_stats;
+int hpm_get_stats(hpm_ctx* ctx, hpm_stats* out);
+/* Enables HIP-event timing of each kernel class (adds no work to the kernels). */
+int hpm_set_timing(hpm_ctx* ctx, int enable);
+
+/* Cost microbenchmark at fixed hypotheses (ComputeMultiViewCostVector,
+ * HPM.cu:490-495 over every pixel): planes = (ref-camera normal, d) with
+ * n.X + d = 0; out = width*height*(num_images-1) costs, pixel-major.
+ * Host pointers. */
+int hpm_cost_fixed(hpm_ctx* ctx, const float* planes, float* out_costs, int precision);
+/* Same on device pointers already in HBM (for timing), asynchronous. */
+int hpm_cost_fixed_device(hpm_ctx* ctx, const void* planes_dev, void* out_costs_dev, int precision);
+
+/* ComputeMultiViewInitialCostandSelectedViews (HPM.cu:450-488) at fixed planes. */
+int hpm_init_cost(hpm_ctx* ctx, const float* planes, float* out_cost, uint32_t* out_masks);
+
+/* ComputeGeomConsistencyCost (HPM.cu:539-564) for every (pixel, source view);
+ * needs hpm_set_src_depths.  out = width*height*(num_images-1). */
+int hpm_geom_cost(hpm_ctx* ctx, const float* planes, float* out_costs);
+
+/* JBU::CudaRun / RunJBU (HPM.cu:3037-3128, HPM.cpp:1266-1319): joint bilateral
+ * upsampling of a coarse depth map (cw x ch) guided by the fine grey image
+ * (w x h).  No NaN patching (RunJBU does that on the host, HPM.cpp:1297-1300). */
+int hpm_jbu_depth(const float* fine_gray, int w, int h, const float* coarse_depth, int cw, int ch, float* out_depth);
+
+/* JBU_prior::CudaRun_prior (HPM.cu:3130-3235): upsamples depth and normal
+ * (coarse_normals = 4 floats per coarse pixel, xyz used). out_normals gets 4
+ * floats per fine pixel. */
+int hpm_jbu_prior(const float* fine_gray, int w, int h, const float* coarse_depth, const float* coarse_normals, int cw,
+                  int ch, float* out_depth, float* out_normals);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HPM_MI355X_H_ */

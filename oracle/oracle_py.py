@@ -1,0 +1,198 @@
+"""ctypes binding of oracle/lib/liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import
+this module; the product never does.  See oracle_pm.cpp for what each entry
+point restates.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "liboracle.so")
+
+
+class orc_camera(ctypes.Structure):
+    _fields_ = [("K", ctypes.c_float * 9), ("R", ctypes.c_float * 9), ("t", ctypes.c_float * 3),
+                ("height", ctypes.c_int32), ("width", ctypes.c_int32),
+                ("depth_min", ctypes.c_float), ("depth_max", ctypes.c_float)]
+
+
+class orc_params(ctypes.Structure):
+    _fields_ = [("patch_size", ctypes.c_int), ("radius_increment", ctypes.c_int),
+                ("sigma_spatial", ctypes.c_float), ("sigma_color", ctypes.c_float), ("top_k", ctypes.c_int),
+                ("num_images", ctypes.c_int), ("depth_min", ctypes.c_float), ("depth_max", ctypes.c_float),
+                ("scaled_cols", ctypes.c_float), ("scaled_rows", ctypes.c_float),
+                ("geom_consistency", ctypes.c_int), ("prior_consistency", ctypes.c_int),
+                ("hierarchy", ctypes.c_int), ("upsample", ctypes.c_int), ("mand_consistency", ctypes.c_int),
+                ("num_iterations", ctypes.c_int)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        f32 = ctypes.c_float
+        L.orc_expf.argtypes = [f32]; L.orc_expf.restype = f32
+        L.orc_exp.argtypes = [ctypes.c_double]; L.orc_exp.restype = ctypes.c_double
+        L.orc_sinf.argtypes = [f32]; L.orc_sinf.restype = f32
+        L.orc_cosf.argtypes = [f32]; L.orc_cosf.restype = f32
+        L.orc_bilinear.restype = f32
+        L.orc_bilinear.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, f32, f32]
+        L.orc_bilinear_f64.restype = ctypes.c_double
+        L.orc_bilinear_f64.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.orc_uniforms.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p]
+        L.orc_philox4x32_10.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_homography_f64.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_depth_from_plane_f64.restype = f32
+        L.orc_depth_from_plane_f64.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.orc_distance_to_origin_f64.restype = ctypes.c_double
+        L.orc_distance_to_origin_f64.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_void_p]
+        L.orc_random_plane_f32.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32,
+                                           f32, f32, ctypes.c_void_p]
+        for n in ("orc_run_patchmatch_f32", "orc_run_patchmatch_f64"):
+            getattr(L, n).argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_uint64] + [ctypes.c_void_p] * 10 + [ctypes.c_int] * 4
+        for n in ("orc_cost_fixed_f32", "orc_cost_fixed_f64"):
+            getattr(L, n).argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.orc_init_cost_f32.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_geom_cost_f32.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        _lib = L
+    return _lib
+
+
+def camera(K, R, t, width, height, depth_min=1.0, depth_max=5.0) -> orc_camera:
+    c = orc_camera()
+    for i, v in enumerate(np.asarray(K, np.float32).reshape(9)):
+        c.K[i] = float(v)
+    for i, v in enumerate(np.asarray(R, np.float32).reshape(9)):
+        c.R[i] = float(v)
+    for i, v in enumerate(np.asarray(t, np.float32).reshape(3)):
+        c.t[i] = float(v)
+    c.width, c.height, c.depth_min, c.depth_max = int(width), int(height), float(depth_min), float(depth_max)
+    return c
+
+
+def camera_from_scene(cam) -> orc_camera:
+    return camera(cam.K, cam.R, cam.t, cam.width, cam.height, cam.depth_min, cam.depth_max)
+
+
+def params(num_images, cam0: orc_camera, **kw) -> orc_params:
+    p = orc_params()
+    p.patch_size, p.radius_increment, p.sigma_spatial, p.sigma_color, p.top_k = 11, 2, 5.0, 3.0, 4
+    p.num_images = num_images
+    # HPM.cpp:685-686, float products
+    p.depth_min = float(np.float32(cam0.depth_min) * np.float32(0.6))
+    p.depth_max = float(np.float32(cam0.depth_max) * np.float32(1.2))
+    p.num_iterations = 3
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def _cams_array(cams):
+    return (orc_camera * len(cams))(*cams)
+
+
+def _img_ptrs(images):
+    imgs = [np.ascontiguousarray(im, np.float32) for im in images]
+    arr = (ctypes.c_void_p * len(imgs))(*[a.ctypes.data for a in imgs])
+    return imgs, arr
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def run_patchmatch(cams, images, prm: orc_params, seed, dtype=np.float32, planes=None, costs=None, pre_costs=None,
+                   selected=None, prior=None, masks=None, conf=None, canny=None, texture=None, scaled=None,
+                   depth_images=None, n_halfpasses=-1, do_output=True, rows=(-1, -1)):
+    """Runs the oracle's RunPatchMatch.  Buffers default to zeros; returns a dict."""
+    W, H = cams[0].width, cams[0].height
+    z = lambda shape, dt=dtype: np.zeros(shape, dt)
+    planes = z((H, W, 4)) if planes is None else np.ascontiguousarray(planes, dtype)
+    costs = z((H, W)) if costs is None else np.ascontiguousarray(costs, dtype)
+    pre_costs = z((H, W)) if pre_costs is None else np.ascontiguousarray(pre_costs, dtype)
+    selected = z((H, W), np.uint32) if selected is None else np.ascontiguousarray(selected, np.uint32)
+    conf = z((H, W)) if conf is None else np.ascontiguousarray(conf, dtype)
+    canny = z((H, W), np.uint32) if canny is None else np.ascontiguousarray(canny, np.uint32)
+    texture = z((H, W)) if texture is None else np.ascontiguousarray(texture, dtype)
+    prior = None if prior is None else np.ascontiguousarray(prior, dtype)
+    masks = None if masks is None else np.ascontiguousarray(masks, np.uint32)
+    scaled = None if scaled is None else np.ascontiguousarray(scaled, dtype)
+    imgs, img_arr = _img_ptrs(images)
+    darr = None
+    if depth_images is not None:
+        dimgs = [np.zeros((1, 1), np.float32) if d is None else np.ascontiguousarray(d, np.float32) for d in depth_images]
+        darr = (ctypes.c_void_p * len(dimgs))(*[a.ctypes.data for a in dimgs])
+    fn = lib().orc_run_patchmatch_f32 if dtype == np.float32 else lib().orc_run_patchmatch_f64
+    rc = fn(_cams_array(cams), len(cams), img_arr, darr, ctypes.byref(prm), ctypes.c_uint64(seed), _p(planes),
+            _p(costs), _p(pre_costs), _p(selected), _p(prior), _p(masks), _p(conf), _p(canny), _p(texture),
+            _p(scaled), n_halfpasses, int(do_output), rows[0], rows[1])
+    if rc != 0:
+        raise RuntimeError(f"oracle run failed ({rc})")
+    return dict(planes=planes, costs=costs, pre_costs=pre_costs, selected=selected, conf=conf, texture=texture)
+
+
+def cost_fixed(cams, images, prm, planes, dtype=np.float32, rows=(-1, -1)):
+    W, H = cams[0].width, cams[0].height
+    V = len(cams) - 1
+    pl = np.ascontiguousarray(planes, dtype)
+    out = np.zeros((H, W, V), dtype)
+    imgs, arr = _img_ptrs(images)
+    fn = lib().orc_cost_fixed_f32 if dtype == np.float32 else lib().orc_cost_fixed_f64
+    fn(_cams_array(cams), len(cams), arr, ctypes.byref(prm), _p(pl), _p(out), rows[0], rows[1])
+    return out
+
+
+def init_cost(cams, images, prm, planes):
+    W, H = cams[0].width, cams[0].height
+    pl = np.ascontiguousarray(planes, np.float32)
+    c = np.zeros((H, W), np.float32)
+    m = np.zeros((H, W), np.uint32)
+    imgs, arr = _img_ptrs(images)
+    lib().orc_init_cost_f32(_cams_array(cams), len(cams), arr, ctypes.byref(prm), _p(pl), _p(c), _p(m))
+    return c, m
+
+
+def geom_cost(cams, depth_images, planes):
+    W, H = cams[0].width, cams[0].height
+    V = len(cams) - 1
+    pl = np.ascontiguousarray(planes, np.float32)
+    out = np.zeros((H, W, V), np.float32)
+    dimgs, arr = _img_ptrs([np.zeros((1, 1)) if d is None else d for d in depth_images])
+    lib().orc_geom_cost_f32(_cams_array(cams), len(cams), arr, _p(pl), _p(out))
+    return out
+
+
+def expf(x):
+    return lib().orc_expf(float(x))
+
+
+def uniforms(seed, pixel, stage, n):
+    out = np.zeros(n, np.float32)
+    lib().orc_uniforms(ctypes.c_uint64(seed), pixel, stage, n, out.ctypes.data)
+    return out
+
+
+def philox4x32_10(ctr, key):
+    c = np.asarray(ctr, np.uint32)
+    k = np.asarray(key, np.uint32)
+    o = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(c.ctypes.data, k.ctypes.data, o.ctypes.data)
+    return o
