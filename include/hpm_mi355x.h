@@ -41,7 +41,7 @@ extern "C" {
 #define HPM_PREC_BF16 2
 #define HPM_PREC_FP32_FAST 3
 
-/* Mirrors struct Camera (main.h:39-47): x_cam = R * X + t. 92 bytes. */
+/* Mirrors struct Camera (main.h:39-47): x_cam = R * X + t. 100 bytes. */
 typedef struct hpm_camera {
   float K[9];
   float R[9];

@@ -16,12 +16,20 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and the built HIP library")
 
 
+_GPU = None
+
+
 def gpu_available():
-    try:
-        import torch
-        return torch.cuda.is_available()
-    except Exception:
-        return False
+    """True when the HIP library loads and a HIP device answers (no torch import)."""
+    global _GPU
+    if _GPU is None:
+        try:
+            import hpm_mi355x as H
+            H.Context(0).close()
+            _GPU = True
+        except Exception:
+            _GPU = False
+    return _GPU
 
 
 @pytest.fixture(scope="session")
