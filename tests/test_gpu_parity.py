@@ -55,12 +55,16 @@ def test_cost_fixed_bitexact_and_fp64(small_scene):
     ref64 = O.cost_fixed(oc, imgs, prm, planes.astype(np.float64), np.float64)
     assert _bits_equal(got, ref32), f"{np.sum(got != ref32)} costs differ from the FP32 oracle"
     err = np.abs(got.astype(np.float64) - ref64)
-    # tolerance vs FP64: 1e-5 absolute on every cost whose variance test agrees;
-    # where float rounding moves a window across var = 1e-5 the cost jumps to 2.
+    # Tolerance of the reference's plain FP32 formula against FP64 (measured on
+    # this scene: median 4e-6, 99.9 % quantile 5.7e-5, max 1.3e-4).  The error is
+    # the cancellation in E[x^2] - E[x]^2 and the absolute-pixel homography; the
+    # mixed-precision path removes both (test_fast_path).  Where float rounding
+    # moves a window across var = 1e-5 the cost jumps to 2, so compare branches.
     same_branch = (got == 2.0) == (ref64 == 2.0)
     assert same_branch.mean() > 0.999
-    assert np.quantile(err[same_branch], 0.999) < 1e-5
-    assert err[same_branch].mean() < 1e-6
+    assert np.median(err[same_branch]) < 1e-5
+    assert np.quantile(err[same_branch], 0.999) < 1e-4
+    assert err[same_branch].max() < 5e-4
 
 
 def test_init_bitexact(small_scene):
