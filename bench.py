@@ -64,6 +64,23 @@ def cpu_baseline(cams, imgs, canny, rows=24):
                       f"(+ init halo), {dt:.2f} s, scaled by {H}/{rows}"}
 
 
+def calibration_copy(nbytes):
+    """One plain device-to-device copy of `nbytes` (the runtime's copy kernel),
+    so that profiled HBM counters can be scaled by a transfer of known size."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return nbytes
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +89,8 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16", "fp32fast"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=400)
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (views are independent; "
@@ -133,7 +152,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.precision.startswith("fp32") else args.precision,
+        "dtype": {"fp32": "f32", "fp32fast": "f32"}.get(args.precision, args.precision),
         "data": "synthetic (seeded ray-cast textured planes, 11 cameras on a 30 deg arc)",
         "config": {"workload": "RunPatchMatch photometric, 1600x1200 reference + 10 source views, 3 iterations",
                    "precision": args.precision, "width": W_FULL, "height": H_FULL, "sources": NSRC},
@@ -146,6 +165,8 @@ def main():
         "depth_within_1pct_of_gt": round(within1, 4),
     }
     ctx.close()
+    if args.calibrate_copy_mb:
+        out["calibration_copy_bytes"] = calibration_copy(args.calibrate_copy_mb << 20)
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline (oracle band) ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(cams, imgs, canny, rows=args.cpu_rows)

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/hpm_mi355x.h"
@@ -14,6 +15,13 @@
 #include "hpm_kernels.h"
 
 using namespace hpm;
+
+// Mixed-precision engine: four lanes per pixel (QuadLaneEngine) or one
+// (FastEngine); both give the same costs to within FP32 rounding.
+#ifndef HPM_FAST_LANES
+#define HPM_FAST_LANES 1
+#endif
+template <class S> using FastFor = typename std::conditional<HPM_FAST_LANES == 4, QuadLaneEngine<S>, FastEngine<S>>::type;
 
 struct hpm_ctx {
   int device = 0;
@@ -39,7 +47,6 @@ struct hpm_ctx {
   int scaled_w = 0, scaled_h = 0;
   bool upsample = false;
   RunConsts* d_consts = nullptr;
-  FastConsts* d_fconsts = nullptr;
   unsigned long long* d_counters = nullptr;
   bool timing = false;
   hipEvent_t ev[8] = {};
@@ -125,7 +132,65 @@ int build_consts(hpm_ctx* ctx, RunConsts& C) {
     C.dimg[i] = i < (int)ctx->d_dimg.size() ? ctx->d_dimg[i] : nullptr;
   }
   for (int v = 1; v < ctx->nimg; ++v) pair_basis(ctx->cams[0], ctx->cams[v], C.pair[v - 1]);
+  // FP64 view-pair basis of the mixed-precision engine: H = A - b m^T with
+  // A = K_s R_rel K_r^-1, b = K_s t_rel, m = K_r^-T n / d (zero skew and
+  // K_r[8] = 1, as HPM.cu:292-310 assumes).
+  const hpm_camera& rc = ctx->cams[0];
+  const double fx = rc.K[0], fy = rc.K[4], cx = rc.K[2], cy = rc.K[5];
+  C.kinv[0] = 1.0 / fx; C.kinv[1] = 1.0 / fy; C.kinv[2] = cx / fx; C.kinv[3] = cy / fy;
+  const double Kri[9] = {1.0 / fx, 0.0, -cx / fx, 0.0, 1.0 / fy, -cy / fy, 0.0, 0.0, 1.0};
+  for (int v = 1; v < ctx->nimg; ++v) {
+    const hpm_camera& sc = ctx->cams[v];
+    double rR[9], sR[9], rC[3], sC[3], Rrel[9], trel[3];
+    for (int i = 0; i < 9; ++i) { rR[i] = rc.R[i]; sR[i] = sc.R[i]; }
+    for (int r = 0; r < 3; ++r) {
+      rC[r] = -(rR[0 + r] * rc.t[0] + rR[3 + r] * rc.t[1] + rR[6 + r] * rc.t[2]);
+      sC[r] = -(sR[0 + r] * sc.t[0] + sR[3 + r] * sc.t[1] + sR[6 + r] * sc.t[2]);
+    }
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        Rrel[3 * a + b] = sR[3 * a] * rR[3 * b] + sR[3 * a + 1] * rR[3 * b + 1] + sR[3 * a + 2] * rR[3 * b + 2];
+    for (int a = 0; a < 3; ++a)
+      trel[a] = sR[3 * a] * (rC[0] - sC[0]) + sR[3 * a + 1] * (rC[1] - sC[1]) + sR[3 * a + 2] * (rC[2] - sC[2]);
+    const double Ks[9] = {sc.K[0], 0.0, sc.K[2], 0.0, sc.K[4], sc.K[5], 0.0, 0.0, sc.K[8]};
+    double KR[9];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) KR[3 * a + b] = Ks[3 * a] * Rrel[b] + Ks[3 * a + 1] * Rrel[3 + b] + Ks[3 * a + 2] * Rrel[6 + b];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) C.A[v - 1][3 * a + b] = KR[3 * a] * Kri[b] + KR[3 * a + 1] * Kri[3 + b] + KR[3 * a + 2] * Kri[6 + b];
+    for (int a = 0; a < 3; ++a) C.bvec[v - 1][a] = Ks[3 * a] * trel[0] + Ks[3 * a + 1] * trel[1] + Ks[3 * a + 2] * trel[2];
+  }
+  C.storage = ctx->lp_kind;
+  for (int i = 0; i < ctx->nimg && i < (int)ctx->d_img_lp.size(); ++i) C.quad[i] = ctx->d_img_lp[i];
   return HPM_OK;
+}
+
+// Quad images of every view for the mixed-precision engine (rebuilt when the
+// storage precision changes).
+template <class S> int build_quads(hpm_ctx* ctx) {
+  for (void* p : ctx->d_img_lp) hipFree(p);
+  ctx->d_img_lp.assign(ctx->nimg, nullptr);
+  for (int i = 0; i < ctx->nimg; ++i) {
+    const int w = ctx->cams[i].width, h = ctx->cams[i].height;
+    const size_t n = (size_t)(w + 1) * (h + 1);
+    hipError_t e = hipMalloc(&ctx->d_img_lp[i], n * sizeof(typename S::Q));
+    if (e != hipSuccess) { ctx->d_img_lp[i] = nullptr; return fail(ctx, HPM_E_NOMEM, "quad image allocation failed"); }
+    const dim3 b(16, 16), g((w + 1 + 15) / 16, (h + 1 + 15) / 16);
+    hipLaunchKernelGGL(k_build_quad<S>, g, b, 0, ctx->stream, ctx->d_img[i], w, h, ctx->d_img_lp[i]);
+    if (hipGetLastError() != hipSuccess) return fail(ctx, HPM_E_DEVICE, "k_build_quad launch failed");
+  }
+  ctx->lp_kind = S::kind;
+  return HPM_OK;
+}
+
+int ensure_quads(hpm_ctx* ctx, int precision) {
+  if (ctx->lp_kind == precision && (int)ctx->d_img_lp.size() == ctx->nimg) return HPM_OK;
+  switch (precision) {
+    case HPM_PREC_FP16: return build_quads<QuadF16>(ctx);
+    case HPM_PREC_BF16: return build_quads<QuadBF16>(ctx);
+    case HPM_PREC_FP32_FAST: return build_quads<QuadF32>(ctx);
+    default: return fail(ctx, HPM_E_INVALID, "unknown precision");
+  }
 }
 
 DevState make_state(hpm_ctx* c) {
@@ -151,6 +216,49 @@ int check_params(hpm_ctx* ctx, const hpm_params& p) {
 size_t npix(const hpm_ctx* c) { return (size_t)c->W * c->H; }
 
 }  // namespace
+
+// Launch sequence of RunPatchMatch (HPM.cu:2987-3035) on the context stream:
+// init, num_iterations x {black, red}, output.  The reference synchronises the
+// device after every launch; stream order is enough here.
+template <class E> int launch_schedule(hpm_ctx* ctx, int n_halfpasses, int do_output) {
+  const DevState S = make_state(ctx);
+  using T = Tiles<E>;
+  const dim3 b256(256);
+  const dim3 gi(((ctx->W + T::sq - 1) / T::sq) * ((ctx->H + T::sq - 1) / T::sq));
+  const dim3 b16(16, 16), g16((ctx->W + 15) / 16, (ctx->H + 15) / 16);
+  const int H2 = (ctx->H / 2) * 2;
+  const dim3 gp((((ctx->W + 1) / 2 + T::cw - 1) / T::cw) * ((H2 + T::ch - 1) / T::ch));
+  hipStream_t st = ctx->stream;
+  if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[0], st));
+  hipLaunchKernelGGL(k_init<E>, gi, b256, 0, st, ctx->d_consts, S);
+  HIP_OK(hipGetLastError());
+  if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[1], st));
+  int done = 0;
+  for (int it = 0; it < ctx->params.num_iterations; ++it) {
+    for (int colour = 0; colour < 2; ++colour) {
+      if (n_halfpasses >= 0 && done >= n_halfpasses) break;
+      hipLaunchKernelGGL(k_propagate<E>, gp, b256, 0, st, ctx->d_consts, S, colour, it, H2);
+      HIP_OK(hipGetLastError());
+      ++done;
+    }
+  }
+  if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[2], st));
+  if (do_output) {
+    hipLaunchKernelGGL(k_output, g16, b16, 0, st, ctx->d_consts, ctx->d_planes);
+    HIP_OK(hipGetLastError());
+  }
+  if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[3], st));
+  ctx->stats.propagate_launches = done;
+  return HPM_OK;
+}
+
+template <class E> int launch_cost_fixed(hpm_ctx* ctx, const void* planes_dev, void* out_dev) {
+  using T = Tiles<E>;
+  const dim3 gi(((ctx->W + T::sq - 1) / T::sq) * ((ctx->H + T::sq - 1) / T::sq));
+  hipLaunchKernelGGL(k_cost_fixed<E>, gi, dim3(256), 0, ctx->stream, ctx->d_consts, (const float4*)planes_dev, (float*)out_dev);
+  HIP_OK(hipGetLastError());
+  return HPM_OK;
+}
 
 extern "C" {
 
@@ -179,9 +287,8 @@ int hpm_create(hpm_ctx** out, int device) {
     return HPM_E_DEVICE;
   }
   if (hipMalloc((void**)&ctx->d_consts, sizeof(RunConsts)) != hipSuccess ||
-      hipMalloc((void**)&ctx->d_fconsts, sizeof(FastConsts)) != hipSuccess ||
       hipMalloc((void**)&ctx->d_counters, 8 * sizeof(unsigned long long)) != hipSuccess) {
-    hipFree(ctx->d_consts); hipFree(ctx->d_fconsts);
+    hipFree(ctx->d_consts);
     hipStreamDestroy(ctx->stream);
     delete ctx;
     return HPM_E_NOMEM;
@@ -197,7 +304,6 @@ void hpm_destroy(hpm_ctx* ctx) {
   hipStreamSynchronize(ctx->stream);
   free_views(ctx);
   hipFree(ctx->d_consts);
-  hipFree(ctx->d_fconsts);
   hipFree(ctx->d_counters);
   for (auto& e : ctx->ev) hipEventDestroy(e);
   hipStreamDestroy(ctx->stream);
@@ -342,39 +448,20 @@ int hpm_run_patchmatch_partial(hpm_ctx* ctx, uint64_t seed, int n_halfpasses, in
   if (P.prior_consistency && (!ctx->d_prior || !ctx->d_masks))
     return fail(ctx, HPM_E_STATE, "prior consistency needs hpm_set_prior");
   if (P.hierarchy && !ctx->d_scaled) return fail(ctx, HPM_E_STATE, "hierarchy needs hpm_set_coarse");
-  if (P.precision != HPM_PREC_FP32_EXACT) return hpm_fast_run(ctx, seed, n_halfpasses, do_output);
   HIP_OK(hipSetDevice(ctx->device));
+  int rc;
+  if (P.precision != HPM_PREC_FP32_EXACT && (rc = ensure_quads(ctx, P.precision))) return rc;
   RunConsts C;
   build_consts(ctx, C);
   C.seed = seed;
   HIP_OK(hipMemcpyAsync(ctx->d_consts, &C, sizeof(C), hipMemcpyHostToDevice, ctx->stream));
   HIP_OK(hipMemsetAsync(ctx->d_counters, 0, 8 * sizeof(unsigned long long), ctx->stream));
-  const DevState S = make_state(ctx);
-  const dim3 b16(16, 16), g16((ctx->W + 15) / 16, (ctx->H + 15) / 16);
-  const int H2 = (ctx->H / 2) * 2;
-  const dim3 bp(32, 8), gp(((ctx->W + 1) / 2 + 31) / 32, (H2 + 7) / 8);
-  hipStream_t st = ctx->stream;
-  if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[0], st));
-  hipLaunchKernelGGL(k_init, g16, b16, 0, st, ctx->d_consts, S);
-  HIP_OK(hipGetLastError());
-  if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[1], st));
-  int done = 0;
-  for (int it = 0; it < P.num_iterations; ++it) {
-    for (int colour = 0; colour < 2; ++colour) {
-      if (n_halfpasses >= 0 && done >= n_halfpasses) break;
-      hipLaunchKernelGGL(k_propagate, gp, bp, 0, st, ctx->d_consts, S, colour, it, H2);
-      HIP_OK(hipGetLastError());
-      ++done;
-    }
+  switch (P.precision) {
+    case HPM_PREC_FP16: return launch_schedule<FastFor<QuadF16>>(ctx, n_halfpasses, do_output);
+    case HPM_PREC_BF16: return launch_schedule<FastFor<QuadBF16>>(ctx, n_halfpasses, do_output);
+    case HPM_PREC_FP32_FAST: return launch_schedule<FastFor<QuadF32>>(ctx, n_halfpasses, do_output);
+    default: return launch_schedule<ExactEngine>(ctx, n_halfpasses, do_output);
   }
-  if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[2], st));
-  if (do_output) {
-    hipLaunchKernelGGL(k_output, g16, b16, 0, st, ctx->d_consts, ctx->d_planes);
-    HIP_OK(hipGetLastError());
-  }
-  if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[3], st));
-  ctx->stats.propagate_launches = done;
-  return HPM_OK;
 }
 
 int hpm_run_patchmatch(hpm_ctx* ctx, uint64_t seed) { return hpm_run_patchmatch_partial(ctx, seed, -1, 1); }
@@ -433,15 +520,19 @@ int hpm_get_size(const hpm_ctx* ctx, int* w, int* h, int* n) {
 int hpm_cost_fixed_device(hpm_ctx* ctx, const void* planes_dev, void* out_dev, int precision) {
   if (!ctx || !planes_dev || !out_dev) return HPM_E_INVALID;
   if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
-  if (precision != HPM_PREC_FP32_EXACT) return hpm_fast_cost_fixed(ctx, planes_dev, out_dev, precision);
+  if (precision < 0 || precision > 3) return fail(ctx, HPM_E_INVALID, "unknown precision");
   HIP_OK(hipSetDevice(ctx->device));
+  int rc;
+  if (precision != HPM_PREC_FP32_EXACT && (rc = ensure_quads(ctx, precision))) return rc;
   RunConsts C;
   build_consts(ctx, C);
   HIP_OK(hipMemcpyAsync(ctx->d_consts, &C, sizeof(C), hipMemcpyHostToDevice, ctx->stream));
-  const dim3 b16(16, 16), g16((ctx->W + 15) / 16, (ctx->H + 15) / 16);
-  hipLaunchKernelGGL(k_cost_fixed, g16, b16, 0, ctx->stream, ctx->d_consts, (const float4*)planes_dev, (float*)out_dev);
-  HIP_OK(hipGetLastError());
-  return HPM_OK;
+  switch (precision) {
+    case HPM_PREC_FP16: return launch_cost_fixed<FastFor<QuadF16>>(ctx, planes_dev, out_dev);
+    case HPM_PREC_BF16: return launch_cost_fixed<FastFor<QuadBF16>>(ctx, planes_dev, out_dev);
+    case HPM_PREC_FP32_FAST: return launch_cost_fixed<FastFor<QuadF32>>(ctx, planes_dev, out_dev);
+    default: return launch_cost_fixed<ExactEngine>(ctx, planes_dev, out_dev);
+  }
 }
 
 int hpm_cost_fixed(hpm_ctx* ctx, const float* planes, float* out, int precision) {
@@ -479,7 +570,7 @@ int hpm_init_cost(hpm_ctx* ctx, const float* planes, float* out_cost, uint32_t* 
   if (!rc) rc = upload(ctx, dp, planes, P * sizeof(float4));
   if (!rc) {
     const dim3 b16(16, 16), g16((ctx->W + 15) / 16, (ctx->H + 15) / 16);
-    hipLaunchKernelGGL(k_init_cost, g16, b16, 0, ctx->stream, ctx->d_consts, dp, dc, dm);
+    hipLaunchKernelGGL(k_init_cost<ExactEngine>, g16, b16, 0, ctx->stream, ctx->d_consts, dp, dc, dm);
     if (hipGetLastError() != hipSuccess) rc = fail(ctx, HPM_E_DEVICE, "k_init_cost launch failed");
   }
   if (!rc) rc = download(ctx, out_cost, dc, P * sizeof(float));
@@ -514,13 +605,3 @@ int hpm_geom_cost(hpm_ctx* ctx, const float* planes, float* out) {
 
 }  // extern "C"
 
-// The mixed-precision path is compiled in hpm_fast.hip when present; this
-// library build carries the FP32-exact path only.
-#ifndef HPM_HAVE_FAST
-int hpm_fast_run(hpm_ctx* ctx, uint64_t, int, int) {
-  return fail(ctx, HPM_E_INVALID, "mixed-precision path is not part of this build");
-}
-int hpm_fast_cost_fixed(hpm_ctx* ctx, const void*, void*, int) {
-  return fail(ctx, HPM_E_INVALID, "mixed-precision path is not part of this build");
-}
-#endif

@@ -45,6 +45,7 @@ struct PairBasis {
 
 struct RunConsts {
   int W, H, V;                 // reference size, number of source views
+  int storage;                 // image storage of the mixed-precision engine (HPM_PREC_*)
   int geom, prior, hierarchy, upsample, mand;
   int num_iterations;
   float depth_min, depth_max;  // cameras[0] range * (0.6, 1.2), HPM.cpp:685-686
@@ -60,6 +61,11 @@ struct RunConsts {
   PairBasis pair[kMaxSrc];
   const float* img[kMaxImages];
   const float* dimg[kMaxImages];
+  // mixed-precision engine: quad texel images and the FP64 homography basis
+  const void* quad[kMaxImages];  // (w+1) x (h+1) quads, origin at texel (-1, -1)
+  double A[kMaxSrc][9];          // K_s R_rel K_r^-1
+  double bvec[kMaxSrc][3];       // K_s t_rel
+  double kinv[4];                // 1/fx, 1/fy, cx/fx, cy/fy of the reference camera
 };
 
 struct DevState {
@@ -264,6 +270,17 @@ __device__ __forceinline__ float ncc_exact(const RunConsts& C, int v, int px, in
   const float den = sqrtf(rw.var_ref * var_s);
   return fmaxf(0.0f, fminf(2.0f, 1.0f - cov / den));
 }
+
+// The exact FP32 cost engine: the reference's arithmetic, bit for bit with the
+// FP32 oracle.
+struct ExactEngine {
+  static constexpr int kLanes = 1;
+  using Win = RefWin;
+  __device__ __forceinline__ static void build(const RunConsts& C, int px, int py, Win& w) { build_refwin(C, px, py, w); }
+  __device__ __forceinline__ static float ncc(const RunConsts& C, int v, int px, int py, float4 pl, const Win& w) {
+    return ncc_exact(C, v, px, py, pl, w);
+  }
+};
 
 // ------------------------------------------------------------------ random planes
 __device__ __forceinline__ float4 random_normal(const DevCam& cam, int px, int py, Philox& rng, float depth) {  // HPM.cu:159-185

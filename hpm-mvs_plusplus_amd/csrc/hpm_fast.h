@@ -1,16 +1,408 @@
-// hpm_fast.h -- mixed-precision (FP16 / BF16 image storage, FP32 accumulation)
-// PatchMatch path.  Declarations only; the kernels live in hpm_fast.hip.
+// hpm_fast.h -- mixed-precision cost engine: FP16 / BF16 / FP32 image storage
+// with FP32 accumulation, plugged into the same PatchMatch control flow as the
+// exact engine (same random draws, same decisions given the costs).
+//
+// What changes against the reference's ComputeBilateralNCC (HPM.cu:375-448):
+//  * Homography.  H = K_s (R_rel - t_rel n^T / d) K_r^-1 = A - b m^T with the
+//    view-pair constants A = K_s R_rel K_r^-1, b = K_s t_rel built once per run
+//    in FP64 on the host, and m = K_r^-T n / d per plane.  The centre
+//    projection (u_c, v_c) is evaluated in FP64 once per (pixel, plane, view).
+//  * Tap positions are offsets from the centre, u = u_c + (i a_u + j b_u) /
+//    (1 + i g6 + j g7), in FP32: small magnitudes keep ~1e-6 px accuracy where
+//    absolute FP32 pixel coordinates carry ~6e-5 px of rounding at 1600 px.
+//  * Texels.  A "quad" image holds, for every texel (x, y) of the (w+1) x (h+1)
+//    grid starting at (-1, -1), the four texels of its bilinear cell with the
+//    reference's border clamp applied; a bilinear tap is one 8-byte load (FP16 /
+//    BF16) or one 16-byte load (FP32).  8-bit images are exact in FP16 and BF16.
+//  * Sums use shifted data: the reference window is centred on its centre
+//    texel, the source window on its first sample, so E[x^2] - E[x]^2 never
+//    cancels large magnitudes.  The reference-side statistics are accumulated
+//    in FP64 once per pixel.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-struct hpm_ctx;
+#include "hpm_device.h"
+
+
+// Build-time knobs (measured variants): noinline keeps one copy of the 36-tap
+// cost body in the propagation kernel instead of one per call site.
+#ifdef HPM_NCC_NOINLINE
+#define HPM_NCC_INLINE __device__ __attribute__((noinline))
+#else
+#define HPM_NCC_INLINE __device__ __forceinline__
+#endif
 
 namespace hpm {
-struct FastConsts {
-  int placeholder;
+
+__device__ __forceinline__ float half_bits_to_float(uint32_t b) {
+  _Float16 h;
+  const uint16_t s = (uint16_t)b;
+  __builtin_memcpy(&h, &s, 2);
+  return (float)h;
+}
+
+// Bilinear interpolation of one cell (t00, t10 / t01, t11) at fractions
+// (ax, ay): top = t00 + ax (t10 - t00), bottom likewise, then along y.
+__device__ __forceinline__ float lerp_cell(float t00, float t10, float t01, float t11, float ax, float ay) {
+  const float top = ffma(ax, t10 - t00, t00);
+  const float bot = ffma(ax, t11 - t01, t01);
+  return ffma(ay, bot - top, top);
+}
+
+#ifndef HPM_MIX_LERP
+#define HPM_MIX_LERP 1
+#endif
+
+struct QuadF16 {
+  using Q = uint2;
+  static constexpr int kind = 1;  // HPM_PREC_FP16
+  __device__ __forceinline__ static uint32_t h(float f) {
+    const _Float16 x = (_Float16)f;  // round to nearest even
+    uint16_t b;
+    __builtin_memcpy(&b, &x, 2);
+    return b;
+  }
+  __device__ __forceinline__ static Q pack(float a, float b, float c, float d) {
+    return make_uint2(h(a) | (h(b) << 16), h(c) | (h(d) << 16));
+  }
+  __device__ __forceinline__ static Q load(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    return make_uint2(v[0], v[1]);
+  }
+  __device__ __forceinline__ static void unpack(Q q, float& a, float& b, float& c, float& d) {
+    a = half_bits_to_float(q.x & 0xffffu);
+    b = half_bits_to_float(q.x >> 16);
+    c = half_bits_to_float(q.y & 0xffffu);
+    d = half_bits_to_float(q.y >> 16);
+  }
+  // top = t00 (1 - ax) + t10 ax as two fused multiply-adds that read the FP16
+  // texels directly (v_fma_mix_f32): fma(ax, t10, fma(-ax, t00, t00)).
+  __device__ __forceinline__ static float bilinear(Q q, float ax, float ay) {
+#if HPM_MIX_LERP
+    const _Float16* h0 = reinterpret_cast<const _Float16*>(&q.x);
+    const _Float16* h1 = reinterpret_cast<const _Float16*>(&q.y);
+    const float top = ffma(ax, (float)h0[1], ffma(-ax, (float)h0[0], (float)h0[0]));
+    const float bot = ffma(ax, (float)h1[1], ffma(-ax, (float)h1[0], (float)h1[0]));
+    return ffma(ay, bot - top, top);
+#else
+    float a, b, c, d;
+    unpack(q, a, b, c, d);
+    return lerp_cell(a, b, c, d, ax, ay);
+#endif
+  }
 };
+struct QuadBF16 {
+  using Q = uint2;
+  static constexpr int kind = 2;  // HPM_PREC_BF16
+  __device__ __forceinline__ static uint32_t h(float f) {  // round to nearest even (finite inputs)
+    const uint32_t u = __float_as_uint(f);
+    return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+  }
+  __device__ __forceinline__ static Q pack(float a, float b, float c, float d) {
+    return make_uint2(h(a) | (h(b) << 16), h(c) | (h(d) << 16));
+  }
+  __device__ __forceinline__ static Q load(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    return make_uint2(v[0], v[1]);
+  }
+  __device__ __forceinline__ static void unpack(Q q, float& a, float& b, float& c, float& d) {
+    a = __uint_as_float(q.x << 16);
+    b = __uint_as_float(q.x & 0xffff0000u);
+    c = __uint_as_float(q.y << 16);
+    d = __uint_as_float(q.y & 0xffff0000u);
+  }
+  __device__ __forceinline__ static float bilinear(Q q, float ax, float ay) {
+    float a, b, c, d;
+    unpack(q, a, b, c, d);
+    return lerp_cell(a, b, c, d, ax, ay);
+  }
+};
+struct QuadF32 {
+  using Q = float4;
+  static constexpr int kind = 3;  // HPM_PREC_FP32_FAST
+  __device__ __forceinline__ static Q pack(float a, float b, float c, float d) { return make_float4(a, b, c, d); }
+  __device__ __forceinline__ static Q load(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
+  }
+  __device__ __forceinline__ static void unpack(Q q, float& a, float& b, float& c, float& d) {
+    a = q.x; b = q.y; c = q.z; d = q.w;
+  }
+  __device__ __forceinline__ static float bilinear(Q q, float ax, float ay) { return lerp_cell(q.x, q.y, q.z, q.w, ax, ay); }
+};
+
+struct FastWin {
+  float w[kTaps];    // bilateral weights (HPM.cu:342-347)
+  float wr[kTaps];   // w * (r - r_centre)
+  float inv_w;       // 1 / sum w
+  float m_r;         // weighted mean of r - r_centre
+  float var_r;       // weighted variance of the reference window
+};
+
+template <class S>
+struct FastEngine {
+  static constexpr int kLanes = 1;
+  using Win = FastWin;
+
+  __device__ __forceinline__ static void build(const RunConsts& C, int px, int py, Win& fw) {
+    const float* ref = C.img[0];
+    const int W = C.cams[0].width, H = C.cams[0].height;
+    const float c = tex_at(ref, W, H, px, py);
+    const float den_s = 2.0f * C.sigma_spatial * C.sigma_spatial;
+    const float den_c = 2.0f * C.sigma_color * C.sigma_color;
+    double sw = 0.0, sr = 0.0, srr = 0.0;
+#pragma unroll
+    for (int ii = 0; ii < kTapsPerAxis; ++ii) {
+#pragma unroll
+      for (int jj = 0; jj < kTapsPerAxis; ++jj) {
+        const int i = -5 + 2 * ii, j = -5 + 2 * jj;
+        const float r = tex_at(ref, W, H, px + i, py + j);
+        const float spatial = sqrtf((float)(i * i + j * j));
+        const float w = expf_hpm(-spatial / den_s - fabsf(r - c) / den_c);
+        const float rs = r - c;  // exact: integer-valued texels
+        fw.w[ii * kTapsPerAxis + jj] = w;
+        fw.wr[ii * kTapsPerAxis + jj] = w * rs;
+        sw += (double)w;
+        sr += (double)w * (double)rs;
+        srr += (double)w * (double)rs * (double)rs;
+      }
+    }
+    const double m = sr / sw;
+    fw.inv_w = (float)(1.0 / sw);
+    fw.m_r = (float)m;
+    fw.var_r = (float)(srr / sw - m * m);
+  }
+
+  HPM_NCC_INLINE static float ncc(const RunConsts& C, int v, int px, int py, float4 pl, const Win& fw) {
+    const double inv_d = 1.0 / (double)pl.w;
+    const double m0 = (double)pl.x * C.kinv[0] * inv_d;
+    const double m1 = (double)pl.y * C.kinv[1] * inv_d;
+    const double m2 = ((double)pl.z - (double)pl.x * C.kinv[2] - (double)pl.y * C.kinv[3]) * inv_d;
+    const double* A = C.A[v - 1];
+    const double* b = C.bvec[v - 1];
+    const double h0 = A[0] - b[0] * m0, h1 = A[1] - b[0] * m1, h2 = A[2] - b[0] * m2;
+    const double h3 = A[3] - b[1] * m0, h4 = A[4] - b[1] * m1, h5 = A[5] - b[1] * m2;
+    const double h6 = A[6] - b[2] * m0, h7 = A[7] - b[2] * m1, h8 = A[8] - b[2] * m2;
+    const double X = h0 * px + h1 * py + h2;
+    const double Y = h3 * px + h4 * py + h5;
+    const double Z = h6 * px + h7 * py + h8;
+    const double rz = 1.0 / Z;
+    const double uc = X * rz, vc = Y * rz;
+    const DevCam& sc = C.cams[v];
+    const int sw = sc.width, sh = sc.height;
+    if (uc >= (double)sw || uc < 0.0 || vc >= (double)sh || vc < 0.0) return 2.0f;
+    if (fw.var_r < 1e-5f) return 2.0f;
+    const double g6 = h6 * rz, g7 = h7 * rz;
+    const float fg6 = (float)g6, fg7 = (float)g7;
+    const float au = (float)(h0 * rz - uc * g6), bu = (float)(h1 * rz - uc * g7);
+    const float av = (float)(h3 * rz - vc * g6), bv = (float)(h4 * rz - vc * g7);
+    const double flu = floor(uc), flv = floor(vc);
+    const int iu = (int)flu + 1, iv = (int)flv + 1;  // +1: the quad grid starts at (-1, -1)
+    const float fu = (float)(uc - flu), fv = (float)(vc - flv);
+    // clamping the cell offset in float keeps the cell inside [0, w] x [0, h]
+    // (the reference's border clamp); a NaN offset converts to 0, also inside.
+    // clamping the cell offset in float keeps the cell inside [0, w] x [0, h]
+    // (the reference's border clamp); a NaN offset converts to 0, also inside.
+    const float lox = (float)(-iu), hix = (float)(sw - iu);
+    const float loy = (float)(-iv), hiy = (float)(sh - iv);
+    const int qp = sw + 1;
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<void*>(C.quad[v]), (short)0, qp * (sh + 1) * (int)sizeof(typename S::Q), 0x00020000);
+    float s0 = 0.0f, S1 = 0.0f, S2 = 0.0f, S3 = 0.0f;
+#pragma unroll
+    for (int ii = 0; ii < kTapsPerAxis; ++ii) {
+#pragma unroll
+      for (int jj = 0; jj < kTapsPerAxis; ++jj) {
+        const float i = (float)(-5 + 2 * ii), j = (float)(-5 + 2 * jj);
+        const float den = ffma(i, fg6, ffma(j, fg7, 1.0f));
+        const float r = __builtin_amdgcn_rcpf(den);
+        const float du = ffma(i, au, j * bu) * r;
+        const float dv = ffma(i, av, j * bv) * r;
+        const float x = fu + du, y = fv + dv;
+        const float fx = floorf(x), fy = floorf(y);
+        const float ax = x - fx, ay = y - fy;
+        const int tx = iu + (int)__builtin_amdgcn_fmed3f(fx, lox, hix);
+        const int ty = iv + (int)__builtin_amdgcn_fmed3f(fy, loy, hiy);
+        const unsigned off = (__umul24((unsigned)ty, (unsigned)qp) + (unsigned)tx) * (unsigned)sizeof(typename S::Q);
+        const float s = S::bilinear(S::load(rsrc, off), ax, ay);
+        if (ii == 0 && jj == 0) s0 = s;
+        const float sp = s - s0;
+        const float ws = fw.w[ii * kTapsPerAxis + jj] * sp;
+        S1 += ws;
+        S2 = ffma(ws, sp, S2);
+        S3 = ffma(fw.wr[ii * kTapsPerAxis + jj], sp, S3);
+      }
+    }
+    const float m_s = S1 * fw.inv_w;
+    const float var_s = ffma(-m_s, m_s, S2 * fw.inv_w);
+    if (var_s < 1e-5f) return 2.0f;
+    const float cov = ffma(-fw.m_r, m_s, S3 * fw.inv_w);
+    const float c = 1.0f - cov * __builtin_amdgcn_rsqf(fw.var_r * var_s);
+    return fmaxf(0.0f, fminf(2.0f, c));
+  }
+};
+
+// Builds the quad image of one view from its FP32 texels.  Thread (x, y) of
+// the (w+1) x (h+1) grid writes the cell whose top-left texel is (x-1, y-1).
+template <class S>
+__global__ __launch_bounds__(256) void k_build_quad(const float* __restrict__ img, int w, int h, void* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  if (x > w || y > h) return;
+  const float a = tex_at(img, w, h, x - 1, y - 1), b = tex_at(img, w, h, x, y - 1);
+  const float c = tex_at(img, w, h, x - 1, y), d = tex_at(img, w, h, x, y);
+  typename S::Q* q = (typename S::Q*)out;
+  q[y * (w + 1) + x] = S::pack(a, b, c, d);
+}
+
 }  // namespace hpm
 
-int hpm_fast_run(hpm_ctx* ctx, uint64_t seed, int n_halfpasses, int do_output);
-int hpm_fast_cost_fixed(hpm_ctx* ctx, const void* planes_dev, void* out_dev, int precision);
+namespace hpm {
+
+// Sum over the four lanes of a quad (lanes 4k..4k+3) with two DPP quad_perm
+// adds.  a + b == b + a in IEEE arithmetic, so every lane of the quad ends
+// with the same bits.
+__device__ __forceinline__ float quad_sum(float x) {
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));  // [1,0,3,2]
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x4E, 0xF, 0xF, false));  // [2,3,0,1]
+  return x;
+}
+
+// Four lanes per pixel: lane q of the quad owns the 3 x 3 block (q & 1, q >> 1)
+// of the 6 x 6 tap grid, i.e. x offsets -5 + 6 (q & 1) + {0, 2, 4} and y offsets
+// -5 + 6 (q >> 1) + {0, 2, 4}.  The window sums are reduced across the quad, so
+// a lane holds 9 weights instead of 36.  All four lanes run the pixel's
+// control flow with identical values.
+template <class S>
+struct QuadLaneEngine {
+  static constexpr int kLanes = 4;
+  struct Win {
+    float w[9];      // bilateral weights of the lane's taps
+    float wr[9];     // w * (r - r_centre)
+    float c_r;       // centre texel (shift of both windows)
+    float inv_w, m_r, var_r;
+    float bi, bj;    // offset of the lane's first tap
+  };
+
+  __device__ __forceinline__ static int quad_lane() { return (int)(__lane_id() & 3u); }
+
+  __device__ __forceinline__ static void build(const RunConsts& C, int px, int py, Win& fw) {
+    const float* ref = C.img[0];
+    const int W = C.cams[0].width, H = C.cams[0].height;
+    const float c = tex_at(ref, W, H, px, py);
+    const float den_s = 2.0f * C.sigma_spatial * C.sigma_spatial;
+    const float den_c = 2.0f * C.sigma_color * C.sigma_color;
+    const int q = quad_lane();
+    const int a = q & 1, b = q >> 1;
+    fw.c_r = c;
+    fw.bi = (float)(-5 + 6 * a);
+    fw.bj = (float)(-5 + 6 * b);
+    // window statistics over all 36 taps
+    double sw = 0.0, sr = 0.0, srr = 0.0;
+#pragma unroll
+    for (int ii = 0; ii < kTapsPerAxis; ++ii) {
+#pragma unroll
+      for (int jj = 0; jj < kTapsPerAxis; ++jj) {
+        const int i = -5 + 2 * ii, j = -5 + 2 * jj;
+        const float r = tex_at(ref, W, H, px + i, py + j);
+        const float spatial = sqrtf((float)(i * i + j * j));
+        const float w = expf_hpm(-spatial / den_s - fabsf(r - c) / den_c);
+        const float rs = r - c;
+        sw += (double)w;
+        sr += (double)w * (double)rs;
+        srr += (double)w * (double)rs * (double)rs;
+      }
+    }
+    // the lane's own 3 x 3 block (same weight formula, same values)
+    const int bi = -5 + 6 * a, bj = -5 + 6 * b;
+#pragma unroll
+    for (int di = 0; di < 3; ++di) {
+#pragma unroll
+      for (int dj = 0; dj < 3; ++dj) {
+        const int i = bi + 2 * di, j = bj + 2 * dj;
+        const float r = tex_at(ref, W, H, px + i, py + j);
+        const float spatial = sqrtf((float)(i * i + j * j));
+        const float w = expf_hpm(-spatial / den_s - fabsf(r - c) / den_c);
+        fw.w[di * 3 + dj] = w;
+        fw.wr[di * 3 + dj] = w * (r - c);
+      }
+    }
+    const double m = sr / sw;
+    fw.inv_w = (float)(1.0 / sw);
+    fw.m_r = (float)m;
+    fw.var_r = (float)(srr / sw - m * m);
+  }
+
+  __device__ __forceinline__ static float ncc(const RunConsts& C, int v, int px, int py, float4 pl, const Win& fw) {
+    const double inv_d = 1.0 / (double)pl.w;
+    const double m0 = (double)pl.x * C.kinv[0] * inv_d;
+    const double m1 = (double)pl.y * C.kinv[1] * inv_d;
+    const double m2 = ((double)pl.z - (double)pl.x * C.kinv[2] - (double)pl.y * C.kinv[3]) * inv_d;
+    const double* A = C.A[v - 1];
+    const double* b = C.bvec[v - 1];
+    const double h0 = A[0] - b[0] * m0, h1 = A[1] - b[0] * m1, h2 = A[2] - b[0] * m2;
+    const double h3 = A[3] - b[1] * m0, h4 = A[4] - b[1] * m1, h5 = A[5] - b[1] * m2;
+    const double h6 = A[6] - b[2] * m0, h7 = A[7] - b[2] * m1, h8 = A[8] - b[2] * m2;
+    const double X = h0 * px + h1 * py + h2;
+    const double Y = h3 * px + h4 * py + h5;
+    const double Z = h6 * px + h7 * py + h8;
+    const double rz = 1.0 / Z;
+    const double uc = X * rz, vc = Y * rz;
+    const DevCam& sc = C.cams[v];
+    const int sw = sc.width, sh = sc.height;
+    if (uc >= (double)sw || uc < 0.0 || vc >= (double)sh || vc < 0.0) return 2.0f;
+    if (fw.var_r < 1e-5f) return 2.0f;
+    const double g6 = h6 * rz, g7 = h7 * rz;
+    const float fg6 = (float)g6, fg7 = (float)g7;
+    const float au = (float)(h0 * rz - uc * g6), bu = (float)(h1 * rz - uc * g7);
+    const float av = (float)(h3 * rz - vc * g6), bv = (float)(h4 * rz - vc * g7);
+    const double flu = floor(uc), flv = floor(vc);
+    const int iu = (int)flu + 1, iv = (int)flv + 1;
+    const float fu = (float)(uc - flu), fv = (float)(vc - flv);
+    const float lox = (float)(-iu), hix = (float)(sw - iu);
+    const float loy = (float)(-iv), hiy = (float)(sh - iv);
+    const int qp = sw + 1;
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<void*>(C.quad[v]), (short)0, qp * (sh + 1) * (int)sizeof(typename S::Q), 0x00020000);
+    // lane-specific bases of the tap terms: value at (bi, bj)
+    const float den0 = ffma(fw.bi, fg6, ffma(fw.bj, fg7, 1.0f));
+    const float nu0 = ffma(fw.bi, au, fw.bj * bu);
+    const float nv0 = ffma(fw.bi, av, fw.bj * bv);
+    float S1 = 0.0f, S2 = 0.0f, S3 = 0.0f;
+#pragma unroll
+    for (int di = 0; di < 3; ++di) {
+#pragma unroll
+      for (int dj = 0; dj < 3; ++dj) {
+        const float oi = (float)(2 * di), oj = (float)(2 * dj);
+        const float den = ffma(oi, fg6, ffma(oj, fg7, den0));
+        const float r = __builtin_amdgcn_rcpf(den);
+        const float du = ffma(oi, au, ffma(oj, bu, nu0)) * r;
+        const float dv = ffma(oi, av, ffma(oj, bv, nv0)) * r;
+        const float x = fu + du, y = fv + dv;
+        const float fx = floorf(x), fy = floorf(y);
+        const float ax = x - fx, ay = y - fy;
+        const int tx = iu + (int)__builtin_amdgcn_fmed3f(fx, lox, hix);
+        const int ty = iv + (int)__builtin_amdgcn_fmed3f(fy, loy, hiy);
+        const unsigned off = (__umul24((unsigned)ty, (unsigned)qp) + (unsigned)tx) * (unsigned)sizeof(typename S::Q);
+        const float sp = S::bilinear(S::load(rsrc, off), ax, ay) - fw.c_r;
+        const float ws = fw.w[di * 3 + dj] * sp;
+        S1 += ws;
+        S2 = ffma(ws, sp, S2);
+        S3 = ffma(fw.wr[di * 3 + dj], sp, S3);
+      }
+    }
+    S1 = quad_sum(S1);
+    S2 = quad_sum(S2);
+    S3 = quad_sum(S3);
+    const float m_s = S1 * fw.inv_w;
+    const float var_s = ffma(-m_s, m_s, S2 * fw.inv_w);
+    if (var_s < 1e-5f) return 2.0f;
+    const float cov = ffma(-fw.m_r, m_s, S3 * fw.inv_w);
+    const float c = 1.0f - cov * __builtin_amdgcn_rsqf(fw.var_r * var_s);
+    return fmaxf(0.0f, fminf(2.0f, c));
+  }
+};
+
+}  // namespace hpm
+

@@ -15,6 +15,13 @@
 #pragma once
 #include "hpm_device.h"
 
+// Occupancy target of the PatchMatch kernels (waves per SIMD); the register
+// budget follows from it (512 / waves VGPRs per lane).
+#ifndef HPM_PROP_WAVES
+#define HPM_PROP_WAVES 1
+#endif
+#define HPM_PROP_ATTR __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_PROP_WAVES)))
+
 namespace hpm {
 
 struct Sample {
@@ -23,6 +30,60 @@ struct Sample {
   bool flag[8];
 };
 
+// Pixel tiles of a 256-thread (1D) block.  Engines with one lane per pixel
+// cover 16 x 16 pixels (2D kernels) or 16 same-colour columns x 16 rows
+// (colour passes); four-lane engines cover 8 x 8 and 8 x 8.
+template <class E> struct Tiles {
+  static constexpr int px_per_block = 256 / E::kLanes;
+  static constexpr int sq = E::kLanes == 1 ? 16 : 8;          // 2D tile side
+#ifndef HPM_COLOUR_TILE_W
+#define HPM_COLOUR_TILE_W 16
+#endif
+  static constexpr int cw = E::kLanes == 1 ? HPM_COLOUR_TILE_W : 8;  // colour tile width (same-colour columns)
+  static constexpr int ch = px_per_block / cw;                // colour tile height
+};
+
+#ifndef HPM_XCD_STRIP
+#define HPM_XCD_STRIP 8
+#endif
+// Workgroup -> tile, XCD-aware.  The dispatcher deals workgroups round-robin to
+// the 8 XCDs (XCD = id % 8), each with its own 4 MB L2.  Every XCD gets one
+// contiguous range of tiles, walked column-major inside horizontal strips of
+// HPM_XCD_STRIP tiles, so the workgroups an XCD runs at the same time cover a
+// compact area and the source texels they gather stay in its L2.
+__device__ __forceinline__ void xcd_tile(int nx, int ny, int& tx, int& ty) {
+  const int id = (int)blockIdx.x;
+  const int N = nx * ny;
+  const int xcd = id & 7, k = id >> 3;
+  const int base = N >> 3, rem = N & 7;
+  const int t = xcd * base + (xcd < rem ? xcd : rem) + k;  // XCD x owns base + (x < rem) tiles
+  const int S = HPM_XCD_STRIP;
+  const int strip = t / (nx * S);
+  const int h = (ny - strip * S) < S ? (ny - strip * S) : S;
+  const int r = t - strip * nx * S;
+  tx = r / h;
+  ty = strip * S + r % h;
+}
+
+template <class E> __device__ __forceinline__ void tile_pixel(int& px, int& py, int W, int H) {
+  const int pix = (int)threadIdx.x / E::kLanes;
+  int tx, ty;
+  xcd_tile((W + Tiles<E>::sq - 1) / Tiles<E>::sq, (H + Tiles<E>::sq - 1) / Tiles<E>::sq, tx, ty);
+  px = tx * Tiles<E>::sq + pix % Tiles<E>::sq;
+  py = ty * Tiles<E>::sq + pix / Tiles<E>::sq;
+}
+template <class E> __device__ __forceinline__ void colour_pixel(int& cx, int& py, int W, int H2) {
+  const int pix = (int)threadIdx.x / E::kLanes;
+  int tx, ty;
+  xcd_tile(((W + 1) / 2 + Tiles<E>::cw - 1) / Tiles<E>::cw, (H2 + Tiles<E>::ch - 1) / Tiles<E>::ch, tx, ty);
+  cx = tx * Tiles<E>::cw + pix % Tiles<E>::cw;
+  py = ty * Tiles<E>::ch + pix / Tiles<E>::cw;
+}
+// counts once per pixel: only the first lane of a multi-lane pixel contributes
+template <class E> __device__ __forceinline__ unsigned first_lane(unsigned x) {
+  return (E::kLanes == 1 || (threadIdx.x % E::kLanes) == 0) ? x : 0u;
+}
+
 __device__ __forceinline__ void wave_count_add(unsigned long long* counter, unsigned long long x) {
   // one atomic per wave
   for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
@@ -30,12 +91,13 @@ __device__ __forceinline__ void wave_count_add(unsigned long long* counter, unsi
 }
 
 // ComputeMultiViewInitialCostandSelectedViews, HPM.cu:450-488
-__device__ float init_cost(const RunConsts& C, int px, int py, float4 pl, const RefWin& rw, uint32_t& sel, unsigned& nncc) {
+template <class E>
+__device__ float init_cost(const RunConsts& C, int px, int py, float4 pl, const typename E::Win& rw, uint32_t& sel, unsigned& nncc) {
   const int V = C.V;
   float cv[kMaxSrc], cvc[kMaxSrc];
   int valid = 0;
   for (int i = 0; i < V; ++i) {
-    const float x = ncc_exact(C, i + 1, px, py, pl, rw);
+    const float x = E::ncc(C, i + 1, px, py, pl, rw);
     cv[i] = x;
     cvc[i] = x;
     if (x < 2.0f) ++valid;
@@ -60,7 +122,8 @@ __device__ float init_cost(const RunConsts& C, int px, int py, float4 pl, const 
   return 2.0f;
 }
 
-__device__ __forceinline__ void reload_plane(const RunConsts& C, const DevState& S, int px, int py, const RefWin& rw, unsigned& nncc) {  // HPM.cu:749-760
+template <class E>
+__device__ __forceinline__ void reload_plane(const RunConsts& C, const DevState& S, int px, int py, const typename E::Win& rw, unsigned& nncc) {  // HPM.cu:749-760
   const int center = py * C.W + px;
   float4 pl = C.hierarchy ? S.scaled[center] : S.planes[center];
   pl = to_ref(C.cams[0], pl);
@@ -68,12 +131,13 @@ __device__ __forceinline__ void reload_plane(const RunConsts& C, const DevState&
   pl.w = dist_to_origin(C.cams[0], px, py, depth, pl);
   S.planes[center] = pl;
   uint32_t sel;
-  S.costs[center] = init_cost(C, px, py, pl, rw, sel, nncc);
+  S.costs[center] = init_cost<E>(C, px, py, pl, rw, sel, nncc);
   S.sel[center] = sel;
 }
 
 // JBU of the coarse normal/cost map inside RandomInitialization, HPM.cu:633-698.
-__device__ void upsample_init(const RunConsts& C, const DevState& S, int px, int py, const RefWin& rw, unsigned& nncc) {
+template <class E>
+__device__ void upsample_init(const RunConsts& C, const DevState& S, int px, int py, const typename E::Win& rw, unsigned& nncc) {
   const int W = C.W, H = C.H;
   const int center = py * W + px;
   const float sc_cols = C.scaled_cols, sc_rows = C.scaled_rows;
@@ -115,20 +179,21 @@ __device__ void upsample_init(const RunConsts& C, const DevState& S, int px, int
   nt.x = nt.x / nf; nt.y = nt.y / nf; nt.z = nt.z / nf;
   normalize3(nt);
   uint32_t sel;
-  float c0 = init_cost(C, px, py, S.planes[center], rw, sel, nncc);
+  float c0 = init_cost<E>(C, px, py, S.planes[center], rw, sel, nncc);
   S.pre_costs[center] = c0;
   float4 pl = to_ref(C.cams[0], nt);
   const float depth = S.planes[center].w;
   pl.w = dist_to_origin(C.cams[0], px, py, depth, pl);
   S.planes[center] = pl;
-  S.costs[center] = init_cost(C, px, py, pl, rw, sel, nncc);
+  S.costs[center] = init_cost<E>(C, px, py, pl, rw, sel, nncc);
   S.sel[center] = sel;
 }
 
-__global__ __launch_bounds__(256) void k_init(const RunConsts* __restrict__ Cp, DevState S) {
+template <class E>
+__global__ HPM_PROP_ATTR void k_init(const RunConsts* __restrict__ Cp, DevState S) {
   const RunConsts& C = *Cp;
-  const int px = blockIdx.x * blockDim.x + threadIdx.x;
-  const int py = blockIdx.y * blockDim.y + threadIdx.y;
+  int px, py;
+  tile_pixel<E>(px, py, C.W, C.H);
   unsigned nncc = 0;
   if (px < C.W && py < C.H) {
     const int center = py * C.W + px;
@@ -136,8 +201,8 @@ __global__ __launch_bounds__(256) void k_init(const RunConsts* __restrict__ Cp, 
     const DevCam& cam = C.cams[0];
     if (!C.prior && !C.mand) S.texture[center] = texture_factor(C, S.canny, px, py);
     bool need_rw = !(C.prior && !(S.masks[center] > 0u && !(S.conf[center] > 0.3f)));
-    RefWin rw;
-    if (need_rw) build_refwin(C, px, py, rw);
+    typename E::Win rw;
+    if (need_rw) E::build(C, px, py, rw);
     if (!C.geom && !C.hierarchy && !C.prior) {
       // GenerateRandomPlaneHypothesis, HPM.cu:224-230
       const float depth = rng.uniform() * (C.depth_max - C.depth_min) + C.depth_min;
@@ -145,7 +210,7 @@ __global__ __launch_bounds__(256) void k_init(const RunConsts* __restrict__ Cp, 
       pl.w = dist_to_origin(cam, px, py, depth, pl);
       S.planes[center] = pl;
       uint32_t sel;
-      S.costs[center] = init_cost(C, px, py, pl, rw, sel, nncc);
+      S.costs[center] = init_cost<E>(C, px, py, pl, rw, sel, nncc);
       S.sel[center] = sel;
     } else if (C.prior) {
       if (S.masks[center] > 0u) {
@@ -166,7 +231,7 @@ __global__ __launch_bounds__(256) void k_init(const RunConsts* __restrict__ Cp, 
           pp.w = dp;  // the depth goes into w here (HPM.cu:609)
           S.planes[center] = pp;
           uint32_t sel;
-          S.costs[center] = init_cost(C, px, py, pp, rw, sel, nncc);
+          S.costs[center] = init_cost<E>(C, px, py, pp, rw, sel, nncc);
           S.sel[center] = sel;
         } else {
           float4 pl = S.planes[center];
@@ -174,7 +239,7 @@ __global__ __launch_bounds__(256) void k_init(const RunConsts* __restrict__ Cp, 
           pl.w = dist_to_origin(cam, px, py, depth, pl);
           S.planes[center] = pl;
           uint32_t sel;
-          S.costs[center] = init_cost(C, px, py, pl, rw, sel, nncc);
+          S.costs[center] = init_cost<E>(C, px, py, pl, rw, sel, nncc);
           S.sel[center] = sel;
         }
       } else {
@@ -186,26 +251,27 @@ __global__ __launch_bounds__(256) void k_init(const RunConsts* __restrict__ Cp, 
       }
     } else {
       if (C.upsample) {
-        upsample_init(C, S, px, py, rw, nncc);
+        upsample_init<E>(C, S, px, py, rw, nncc);
       } else if (C.mand) {
         if (!(S.conf[center] >= 0.3f)) {
           const float tex = texture_factor(C, S.canny, px, py);
           if (tex > 0.5f) S.conf[center] = 0.1f;
         }
-        reload_plane(C, S, px, py, rw, nncc);
+        reload_plane<E>(C, S, px, py, rw, nncc);
       } else {
-        reload_plane(C, S, px, py, rw, nncc);
+        reload_plane<E>(C, S, px, py, rw, nncc);
       }
     }
   }
-  wave_count_add(&S.counters[0], nncc);
+  wave_count_add(&S.counters[0], first_lane<E>(nncc));
 }
 
 // ------------------------------------------------------------------ propagation
 // Neighbour search (HPM.cu:1385-1608) and left-up extension (HPM.cu:1609-1680,
 // 1063-1126, 1322-1345).  Only direction 0 extends: the reference's
 // `bool symbol_eight_orientations[10] = { true }` enables index 0 only.
-__device__ void sample_neighbours(const RunConsts& C, const DevState& S, int px, int py, int iter, const RefWin& rw,
+template <class E>
+__device__ void sample_neighbours(const RunConsts& C, const DevState& S, int px, int py, int iter, const typename E::Win& rw,
                                   Sample& s, unsigned& nncc, unsigned& n_ext) {
   const int W = C.W, H = C.H, V = C.V;
   const int center = py * W + px;
@@ -275,7 +341,7 @@ __device__ void sample_neighbours(const RunConsts& C, const DevState& S, int px,
   for (int d = 0; d < 8; ++d) {
     if (!s.flag[d]) continue;
     const float4 pl = S.planes[pos[d]];
-    for (int v = 0; v < V; ++v) s.ca[d * V + v] = ncc_exact(C, v + 1, px, py, pl, rw);
+    for (int v = 0; v < V; ++v) s.ca[d * V + v] = E::ncc(C, v + 1, px, py, pl, rw);
     nncc += V;
   }
   // extension of direction 0
@@ -306,7 +372,7 @@ __device__ void sample_neighbours(const RunConsts& C, const DevState& S, int px,
       } else {
         pos[0] = mp;
         const float4 pl = S.planes[mp];
-        for (int v = 0; v < V; ++v) s.ca[v] = ncc_exact(C, v + 1, px, py, pl, rw);
+        for (int v = 0; v < V; ++v) s.ca[v] = E::ncc(C, v + 1, px, py, pl, rw);
         nncc += V;
         n_ext += 1;
       }
@@ -379,13 +445,14 @@ __device__ __forceinline__ void final_costs(const RunConsts& C, const DevState& 
   }
 }
 
+template <class E>
 __device__ __forceinline__ float current_cost(const RunConsts& C, int px, int py, float4 pl, const float* vw, float wn,
-                                              const RefWin& rw, unsigned& nncc) {  // HPM.cu:1775-1786
+                                              const typename E::Win& rw, unsigned& nncc) {  // HPM.cu:1775-1786
   const int V = C.V;
   float cn = 0.0f;
   for (int i = 0; i < V; ++i) {
     if (!(vw[i] > 0.0f)) continue;
-    const float ci = ncc_exact(C, i + 1, px, py, pl, rw);
+    const float ci = E::ncc(C, i + 1, px, py, pl, rw);
     ++nncc;
     if (C.geom) cn += vw[i] * (ci + 0.2f * geom_cost(C, i + 1, pl, px, py));
     else cn += vw[i] * ci;
@@ -405,8 +472,9 @@ __device__ __forceinline__ int find_max_idx(const float* x) {  // HPM.cu:54-65
 }
 
 // PlaneHypothesisRefinement, HPM.cu:765-841
+template <class E>
 __device__ void refine(const RunConsts& C, const DevState& S, float4& plane, float& depth, float& cost, Philox& rng,
-                       const float* vw, float wn, float& rcost, int px, int py, float texture, const RefWin& rw,
+                       const float* vw, float wn, float& rcost, int px, int py, float texture, const typename E::Win& rw,
                        unsigned& nncc) {
   const DevCam& cam = C.cams[0];
   const int center = py * C.W + px;
@@ -437,7 +505,7 @@ __device__ void refine(const RunConsts& C, const DevState& S, float4& plane, flo
     float tc = 0.0f;
     for (int j = 0; j < V; ++j) {
       if (!(vw[j] > 0.0f)) continue;
-      const float cj = ncc_exact(C, j + 1, px, py, tp, rw);
+      const float cj = E::ncc(C, j + 1, px, py, tp, rw);
       ++nncc;
       if (C.geom) tc += vw[j] * (cj + 0.2f * geom_cost(C, j + 1, tp, px, py));
       else tc += vw[j] * cj;
@@ -457,18 +525,19 @@ __device__ void refine(const RunConsts& C, const DevState& S, float4& plane, flo
 }
 
 // CheckerboardPropagation, HPM.cu:1347-1873
-__device__ void propagate_pixel(const RunConsts& C, const DevState& S, int px, int py, int iter, const RefWin& rw,
+template <class E>
+__device__ void propagate_pixel(const RunConsts& C, const DevState& S, int px, int py, int iter, const typename E::Win& rw,
                                 Sample& s, unsigned& nncc, unsigned& n_ext) {
   const DevCam& cam = C.cams[0];
   const int center = py * C.W + px;
   Philox rng(C.seed, (uint32_t)center, (uint32_t)(1 + iter));
-  sample_neighbours(C, S, px, py, iter, rw, s, nncc, n_ext);
+  sample_neighbours<E>(C, S, px, py, iter, rw, s, nncc, n_ext);
   float vw[kMaxSrc]; uint32_t tsel; float wn; int nsel;
   view_selection(C, S, px, py, iter, s, rng, vw, tsel, wn, nsel);
   float fc[8];
   final_costs(C, S, px, py, s, vw, wn, fc);
   const int min_idx = find_min_idx(fc);
-  float cost_now = current_cost(C, px, py, S.planes[center], vw, wn, rw, nncc);
+  float cost_now = current_cost<E>(C, px, py, S.planes[center], vw, wn, rw, nncc);
   S.costs[center] = cost_now;
   float depth_now = depth_from_plane(cam, S.planes[center], px, py);
   float rcost = 0.0f, texture = 0.0f;
@@ -525,7 +594,7 @@ __device__ void propagate_pixel(const RunConsts& C, const DevState& S, int px, i
       depth_now = db; plane_now = q; cost_now = fc[min_idx]; S.sel[center] = tsel;
     }
   }
-  refine(C, S, plane_now, depth_now, cost_now, rng, vw, wn, rcost, px, py, texture, rw, nncc);
+  refine<E>(C, S, plane_now, depth_now, cost_now, rng, vw, wn, rcost, px, py, texture, rw, nncc);
   if (C.hierarchy) {
     if (cost_now < S.pre_costs[center] - 0.1f) {
       S.costs[center] = cost_now;
@@ -538,20 +607,21 @@ __device__ void propagate_pixel(const RunConsts& C, const DevState& S, int px, i
 }
 
 // CheckerboardPropagation_MandatoryConsistency, HPM.cu:1875-2791
-__device__ void propagate_pixel_mand(const RunConsts& C, const DevState& S, int px, int py, int iter, const RefWin& rw,
+template <class E>
+__device__ void propagate_pixel_mand(const RunConsts& C, const DevState& S, int px, int py, int iter, const typename E::Win& rw,
                                      Sample& s, unsigned& nncc, unsigned& n_ext) {
   const DevCam& cam = C.cams[0];
   const int V = C.V;
   const int center = py * C.W + px;
   Philox rng(C.seed, (uint32_t)center, (uint32_t)(1 + iter));
-  sample_neighbours(C, S, px, py, iter, rw, s, nncc, n_ext);
+  sample_neighbours<E>(C, S, px, py, iter, rw, s, nncc, n_ext);
   float vw[kMaxSrc]; uint32_t tsel; float wn; int nsel;
   view_selection(C, S, px, py, iter, s, rng, vw, tsel, wn, nsel);
   if (S.conf[center] >= 0.3f) {
     float fc[8];
     final_costs(C, S, px, py, s, vw, wn, fc);
     const int min_idx = find_min_idx(fc);
-    float cost_now = current_cost(C, px, py, S.planes[center], vw, wn, rw, nncc);
+    float cost_now = current_cost<E>(C, px, py, S.planes[center], vw, wn, rw, nncc);
     S.costs[center] = cost_now;
     float depth_now = depth_from_plane(cam, S.planes[center], px, py);
     float4 plane_now = S.planes[center];
@@ -565,7 +635,7 @@ __device__ void propagate_pixel_mand(const RunConsts& C, const DevState& S, int 
       if (cc != cc) { plane_now = q; cost_now = fc[min_idx]; S.sel[center] = tsel; }
     }
     float unused = 0.0f;
-    refine(C, S, plane_now, depth_now, cost_now, rng, vw, wn, unused, px, py, 0.0f, rw, nncc);
+    refine<E>(C, S, plane_now, depth_now, cost_now, rng, vw, wn, unused, px, py, 0.0f, rw, nncc);
     S.costs[center] = cost_now;
     S.planes[center] = plane_now;
     return;
@@ -598,7 +668,7 @@ __device__ void propagate_pixel_mand(const RunConsts& C, const DevState& S, int 
     fc[i] = f;
   }
   const int max_idx = find_max_idx(rfc);
-  float cost_now = current_cost(C, px, py, S.planes[center], vw, wn, rw, nncc);
+  float cost_now = current_cost<E>(C, px, py, S.planes[center], vw, wn, rw, nncc);
   S.costs[center] = cost_now;
   const float gce = expf_hpm(-cost_now * cost_now / 0.18f);
   const float mmc = (float)nsel * 0.5f;
@@ -629,10 +699,11 @@ __device__ void propagate_pixel_mand(const RunConsts& C, const DevState& S, int 
 
 // One colour pass.  Thread (tx, y) of a 32 x 8 block handles the tx-th pixel of
 // the pass's colour in row y: x = 2 tx + ((y + colour) & 1).
-__global__ __launch_bounds__(256) void k_propagate(const RunConsts* __restrict__ Cp, DevState S, int colour, int iter, int H2) {
+template <class E>
+__global__ HPM_PROP_ATTR void k_propagate(const RunConsts* __restrict__ Cp, DevState S, int colour, int iter, int H2) {
   const RunConsts& C = *Cp;
-  const int tx = blockIdx.x * blockDim.x + threadIdx.x;
-  const int py = blockIdx.y * blockDim.y + threadIdx.y;
+  int tx, py;
+  colour_pixel<E>(tx, py, C.W, H2);
   const int px = 2 * tx + ((py + colour) & 1);
   unsigned nncc = 0, n_ext = 0;
   bool active = px < C.W && py < H2;
@@ -641,16 +712,16 @@ __global__ __launch_bounds__(256) void k_propagate(const RunConsts* __restrict__
     if (S.conf[center] > 0.3f || S.masks[center] == 0u) active = false;
   }
   if (active) {
-    RefWin rw;
-    build_refwin(C, px, py, rw);
+    typename E::Win rw;
+    E::build(C, px, py, rw);
     Sample s;
-    if (C.mand) propagate_pixel_mand(C, S, px, py, iter, rw, s, nncc, n_ext);
-    else propagate_pixel(C, S, px, py, iter, rw, s, nncc, n_ext);
+    if (C.mand) propagate_pixel_mand<E>(C, S, px, py, iter, rw, s, nncc, n_ext);
+    else propagate_pixel<E>(C, S, px, py, iter, rw, s, nncc, n_ext);
   }
-  wave_count_add(&S.counters[1], nncc);
+  wave_count_add(&S.counters[1], first_lane<E>(nncc));
   // evaluations the reference performs for this pixel: (8 + E + 1 + 5) * V
   const unsigned ref_count = active ? (unsigned)((14 + n_ext) * C.V) : 0u;
-  wave_count_add(&S.counters[2], ref_count);
+  wave_count_add(&S.counters[2], first_lane<E>(ref_count));
 }
 
 __global__ __launch_bounds__(256) void k_output(const RunConsts* __restrict__ Cp, float4* planes) {  // HPM.cu:2829-2842
@@ -665,19 +736,24 @@ __global__ __launch_bounds__(256) void k_output(const RunConsts* __restrict__ Cp
 }
 
 // ComputeMultiViewCostVector over all pixels at fixed planes (cost microbench).
+template <class E>
 __global__ __launch_bounds__(256) void k_cost_fixed(const RunConsts* __restrict__ Cp, const float4* __restrict__ planes,
                                                     float* __restrict__ out) {
   const RunConsts& C = *Cp;
-  const int px = blockIdx.x * blockDim.x + threadIdx.x;
-  const int py = blockIdx.y * blockDim.y + threadIdx.y;
+  int px, py;
+  tile_pixel<E>(px, py, C.W, C.H);
   if (px >= C.W || py >= C.H) return;
   const int p = py * C.W + px;
-  RefWin rw;
-  build_refwin(C, px, py, rw);
+  typename E::Win rw;
+  E::build(C, px, py, rw);
   const float4 pl = planes[p];
-  for (int v = 0; v < C.V; ++v) out[(size_t)p * C.V + v] = ncc_exact(C, v + 1, px, py, pl, rw);
+  for (int v = 0; v < C.V; ++v) {
+    const float c = E::ncc(C, v + 1, px, py, pl, rw);
+    if (first_lane<E>(1u)) out[(size_t)p * C.V + v] = c;
+  }
 }
 
+template <class E>
 __global__ __launch_bounds__(256) void k_init_cost(const RunConsts* __restrict__ Cp, const float4* __restrict__ planes,
                                                    float* __restrict__ out, uint32_t* __restrict__ sel) {
   const RunConsts& C = *Cp;
@@ -685,11 +761,11 @@ __global__ __launch_bounds__(256) void k_init_cost(const RunConsts* __restrict__
   const int py = blockIdx.y * blockDim.y + threadIdx.y;
   if (px >= C.W || py >= C.H) return;
   const int p = py * C.W + px;
-  RefWin rw;
-  build_refwin(C, px, py, rw);
+  typename E::Win rw;
+  E::build(C, px, py, rw);
   unsigned n = 0;
   uint32_t m;
-  out[p] = init_cost(C, px, py, planes[p], rw, m, n);
+  out[p] = init_cost<E>(C, px, py, planes[p], rw, m, n);
   sel[p] = m;
 }
 

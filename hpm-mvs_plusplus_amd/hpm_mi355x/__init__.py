@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libhpm_mi355x.so")
+# HPM_MI355X_LIB selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("HPM_MI355X_LIB") or os.path.join(os.path.dirname(_HERE), "lib", "libhpm_mi355x.so")
 
 HPM_OK = 0
 PREC_FP32_EXACT = 0

@@ -1,0 +1,86 @@
+"""Mixed-precision engine (FP16 / BF16 / FP32 quad storage) against the FP64
+oracle (needs an MI355X).
+
+Tolerances, written per test:
+  * cost at fixed hypotheses: |cost - cost_fp64| <= 1e-5 on >= 99.9 % of the
+    (pixel, view) pairs whose var >= 1e-5 decision agrees, mean <= 2e-6;
+  * after a full RunPatchMatch: depth within 1 % of the FP64 oracle on >= 99 %
+    of pixels and normals within 5 degrees on >= 99 %.
+"""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="no HIP device")]
+
+PRECISIONS = ["fp16", "bf16", "fp32fast"]
+
+
+def _ctx(cams, imgs, **params):
+    import hpm_mi355x as H
+    ctx = H.Context(0)
+    ctx.set_params(H.default_params(**params))
+    ctx.set_views([H.camera_from_scene(c) for c in cams], imgs)
+    return ctx
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+def test_fast_cost_vs_fp64(small_scene, prec):
+    import hpm_mi355x as H
+    from hpm_mi355x import scenes
+    sc, ids, cams, imgs = small_scene
+    planes = scenes.random_planes_per_pixel(cams[0], seed=4)
+    print(f"case: cost_fixed {prec} 160x120 V=4", flush=True)
+    ctx = _ctx(cams, imgs)
+    got = ctx.cost_fixed(planes, precision=H.PRECISIONS[prec]).astype(np.float64)
+    oc = [O.camera_from_scene(c) for c in cams]
+    ref64 = O.cost_fixed(oc, imgs, O.params(len(oc), oc[0]), planes.astype(np.float64), np.float64)
+    same = (got == 2.0) == (ref64 == 2.0)
+    assert same.mean() > 0.999
+    err = np.abs(got - ref64)[same]
+    print(f"{prec}: median {np.median(err):.2e} q999 {np.quantile(err, 0.999):.2e} max {err.max():.2e}")
+    assert np.quantile(err, 0.999) <= 1e-5
+    assert err.mean() <= 2e-6
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+def test_fast_full_run_vs_fp64(wide_scene, prec):
+    sc, ids, cams, imgs = wide_scene
+    print(f"case: RunPatchMatch {prec} 128x96 V=10", flush=True)
+    ctx = _ctx(cams, imgs, precision=prec)
+    ctx.run(5)
+    gp = ctx.planes().astype(np.float64)
+    oc = [O.camera_from_scene(c) for c in cams]
+    r64 = O.run_patchmatch(oc, imgs, O.params(len(oc), oc[0]), 5, dtype=np.float64)
+    d64 = r64["planes"][..., 3]
+    rel = np.abs(gp[..., 3] - d64) / np.abs(d64)
+    n, n64 = gp[..., :3], r64["planes"][..., :3]
+    cosang = np.sum(n * n64, -1) / (np.linalg.norm(n, axis=-1) * np.linalg.norm(n64, axis=-1))
+    ang = np.degrees(np.arccos(np.clip(cosang, -1, 1)))
+    print(f"{prec}: depth within 1% {np.mean(rel <= 0.01):.4f}, normals within 5 deg {np.mean(ang <= 5):.4f}")
+    assert np.mean(rel <= 0.01) >= 0.99
+    assert np.mean(ang <= 5.0) >= 0.99
+
+
+def test_fast_geom_and_mand_passes_run(small_scene):
+    """The mixed-precision engine drives every pass mode; results stay close to
+    the exact engine's (same random draws, decisions differ only where the
+    costs' rounding does)."""
+    sc, ids, cams, imgs = small_scene
+    oc = [O.camera_from_scene(c) for c in cams]
+    base = O.run_patchmatch(oc, imgs, O.params(len(oc), oc[0]), 5)
+    depths = [None] + [sc.depths[i] for i in ids[1:]]
+    conf = np.full(imgs[0].shape, 0.5, np.float32)
+    conf[::3] = 0.1
+    out = {}
+    for prec in ("fp32", "fp16"):
+        ctx = _ctx(cams, imgs, precision=prec, geom_consistency=1, mand_consistency=1)
+        ctx.set_src_depths(depths)
+        ctx.set_confidence(conf)
+        ctx.reload_hypotheses(base["planes"], base["costs"])
+        ctx.run(6)
+        out[prec] = ctx.planes()
+    rel = np.abs(out["fp16"][..., 3] - out["fp32"][..., 3]) / np.abs(out["fp32"][..., 3])
+    assert np.mean(rel <= 0.01) >= 0.98
