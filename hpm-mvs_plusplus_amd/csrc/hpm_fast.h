@@ -209,29 +209,57 @@ struct FastEngine {
     const int qp = sw + 1;
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<void*>(C.quad[v]), (short)0, qp * (sh + 1) * (int)sizeof(typename S::Q), 0x00020000);
+    // Tap (ii, jj): cell offset and bilinear fractions.
+    auto tap = [&](int ii, int jj, unsigned& off, float& ax, float& ay) {
+      const float i = (float)(-5 + 2 * ii), j = (float)(-5 + 2 * jj);
+      const float den = ffma(i, fg6, ffma(j, fg7, 1.0f));
+      const float r = __builtin_amdgcn_rcpf(den);
+      const float du = ffma(i, au, j * bu) * r;
+      const float dv = ffma(i, av, j * bv) * r;
+      const float x = fu + du, y = fv + dv;
+      const float fx = floorf(x), fy = floorf(y);
+      ax = x - fx;
+      ay = y - fy;
+      const int tx = iu + (int)__builtin_amdgcn_fmed3f(fx, lox, hix);
+      const int ty = iv + (int)__builtin_amdgcn_fmed3f(fy, loy, hiy);
+      off = (__umul24((unsigned)ty, (unsigned)qp) + (unsigned)tx) * (unsigned)sizeof(typename S::Q);
+    };
+    // Software pipeline over the six columns: the loads of column ii + 1 are
+    // issued before column ii is consumed, so twelve gathers are in flight.
+    typename S::Q qa[kTapsPerAxis], qb[kTapsPerAxis];
+    float axa[kTapsPerAxis], aya[kTapsPerAxis], axb[kTapsPerAxis], ayb[kTapsPerAxis];
+#pragma unroll
+    for (int jj = 0; jj < kTapsPerAxis; ++jj) {
+      unsigned off;
+      tap(0, jj, off, axa[jj], aya[jj]);
+      qa[jj] = S::load(rsrc, off);
+    }
     float s0 = 0.0f, S1 = 0.0f, S2 = 0.0f, S3 = 0.0f;
 #pragma unroll
     for (int ii = 0; ii < kTapsPerAxis; ++ii) {
+      if (ii + 1 < kTapsPerAxis) {
+#pragma unroll
+        for (int jj = 0; jj < kTapsPerAxis; ++jj) {
+          unsigned off;
+          tap(ii + 1, jj, off, axb[jj], ayb[jj]);
+          qb[jj] = S::load(rsrc, off);
+        }
+      }
 #pragma unroll
       for (int jj = 0; jj < kTapsPerAxis; ++jj) {
-        const float i = (float)(-5 + 2 * ii), j = (float)(-5 + 2 * jj);
-        const float den = ffma(i, fg6, ffma(j, fg7, 1.0f));
-        const float r = __builtin_amdgcn_rcpf(den);
-        const float du = ffma(i, au, j * bu) * r;
-        const float dv = ffma(i, av, j * bv) * r;
-        const float x = fu + du, y = fv + dv;
-        const float fx = floorf(x), fy = floorf(y);
-        const float ax = x - fx, ay = y - fy;
-        const int tx = iu + (int)__builtin_amdgcn_fmed3f(fx, lox, hix);
-        const int ty = iv + (int)__builtin_amdgcn_fmed3f(fy, loy, hiy);
-        const unsigned off = (__umul24((unsigned)ty, (unsigned)qp) + (unsigned)tx) * (unsigned)sizeof(typename S::Q);
-        const float s = S::bilinear(S::load(rsrc, off), ax, ay);
+        const float s = S::bilinear(qa[jj], axa[jj], aya[jj]);
         if (ii == 0 && jj == 0) s0 = s;
         const float sp = s - s0;
         const float ws = fw.w[ii * kTapsPerAxis + jj] * sp;
         S1 += ws;
         S2 = ffma(ws, sp, S2);
         S3 = ffma(fw.wr[ii * kTapsPerAxis + jj], sp, S3);
+      }
+#pragma unroll
+      for (int jj = 0; jj < kTapsPerAxis; ++jj) {
+        qa[jj] = qb[jj];
+        axa[jj] = axb[jj];
+        aya[jj] = ayb[jj];
       }
     }
     const float m_s = S1 * fw.inv_w;
@@ -369,7 +397,9 @@ struct QuadLaneEngine {
     const float den0 = ffma(fw.bi, fg6, ffma(fw.bj, fg7, 1.0f));
     const float nu0 = ffma(fw.bi, au, fw.bj * bu);
     const float nv0 = ffma(fw.bi, av, fw.bj * bv);
-    float S1 = 0.0f, S2 = 0.0f, S3 = 0.0f;
+    // all nine gathers of the lane are issued before any is consumed
+    typename S::Q q[9];
+    float axs[9], ays[9];
 #pragma unroll
     for (int di = 0; di < 3; ++di) {
 #pragma unroll
@@ -381,16 +411,22 @@ struct QuadLaneEngine {
         const float dv = ffma(oi, av, ffma(oj, bv, nv0)) * r;
         const float x = fu + du, y = fv + dv;
         const float fx = floorf(x), fy = floorf(y);
-        const float ax = x - fx, ay = y - fy;
+        axs[di * 3 + dj] = x - fx;
+        ays[di * 3 + dj] = y - fy;
         const int tx = iu + (int)__builtin_amdgcn_fmed3f(fx, lox, hix);
         const int ty = iv + (int)__builtin_amdgcn_fmed3f(fy, loy, hiy);
         const unsigned off = (__umul24((unsigned)ty, (unsigned)qp) + (unsigned)tx) * (unsigned)sizeof(typename S::Q);
-        const float sp = S::bilinear(S::load(rsrc, off), ax, ay) - fw.c_r;
-        const float ws = fw.w[di * 3 + dj] * sp;
-        S1 += ws;
-        S2 = ffma(ws, sp, S2);
-        S3 = ffma(fw.wr[di * 3 + dj], sp, S3);
+        q[di * 3 + dj] = S::load(rsrc, off);
       }
+    }
+    float S1 = 0.0f, S2 = 0.0f, S3 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const float sp = S::bilinear(q[k], axs[k], ays[k]) - fw.c_r;
+      const float ws = fw.w[k] * sp;
+      S1 += ws;
+      S2 = ffma(ws, sp, S2);
+      S3 = ffma(fw.wr[k], sp, S3);
     }
     S1 = quad_sum(S1);
     S2 = quad_sum(S2);

@@ -18,7 +18,7 @@
 // Occupancy target of the PatchMatch kernels (waves per SIMD); the register
 // budget follows from it (512 / waves VGPRs per lane).
 #ifndef HPM_PROP_WAVES
-#define HPM_PROP_WAVES 1
+#define HPM_PROP_WAVES 2
 #endif
 #define HPM_PROP_ATTR __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_PROP_WAVES)))
 
@@ -44,7 +44,7 @@ template <class E> struct Tiles {
 };
 
 #ifndef HPM_XCD_STRIP
-#define HPM_XCD_STRIP 8
+#define HPM_XCD_STRIP 4
 #endif
 // Workgroup -> tile, XCD-aware.  The dispatcher deals workgroups round-robin to
 // the 8 XCDs (XCD = id % 8), each with its own 4 MB L2.  Every XCD gets one
