@@ -229,6 +229,11 @@ template <class E> int launch_schedule(hpm_ctx* ctx, int n_halfpasses, int do_ou
   const int H2 = (ctx->H / 2) * 2;
   const dim3 gp((((ctx->W + 1) / 2 + T::cw - 1) / T::cw) * ((H2 + T::ch - 1) / T::ch));
   hipStream_t st = ctx->stream;
+  const int V = ctx->nimg - 1;
+  const size_t lds_bytes = (size_t)8 * V * kRowStride * sizeof(float);
+  if (V <= kLdsMaxSrc)  // the rows need up to 160 KiB of dynamic LDS
+    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_propagate<E, true>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
   if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[0], st));
   hipLaunchKernelGGL(k_init<E>, gi, b256, 0, st, ctx->d_consts, S);
   HIP_OK(hipGetLastError());
@@ -237,7 +242,10 @@ template <class E> int launch_schedule(hpm_ctx* ctx, int n_halfpasses, int do_ou
   for (int it = 0; it < ctx->params.num_iterations; ++it) {
     for (int colour = 0; colour < 2; ++colour) {
       if (n_halfpasses >= 0 && done >= n_halfpasses) break;
-      hipLaunchKernelGGL(k_propagate<E>, gp, b256, 0, st, ctx->d_consts, S, colour, it, H2);
+      if (V <= kLdsMaxSrc)
+        hipLaunchKernelGGL((k_propagate<E, true>), gp, b256, lds_bytes, st, ctx->d_consts, S, colour, it, H2);
+      else
+        hipLaunchKernelGGL((k_propagate<E, false>), gp, b256, 0, st, ctx->d_consts, S, colour, it, H2);
       HIP_OK(hipGetLastError());
       ++done;
     }

@@ -53,6 +53,10 @@ __device__ __forceinline__ float lerp_cell(float t00, float t10, float t01, floa
 #ifndef HPM_MIX_LERP
 #define HPM_MIX_LERP 1
 #endif
+// columns of taps whose gathers are issued ahead of the column being summed
+#ifndef HPM_PIPE_COLS
+#define HPM_PIPE_COLS 1
+#endif
 
 struct QuadF16 {
   using Q = uint2;
@@ -224,42 +228,41 @@ struct FastEngine {
       const int ty = iv + (int)__builtin_amdgcn_fmed3f(fy, loy, hiy);
       off = (__umul24((unsigned)ty, (unsigned)qp) + (unsigned)tx) * (unsigned)sizeof(typename S::Q);
     };
-    // Software pipeline over the six columns: the loads of column ii + 1 are
-    // issued before column ii is consumed, so twelve gathers are in flight.
-    typename S::Q qa[kTapsPerAxis], qb[kTapsPerAxis];
-    float axa[kTapsPerAxis], aya[kTapsPerAxis], axb[kTapsPerAxis], ayb[kTapsPerAxis];
+    // Software pipeline over the six columns: the loads of columns
+    // ii + 1 .. ii + HPM_PIPE_COLS are in flight while column ii is consumed.
+    constexpr int L = HPM_PIPE_COLS;
+    constexpr int R = L + 1;  // ring of column buffers
+    typename S::Q q[R][kTapsPerAxis];
+    float axr[R][kTapsPerAxis], ayr[R][kTapsPerAxis];
 #pragma unroll
-    for (int jj = 0; jj < kTapsPerAxis; ++jj) {
-      unsigned off;
-      tap(0, jj, off, axa[jj], aya[jj]);
-      qa[jj] = S::load(rsrc, off);
+    for (int c = 0; c < L && c < kTapsPerAxis; ++c) {
+#pragma unroll
+      for (int jj = 0; jj < kTapsPerAxis; ++jj) {
+        unsigned off;
+        tap(c, jj, off, axr[c % R][jj], ayr[c % R][jj]);
+        q[c % R][jj] = S::load(rsrc, off);
+      }
     }
     float s0 = 0.0f, S1 = 0.0f, S2 = 0.0f, S3 = 0.0f;
 #pragma unroll
     for (int ii = 0; ii < kTapsPerAxis; ++ii) {
-      if (ii + 1 < kTapsPerAxis) {
+      if (ii + L < kTapsPerAxis) {
 #pragma unroll
         for (int jj = 0; jj < kTapsPerAxis; ++jj) {
           unsigned off;
-          tap(ii + 1, jj, off, axb[jj], ayb[jj]);
-          qb[jj] = S::load(rsrc, off);
+          tap(ii + L, jj, off, axr[(ii + L) % R][jj], ayr[(ii + L) % R][jj]);
+          q[(ii + L) % R][jj] = S::load(rsrc, off);
         }
       }
 #pragma unroll
       for (int jj = 0; jj < kTapsPerAxis; ++jj) {
-        const float s = S::bilinear(qa[jj], axa[jj], aya[jj]);
+        const float s = S::bilinear(q[ii % R][jj], axr[ii % R][jj], ayr[ii % R][jj]);
         if (ii == 0 && jj == 0) s0 = s;
         const float sp = s - s0;
         const float ws = fw.w[ii * kTapsPerAxis + jj] * sp;
         S1 += ws;
         S2 = ffma(ws, sp, S2);
         S3 = ffma(fw.wr[ii * kTapsPerAxis + jj], sp, S3);
-      }
-#pragma unroll
-      for (int jj = 0; jj < kTapsPerAxis; ++jj) {
-        qa[jj] = qb[jj];
-        axa[jj] = axb[jj];
-        aya[jj] = ayb[jj];
       }
     }
     const float m_s = S1 * fw.inv_w;

@@ -24,8 +24,15 @@
 
 namespace hpm {
 
+// Cost rows of the 8 sampled directions, direction-major (row d, view v at
+// d * V + v).  They live in LDS, thread-interleaved (element k of a thread at
+// ca[k * kRowStride]) so that a wave's accesses are bank-conflict free; views
+// beyond kLdsMaxSrc fall back to a private array.
+constexpr int kRowStride = 256;  // threads per block
+constexpr int kLdsMaxSrc = 20;   // 8 * 20 * 256 * 4 B = 160 KiB
 struct Sample {
-  float ca[8 * kMaxSrc];   // cost rows, direction-major: ca[d * V + v]
+  float* ca;
+  int cs;  // element stride: kRowStride in LDS, 1 in a private array
   int pos[8];
   bool flag[8];
 };
@@ -277,8 +284,8 @@ __device__ void sample_neighbours(const RunConsts& C, const DevState& S, int px,
   const int center = py * W + px;
   const float* __restrict__ costs = S.costs;
   for (int d = 0; d < 8; ++d) s.flag[d] = false;
-  for (int k = 0; k < 8 * V; ++k) s.ca[k] = 0.0f;
-  s.ca[0] = 2.0f;  // `float cost_array[8][32] = { 2.0f }`
+  for (int k = 0; k < 8 * V; ++k) s.ca[s.cs * (k)] = 0.0f;
+  s.ca[s.cs * (0)] = 2.0f;  // `float cost_array[8][32] = { 2.0f }`
   int pos[8] = {center - 5 - 6 * W, center - W * 5, center + 6 - 5 * W, center + W * 5,
                 center + 5 + 6 * W, center - 5,     center - 6 + 5 * W, center + 5};
   // axis directions: 1 up, 3 down, 5 left, 7 right (7 keeps the maximum)
@@ -341,7 +348,7 @@ __device__ void sample_neighbours(const RunConsts& C, const DevState& S, int px,
   for (int d = 0; d < 8; ++d) {
     if (!s.flag[d]) continue;
     const float4 pl = S.planes[pos[d]];
-    for (int v = 0; v < V; ++v) s.ca[d * V + v] = E::ncc(C, v + 1, px, py, pl, rw);
+    for (int v = 0; v < V; ++v) s.ca[s.cs * (d * V + v)] = E::ncc(C, v + 1, px, py, pl, rw);
     nncc += V;
   }
   // extension of direction 0
@@ -350,7 +357,7 @@ __device__ void sample_neighbours(const RunConsts& C, const DevState& S, int px,
     const float good = C.judge_thr[iter][it];
     int gs = 0, bs = 0;
     for (int v = 0; v < V; v++) {
-      const float x = s.ca[v];
+      const float x = s.ca[s.cs * (v)];
       if (x < good) gs++;
       if (x > 1.2f) bs++;
     }
@@ -372,7 +379,7 @@ __device__ void sample_neighbours(const RunConsts& C, const DevState& S, int px,
       } else {
         pos[0] = mp;
         const float4 pl = S.planes[mp];
-        for (int v = 0; v < V; ++v) s.ca[v] = E::ncc(C, v + 1, px, py, pl, rw);
+        for (int v = 0; v < V; ++v) s.ca[s.cs * (v)] = E::ncc(C, v + 1, px, py, pl, rw);
         nncc += V;
         n_ext += 1;
       }
@@ -381,63 +388,91 @@ __device__ void sample_neighbours(const RunConsts& C, const DevState& S, int px,
   for (int d = 0; d < 8; ++d) s.pos[d] = pos[d];
 }
 
-// Joint view selection, HPM.cu:1686-1751
+// View weights of the joint view selection: counts 0..15 (15 draws) packed
+// four bits per view, views 0-15 in `lo`, 16-31 in `hi`, so that no
+// per-thread array (and no scratch memory) is needed.
+struct ViewWeights {
+  uint64_t lo = 0, hi = 0;
+  __device__ __forceinline__ int get(int v) const { return (int)(((v < 16 ? lo : hi) >> (4 * (v & 15))) & 15ull); }
+  __device__ __forceinline__ bool on(int v) const { return get(v) > 0; }
+  __device__ __forceinline__ float w(int v) const { return (float)get(v); }
+  __device__ __forceinline__ void inc(int v) {
+    if (v < 16) lo += 1ull << (4 * v); else hi += 1ull << (4 * (v - 16));
+  }
+};
+
+// Sampling weight of view i before normalisation: the view's probability from
+// the 8 direction costs times its prior from the neighbours' masks
+// (HPM.cu:1689-1727).
+__device__ __forceinline__ float view_prob(const RunConsts& C, const Sample& s, int i, int V, float thr, float fb,
+                                           const uint32_t* nm, int nflags) {
+  float prior = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if ((nflags >> k) & 1) prior += ((nm[k] >> i) & 1u) ? 0.9f : 0.1f;
+  float count = 0.0f, tmpw = 0.0f;
+  int count_false = 0;
+  for (int j = 0; j < 8; j++) {
+    const float x = s.ca[s.cs * (j * V + i)];
+    if (x < thr) { tmpw += expf_hpm(x * x / (-0.18f)); count++; }
+    if (x > 1.2f) count_false++;
+  }
+  float p = 0.0f;
+  if (count > 2 && count_false < 3) p = tmpw / count;
+  else if (count_false < 3) p = fb;
+  return p * prior;
+}
+
+// Joint view selection, HPM.cu:1686-1751.  The 15 draws are taken first (the
+// scans between them draw nothing) and resolved in one sweep over the CDF:
+// draw k goes to the first view whose cumulative probability exceeds it.
 __device__ void view_selection(const RunConsts& C, const DevState& S, int px, int py, int iter, const Sample& s, Philox& rng,
-                               float* vw, uint32_t& tsel, float& wn, int& nsel) {
+                               ViewWeights& vw, uint32_t& tsel, float& wn, int& nsel) {
   const int W = C.W, V = C.V;
   const int center = py * W + px;
-  float prior[kMaxSrc];
-  for (int v = 0; v < V; ++v) { vw[v] = 0.0f; prior[v] = 0.0f; }
   const int nbr[4] = {center - W * 5, center + W * 5, center - 5, center + 5};
+  uint32_t nm[4];
+  int nflags = 0;
+#pragma unroll
   for (int i = 0; i < 4; ++i) {
-    if (!s.flag[2 * i + 1]) continue;
-    const uint32_t m = S.sel[nbr[i]];
-    for (int j = 0; j < V; ++j) prior[j] += ((m >> j) & 1u) ? 0.9f : 0.1f;
+    nm[i] = 0u;
+    if (s.flag[2 * i + 1]) { nm[i] = S.sel[nbr[i]]; nflags |= 1 << i; }
   }
-  float probs[kMaxSrc];
-  const float thr = C.vs_thr[iter];
-  for (int i = 0; i < V; i++) {
-    float count = 0.0f, tmpw = 0.0f;
-    int count_false = 0;
-    for (int j = 0; j < 8; j++) {
-      const float x = s.ca[j * V + i];
-      if (x < thr) { tmpw += expf_hpm(x * x / (-0.18f)); count++; }
-      if (x > 1.2f) count_false++;
-    }
-    float p = 0.0f;
-    if (count > 2 && count_false < 3) p = tmpw / count;
-    else if (count_false < 3) p = C.vs_fb[iter];
-    probs[i] = p * prior[i];
-  }
+  const float thr = C.vs_thr[iter], fb = C.vs_fb[iter];
   float sum = 0.0f;
-  for (int i = 0; i < V; ++i) sum += probs[i];
+  for (int i = 0; i < V; ++i) sum += view_prob(C, s, i, V, thr, fb, nm, nflags);
   const float inv = 1.0f / sum;
+  float r[15];
+#pragma unroll
+  for (int k = 0; k < 15; ++k) r[k] = rng.uniform() - 1.19209290e-07f;
+  vw = ViewWeights();
+  uint32_t assigned = 0u;
   float cum = 0.0f;
-  for (int i = 0; i < V; ++i) { cum += probs[i] * inv; probs[i] = cum; }
-  for (int k = 0; k < 15; ++k) {
-    const float r = rng.uniform() - 1.19209290e-07f;
-    for (int v = 0; v < V; ++v) {
-      if (probs[v] > r) { vw[v] += 1.0f; break; }
+  for (int v = 0; v < V; ++v) {
+    cum += view_prob(C, s, v, V, thr, fb, nm, nflags) * inv;
+#pragma unroll
+    for (int k = 0; k < 15; ++k) {
+      if (!((assigned >> k) & 1u) && cum > r[k]) { assigned |= 1u << k; vw.inc(v); }
     }
   }
   tsel = 0u; wn = 0.0f; nsel = 0;
   for (int i = 0; i < V; ++i)
-    if (vw[i] > 0.0f) { tsel |= (1u << i); wn += vw[i]; nsel++; }
+    if (vw.on(i)) { tsel |= (1u << i); wn += vw.w(i); nsel++; }
 }
 
 __device__ __forceinline__ void final_costs(const RunConsts& C, const DevState& S, int px, int py, const Sample& s,
-                                            const float* vw, float wn, float* fc) {  // HPM.cu:1753-1771
+                                            const ViewWeights& vw, float wn, float* fc) {  // HPM.cu:1753-1771
   const int V = C.V;
   for (int i = 0; i < 8; ++i) {
     float f = 0.0f;
     for (int j = 0; j < V; ++j) {
-      if (vw[j] > 0.0f) {
-        const float c = s.ca[i * V + j];
+      if (vw.on(j)) {
+        const float c = s.ca[s.cs * (i * V + j)];
         if (C.geom) {
-          if (s.flag[i]) f += vw[j] * (c + 0.2f * geom_cost(C, j + 1, S.planes[s.pos[i]], px, py));
-          else f += vw[j] * (c + 0.1f * 3.0f);
+          if (s.flag[i]) f += vw.w(j) * (c + 0.2f * geom_cost(C, j + 1, S.planes[s.pos[i]], px, py));
+          else f += vw.w(j) * (c + 0.1f * 3.0f);
         } else {
-          f += vw[j] * c;
+          f += vw.w(j) * c;
         }
       }
     }
@@ -446,16 +481,16 @@ __device__ __forceinline__ void final_costs(const RunConsts& C, const DevState& 
 }
 
 template <class E>
-__device__ __forceinline__ float current_cost(const RunConsts& C, int px, int py, float4 pl, const float* vw, float wn,
+__device__ __forceinline__ float current_cost(const RunConsts& C, int px, int py, float4 pl, const ViewWeights& vw, float wn,
                                               const typename E::Win& rw, unsigned& nncc) {  // HPM.cu:1775-1786
   const int V = C.V;
   float cn = 0.0f;
   for (int i = 0; i < V; ++i) {
-    if (!(vw[i] > 0.0f)) continue;
+    if (!vw.on(i)) continue;
     const float ci = E::ncc(C, i + 1, px, py, pl, rw);
     ++nncc;
-    if (C.geom) cn += vw[i] * (ci + 0.2f * geom_cost(C, i + 1, pl, px, py));
-    else cn += vw[i] * ci;
+    if (C.geom) cn += vw.w(i) * (ci + 0.2f * geom_cost(C, i + 1, pl, px, py));
+    else cn += vw.w(i) * ci;
   }
   return cn / wn;
 }
@@ -474,7 +509,7 @@ __device__ __forceinline__ int find_max_idx(const float* x) {  // HPM.cu:54-65
 // PlaneHypothesisRefinement, HPM.cu:765-841
 template <class E>
 __device__ void refine(const RunConsts& C, const DevState& S, float4& plane, float& depth, float& cost, Philox& rng,
-                       const float* vw, float wn, float& rcost, int px, int py, float texture, const typename E::Win& rw,
+                       const ViewWeights& vw, float wn, float& rcost, int px, int py, float texture, const typename E::Win& rw,
                        unsigned& nncc) {
   const DevCam& cam = C.cams[0];
   const int center = py * C.W + px;
@@ -496,24 +531,26 @@ __device__ void refine(const RunConsts& C, const DevState& S, float4& plane, flo
   const float dmaxp = (1 + perturbation) * depth_pert;
   depth_pert = rng.uniform() * (dmaxp - dminp) + dminp;
   const float4 plane_pert = perturbed_normal(cam, px, py, plane, rng, C.pert_refine);
-  const float depths[5] = {depth_rand, depth, depth_rand, depth, depth_pert};
-  const float4 normals[5] = {plane, plane_rand, plane_rand, plane_pert, plane};
+  // candidates (depth, normal): (rand, cur) (cur, rand) (rand, rand) (cur, pert) (pert, cur), HPM.cu:797-799
+  const float depth0 = depth;
+  const float4 plane0 = plane;
   const int V = C.V;
   for (int i = 0; i < 5; ++i) {
-    float4 tp = normals[i];
-    tp.w = dist_to_origin(cam, px, py, depths[i], tp);
+    const float di = (i == 0 || i == 2) ? depth_rand : (i == 4 ? depth_pert : depth0);
+    float4 tp = (i == 0 || i == 4) ? plane0 : (i == 3 ? plane_pert : plane_rand);
+    tp.w = dist_to_origin(cam, px, py, di, tp);
     float tc = 0.0f;
     for (int j = 0; j < V; ++j) {
-      if (!(vw[j] > 0.0f)) continue;
+      if (!vw.on(j)) continue;
       const float cj = E::ncc(C, j + 1, px, py, tp, rw);
       ++nncc;
-      if (C.geom) tc += vw[j] * (cj + 0.2f * geom_cost(C, j + 1, tp, px, py));
-      else tc += vw[j] * cj;
+      if (C.geom) tc += vw.w(j) * (cj + 0.2f * geom_cost(C, j + 1, tp, px, py));
+      else tc += vw.w(j) * cj;
     }
     tc /= wn;
     const float db = depth_from_plane(cam, tp, px, py);
     if (use_prior) {
-      const float dd = depths[i] - depth_prior;
+      const float dd = di - depth_prior;
       const float nd = norm_diff(S.prior[center], tp);
       const float pr = prior_term(dd, nd, texture, C.two_dss, C.two_ass);
       const float rtc = restricted_cost(tc, 0.18f, texture, pr);
@@ -532,7 +569,7 @@ __device__ void propagate_pixel(const RunConsts& C, const DevState& S, int px, i
   const int center = py * C.W + px;
   Philox rng(C.seed, (uint32_t)center, (uint32_t)(1 + iter));
   sample_neighbours<E>(C, S, px, py, iter, rw, s, nncc, n_ext);
-  float vw[kMaxSrc]; uint32_t tsel; float wn; int nsel;
+  ViewWeights vw; uint32_t tsel; float wn; int nsel;
   view_selection(C, S, px, py, iter, s, rng, vw, tsel, wn, nsel);
   float fc[8];
   final_costs(C, S, px, py, s, vw, wn, fc);
@@ -615,7 +652,7 @@ __device__ void propagate_pixel_mand(const RunConsts& C, const DevState& S, int 
   const int center = py * C.W + px;
   Philox rng(C.seed, (uint32_t)center, (uint32_t)(1 + iter));
   sample_neighbours<E>(C, S, px, py, iter, rw, s, nncc, n_ext);
-  float vw[kMaxSrc]; uint32_t tsel; float wn; int nsel;
+  ViewWeights vw; uint32_t tsel; float wn; int nsel;
   view_selection(C, S, px, py, iter, s, rng, vw, tsel, wn, nsel);
   if (S.conf[center] >= 0.3f) {
     float fc[8];
@@ -644,13 +681,13 @@ __device__ void propagate_pixel_mand(const RunConsts& C, const DevState& S, int 
   for (int i = 0; i < 8; i++) {
     float f = 0.0f;
     for (int j = 0; j < V; ++j) {
-      if (vw[j] > 0.0f) {
-        const float c = s.ca[i * V + j];
+      if (vw.on(j)) {
+        const float c = s.ca[s.cs * (i * V + j)];
         if (C.geom) {
-          if (s.flag[i]) f += vw[j] * (c + 0.2f * geom_cost(C, j + 1, S.planes[s.pos[i]], px, py));
-          else f += vw[j] * (c + 0.1f * 3.0f);
+          if (s.flag[i]) f += vw.w(j) * (c + 0.2f * geom_cost(C, j + 1, S.planes[s.pos[i]], px, py));
+          else f += vw.w(j) * (c + 0.1f * 3.0f);
         } else {
-          f += vw[j] * c;
+          f += vw.w(j) * c;
         }
       }
     }
@@ -699,9 +736,11 @@ __device__ void propagate_pixel_mand(const RunConsts& C, const DevState& S, int 
 
 // One colour pass.  Thread (tx, y) of a 32 x 8 block handles the tx-th pixel of
 // the pass's colour in row y: x = 2 tx + ((y + colour) & 1).
-template <class E>
+template <class E, bool kLds>
 __global__ HPM_PROP_ATTR void k_propagate(const RunConsts* __restrict__ Cp, DevState S, int colour, int iter, int H2) {
   const RunConsts& C = *Cp;
+  extern __shared__ float lds_rows[];
+  float private_rows[kLds ? 1 : 8 * kMaxSrc];
   int tx, py;
   colour_pixel<E>(tx, py, C.W, H2);
   const int px = 2 * tx + ((py + colour) & 1);
@@ -715,6 +754,8 @@ __global__ HPM_PROP_ATTR void k_propagate(const RunConsts* __restrict__ Cp, DevS
     typename E::Win rw;
     E::build(C, px, py, rw);
     Sample s;
+    s.ca = kLds ? lds_rows + threadIdx.x : private_rows;
+    s.cs = kLds ? kRowStride : 1;
     if (C.mand) propagate_pixel_mand<E>(C, S, px, py, iter, rw, s, nncc, n_ext);
     else propagate_pixel<E>(C, S, px, py, iter, rw, s, nncc, n_ext);
   }
