@@ -99,7 +99,7 @@ __device__ __forceinline__ void wave_count_add(unsigned long long* counter, unsi
 
 // ComputeMultiViewInitialCostandSelectedViews, HPM.cu:450-488
 template <class E>
-__device__ float init_cost(const RunConsts& C, int px, int py, float4 pl, const typename E::Win& rw, uint32_t& sel, unsigned& nncc) {
+__device__ __forceinline__ float init_cost(const RunConsts& C, int px, int py, float4 pl, const typename E::Win& rw, uint32_t& sel, unsigned& nncc) {
   const int V = C.V;
   float cv[kMaxSrc], cvc[kMaxSrc];
   int valid = 0;
@@ -144,7 +144,7 @@ __device__ __forceinline__ void reload_plane(const RunConsts& C, const DevState&
 
 // JBU of the coarse normal/cost map inside RandomInitialization, HPM.cu:633-698.
 template <class E>
-__device__ void upsample_init(const RunConsts& C, const DevState& S, int px, int py, const typename E::Win& rw, unsigned& nncc) {
+__device__ __forceinline__ void upsample_init(const RunConsts& C, const DevState& S, int px, int py, const typename E::Win& rw, unsigned& nncc) {
   const int W = C.W, H = C.H;
   const int center = py * W + px;
   const float sc_cols = C.scaled_cols, sc_rows = C.scaled_rows;
@@ -278,7 +278,7 @@ __global__ HPM_PROP_ATTR void k_init(const RunConsts* __restrict__ Cp, DevState 
 // 1063-1126, 1322-1345).  Only direction 0 extends: the reference's
 // `bool symbol_eight_orientations[10] = { true }` enables index 0 only.
 template <class E>
-__device__ void sample_neighbours(const RunConsts& C, const DevState& S, int px, int py, int iter, const typename E::Win& rw,
+__device__ __forceinline__ void sample_neighbours(const RunConsts& C, const DevState& S, int px, int py, int iter, const typename E::Win& rw,
                                   Sample& s, unsigned& nncc, unsigned& n_ext) {
   const int W = C.W, H = C.H, V = C.V;
   const int center = py * W + px;
@@ -426,7 +426,7 @@ __device__ __forceinline__ float view_prob(const RunConsts& C, const Sample& s, 
 // Joint view selection, HPM.cu:1686-1751.  The 15 draws are taken first (the
 // scans between them draw nothing) and resolved in one sweep over the CDF:
 // draw k goes to the first view whose cumulative probability exceeds it.
-__device__ void view_selection(const RunConsts& C, const DevState& S, int px, int py, int iter, const Sample& s, Philox& rng,
+__device__ __forceinline__ void view_selection(const RunConsts& C, const DevState& S, int px, int py, int iter, const Sample& s, Philox& rng,
                                ViewWeights& vw, uint32_t& tsel, float& wn, int& nsel) {
   const int W = C.W, V = C.V;
   const int center = py * W + px;
@@ -508,7 +508,7 @@ __device__ __forceinline__ int find_max_idx(const float* x) {  // HPM.cu:54-65
 
 // PlaneHypothesisRefinement, HPM.cu:765-841
 template <class E>
-__device__ void refine(const RunConsts& C, const DevState& S, float4& plane, float& depth, float& cost, Philox& rng,
+__device__ __forceinline__ void refine(const RunConsts& C, const DevState& S, float4& plane, float& depth, float& cost, Philox& rng,
                        const ViewWeights& vw, float wn, float& rcost, int px, int py, float texture, const typename E::Win& rw,
                        unsigned& nncc) {
   const DevCam& cam = C.cams[0];
@@ -563,7 +563,7 @@ __device__ void refine(const RunConsts& C, const DevState& S, float4& plane, flo
 
 // CheckerboardPropagation, HPM.cu:1347-1873
 template <class E>
-__device__ void propagate_pixel(const RunConsts& C, const DevState& S, int px, int py, int iter, const typename E::Win& rw,
+__device__ __forceinline__ void propagate_pixel(const RunConsts& C, const DevState& S, int px, int py, int iter, const typename E::Win& rw,
                                 Sample& s, unsigned& nncc, unsigned& n_ext) {
   const DevCam& cam = C.cams[0];
   const int center = py * C.W + px;
@@ -645,7 +645,7 @@ __device__ void propagate_pixel(const RunConsts& C, const DevState& S, int px, i
 
 // CheckerboardPropagation_MandatoryConsistency, HPM.cu:1875-2791
 template <class E>
-__device__ void propagate_pixel_mand(const RunConsts& C, const DevState& S, int px, int py, int iter, const typename E::Win& rw,
+__device__ __forceinline__ void propagate_pixel_mand(const RunConsts& C, const DevState& S, int px, int py, int iter, const typename E::Win& rw,
                                      Sample& s, unsigned& nncc, unsigned& n_ext) {
   const DevCam& cam = C.cams[0];
   const int V = C.V;
