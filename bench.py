@@ -3,14 +3,26 @@
 with 10 source views (photometric pass: init + 3 red/black iterations + the
 plane -> depth/normal output), inputs resident in HBM.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32|fp16|bf16]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp16|bf16|fp32fast|fp32]
 
-One "step" is one RunPatchMatch of one reference view.  Prints one JSON line.
+One "step" is one RunPatchMatch of one reference view.  The headline `value`
+is the mixed-precision path (FP16 texel storage, FP32 accumulation by
+default); the plain FP32 path (bit-exact with the FP32 oracle) is timed beside
+it on the same view and reported as `plain_fp32`, together with the fraction of
+pixels whose depth agrees with it within 1 %.  Prints one JSON line.
+
 The roofline object prices the dominant kernel (the red/black propagation
 kernel) by its algorithmic bytes: 181 texels per NCC evaluation (37 reference
 texels + 36 bilinear samples x 4 source texels) x bytes per stored texel, plus
 16 B plane in and 4 B per view cost out per hypothesis, divided by its average
-launch time measured with HIP events on the context stream.
+launch time measured with HIP events on the context stream.  `traffic` is the
+HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by
+the same counters on a copy kernel of known size, read from
+profiles/hbm_traffic.json when that file holds the precision (else null).
+
+    python bench.py --calibration-copy-mib 4096
+
+runs only that copy kernel (4 GiB read + 4 GiB written), for the counter runs.
 """
 import argparse
 import json
@@ -44,7 +56,7 @@ def canny_like(img):
     return (g > 60).astype(np.uint32)
 
 
-def cpu_baseline(cams, imgs, canny, rows=24):
+def cpu_baseline(cams, imgs, canny, rows=1200):
     """The FP32 oracle on a band of `rows` rows of the same view (all host
     threads up to 16), scaled to views/s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -64,38 +76,76 @@ def cpu_baseline(cams, imgs, canny, rows=24):
                       f"(+ init halo), {dt:.2f} s, scaled by {H}/{rows}"}
 
 
-def calibration_copy(nbytes):
-    """One plain device-to-device copy of `nbytes` (the runtime's copy kernel),
-    so that profiled HBM counters can be scaled by a transfer of known size."""
-    import ctypes
-    hip = ctypes.CDLL("libamdhip64.so")
-    a, b = ctypes.c_void_p(), ctypes.c_void_p()
-    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
-    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
-    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
-    assert hip.hipDeviceSynchronize() == 0
-    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
-    assert hip.hipDeviceSynchronize() == 0
-    hip.hipFree(a)
-    hip.hipFree(b)
-    return nbytes
+def calibration_copy(mib):
+    """One elementwise copy kernel (y = x * 1) over `mib` MiB of float32: reads
+    and writes exactly mib MiB each, so that profiled HBM counters can be
+    scaled by a transfer of known size.  Sizes well above the 256 MB Infinity
+    Cache keep the counted traffic in HBM."""
+    import torch
+    n = (mib << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    ok = bool(y[-1].item() == 1.0)
+    return {"calibration_copy_bytes_read": n * 4, "calibration_copy_bytes_written": n * 4, "ok": ok}
+
+
+def hbm_traffic(precision):
+    """Measured HBM bytes per k_propagate launch for this precision, from the
+    committed rocprofv3 counter summary (None when absent)."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(precision)
+    except (OSError, ValueError):
+        return None
+
+
+def run_views(ctx, steps, warmup, seed0):
+    """Times `steps` RunPatchMatch calls after `warmup` untimed ones; returns
+    (views/s, ms per step, propagate ms, launches, NCC counts)."""
+    for i in range(warmup):
+        ctx.run(seed0 + i)
+        print(f"[bench] warmup {i} done", file=sys.stderr, flush=True)
+    ctx.set_timing(True)
+    prop_ms, launches, ncc_prop, ncc_ref = 0.0, 0, 0, 0
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ctx.run(seed0 + 100 + k, sync=False)
+        st = ctx.stats()  # synchronises the stream
+        prop_ms += st.ms_propagate
+        launches += st.propagate_launches
+        ncc_prop += st.ncc_propagate
+        ncc_ref += st.ncc_reference
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.set_timing(False)
+    return {"views_s": steps / elapsed, "elapsed": elapsed, "prop_ms": prop_ms, "launches": launches,
+            "ncc_prop": ncc_prop, "ncc_ref": ncc_ref}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16", "fp32fast"])
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default="fp16", choices=["fp16", "bf16", "fp32fast", "fp32"])
+    ap.add_argument("--no-plain-fp32", action="store_true", help="skip timing the plain FP32 path beside it")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=400)
-    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
-                    help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
+    ap.add_argument("--cpu-rows", type=int, default=1200)
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="run only an elementwise copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (views are independent; "
-              "the multi-GPU design shards reference views, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (reference views are "
+              "independent; a multi-GPU run would shard them, one process per GPU)", file=sys.stderr)
         sys.exit(2)
+    if args.calibration_copy_mib:
+        print(json.dumps(calibration_copy(args.calibration_copy_mib)), flush=True)
+        return
 
     import hpm_mi355x as Hm
     t_setup = time.perf_counter()
@@ -107,66 +157,68 @@ def main():
     ctx.set_canny(canny)
     print(f"[bench] workload ready in {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
 
-    for i in range(args.warmup):
-        ctx.run(SEED + i)
-        print(f"[bench] warmup {i} done", file=sys.stderr, flush=True)
-    ctx.set_timing(True)
-    prop_ms, launches, ncc_prop, ncc_ref = 0.0, 0, 0, 0
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    per_step = []
-    for k in range(args.steps):
-        ts = time.perf_counter()
-        ctx.run(SEED + 100 + k, sync=False)
-        st = ctx.stats()  # synchronises the stream
-        per_step.append(time.perf_counter() - ts)
-        prop_ms += st.ms_propagate
-        launches += st.propagate_launches
-        ncc_prop += st.ncc_propagate
-        ncc_ref += st.ncc_reference
-    ctx.synchronize()
-    elapsed = time.perf_counter() - t0
-    views_s = args.steps / elapsed
-
-    # accuracy of the last run against ground truth
+    r = run_views(ctx, args.steps, args.warmup, SEED)
     planes = ctx.planes()
     gt = sc.depths[ids[0]]
-    rel = np.abs(planes[..., 3] - gt) / gt
-    within1 = float(np.mean(rel <= 0.01))
+    within1 = float(np.mean(np.abs(planes[..., 3] - gt) / gt <= 0.01))
+    print(f"[bench] {args.precision}: {r['views_s']:.2f} views/s", file=sys.stderr, flush=True)
+
+    plain = None
+    if not args.no_plain_fp32 and args.precision != "fp32":
+        ctx.set_params(Hm.default_params(precision="fp32"))
+        rp = run_views(ctx, max(2, args.steps // 2), 1, SEED)
+        pp = ctx.planes()
+        # same seed as the last timed step of the headline run
+        ctx.set_params(Hm.default_params(precision=args.precision))
+        ctx.run(SEED + 100 + args.steps - 1)
+        pf = ctx.planes()
+        ctx.set_params(Hm.default_params(precision="fp32"))
+        ctx.run(SEED + 100 + args.steps - 1)
+        pe = ctx.planes()
+        agree = float(np.mean(np.abs(pf[..., 3] - pe[..., 3]) / np.abs(pe[..., 3]) <= 0.01))
+        plain = {"views_s": round(rp["views_s"], 4), "ms_per_step": round(1e3 * rp["elapsed"] / max(2, args.steps // 2), 3),
+                 "k_propagate_avg_ms": round(rp["prop_ms"] / max(rp["launches"], 1), 3),
+                 "depth_within_1pct_of_gt": round(float(np.mean(np.abs(pp[..., 3] - gt) / gt <= 0.01)), 4),
+                 "speedup": round(r["views_s"] / rp["views_s"], 3),
+                 "depth_within_1pct_of_plain_fp32": round(agree, 4)}
+        print(f"[bench] plain fp32: {rp['views_s']:.2f} views/s", file=sys.stderr, flush=True)
 
     bytes_per_texel = {"fp32": 4, "fp32fast": 4, "fp16": 2, "bf16": 2}[args.precision]
     V = NSRC
-    per_launch_evals = ncc_prop / max(launches, 1)
+    launches = max(r["launches"], 1)
+    per_launch_evals = r["ncc_prop"] / launches
     alg_bytes = per_launch_evals * (181 * bytes_per_texel) + (per_launch_evals / V) * (16 + 4 * V)
-    avg_launch_s = (prop_ms / max(launches, 1)) / 1e3
+    avg_launch_s = (r["prop_ms"] / launches) / 1e3
     achieved = alg_bytes / avg_launch_s / 1e9
+    traffic = hbm_traffic(args.precision)
 
     out = {
         "metric": "views/s @1600x1200, 10 src: plain FP32 vs FP16+batched, at <=1% depth err",
-        "value": round(views_s, 4),
+        "value": round(r["views_s"], 4),
         "unit": "views/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "ms_per_step": round(1e3 * r["elapsed"] / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp32": "f32", "fp32fast": "f32"}.get(args.precision, args.precision),
         "data": "synthetic (seeded ray-cast textured planes, 11 cameras on a 30 deg arc)",
         "config": {"workload": "RunPatchMatch photometric, 1600x1200 reference + 10 source views, 3 iterations",
-                   "precision": args.precision, "width": W_FULL, "height": H_FULL, "sources": NSRC},
+                   "precision": args.precision, "width": W_FULL, "height": H_FULL, "sources": NSRC,
+                   "views_per_launch": 1},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None if traffic is None else traffic["bytes_per_launch"],
                      "kernel": "k_propagate", "avg_launch_ms": round(avg_launch_s * 1e3, 3),
-                     "alg_bytes_per_launch": int(alg_bytes)},
-        "mpix_hyp_view_per_s": round(ncc_prop / (prop_ms / 1e3) / 1e6, 1) if prop_ms > 0 else None,
-        "mpix_hyp_view_per_s_reference_count": round(ncc_ref / elapsed / 1e6, 1),
+                     "alg_bytes_per_launch": int(alg_bytes), "ncc_evals_per_launch": int(per_launch_evals)},
+        "plain_fp32": plain,
+        "mpix_hyp_view_per_s": round(r["ncc_prop"] / (r["prop_ms"] / 1e3) / 1e6, 1) if r["prop_ms"] > 0 else None,
+        "mpix_hyp_view_per_s_reference_count": round(r["ncc_ref"] / r["elapsed"] / 1e6, 1),
         "depth_within_1pct_of_gt": round(within1, 4),
     }
     ctx.close()
-    if args.calibrate_copy_mb:
-        out["calibration_copy_bytes"] = calibration_copy(args.calibrate_copy_mb << 20)
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline (oracle band) ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(cams, imgs, canny, rows=args.cpu_rows)
