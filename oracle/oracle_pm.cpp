@@ -7,14 +7,18 @@
 // Each function follows the reference function cited beside it (HPM.cu,
 // HPM.cpp).  It is templated on the arithmetic type T:
 //   T = float  : the FP32 oracle.  Operations happen in the order the reference
-//                writes them, with the canonical fused multiply-adds listed in
-//                DESIGN.md ("canonical FP32 arithmetic"), IEEE division and
-//                square root, and the oracle's own exp/sin/cos (oracle_math.h).
-//                The GPU's exact FP32 path performs the same operations, so the
-//                two agree bit for bit.
+//                writes them, with fused multiply-adds at exactly these sites
+//                (written FMA(...) below): the homography projection
+//                fma(H0, x, fma(H1, y, H2)), the two bilinear lerps and the
+//                vertical lerp, the per-tap weighted accumulations, and the
+//                variance/covariance differences; everything else is a plain
+//                IEEE operation (build with -ffp-contract=off), division and
+//                square root are IEEE, and exp/sin/cos are the oracle's own
+//                (oracle_math.h).  The GPU's exact FP32 path performs the same
+//                operations, so the two agree bit for bit.
 //   T = double : the FP64 oracle (error reference).  Same algorithm, same random
 //                draws, libm transcendentals.
-// Documented resolutions of reference undefined behaviour ("quirks"):
+// Resolutions of reference undefined behaviour and quirks:
 //   Q1  plane_hypotheses_now starts as the pixel's current plane.
 //   Q3  invalid-direction cost rows keep the reference's aggregate-initialiser
 //       contents (row 0 starts {2, 0, 0, ...}, other rows 0).
