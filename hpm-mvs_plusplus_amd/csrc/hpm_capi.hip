@@ -11,6 +11,7 @@
 
 #include "../../include/hpm_mi355x.h"
 #include "hpm_fast.h"
+#include "hpm_fusion.h"
 #include "hpm_jbu.h"
 #include "hpm_kernels.h"
 

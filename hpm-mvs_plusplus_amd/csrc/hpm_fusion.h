@@ -1,0 +1,263 @@
+// hpm_fusion.h -- multi-view consistency check of fusion and confidence
+// evaluation (RunFusion main.cpp:823-957, ConfidenceEvaluation
+// main.cpp:959-1109).
+//
+// The reference walks images in order and pixels row-major, and every accepted
+// pixel masks the source pixels it matched, so later pixels skip them.  The
+// expensive part does not depend on the masks: for each (reference pixel,
+// source view) it is two projections, a reprojection, a normal angle and an
+// exp.  k_consistency computes it for all pixels of one image at once and
+// stores, per (source slot j, pixel), the matched source pixel (or -1 when the
+// pair fails any test) and its score exp(-(e + 200 dd + 10 angle)).  The
+// greedy, ordered part -- masks, the score sum over unmasked matches, the
+// acceptance test and the mask updates -- is a short sequential pass on the
+// host over those arrays, with the reference's float operations in the
+// reference's order.
+//
+// Arithmetic: plain IEEE float operations in the reference's order (no fused
+// multiply-adds; the library is built with -ffp-contract=off); pow(x, 2) of a
+// float is x * x, exp of a float is expf_hpm, acosf is acosf_hpm.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "hpm_device.h"
+
+namespace hpm {
+
+// Get3DPointonWorld, HPM.cpp:221-245.
+HPM_HD void point_on_world(int x, int y, float depth, const DevCam& c, float X[3]) {
+  const float px = depth * ((float)x - c.K[2]) / c.K[0];
+  const float py = depth * ((float)y - c.K[5]) / c.K[4];
+  const float pz = depth;
+  const float tx = c.R[0] * px + c.R[3] * py + c.R[6] * pz;
+  const float ty = c.R[1] * px + c.R[4] * py + c.R[7] * pz;
+  const float tz = c.R[2] * px + c.R[5] * py + c.R[8] * pz;
+  const float Cx = -(c.R[0] * c.t[0] + c.R[3] * c.t[1] + c.R[6] * c.t[2]);
+  const float Cy = -(c.R[1] * c.t[0] + c.R[4] * c.t[1] + c.R[7] * c.t[2]);
+  const float Cz = -(c.R[2] * c.t[0] + c.R[5] * c.t[1] + c.R[8] * c.t[2]);
+  X[0] = tx + Cx;
+  X[1] = ty + Cy;
+  X[2] = tz + Cz;
+}
+
+// ProjectonCamera, HPM.cpp:269-279.
+HPM_HD void project_on_camera(const float X[3], const DevCam& c, float& u, float& v, float& depth) {
+  const float tx = c.R[0] * X[0] + c.R[1] * X[1] + c.R[2] * X[2] + c.t[0];
+  const float ty = c.R[3] * X[0] + c.R[4] * X[1] + c.R[5] * X[2] + c.t[1];
+  const float tz = c.R[6] * X[0] + c.R[7] * X[1] + c.R[8] * X[2] + c.t[2];
+  depth = c.K[6] * tx + c.K[7] * ty + c.K[8] * tz;
+  u = (c.K[0] * tx + c.K[1] * ty + c.K[2] * tz) / depth;
+  v = (c.K[3] * tx + c.K[4] * ty + c.K[5] * tz) / depth;
+}
+
+// (int)(v + 0.5f) of main.cpp:908-909,1046-1047 with a defined result when v
+// is NaN or beyond the int range (x86's conversion gives INT_MIN there, which
+// every bounds test rejects): -1.
+HPM_HD int round_index(float v) {
+  const float x = v + 0.5f;
+  return (x > -2147483648.0f && x < 2147483648.0f) ? (int)x : -1;
+}
+
+struct FusionImage {
+  const float* depth;   // w * h
+  const float* normal;  // 3 floats per pixel
+};
+
+constexpr int kFusionMaxImages = 1024;
+
+// One thread per pixel of reference image `ref`; writes match[j * P + p] and
+// term[j * P + p] for its num_ngb source slots.
+__global__ __launch_bounds__(256) void k_consistency(const DevCam* __restrict__ cams, const FusionImage* __restrict__ imgs,
+                                                     const int* __restrict__ srcs, int num_ngb, int ref, int mode,
+                                                     int* __restrict__ match, float* __restrict__ term) {
+  const DevCam& rc = cams[ref];
+  const int cols = rc.width, rows = rc.height;
+  const int P = cols * rows;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const int r = p / cols, c = p - r * cols;
+  const float ref_depth = imgs[ref].depth[p];
+  const float* rn = imgs[ref].normal + 3 * (size_t)p;
+  bool ref_ok = ref_depth > 0.0f;
+  if (mode == 1 && (ref_depth < rc.depth_min || ref_depth > rc.depth_max)) ref_ok = false;
+  float X[3];
+  point_on_world(c, r, ref_depth, rc, X);
+  for (int j = 0; j < num_ngb; ++j) {
+    int m = -1;
+    float score = 0.0f;
+    const int s = srcs[j];
+    const DevCam& sc = cams[s];
+    if (ref_ok) {
+      float pu, pv, proj_depth;
+      project_on_camera(X, sc, pu, pv, proj_depth);
+      const int src_r = round_index(pv);
+      const int src_c = round_index(pu);
+      if (src_c >= 0 && src_c < sc.width && src_r >= 0 && src_r < sc.height) {
+        const int sp = src_r * sc.width + src_c;
+        const float src_depth = imgs[s].depth[sp];
+        const float* sn = imgs[s].normal + 3 * (size_t)sp;
+        bool ok = src_depth > 0.0f;
+        if (mode == 1 && (src_depth < rc.depth_min || src_depth > rc.depth_max)) ok = false;
+        if (ok) {
+          float tX[3];
+          point_on_world(src_c, src_r, src_depth, sc, tX);
+          float tu, tv;
+          project_on_camera(tX, rc, tu, tv, proj_depth);
+          const float dx = (float)c - tu, dy = (float)r - tv;
+          const float reproj_error = sqrtf(dx * dx + dy * dy);
+          const float relative_depth_diff = fabsf(proj_depth - ref_depth) / ref_depth;
+          const float dot = rn[0] * sn[0] + rn[1] * sn[1] + rn[2] * sn[2];
+          float angle = acosf_hpm(dot);  // GetAngle, HPM.cpp:281-290
+          if (angle != angle) angle = 0.0f;
+          if (reproj_error < 2.0f && relative_depth_diff < 0.01f && angle < 0.174533f) {
+            m = sp;
+            const float tmp_index = reproj_error + 200 * relative_depth_diff + angle * 10;
+            score = expf_hpm(-tmp_index);
+          }
+        }
+      }
+    }
+    match[(size_t)j * P + p] = m;
+    term[(size_t)j * P + p] = score;
+  }
+}
+
+}  // namespace hpm
+
+extern "C" {
+
+int hpm_consistency_check(int mode, int num_images, const hpm_camera* cameras, const float* const* depths,
+                          const float* const* normals, const int32_t* src_offsets, const int32_t* src_ids,
+                          float* const* out_confidence, int32_t* out_pairs, float* out_points, int64_t capacity,
+                          int64_t* out_count) {
+  using namespace hpm;
+  if ((mode != HPM_CHECK_FUSION && mode != HPM_CHECK_CONFIDENCE) || num_images < 1 || num_images > kFusionMaxImages ||
+      !cameras || !depths || !normals || !src_offsets || !src_ids || !out_count)
+    return HPM_E_INVALID;
+  if (mode == HPM_CHECK_CONFIDENCE) {
+    if (!out_confidence) return HPM_E_INVALID;
+    for (int i = 0; i < num_images; ++i)
+      if (!out_confidence[i]) return HPM_E_INVALID;
+  }
+  if (mode == HPM_CHECK_FUSION && capacity > 0 && !out_pairs) return HPM_E_INVALID;
+  if (src_offsets[0] != 0) return HPM_E_INVALID;
+  size_t max_p = 0;
+  int max_ngb = 0;
+  for (int i = 0; i < num_images; ++i) {
+    if (cameras[i].width <= 0 || cameras[i].height <= 0 || !depths[i] || !normals[i]) return HPM_E_INVALID;
+    if ((size_t)cameras[i].width * cameras[i].height > ((size_t)1 << 28)) return HPM_E_INVALID;
+    const int n = src_offsets[i + 1] - src_offsets[i];
+    if (n < 0) return HPM_E_INVALID;
+    for (int j = src_offsets[i]; j < src_offsets[i + 1]; ++j)
+      if (src_ids[j] < 0 || src_ids[j] >= num_images) return HPM_E_INVALID;
+    max_p = std::max(max_p, (size_t)cameras[i].width * cameras[i].height);
+    max_ngb = std::max(max_ngb, n);
+  }
+  std::vector<DevCam> hc(num_images);
+  for (int i = 0; i < num_images; ++i) std::memcpy(&hc[i], &cameras[i], sizeof(DevCam));
+
+  DevCam* d_cams = nullptr;
+  FusionImage* d_imgs = nullptr;
+  int* d_srcs = nullptr;
+  int* d_match = nullptr;
+  float* d_term = nullptr;
+  std::vector<float*> d_buf(2 * num_images, nullptr);
+  int rc = HPM_OK;
+  auto ok = [&](hipError_t e) {
+    if (e != hipSuccess && rc == HPM_OK) rc = (e == hipErrorOutOfMemory ? HPM_E_NOMEM : HPM_E_DEVICE);
+    return rc == HPM_OK;
+  };
+  std::vector<FusionImage> himgs(num_images);
+  const int total_src = src_offsets[num_images];
+  if (ok(hipMalloc((void**)&d_cams, sizeof(DevCam) * num_images)) &&
+      ok(hipMalloc((void**)&d_imgs, sizeof(FusionImage) * num_images)) &&
+      ok(hipMalloc((void**)&d_srcs, sizeof(int) * std::max(total_src, 1))) &&
+      ok(hipMalloc((void**)&d_match, sizeof(int) * max_p * std::max(max_ngb, 1))) &&
+      ok(hipMalloc((void**)&d_term, sizeof(float) * max_p * std::max(max_ngb, 1)))) {
+    for (int i = 0; i < num_images && rc == HPM_OK; ++i) {
+      const size_t P = (size_t)cameras[i].width * cameras[i].height;
+      if (ok(hipMalloc((void**)&d_buf[2 * i], P * 4)) && ok(hipMalloc((void**)&d_buf[2 * i + 1], P * 12)) &&
+          ok(hipMemcpy(d_buf[2 * i], depths[i], P * 4, hipMemcpyHostToDevice)) &&
+          ok(hipMemcpy(d_buf[2 * i + 1], normals[i], P * 12, hipMemcpyHostToDevice)))
+        himgs[i] = FusionImage{d_buf[2 * i], d_buf[2 * i + 1]};
+    }
+  }
+  if (rc == HPM_OK && ok(hipMemcpy(d_cams, hc.data(), sizeof(DevCam) * num_images, hipMemcpyHostToDevice)) &&
+      ok(hipMemcpy(d_imgs, himgs.data(), sizeof(FusionImage) * num_images, hipMemcpyHostToDevice)) && total_src > 0)
+    ok(hipMemcpy(d_srcs, src_ids, sizeof(int) * total_src, hipMemcpyHostToDevice));
+
+  std::vector<std::vector<uint8_t>> masks(num_images);
+  std::vector<int> match;
+  std::vector<float> term;
+  int64_t npts = 0;
+  for (int i = 0; i < num_images && rc == HPM_OK; ++i) {
+    masks[i].assign((size_t)cameras[i].width * cameras[i].height, 0);
+    if (mode == HPM_CHECK_CONFIDENCE) std::memset(out_confidence[i], 0, masks[i].size() * sizeof(float));
+  }
+  for (int i = 0; i < num_images && rc == HPM_OK; ++i) {
+    const int cols = cameras[i].width, rows = cameras[i].height;
+    const size_t P = (size_t)cols * rows;
+    const int num_ngb = src_offsets[i + 1] - src_offsets[i];
+    const int* srcs = src_ids + src_offsets[i];
+    if (num_ngb > 0) {
+      hipLaunchKernelGGL(k_consistency, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, 0, d_cams, d_imgs,
+                         d_srcs + src_offsets[i], num_ngb, i, mode, d_match, d_term);
+      if (!ok(hipGetLastError())) break;
+      match.resize(P * num_ngb);
+      term.resize(P * num_ngb);
+      if (!ok(hipMemcpy(match.data(), d_match, P * num_ngb * sizeof(int), hipMemcpyDeviceToHost)) ||
+          !ok(hipMemcpy(term.data(), d_term, P * num_ngb * sizeof(float), hipMemcpyDeviceToHost)))
+        break;
+    }
+    // The ordered greedy pass (main.cpp:884-952 / 1019-1088); used_list lives
+    // for the whole image, as in the reference.
+    std::vector<int> used(num_ngb, -1);
+    for (size_t p = 0; p < P; ++p) {
+      if (masks[i][p] == 1) continue;
+      int num_consistent = 0;
+      float dynamic_consistency = 0;
+      for (int j = 0; j < num_ngb; ++j) {
+        const int m = match[(size_t)j * P + p];
+        if (m < 0) continue;
+        if (masks[srcs[j]][m] == 1) continue;
+        used[j] = m;
+        dynamic_consistency += term[(size_t)j * P + p];
+        num_consistent++;
+      }
+      if (num_consistent >= 1 && ((double)dynamic_consistency > 0.3 * num_consistent)) {
+        if (mode == HPM_CHECK_FUSION) {
+          if (npts < capacity) {
+            out_pairs[2 * npts] = i;
+            out_pairs[2 * npts + 1] = (int32_t)p;
+            if (out_points) {
+              const int r = (int)(p / cols), c = (int)(p - (size_t)r * cols);
+              float X[3];
+              point_on_world(c, r, depths[i][p], hc[i], X);
+              float* q = out_points + 6 * npts;
+              q[0] = X[0]; q[1] = X[1]; q[2] = X[2];
+              q[3] = normals[i][3 * p]; q[4] = normals[i][3 * p + 1]; q[5] = normals[i][3 * p + 2];
+            }
+          }
+          ++npts;
+        } else {
+          out_confidence[i][p] = dynamic_consistency;
+        }
+        for (int j = 0; j < num_ngb; ++j) {
+          if (used[j] < 0) continue;
+          masks[srcs[j]][used[j]] = 1;
+          if (mode == HPM_CHECK_CONFIDENCE) out_confidence[srcs[j]][used[j]] = dynamic_consistency;
+        }
+      }
+    }
+  }
+  for (float* b : d_buf) hipFree(b);
+  hipFree(d_cams); hipFree(d_imgs); hipFree(d_srcs); hipFree(d_match); hipFree(d_term);
+  *out_count = npts;
+  return rc;
+}
+
+}  // extern "C"

@@ -57,7 +57,7 @@ EXPORTS = [
     "hpm_set_coarse", "hpm_run_patchmatch", "hpm_run_patchmatch_partial", "hpm_synchronize", "hpm_get_planes",
     "hpm_get_costs", "hpm_get_texture", "hpm_get_selected_views", "hpm_get_confidence", "hpm_get_size",
     "hpm_get_stats", "hpm_set_timing", "hpm_cost_fixed", "hpm_cost_fixed_device", "hpm_init_cost",
-    "hpm_geom_cost", "hpm_jbu_depth", "hpm_jbu_prior",
+    "hpm_geom_cost", "hpm_jbu_depth", "hpm_jbu_prior", "hpm_consistency_check",
 ]
 
 _lib = None
@@ -104,6 +104,9 @@ def lib():
     L.hpm_geom_cost.argtypes = [P, fp, fp]
     L.hpm_jbu_depth.argtypes = [fp, ctypes.c_int, ctypes.c_int, fp, ctypes.c_int, ctypes.c_int, fp]
     L.hpm_jbu_prior.argtypes = [fp, ctypes.c_int, ctypes.c_int, fp, fp, ctypes.c_int, ctypes.c_int, fp, fp]
+    ip = ctypes.POINTER(ctypes.c_int32)
+    L.hpm_consistency_check.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(hpm_camera), P, P, ip, ip, P, ip,
+                                        fp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
     _lib = L
     return L
 
@@ -111,6 +114,11 @@ def lib():
 def _fptr(a: np.ndarray):
     assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
 
 
 def _uptr(a: np.ndarray):
@@ -311,4 +319,48 @@ def jbu_prior(fine_gray, coarse_depth, coarse_normals):
     if rc != HPM_OK:
         raise HpmError(f"hpm_jbu_prior failed ({rc})")
     return od, on
+
+
+CHECK_FUSION = 0
+CHECK_CONFIDENCE = 1
+
+
+def _pack_views(cameras, depths, normals, sources):
+    n = len(cameras)
+    cams = (hpm_camera * n)(*cameras)
+    d = [np.ascontiguousarray(x, np.float32) for x in depths]
+    nm = [np.ascontiguousarray(np.asarray(x, np.float32)[..., :3]) for x in normals]
+    dptr = (ctypes.c_void_p * n)(*[x.ctypes.data for x in d])
+    nptr = (ctypes.c_void_p * n)(*[x.ctypes.data for x in nm])
+    off = np.zeros(n + 1, np.int32)
+    off[1:] = np.cumsum([len(s) for s in sources])
+    ids = np.array([int(v) for s in sources for v in s] or [0], np.int32)
+    return cams, d, nm, dptr, nptr, off, ids
+
+
+def consistency_check(mode, cameras, depths, normals, sources, capacity=None):
+    """RunFusion (mode CHECK_FUSION) / ConfidenceEvaluation (CHECK_CONFIDENCE).
+    Returns (pairs[k, 2], points[k, 6]) for fusion, a list of confidence maps
+    for confidence evaluation."""
+    cams, d, nm, dptr, nptr, off, ids = _pack_views(cameras, depths, normals, sources)
+    n = len(cameras)
+    count = ctypes.c_int64(0)
+    L = lib()
+    if mode == CHECK_CONFIDENCE:
+        conf = [np.zeros(x.shape, np.float32) for x in d]
+        cptr = (ctypes.c_void_p * n)(*[x.ctypes.data for x in conf])
+        rc = L.hpm_consistency_check(mode, n, cams, dptr, nptr, _iptr(off), _iptr(ids), cptr, None, None, 0,
+                                     ctypes.byref(count))
+        if rc != HPM_OK:
+            raise HpmError(f"hpm_consistency_check failed ({rc})")
+        return conf
+    cap = int(capacity if capacity is not None else sum(x.size for x in d))
+    pairs = np.zeros((max(cap, 1), 2), np.int32)
+    pts = np.zeros((max(cap, 1), 6), np.float32)
+    rc = L.hpm_consistency_check(mode, n, cams, dptr, nptr, _iptr(off), _iptr(ids), None, _iptr(pairs), _fptr(pts),
+                                 cap, ctypes.byref(count))
+    if rc != HPM_OK:
+        raise HpmError(f"hpm_consistency_check failed ({rc})")
+    k = min(count.value, cap)
+    return pairs[:k], pts[:k]
 

@@ -181,6 +181,32 @@ int hpm_jbu_depth(const float* fine_gray, int w, int h, const float* coarse_dept
 int hpm_jbu_prior(const float* fine_gray, int w, int h, const float* coarse_depth, const float* coarse_normals, int cw,
                   int ch, float* out_depth, float* out_normals);
 
+/* Multi-view consistency check of RunFusion (mode HPM_CHECK_FUSION,
+ * main.cpp:823-957) and ConfidenceEvaluation (mode HPM_CHECK_CONFIDENCE,
+ * main.cpp:959-1109), with the reference's greedy masks: images in order,
+ * pixels row-major, sources in pair order.
+ *   cameras[i]  camera of image i, already rescaled to its depth map
+ *               (RescaleImageAndCamera, HPM.cpp:186-207);
+ *   depths[i]   width*height depths; normals[i] 3 floats per pixel (world
+ *               normal, as normals.dmb holds);
+ *   src_ids[src_offsets[i] .. src_offsets[i+1]) are image i's source images
+ *               (Problem::src_image_ids), src_offsets has num_images+1 entries.
+ * HPM_CHECK_FUSION: out_pairs receives (image, pixel) of every fused point in
+ *   the reference's PointCloud order, out_points (may be NULL) 6 floats per
+ *   point (world position, normal), both up to `capacity` points;
+ *   *out_count is the total number of fused points.
+ * HPM_CHECK_CONFIDENCE: out_confidence[i] (width*height floats each) receives
+ *   the confidence map of image i (zeros where not consistent); the depth-range
+ *   filter uses cameras[i].depth_min/max for reference and source depths, as the
+ *   reference does (main.cpp:1032,1057).
+ * Runs on the current HIP device; blocking. */
+#define HPM_CHECK_FUSION 0
+#define HPM_CHECK_CONFIDENCE 1
+int hpm_consistency_check(int mode, int num_images, const hpm_camera* cameras, const float* const* depths,
+                          const float* const* normals, const int32_t* src_offsets, const int32_t* src_ids,
+                          float* const* out_confidence, int32_t* out_pairs, float* out_points, int64_t capacity,
+                          int64_t* out_count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -176,4 +176,31 @@ inline float cosf_hpm(float x) {
   }
 }
 
+// acos of a float, evaluated in double and rounded once (GetAngle's acosf,
+// HPM.cpp:281-290).  asin(z) = z + sum_n c_n z^(2n+1), c_n = C(2n,n) / (4^n (2n+1))
+// (the Taylor series of arcsin), truncated after 18 terms for |z| <= 1/2, where
+// the first omitted term is below 2e-13 relative.  |x| <= 1/2 uses
+// acos x = pi/2 - asin x; otherwise acos |x| = 2 asin(sqrt((1 - |x|) / 2)) and
+// acos(-|x|) = pi - acos |x|.  Out-of-domain and NaN inputs give NaN.
+inline double asin_series(double z) {
+  const double c[18] = {0.16666666666666666, 0.075, 0.044642857142857144, 0.030381944444444444, 0.022372159090909092,
+      0.017352764423076924, 0.01396484375, 0.011551800896139705, 0.009761609529194078,
+      0.008390335809616815, 0.0073125258735988454, 0.006447210311889649, 0.005740037670841924,
+      0.005153309682319905, 0.004660143486915096, 0.004240907093679363, 0.003880964558837669,
+      0.0035692053938259347};
+  const double z2 = z * z;
+  double p = c[17];
+  for (int k = 16; k >= 0; --k) p = std::fma(p, z2, c[k]);
+  return std::fma(z * z2, p, z);
+}
+inline float acosf_hpm(float xf) {
+  const double x = (double)xf;
+  const double ax = x < 0.0 ? -x : x;
+  if (!(ax <= 1.0)) return std::nanf("");
+  const double pio2 = 1.5707963267948966, pi = 3.141592653589793;
+  if (ax <= 0.5) return (float)(pio2 - asin_series(x));
+  const double a = 2.0 * asin_series(std::sqrt((1.0 - ax) * 0.5));
+  return (float)(x > 0.0 ? a : pi - a);
+}
+
 }  // namespace orc

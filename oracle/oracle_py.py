@@ -47,6 +47,14 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         f32 = ctypes.c_float
         L.orc_expf.argtypes = [f32]; L.orc_expf.restype = f32
+        L.orc_acosf.argtypes = [f32]; L.orc_acosf.restype = f32
+        L.orc_jbu_depth.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_void_p]
+        L.orc_jbu_prior.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_consistency.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
         L.orc_exp.argtypes = [ctypes.c_double]; L.orc_exp.restype = ctypes.c_double
         L.orc_sinf.argtypes = [f32]; L.orc_sinf.restype = f32
         L.orc_cosf.argtypes = [f32]; L.orc_cosf.restype = f32
@@ -196,3 +204,52 @@ def philox4x32_10(ctr, key):
     o = np.zeros(4, np.uint32)
     lib().orc_philox4x32_10(c.ctypes.data, k.ctypes.data, o.ctypes.data)
     return o
+
+
+def acosf(x):
+    return lib().orc_acosf(float(x))
+
+
+def jbu_depth(fine, coarse):
+    f = np.ascontiguousarray(fine, np.float32)
+    c = np.ascontiguousarray(coarse, np.float32)
+    out = np.empty_like(f)
+    lib().orc_jbu_depth(_p(f), f.shape[1], f.shape[0], _p(c), c.shape[1], c.shape[0], _p(out))
+    return out
+
+
+def jbu_prior(fine, coarse, coarse_normals):
+    f = np.ascontiguousarray(fine, np.float32)
+    c = np.ascontiguousarray(coarse, np.float32)
+    cn = np.zeros(c.shape + (4,), np.float32)
+    cn[..., :3] = np.asarray(coarse_normals)[..., :3]
+    od = np.empty_like(f)
+    on = np.empty(f.shape + (4,), np.float32)
+    lib().orc_jbu_prior(_p(f), f.shape[1], f.shape[0], _p(c), _p(cn), c.shape[1], c.shape[0], _p(od), _p(on))
+    return od, on
+
+
+def consistency(mode, cams, depths, normals, sources):
+    """RunFusion (mode 0) -> (pairs[k, 2], points[k, 6]); ConfidenceEvaluation
+    (mode 1) -> list of confidence maps."""
+    n = len(cams)
+    ca = _cams_array(cams)
+    d = [np.ascontiguousarray(x, np.float32) for x in depths]
+    nm = [np.ascontiguousarray(np.asarray(x, np.float32)[..., :3]) for x in normals]
+    dptr = (ctypes.c_void_p * n)(*[x.ctypes.data for x in d])
+    nptr = (ctypes.c_void_p * n)(*[x.ctypes.data for x in nm])
+    off = np.zeros(n + 1, np.int32)
+    off[1:] = np.cumsum([len(s) for s in sources])
+    ids = np.array([int(v) for s in sources for v in s] or [0], np.int32)
+    count = ctypes.c_int64(0)
+    if mode == 1:
+        conf = [np.zeros(x.shape, np.float32) for x in d]
+        cptr = (ctypes.c_void_p * n)(*[x.ctypes.data for x in conf])
+        lib().orc_consistency(1, n, ca, dptr, nptr, _p(off), _p(ids), cptr, None, None, 0, ctypes.byref(count))
+        return conf
+    cap = sum(x.size for x in d)
+    pairs = np.zeros((cap, 2), np.int32)
+    pts = np.zeros((cap, 6), np.float32)
+    lib().orc_consistency(0, n, ca, dptr, nptr, _p(off), _p(ids), None, _p(pairs), _p(pts), cap, ctypes.byref(count))
+    return pairs[:count.value], pts[:count.value]
+
