@@ -1,6 +1,8 @@
-"""Builds libhpm_mi355x.so (the HIP kernels + C ABI) for gfx950, in-tree.
+"""Builds, in-tree, libhpm_mi355x.so (the HIP kernels + C ABI) for gfx950
+and libhpm_host.so (the C++ host passes: dense-folder pipeline, planar prior,
+fusion driver), which links the first.
 
-    python build.py            # -> lib/libhpm_mi355x.so
+    python build.py            # -> lib/libhpm_mi355x.so, lib/libhpm_host.so
 
 Flags: -ffp-contract=off keeps every fused multiply-add explicit (the exact
 FP32 path must perform the operations its canonical order names); packed FP32
@@ -13,7 +15,10 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "lib", "libhpm_mi355x.so")
 SOURCES = [os.path.join(HERE, "csrc", "hpm_capi.hip")]
-HEADERS = ["hpm_math.h", "hpm_device.h", "hpm_kernels.h", "hpm_jbu.h", "hpm_fast.h"]
+HEADERS = ["hpm_math.h", "hpm_device.h", "hpm_kernels.h", "hpm_jbu.h", "hpm_fast.h", "hpm_fusion.h"]
+HOST_LIB = os.path.join(HERE, "lib", "libhpm_host.so")
+HOST_SOURCES = [os.path.join(HERE, "host", "hpm_host.cpp"), os.path.join(HERE, "host", "hpm_host_capi.cpp")]
+GXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wextra"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -37,6 +42,13 @@ def build(verbose=False, extra=()):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True, cwd=HERE)
     os.replace(tmp, LIB)
+    tmp = HOST_LIB + ".tmp"
+    cmd = (["g++"] + GXX_FLAGS + HOST_SOURCES + ["-o", tmp, "-L" + os.path.dirname(LIB), "-lhpm_mi355x",
+           "-Wl,-rpath,$ORIGIN"])
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=HERE)
+    os.replace(tmp, HOST_LIB)
     return LIB
 
 

@@ -1,0 +1,840 @@
+// hpm_host.cpp -- formats, planar-prior construction, edge map, the HPM facade
+// and the pass driver (see hpm_host.hpp).
+//
+// Float arithmetic follows the reference's expressions: int and float operands
+// stay float, double literals promote (C++ rules), pow(x, 2) of a float is x*x;
+// built with -ffp-contract=off.
+#include "hpm_host.hpp"
+
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <sstream>
+
+namespace hpmhost {
+
+// ------------------------------------------------------------------ formats
+
+std::vector<Problem> ReadPairFile(const std::string& path) {
+  std::ifstream f(path);
+  if (!f) throw Error("cannot open " + path);
+  int n = 0;
+  f >> n;
+  std::vector<Problem> out;
+  for (int i = 0; i < n && f; ++i) {
+    Problem p;
+    int m = 0;
+    f >> p.ref_image_id >> m;
+    for (int j = 0; j < m; ++j) {
+      int id;
+      float score;
+      f >> id >> score;
+      if (score <= 0.0f) continue;  // main.cpp:193-195
+      p.src_image_ids.push_back(id);
+    }
+    out.push_back(p);
+  }
+  if (!f) throw Error("truncated pair file " + path);
+  return out;
+}
+
+hpm_camera ReadCamera(const std::string& path) {
+  std::ifstream f(path);
+  if (!f) throw Error("cannot open " + path);
+  hpm_camera c;
+  std::memset(&c, 0, sizeof(c));
+  std::string word;
+  f >> word;  // "extrinsic"
+  for (int i = 0; i < 3; ++i) f >> c.R[3 * i + 0] >> c.R[3 * i + 1] >> c.R[3 * i + 2] >> c.t[i];
+  float row4[4];
+  f >> row4[0] >> row4[1] >> row4[2] >> row4[3];
+  f >> word;  // "intrinsic"
+  for (int i = 0; i < 3; ++i) f >> c.K[3 * i + 0] >> c.K[3 * i + 1] >> c.K[3 * i + 2];
+  float interval, depth_num;
+  f >> c.depth_min >> interval >> depth_num >> c.depth_max;
+  if (!f) throw Error("malformed camera file " + path);
+  return c;
+}
+
+bool ReadDmb(const std::string& path, MapF& out) {
+  FILE* fp = std::fopen(path.c_str(), "rb");
+  if (!fp) return false;
+  int32_t hdr[4] = {-1, 0, 0, 0};
+  bool ok = std::fread(hdr, sizeof(int32_t), 4, fp) == 4 && hdr[0] == 1 && hdr[1] > 0 && hdr[2] > 0 && hdr[3] > 0;
+  if (ok) {
+    out = MapF(hdr[2], hdr[1], hdr[3]);
+    ok = std::fread(out.data.data(), sizeof(float), out.data.size(), fp) == out.data.size();
+  }
+  std::fclose(fp);
+  return ok;
+}
+
+bool WriteDmb(const std::string& path, const MapF& m) {
+  FILE* fp = std::fopen(path.c_str(), "wb");
+  if (!fp) return false;
+  const int32_t hdr[4] = {1, m.height, m.width, m.channels};
+  bool ok = std::fwrite(hdr, sizeof(int32_t), 4, fp) == 4 &&
+            std::fwrite(m.data.data(), sizeof(float), m.data.size(), fp) == m.data.size();
+  return std::fclose(fp) == 0 && ok;
+}
+
+static bool next_token(FILE* fp, std::string& tok) {
+  tok.clear();
+  int ch;
+  while ((ch = std::fgetc(fp)) != EOF) {
+    if (ch == '#') {
+      while ((ch = std::fgetc(fp)) != EOF && ch != '\n') {
+      }
+      continue;
+    }
+    if (!std::isspace(ch)) break;
+  }
+  if (ch == EOF) return false;
+  tok.push_back((char)ch);
+  while ((ch = std::fgetc(fp)) != EOF && !std::isspace(ch)) tok.push_back((char)ch);
+  return true;  // one whitespace byte after the token has been consumed
+}
+
+bool ReadNetpbm(const std::string& path, MapU8& out) {
+  FILE* fp = std::fopen(path.c_str(), "rb");
+  if (!fp) return false;
+  std::string magic, sw, sh, sm;
+  bool ok = next_token(fp, magic) && (magic == "P5" || magic == "P6") && next_token(fp, sw) && next_token(fp, sh) &&
+            next_token(fp, sm) && std::atoi(sm.c_str()) == 255;
+  if (ok) {
+    const int w = std::atoi(sw.c_str()), h = std::atoi(sh.c_str()), c = magic == "P6" ? 3 : 1;
+    ok = w > 0 && h > 0;
+    if (ok) {
+      out = MapU8(w, h, c);
+      ok = std::fread(out.data.data(), 1, out.data.size(), fp) == out.data.size();
+    }
+  }
+  std::fclose(fp);
+  return ok;
+}
+
+MapF ToGrayFloat(const MapU8& img) {
+  MapF g(img.width, img.height);
+  for (size_t i = 0, n = (size_t)img.width * img.height; i < n; ++i) {
+    if (img.channels == 1) {
+      g.data[i] = (float)img.data[i];
+    } else {  // P6 holds R G B
+      const double y = 0.299 * img.data[3 * i] + 0.587 * img.data[3 * i + 1] + 0.114 * img.data[3 * i + 2];
+      g.data[i] = (float)std::floor(y + 0.5);
+    }
+  }
+  return g;
+}
+
+bool ExportPointCloud(const std::string& path, const std::vector<FusedPoint>& pts) {
+  FILE* fp = std::fopen(path.c_str(), "wb");
+  if (!fp) return false;
+  std::fprintf(fp,
+               "ply\nformat binary_little_endian 1.0\nelement vertex %zu\nproperty float x\nproperty float y\n"
+               "property float z\nproperty uchar red\nproperty uchar green\nproperty uchar blue\nend_header\n",
+               pts.size());
+  bool ok = true;
+  for (const FusedPoint& p : pts) {
+    float X[3] = {p.coord[0], p.coord[1], p.coord[2]};
+    const bool finite = std::isfinite(X[0]) && std::isfinite(X[1]) && std::isfinite(X[2]);
+    if (!finite) X[0] = X[1] = X[2] = 0.0f;  // HPM.cpp:477-481
+    const char rgb[3] = {(char)(int)p.color[2], (char)(int)p.color[1], (char)(int)p.color[0]};
+    ok = ok && std::fwrite(X, sizeof(float), 3, fp) == 3 && std::fwrite(rgb, 1, 3, fp) == 3;
+  }
+  return std::fclose(fp) == 0 && ok;
+}
+
+bool ExportFusedPointCloud(const std::string& path, const std::vector<FusedPoint>& pts) {
+  auto norm2 = [](const FusedPoint& p) {
+    return (double)p.normal[0] * p.normal[0] + (double)p.normal[1] * p.normal[1] + (double)p.normal[2] * p.normal[2];
+  };
+  auto valid = [&](const FusedPoint& p) {
+    const double n = norm2(p);
+    return std::isfinite(p.coord[0]) && std::isfinite(p.coord[1]) && std::isfinite(p.coord[2]) && std::isfinite(n) &&
+           n > 1e-20;
+  };
+  size_t count = 0;
+  for (const FusedPoint& p : pts) count += valid(p);
+  FILE* fp = std::fopen(path.c_str(), "wb");
+  if (!fp) return false;
+  std::fprintf(fp,
+               "ply\nformat binary_little_endian 1.0\nelement vertex %zu\nproperty float x\nproperty float y\n"
+               "property float z\nproperty float nx\nproperty float ny\nproperty float nz\nproperty uchar red\n"
+               "property uchar green\nproperty uchar blue\nend_header\n",
+               count);
+  bool ok = true;
+  auto channel = [](float v) -> unsigned char {
+    return std::isfinite(v) ? (unsigned char)std::max(0.0f, std::min(255.0f, v)) : 0;
+  };
+  for (const FusedPoint& p : pts) {
+    if (!valid(p)) continue;
+    const double len = std::sqrt(norm2(p));
+    const float v[6] = {p.coord[0], p.coord[1], p.coord[2], (float)(p.normal[0] / len), (float)(p.normal[1] / len),
+                        (float)(p.normal[2] / len)};
+    const unsigned char rgb[3] = {channel(p.color[2]), channel(p.color[1]), channel(p.color[0])};
+    ok = ok && std::fwrite(v, sizeof(float), 6, fp) == 6 && std::fwrite(rgb, 1, 3, fp) == 3;
+  }
+  return std::fclose(fp) == 0 && ok;
+}
+
+// ------------------------------------------------------------- planar prior
+
+std::vector<Point2i> GetSupportPoints(const MapF& costs, const MapF& confidences, int width, int height) {
+  std::vector<Point2i> pts;
+  const int step_size = 5;
+  for (int col = 0; col < width; col += step_size) {
+    for (int row = 0; row < height; row += step_size) {
+      float min_cost_no_texture = 2.0f, min_cost_texture = 2.0f;
+      Point2i p_no_texture, p_texture;
+      const int c_bound = std::min(width, col + step_size), r_bound = std::min(height, row + step_size);
+      for (int c = col; c < c_bound; ++c) {
+        for (int r = row; r < r_bound; ++r) {
+          const float cost = costs.at(c, r), conf = confidences.at(c, r);
+          if (cost < 2.0f) {
+            const float f_no_texture = cost - conf;
+            const float f_texture = (float)((double)cost + 0.2 - (double)conf);
+            if (min_cost_no_texture > f_no_texture) { p_no_texture = {c, r}; min_cost_no_texture = f_no_texture; }
+            if (min_cost_texture > f_texture) { p_texture = {c, r}; min_cost_texture = f_texture; }
+          }
+        }
+      }
+      if ((double)min_cost_texture < 0.1) pts.push_back(p_texture);
+      if ((double)min_cost_no_texture < 0.1) pts.push_back(p_no_texture);
+    }
+  }
+  return pts;
+}
+
+namespace {
+
+// Incremental Delaunay triangulation (Bowyer-Watson) with exact predicates on
+// integer coordinates: orientation in int64, in-circle in __int128.
+struct Delaunay {
+  struct V { int64_t x, y; };
+  struct T { int v[3]; int nb[3]; bool alive; };  // nb[i] is across the edge opposite v[i]
+  std::vector<V> vs;
+  std::vector<T> ts;
+  int last = 0;
+
+  static int64_t orient(const V& a, const V& b, const V& c) {
+    return (b.x - a.x) * (c.y - a.y) - (b.y - a.y) * (c.x - a.x);
+  }
+  // > 0 when d lies strictly inside the circumcircle of the CCW triangle abc
+  static int incircle(const V& a, const V& b, const V& c, const V& d) {
+    const __int128 adx = a.x - d.x, ady = a.y - d.y, bdx = b.x - d.x, bdy = b.y - d.y, cdx = c.x - d.x,
+                   cdy = c.y - d.y;
+    const __int128 ad = adx * adx + ady * ady, bd = bdx * bdx + bdy * bdy, cd = cdx * cdx + cdy * cdy;
+    const __int128 det = adx * (bdy * cd - bd * cdy) - ady * (bdx * cd - bd * cdx) + ad * (bdx * cdy - bdy * cdx);
+    return det > 0 ? 1 : (det < 0 ? -1 : 0);
+  }
+
+  // Coordinates are stored doubled so that the bounding triangle is integral.
+  // The bounding triangle is the one cv::Subdiv2D starts from (centre of the
+  // rectangle +- 3 x its larger side), so that, as in the reference, thin
+  // triangles along the convex hull whose circumcircle reaches a bounding
+  // vertex are not produced.
+  Delaunay(int64_t w, int64_t h) {
+    const int64_t big = 3 * std::max(w, h), cx = w, cy = h;  // doubled centre
+    vs = {{cx + 2 * big, cy}, {cx, cy + 2 * big}, {cx - 2 * big, cy - 2 * big}};
+    ts.push_back(T{{0, 1, 2}, {-1, -1, -1}, true});
+  }
+
+  int locate(const V& p) {
+    int t = last;
+    if (!ts[t].alive) t = (int)ts.size() - 1;
+    for (size_t steps = 0; steps < 4 * ts.size() + 16; ++steps) {
+      const T& tr = ts[t];
+      int next = -1;
+      for (int k = 0; k < 3 && next < 0; ++k) {
+        const int i = (k + (int)(steps % 3)) % 3;  // rotate the test order so a walk cannot cycle
+        if (orient(vs[tr.v[(i + 1) % 3]], vs[tr.v[(i + 2) % 3]], p) < 0) next = tr.nb[i];
+      }
+      if (next < 0) return t;
+      t = next;
+    }
+    for (size_t i = 0; i < ts.size(); ++i) {  // fallback: scan
+      if (!ts[i].alive) continue;
+      const T& tr = ts[i];
+      if (orient(vs[tr.v[0]], vs[tr.v[1]], p) >= 0 && orient(vs[tr.v[1]], vs[tr.v[2]], p) >= 0 &&
+          orient(vs[tr.v[2]], vs[tr.v[0]], p) >= 0)
+        return (int)i;
+    }
+    return -1;
+  }
+
+  void insert(int64_t x, int64_t y) {
+    const V p{x, y};
+    const int t0 = locate(p);
+    if (t0 < 0) return;
+    for (int k = 0; k < 3; ++k)
+      if (vs[ts[t0].v[k]].x == x && vs[ts[t0].v[k]].y == y) return;  // duplicate point
+    const int pi = (int)vs.size();
+    vs.push_back(p);
+    // cavity: triangles whose circumcircle strictly contains p, grown from t0
+    std::vector<int> cavity{t0}, stack{t0};
+    std::vector<char> in(ts.size(), 0);
+    in[t0] = 1;
+    while (!stack.empty()) {
+      const int t = stack.back();
+      stack.pop_back();
+      for (int k = 0; k < 3; ++k) {
+        const int n = ts[t].nb[k];
+        if (n < 0 || in[n]) continue;
+        const T& tn = ts[n];
+        if (incircle(vs[tn.v[0]], vs[tn.v[1]], vs[tn.v[2]], p) > 0) {
+          in[n] = 1;
+          cavity.push_back(n);
+          stack.push_back(n);
+        }
+      }
+    }
+    // boundary edges (a, b) in CCW order of their cavity triangle, with the
+    // outside neighbour; each becomes the triangle (a, b, p)
+    struct E { int a, b, out; };
+    std::vector<E> boundary;
+    for (int t : cavity)
+      for (int k = 0; k < 3; ++k) {
+        const int n = ts[t].nb[k];
+        if (n < 0 || !in[n]) boundary.push_back({ts[t].v[(k + 1) % 3], ts[t].v[(k + 2) % 3], n});
+      }
+    for (int t : cavity) ts[t].alive = false;
+    std::vector<int> starts(vs.size(), -1), ends(vs.size(), -1);
+    std::vector<int> created;
+    for (const E& e : boundary) {
+      const int id = (int)ts.size();
+      ts.push_back(T{{e.a, e.b, pi}, {-1, -1, e.out}, true});
+      created.push_back(id);
+      starts[e.a] = id;
+      ends[e.b] = id;
+      if (e.out >= 0)
+        for (int k = 0; k < 3; ++k) {
+          T& o = ts[e.out];
+          // the outside triangle's edge (b, a) is opposite its vertex k
+          if (o.v[(k + 1) % 3] == e.b && o.v[(k + 2) % 3] == e.a) o.nb[k] = id;
+        }
+    }
+    for (int id : created) {
+      T& t = ts[id];
+      t.nb[0] = starts[t.v[1]];  // edge (b, p) is shared with the triangle that starts at b
+      t.nb[1] = ends[t.v[0]];    // edge (p, a) with the triangle that ends at a
+    }
+    last = created.empty() ? 0 : created.back();
+  }
+};
+
+}  // namespace
+
+std::vector<Triangle> DelaunayTriangulation(int width, int height, const std::vector<Point2i>& points) {
+  std::vector<Triangle> out;
+  if (points.empty()) return out;
+  Delaunay d(width, height);
+  for (const Point2i& p : points)
+    if (p.x >= 0 && p.x < width && p.y >= 0 && p.y < height) d.insert(2 * (int64_t)p.x, 2 * (int64_t)p.y);
+  for (const auto& t : d.ts) {
+    if (!t.alive || t.v[0] < 3 || t.v[1] < 3 || t.v[2] < 3) continue;
+    const auto& a = d.vs[t.v[0]];
+    const auto& b = d.vs[t.v[1]];
+    const auto& c = d.vs[t.v[2]];
+    out.push_back(Triangle{{(int)(a.x / 2), (int)(a.y / 2)}, {(int)(b.x / 2), (int)(b.y / 2)},
+                           {(int)(c.x / 2), (int)(c.y / 2)}});
+  }
+  return out;
+}
+
+void RasterTriangles(const std::vector<Triangle>& tris, MapF& mask) {
+  uint32_t idx = 0;
+  for (const Triangle& t : tris) {
+    auto in = [&](const Point2i& p) { return p.x >= 0 && p.x < mask.width && p.y >= 0 && p.y < mask.height; };
+    if (!(in(t.pt1) && in(t.pt2) && in(t.pt3))) continue;
+    auto len = [](const Point2i& a, const Point2i& b) {
+      const double dx = a.x - b.x, dy = a.y - b.y;  // pow(int, 2) is a double
+      return (float)std::sqrt(dx * dx + dy * dy);
+    };
+    const float max_edge = std::max(len(t.pt1, t.pt2), std::max(len(t.pt1, t.pt3), len(t.pt2, t.pt3)));
+    const float step = (float)(1.0 / max_edge);
+    for (float p = 0; p < 1.0; p += step) {
+      for (float q = 0; (double)q < 1.0 - p; q += step) {
+        const int x = (int)((double)(p * t.pt1.x + q * t.pt2.x) + (1.0 - p - q) * t.pt3.x);
+        const int y = (int)((double)(p * t.pt1.y + q * t.pt2.y) + (1.0 - p - q) * t.pt3.y);
+        if (x >= 0 && x < mask.width && y >= 0 && y < mask.height) mask.at(x, y) = (float)(idx + 1.0);
+      }
+    }
+    ++idx;
+  }
+}
+
+void PlaneThroughPoints(const float X1[3], const float X2[3], const float X3[3], float plane[4]) {
+  // Null vector of [X 1] (3 x 4): the generalised cross product of its rows,
+  // component k = (-1)^k det(rows without column k), in double.
+  const double A[3][4] = {{X1[0], X1[1], X1[2], 1.0}, {X2[0], X2[1], X2[2], 1.0}, {X3[0], X3[1], X3[2], 1.0}};
+  double n[4];
+  for (int k = 0; k < 4; ++k) {
+    int c[3], m = 0;
+    for (int j = 0; j < 4; ++j)
+      if (j != k) c[m++] = j;
+    const double det = A[0][c[0]] * (A[1][c[1]] * A[2][c[2]] - A[1][c[2]] * A[2][c[1]]) -
+                       A[0][c[1]] * (A[1][c[0]] * A[2][c[2]] - A[1][c[2]] * A[2][c[0]]) +
+                       A[0][c[2]] * (A[1][c[0]] * A[2][c[1]] - A[1][c[1]] * A[2][c[0]]);
+    n[k] = (k % 2 ? -det : det);
+  }
+  const double s = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2] + n[3] * n[3]);
+  float n4[4];
+  for (int k = 0; k < 4; ++k) n4[k] = (float)(n[k] / s);  // unit null vector, as SVD::solveZ returns
+  float norm2 = (float)std::sqrt((double)n4[0] * n4[0] + (double)n4[1] * n4[1] + (double)n4[2] * n4[2]);
+  if (n4[3] < 0) norm2 *= -1;
+  for (int k = 0; k < 4; ++k) plane[k] = n4[k] / norm2;
+}
+
+// ----------------------------------------------------------------- edge map
+
+MapU8 CannyEdges(const MapU8& gray, float low, float high) {
+  const int W = gray.width, H = gray.height;
+  auto px = [&](int x, int y) {
+    x = std::min(std::max(x, 0), W - 1);
+    y = std::min(std::max(y, 0), H - 1);
+    return (int)gray.data[(size_t)y * W + x];
+  };
+  std::vector<int> gx((size_t)W * H), gy((size_t)W * H), mag((size_t)W * H);
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) {
+      const int dx = (px(x + 1, y - 1) + 2 * px(x + 1, y) + px(x + 1, y + 1)) -
+                     (px(x - 1, y - 1) + 2 * px(x - 1, y) + px(x - 1, y + 1));
+      const int dy = (px(x - 1, y + 1) + 2 * px(x, y + 1) + px(x + 1, y + 1)) -
+                     (px(x - 1, y - 1) + 2 * px(x, y - 1) + px(x + 1, y - 1));
+      const size_t i = (size_t)y * W + x;
+      gx[i] = dx;
+      gy[i] = dy;
+      mag[i] = std::abs(dx) + std::abs(dy);
+    }
+  auto m = [&](int x, int y) { return (x < 0 || y < 0 || x >= W || y >= H) ? 0 : mag[(size_t)y * W + x]; };
+  // 0 none, 1 weak, 2 strong
+  std::vector<uint8_t> cls((size_t)W * H, 0);
+  const double t22 = 0.41421356237309503, t67 = 2.414213562373095;  // tan 22.5, tan 67.5 degrees
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) {
+      const size_t i = (size_t)y * W + x;
+      const int v = mag[i];
+      if (v <= low) continue;
+      const double ax = std::abs(gx[i]), ay = std::abs(gy[i]);
+      bool peak;
+      if (ay <= ax * t22) peak = v > m(x - 1, y) && v >= m(x + 1, y);
+      else if (ay >= ax * t67) peak = v > m(x, y - 1) && v >= m(x, y + 1);
+      else if ((gx[i] ^ gy[i]) < 0) peak = v > m(x + 1, y - 1) && v >= m(x - 1, y + 1);
+      else peak = v > m(x - 1, y - 1) && v >= m(x + 1, y + 1);
+      if (peak) cls[i] = v > high ? 2 : 1;
+    }
+  MapU8 out(W, H);
+  std::vector<size_t> stack;
+  for (size_t i = 0; i < cls.size(); ++i)
+    if (cls[i] == 2) { out.data[i] = 255; stack.push_back(i); }
+  while (!stack.empty()) {
+    const size_t i = stack.back();
+    stack.pop_back();
+    const int x = (int)(i % W), y = (int)(i / W);
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int xx = x + dx, yy = y + dy;
+        if (xx < 0 || yy < 0 || xx >= W || yy >= H) continue;
+        const size_t j = (size_t)yy * W + xx;
+        if (cls[j] == 1 && !out.data[j]) { out.data[j] = 255; stack.push_back(j); }
+      }
+  }
+  return out;
+}
+
+// --------------------------------------------------------------- HPM facade
+
+std::string ResultFolder(const std::string& dense_folder, int ref_image_id) {
+  std::ostringstream s;
+  s << dense_folder << "/HPM_MVS_plusplus/2333_" << std::setw(8) << std::setfill('0') << ref_image_id;
+  return s.str();
+}
+
+static std::string ImagePath(const std::string& dense_folder, int id, const char* ext) {
+  std::ostringstream s;
+  s << dense_folder << "/images/" << std::setw(8) << std::setfill('0') << id << ext;
+  return s.str();
+}
+
+static MapU8 LoadImage(const std::string& dense_folder, int id) {
+  MapU8 img;
+  if (ReadNetpbm(ImagePath(dense_folder, id, ".pgm"), img) || ReadNetpbm(ImagePath(dense_folder, id, ".ppm"), img))
+    return img;
+  throw Error("cannot read image " + ImagePath(dense_folder, id, ".pgm") + " (binary PGM/PPM)");
+}
+
+static MapF LoadDmb(const std::string& path) {
+  MapF m;
+  if (!ReadDmb(path, m)) throw Error("cannot read " + path);
+  return m;
+}
+
+HPM::HPM() {
+  hpm_default_params(&params_);
+  if (hpm_create(&ctx_, 0) != HPM_OK) throw Error("hpm_create failed (no HIP device?)");
+}
+
+HPM::~HPM() { hpm_destroy(ctx_); }
+
+void HPM::check(int rc, const char* what) const {
+  if (rc != HPM_OK) throw Error(std::string(what) + ": " + hpm_last_error(ctx_));
+}
+
+void HPM::SetGeomConsistencyParams(bool multi_geometry) {  // HPM.cpp:565-569
+  params_.geom_consistency = 1;
+  multi_geometry_ = multi_geometry;
+}
+void HPM::SetPlanarPriorParams() { params_.prior_consistency = 1; }
+void HPM::SetHierarchyParams() { params_.hierarchy = 1; }
+void HPM::SetMandConsistencyParams(bool flag) { params_.mand_consistency = flag ? 1 : 0; }
+void HPM::SetPrecision(int precision) { params_.precision = precision; }
+
+void HPM::InuputInitialization(const std::string& dense_folder, const std::vector<Problem>& problems, int idx) {
+  images_.clear();
+  cams_.clear();
+  depths_.clear();
+  const Problem& problem = problems.at(idx);
+  std::vector<int> ids{problem.ref_image_id};
+  ids.insert(ids.end(), problem.src_image_ids.begin(), problem.src_image_ids.end());
+  for (int id : ids) {
+    MapU8 img = LoadImage(dense_folder, id);
+    std::ostringstream cp;
+    cp << dense_folder << "/cams/" << std::setw(8) << std::setfill('0') << id << "_cam.txt";
+    hpm_camera c = ReadCamera(cp.str());
+    c.width = img.width;
+    c.height = img.height;
+    cams_.push_back(c);
+    images_.push_back(ToGrayFloat(img));
+  }
+  if (params_.geom_consistency) {  // HPM.cpp:693-732
+    const char* suffix = multi_geometry_ ? "/depths_geom.dmb" : "/depths.dmb";
+    for (int id : ids) depths_.push_back(LoadDmb(ResultFolder(dense_folder, id) + suffix));
+  }
+}
+
+void HPM::CudaSpaceInitialization(const std::string& dense_folder, const Problem& problem) {
+  check(hpm_set_params(ctx_, &params_), "hpm_set_params");
+  std::vector<const float*> ptrs;
+  for (const MapF& m : images_) ptrs.push_back(m.data.data());
+  check(hpm_set_views(ctx_, (int)cams_.size(), cams_.data(), ptrs.data()), "hpm_set_views");
+  if (params_.geom_consistency) {
+    std::vector<const float*> dp;
+    for (const MapF& m : depths_) dp.push_back(m.data.data());
+    check(hpm_set_src_depths(ctx_, dp.data()), "hpm_set_src_depths");
+    // reload of the reference's own maps, HPM.cpp:813-846
+    const std::string rf = ResultFolder(dense_folder, problem.ref_image_id);
+    CudaHypothesesReload(depths_[0], LoadDmb(rf + "/costs.dmb"), LoadDmb(rf + "/normals.dmb"));
+  }
+}
+
+void HPM::CudaCannyInitialization(const MapU8& canny) {
+  std::vector<uint32_t> c(canny.data.begin(), canny.data.end());
+  check(hpm_set_canny(ctx_, c.data()), "hpm_set_canny");
+}
+
+void HPM::CudaConfidenceInitialization(const std::string& dense_folder, const std::vector<Problem>& problems, int idx) {
+  const MapF conf = LoadDmb(ResultFolder(dense_folder, problems.at(idx).ref_image_id) + "/confidence.dmb");
+  check(hpm_set_confidence(ctx_, conf.data.data()), "hpm_set_confidence");
+}
+
+void HPM::CudaHypothesesReload(const MapF& depths, const MapF& costs, const MapF& normals) {
+  const size_t n = (size_t)depths.width * depths.height;
+  std::vector<float> planes(4 * n);
+  for (size_t i = 0; i < n; ++i) {
+    planes[4 * i + 0] = normals.data[3 * i + 0];
+    planes[4 * i + 1] = normals.data[3 * i + 1];
+    planes[4 * i + 2] = normals.data[3 * i + 2];
+    planes[4 * i + 3] = depths.data[i];
+  }
+  check(hpm_reload_hypotheses(ctx_, planes.data(), costs.data.data()), "hpm_reload_hypotheses");
+}
+
+void HPM::CudaPlanarPriorInitialization(const std::vector<float>& plane_params, const MapF& masks) {
+  const size_t n = (size_t)masks.width * masks.height;
+  std::vector<float> prior(4 * n, 0.0f);
+  std::vector<uint32_t> m(n);
+  for (size_t i = 0; i < n; ++i) {
+    m[i] = (uint32_t)masks.data[i];
+    if (masks.data[i] > 0) std::memcpy(&prior[4 * i], &plane_params[4 * (m[i] - 1)], 16);
+  }
+  check(hpm_set_prior(ctx_, prior.data(), m.data()), "hpm_set_prior");
+}
+
+void HPM::ReloadPlanarPriorInitialization(const MapF& masks, const std::vector<float>& prior_planes) {
+  const size_t n = (size_t)masks.width * masks.height;
+  std::vector<float> prior(4 * n, 0.0f);
+  std::vector<uint32_t> m(n);
+  for (size_t i = 0; i < n; ++i) {
+    m[i] = (uint32_t)masks.data[i];
+    if (masks.data[i] > 0) std::memcpy(&prior[4 * i], &prior_planes[4 * i], 16);
+  }
+  check(hpm_set_prior(ctx_, prior.data(), m.data()), "hpm_set_prior");
+}
+
+void HPM::RunPatchMatch(uint64_t seed) {
+  check(hpm_set_params(ctx_, &params_), "hpm_set_params");
+  check(hpm_run_patchmatch(ctx_, seed), "hpm_run_patchmatch");
+  check(hpm_synchronize(ctx_), "hpm_synchronize");
+}
+
+float HPM::GetMinDepth() const { return cams_.at(0).depth_min * 0.6f; }  // params.depth_min, HPM.cpp:685
+float HPM::GetMaxDepth() const { return cams_.at(0).depth_max * 1.2f; }  // params.depth_max, HPM.cpp:686
+
+void HPM::GetPlanes(std::vector<float>& planes) const {
+  planes.resize((size_t)4 * GetReferenceImageWidth() * GetReferenceImageHeight());
+  check(hpm_get_planes(ctx_, planes.data()), "hpm_get_planes");
+}
+void HPM::GetCosts(MapF& costs) const {
+  costs = MapF(GetReferenceImageWidth(), GetReferenceImageHeight());
+  check(hpm_get_costs(ctx_, costs.data.data()), "hpm_get_costs");
+}
+void HPM::GetTexture(MapF& texture) const {
+  texture = MapF(GetReferenceImageWidth(), GetReferenceImageHeight());
+  check(hpm_get_texture(ctx_, texture.data.data()), "hpm_get_texture");
+}
+
+// Get3DPointonRefCam, HPM.cpp:258-267.
+static void PointOnRefCam(int x, int y, float depth, const hpm_camera& c, float X[3]) {
+  X[0] = depth * ((float)x - c.K[2]) / c.K[0];
+  X[1] = depth * ((float)y - c.K[5]) / c.K[4];
+  X[2] = depth;
+}
+
+void HPM::GetPriorPlaneParams(const Triangle& t, const MapF& depths, float plane[4]) const {
+  float X1[3], X2[3], X3[3];
+  PointOnRefCam(t.pt1.x, t.pt1.y, depths.at(t.pt1.x, t.pt1.y), cams_[0], X1);
+  PointOnRefCam(t.pt2.x, t.pt2.y, depths.at(t.pt2.x, t.pt2.y), cams_[0], X2);
+  PointOnRefCam(t.pt3.x, t.pt3.y, depths.at(t.pt3.x, t.pt3.y), cams_[0], X3);
+  PlaneThroughPoints(X1, X2, X3, plane);
+}
+
+float HPM::GetDepthFromPlaneParam(const float pl[4], int x, int y) const {  // HPM.cpp:1169-1172
+  const float* K = cams_[0].K;
+  return -pl[3] * K[0] / (((float)x - K[2]) * pl[0] + (K[0] / K[4]) * ((float)y - K[5]) * pl[1] + K[0] * pl[2]);
+}
+
+void HPM::TransformNormal(const float in[4], float out[4]) const {  // HPM.cpp:1179-1187
+  const float* R = cams_[0].R;
+  out[0] = R[0] * in[0] + R[3] * in[1] + R[6] * in[2];
+  out[1] = R[1] * in[0] + R[4] * in[1] + R[7] * in[2];
+  out[2] = R[2] * in[0] + R[5] * in[1] + R[8] * in[2];
+  out[3] = in[3];
+}
+
+void HPM::TransformNormal2RefCam(const float in[4], float out[4]) const {  // HPM.cpp:1189-1197
+  const float* R = cams_[0].R;
+  out[0] = R[0] * in[0] + R[1] * in[1] + R[2] * in[2];
+  out[1] = R[3] * in[0] + R[4] * in[1] + R[5] * in[2];
+  out[2] = R[6] * in[0] + R[7] * in[1] + R[8] * in[2];
+  out[3] = in[3];
+}
+
+float HPM::GetDistance2Origin(int x, int y, float depth, const float n[4]) const {  // HPM.cpp:1199-1206
+  float X[3];
+  PointOnRefCam(x, y, depth, cams_[0], X);
+  return -(n[0] * X[0] + n[1] * X[1] + n[2] * X[2]);
+}
+
+// -------------------------------------------------------------- pass driver
+
+static void MakeDir(const std::string& path) { ::mkdir(path.c_str(), 0755); }
+
+void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>& problems, int idx,
+                    const PassFlags& flags, int precision, uint64_t seed) {
+  const Problem& problem = problems.at(idx);
+  MakeDir(dense_folder + "/HPM_MVS_plusplus");
+  const std::string rf = ResultFolder(dense_folder, problem.ref_image_id);
+  MakeDir(rf);
+  HPM hpm;
+  hpm.SetPrecision(precision);
+  if (flags.geom_consistency) hpm.SetGeomConsistencyParams(flags.multi_geometry);
+  hpm.InuputInitialization(dense_folder, problems, idx);
+  hpm.SetMandConsistencyParams(flags.mand_consistency);
+  if (flags.prior_consistency) hpm.SetPlanarPriorParams();
+  hpm.CudaSpaceInitialization(dense_folder, problem);
+  const int width = hpm.GetReferenceImageWidth(), height = hpm.GetReferenceImageHeight();
+
+  // cv::Canny(image, 50, 150), main.cpp:287
+  MapU8 gray8(width, height);
+  for (size_t i = 0; i < gray8.data.size(); ++i) gray8.data[i] = (uint8_t)hpm.GetReferenceImage().data[i];
+  hpm.CudaCannyInitialization(CannyEdges(gray8, 50.0f, 150.0f));
+  if (flags.mand_consistency || flags.prior_consistency) hpm.CudaConfidenceInitialization(dense_folder, problems, idx);
+
+  MapF depths(width, height), normals(width, height, 3), costs(width, height), texture(width, height);
+  if (flags.prior_consistency) {  // main.cpp:324-441, distance 0
+    const char* suffix = flags.multi_geometry ? "/depths_geom.dmb" : "/depths.dmb";
+    depths = LoadDmb(rf + suffix);
+    normals = LoadDmb(rf + "/normals.dmb");
+    const MapF confidences = LoadDmb(rf + "/confidence.dmb");
+    costs = LoadDmb(rf + "/costs.dmb");
+    const std::vector<Point2i> support = GetSupportPoints(costs, confidences, width, height);
+    const std::vector<Triangle> tris = DelaunayTriangulation(width, height, support);
+    MapF mask(width, height);
+    std::vector<float> planes;
+    for (const Triangle& t : tris) {
+      float pl[4];
+      hpm.GetPriorPlaneParams(t, depths, pl);
+      planes.insert(planes.end(), pl, pl + 4);
+    }
+    RasterTriangles(tris, mask);
+    for (int i = 0; i < width; ++i)
+      for (int j = 0; j < height; ++j) {
+        if (mask.at(i, j) > 0) {
+          const float d = hpm.GetDepthFromPlaneParam(&planes[4 * ((size_t)mask.at(i, j) - 1)], i, j);
+          if (!(d <= hpm.GetMaxDepth() && d >= hpm.GetMinDepth())) mask.at(i, j) = 0;
+        }
+      }
+    hpm.CudaPlanarPriorInitialization(planes, mask);
+    hpm.CudaHypothesesReload(depths, costs, normals);
+  }
+  hpm.RunPatchMatch(seed);
+  std::vector<float> planes;
+  hpm.GetPlanes(planes);
+  for (size_t i = 0; i < (size_t)width * height; ++i) {
+    depths.data[i] = planes[4 * i + 3];
+    normals.data[3 * i + 0] = planes[4 * i + 0];
+    normals.data[3 * i + 1] = planes[4 * i + 1];
+    normals.data[3 * i + 2] = planes[4 * i + 2];
+  }
+  hpm.GetCosts(costs);
+  const std::string dpath = rf + (flags.geom_consistency ? "/depths_geom.dmb" : "/depths.dmb");
+  if (!WriteDmb(dpath, depths) || !WriteDmb(rf + "/normals.dmb", normals) || !WriteDmb(rf + "/costs.dmb", costs))
+    throw Error("cannot write results under " + rf);
+  if (!flags.mand_consistency && !flags.prior_consistency) {
+    hpm.GetTexture(texture);
+    if (!WriteDmb(rf + "/texture" + std::to_string(flags.image_scale) + ".dmb", texture))
+      throw Error("cannot write texture under " + rf);
+  }
+}
+
+struct ViewMaps {
+  std::vector<hpm_camera> cams;
+  std::vector<MapF> depths, normals;
+  std::vector<int32_t> offsets, ids;
+};
+
+static ViewMaps LoadViewMaps(const std::string& dense_folder, const std::vector<Problem>& problems, bool geom) {
+  // The cameras are rescaled to the depth-map size (RescaleImageAndCamera,
+  // HPM.cpp:186-207); here the maps have the image size.
+  ViewMaps v;
+  for (const Problem& p : problems) {
+    const std::string rf = ResultFolder(dense_folder, p.ref_image_id);
+    v.depths.push_back(LoadDmb(rf + (geom ? "/depths_geom.dmb" : "/depths.dmb")));
+    v.normals.push_back(LoadDmb(rf + "/normals.dmb"));
+    std::ostringstream cp;
+    cp << dense_folder << "/cams/" << std::setw(8) << std::setfill('0') << p.ref_image_id << "_cam.txt";
+    hpm_camera c = ReadCamera(cp.str());
+    MapU8 img = LoadImage(dense_folder, p.ref_image_id);
+    const MapF& d = v.depths.back();
+    if (img.width != d.width || img.height != d.height) {
+      const float sx = d.width / (float)img.width, sy = d.height / (float)img.height;
+      c.K[0] *= sx; c.K[2] *= sx; c.K[4] *= sy; c.K[5] *= sy;
+    }
+    c.width = d.width;
+    c.height = d.height;
+    v.cams.push_back(c);
+    v.offsets.push_back(0);  // filled below
+  }
+  // main.cpp:897 indexes the loaded maps with the source image id; map ids to
+  // positions in pair.txt order
+  std::vector<int> pos;
+  for (size_t i = 0; i < problems.size(); ++i) {
+    const int id = problems[i].ref_image_id;
+    if (id < 0) throw Error("negative image id in pair.txt");
+    if ((size_t)id >= pos.size()) pos.resize(id + 1, -1);
+    pos[id] = (int)i;
+  }
+  v.offsets.assign(1, 0);
+  for (const Problem& p : problems) {
+    for (int s : p.src_image_ids) {
+      if (s < 0 || (size_t)s >= pos.size() || pos[s] < 0) throw Error("source image without a pair.txt entry");
+      v.ids.push_back(pos[s]);
+    }
+    v.offsets.push_back((int32_t)v.ids.size());
+  }
+  return v;
+}
+
+void ConfidenceEvaluation(const std::string& dense_folder, const std::vector<Problem>& problems, bool geom) {
+  ViewMaps v = LoadViewMaps(dense_folder, problems, geom);
+  const int n = (int)problems.size();
+  std::vector<MapF> conf;
+  std::vector<const float*> dp, np;
+  std::vector<float*> cp;
+  for (int i = 0; i < n; ++i) {
+    conf.emplace_back(v.depths[i].width, v.depths[i].height);
+    dp.push_back(v.depths[i].data.data());
+    np.push_back(v.normals[i].data.data());
+  }
+  for (auto& c : conf) cp.push_back(c.data.data());
+  int64_t count = 0;
+  const int rc = hpm_consistency_check(HPM_CHECK_CONFIDENCE, n, v.cams.data(), dp.data(), np.data(), v.offsets.data(),
+                                       v.ids.data(), cp.data(), nullptr, nullptr, 0, &count);
+  if (rc != HPM_OK) throw Error("hpm_consistency_check (confidence) failed");
+  for (int i = 0; i < n; ++i)
+    if (!WriteDmb(ResultFolder(dense_folder, problems[i].ref_image_id) + "/confidence.dmb", conf[i]))
+      throw Error("cannot write confidence.dmb");
+}
+
+std::vector<FusedPoint> RunFusion(const std::string& dense_folder, const std::vector<Problem>& problems, bool geom) {
+  ViewMaps v = LoadViewMaps(dense_folder, problems, geom);
+  const int n = (int)problems.size();
+  std::vector<const float*> dp, np;
+  size_t cap = 0;
+  for (int i = 0; i < n; ++i) {
+    dp.push_back(v.depths[i].data.data());
+    np.push_back(v.normals[i].data.data());
+    cap += v.depths[i].data.size();
+  }
+  std::vector<int32_t> pairs(2 * cap);
+  std::vector<float> xyzn(6 * cap);
+  int64_t count = 0;
+  const int rc = hpm_consistency_check(HPM_CHECK_FUSION, n, v.cams.data(), dp.data(), np.data(), v.offsets.data(),
+                                       v.ids.data(), nullptr, pairs.data(), xyzn.data(), (int64_t)cap, &count);
+  if (rc != HPM_OK) throw Error("hpm_consistency_check (fusion) failed");
+  std::vector<MapU8> imgs;
+  for (const Problem& p : problems) imgs.push_back(LoadImage(dense_folder, p.ref_image_id));
+  std::vector<FusedPoint> pts((size_t)count);
+  for (int64_t k = 0; k < count; ++k) {
+    FusedPoint& fp = pts[k];
+    std::memcpy(fp.coord, &xyzn[6 * k], 12);
+    std::memcpy(fp.normal, &xyzn[6 * k + 3], 12);
+    const MapU8& img = imgs[pairs[2 * k]];
+    const size_t p = (size_t)pairs[2 * k + 1];
+    for (int c = 0; c < 3; ++c)  // B, G, R (a grey image repeats its value)
+      fp.color[c] = img.channels == 1 ? img.data[p] : img.data[3 * p + (2 - c)];
+  }
+  const std::string out = dense_folder + "/HPM_MVS_plusplus";
+  MakeDir(out);
+  if (!ExportPointCloud(out + "/HPM_MVS_plusplus.ply", pts) || !ExportFusedPointCloud(out + "/fused.ply", pts))
+    throw Error("cannot write the point clouds under " + out);
+  return pts;
+}
+
+std::vector<FusedPoint> RunPipeline(const std::string& dense_folder, int precision, uint64_t seed) {
+  const std::vector<Problem> problems = ReadPairFile(dense_folder + "/pair.txt");
+  const int n = (int)problems.size();
+  PassFlags f;
+  for (int i = 0; i < n; ++i) ProcessProblem(dense_folder, problems, i, f, precision, seed + i);  // photometric
+  ConfidenceEvaluation(dense_folder, problems, false);
+  f.prior_consistency = true;
+  for (int i = 0; i < n; ++i) ProcessProblem(dense_folder, problems, i, f, precision, seed + 1000 + i);
+  f = PassFlags();
+  for (int it = 0; it < 3; ++it) {  // main.cpp:1269-1290
+    f.multi_geometry = it > 0;
+    if (it > 0) {
+      f.mand_consistency = true;
+      ConfidenceEvaluation(dense_folder, problems, f.geom_consistency);
+    }
+    f.geom_consistency = true;
+    for (int i = 0; i < n; ++i) ProcessProblem(dense_folder, problems, i, f, precision, seed + 2000 * (it + 2) + i);
+  }
+  return RunFusion(dense_folder, problems, true);
+}
+
+}  // namespace hpmhost

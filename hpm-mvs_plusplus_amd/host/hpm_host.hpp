@@ -1,0 +1,190 @@
+// hpm_host.hpp -- C++ host side of the MI355X PatchMatch library: the dense
+// folder formats, the planar-prior construction, an edge detector, the `HPM`
+// facade with the reference class's method names, and the per-view pass
+// driver with its single-level schedule.  Everything that touches the GPU
+// goes through the C ABI of include/hpm_mi355x.h.
+#pragma once
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/hpm_mi355x.h"
+
+namespace hpmhost {
+
+// ---------------------------------------------------------------- formats --
+
+struct Problem {  // main.h:49-58 (the fields the pass driver uses)
+  int ref_image_id = 0;
+  std::vector<int> src_image_ids;
+};
+
+template <class T>
+struct Map {  // row-major image / per-pixel map
+  int width = 0, height = 0, channels = 1;
+  std::vector<T> data;
+  Map() = default;
+  Map(int w, int h, int c = 1, T v = T()) : width(w), height(h), channels(c), data((size_t)w * h * c, v) {}
+  T& at(int x, int y, int c = 0) { return data[((size_t)y * width + x) * channels + c]; }
+  const T& at(int x, int y, int c = 0) const { return data[((size_t)y * width + x) * channels + c]; }
+  bool empty() const { return data.empty(); }
+};
+using MapF = Map<float>;
+using MapU8 = Map<uint8_t>;
+
+// GenerateSampleList, main.cpp:171-200: sources with score <= 0 are dropped.
+std::vector<Problem> ReadPairFile(const std::string& path);
+// ReadCamera, HPM.cpp:159-184 ("extrinsic" 4x4 of which 3 rows are read,
+// "intrinsic" 3x3, then depth_min interval depth_num depth_max).  Width and
+// height are not in the file; they stay 0 until the image is known.
+hpm_camera ReadCamera(const std::string& path);
+// readDepthDmb / readNormalDmb (HPM.cpp:291-380): int32 type (1), h, w, nb,
+// then h*w*nb floats.  Return false on a missing file or another type.
+bool ReadDmb(const std::string& path, MapF& out);
+// writeDepthDmb / writeNormalDmb (HPM.cpp:323-407).
+bool WriteDmb(const std::string& path, const MapF& m);
+// Binary PGM (P5) or PPM (P6), 8 bit.  The reference decodes JPEG with
+// OpenCV; this build reads the uncompressed Netpbm formats.
+bool ReadNetpbm(const std::string& path, MapU8& out);
+// Grey float image of an 8-bit image: P5 as is; P6 with the ITU-R BT.601
+// weights 0.299 R + 0.587 G + 0.114 B rounded to the nearest integer.
+MapF ToGrayFloat(const MapU8& img);
+
+struct FusedPoint {  // PointList, main.h:92-96
+  float coord[3];
+  float normal[3];
+  float color[3];  // B, G, R as the reference stores them
+};
+// ExportPointCloud (HPM.cpp:445-498): x y z + uchar r g b, in point order.
+bool ExportPointCloud(const std::string& path, const std::vector<FusedPoint>& pts);
+// ExportFusedPointCloud (HPM.cpp:409-443): finite points with a non-zero
+// normal; x y z, unit normal, uchar r g b.
+bool ExportFusedPointCloud(const std::string& path, const std::vector<FusedPoint>& pts);
+
+// ------------------------------------------------------------ planar prior --
+
+struct Point2i {
+  int x = 0, y = 0;
+  bool operator==(const Point2i& o) const { return x == o.x && y == o.y; }
+};
+struct Triangle {  // main.h:87-90
+  Point2i pt1, pt2, pt3;
+};
+
+// HPM::GetSupportPoints_Classify_Check, HPM.cpp:1026-1073.
+std::vector<Point2i> GetSupportPoints(const MapF& costs, const MapF& confidences, int width, int height);
+// HPM::DelaunayTriangulation, HPM.cpp:1075-1097: Delaunay triangles of the
+// points inside [0, w) x [0, h) (duplicates ignored), grown inside the same
+// bounding triangle as cv::Subdiv2D, whose vertices are then dropped.  Exact
+// integer predicates; among co-circular configurations any valid
+// triangulation.
+std::vector<Triangle> DelaunayTriangulation(int width, int height, const std::vector<Point2i>& points);
+// Pixels of each triangle in the reference's barycentric raster
+// (main.cpp:389-404): mask(x, y) = index + 1 of the last triangle that wrote it.
+void RasterTriangles(const std::vector<Triangle>& tris, MapF& mask);
+// HPM::GetPriorPlaneParams, HPM.cpp:1134-1167: the plane n.X + w = 0 through
+// the three back-projected corners, |n| = 1, w >= 0.
+void PlaneThroughPoints(const float X1[3], const float X2[3], const float X3[3], float plane[4]);
+
+// -------------------------------------------------------------- edge map --
+
+// Edge map for the texture term (main.cpp:287 calls cv::Canny(image, 50, 150)):
+// 3x3 Sobel gradients, L1 magnitude, non-maximum suppression along the
+// quantised gradient direction and hysteresis between the two thresholds
+// (Canny 1986).  255 = edge, 0 = none.
+MapU8 CannyEdges(const MapU8& gray, float low, float high);
+
+// ------------------------------------------------------------ HPM facade --
+
+struct Error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// The reference's `class HPM` (HPM.h:57-140) over the C ABI: same method names,
+// same meaning; failures throw hpmhost::Error instead of exiting.
+class HPM {
+ public:
+  HPM();
+  ~HPM();
+  HPM(const HPM&) = delete;
+  HPM& operator=(const HPM&) = delete;
+
+  void SetGeomConsistencyParams(bool multi_geometry);
+  void SetPlanarPriorParams();
+  void SetHierarchyParams();
+  void SetMandConsistencyParams(bool flag);
+  void SetPrecision(int precision);  // HPM_PREC_*; the reference has FP32 only
+
+  // InuputInitialization (HPM.cpp:644-733): grey images and cameras of the
+  // problem's reference and sources from `images/%08d.{pgm,ppm}` and
+  // `cams/%08d_cam.txt`; source depth maps in geometric mode.
+  void InuputInitialization(const std::string& dense_folder, const std::vector<Problem>& problems, int idx);
+  // CudaSpaceInitialization (HPM.cpp:741-906): uploads; reloads the
+  // reference's own previous maps in geometric mode.
+  void CudaSpaceInitialization(const std::string& dense_folder, const Problem& problem);
+  void CudaCannyInitialization(const MapU8& canny);
+  void CudaConfidenceInitialization(const std::string& dense_folder, const std::vector<Problem>& problems, int idx);
+  void TextureInformationInitialization() {}
+  void CudaHypothesesReload(const MapF& depths, const MapF& costs, const MapF& normals);
+  void CudaPlanarPriorInitialization(const std::vector<float>& plane_params, const MapF& masks);
+  void ReloadPlanarPriorInitialization(const MapF& masks, const std::vector<float>& prior_planes);
+  void RunPatchMatch(uint64_t seed);
+
+  int GetReferenceImageWidth() const { return cams_.empty() ? 0 : cams_[0].width; }
+  int GetReferenceImageHeight() const { return cams_.empty() ? 0 : cams_[0].height; }
+  const MapF& GetReferenceImage() const { return images_.at(0); }
+  const hpm_camera& GetCamera(int i) const { return cams_.at(i); }
+  float GetMinDepth() const;
+  float GetMaxDepth() const;
+  // Bulk getters (GetPlaneHypothesis / GetCost / GetTexture for every pixel).
+  void GetPlanes(std::vector<float>& planes) const;  // (world normal, depth)
+  void GetCosts(MapF& costs) const;
+  void GetTexture(MapF& texture) const;
+
+  // GetPriorPlaneParams (HPM.cpp:1134-1167): plane of a triangle's corners in
+  // the reference camera frame.
+  void GetPriorPlaneParams(const Triangle& t, const MapF& depths, float plane[4]) const;
+  float GetDepthFromPlaneParam(const float plane[4], int x, int y) const;
+  void TransformNormal(const float in[4], float out[4]) const;
+  void TransformNormal2RefCam(const float in[4], float out[4]) const;
+  float GetDistance2Origin(int x, int y, float depth, const float normal[4]) const;
+
+ private:
+  void check(int rc, const char* what) const;
+  hpm_ctx* ctx_ = nullptr;
+  hpm_params params_{};
+  bool multi_geometry_ = false;
+  std::vector<hpm_camera> cams_;
+  std::vector<MapF> images_;
+  std::vector<MapF> depths_;
+};
+
+// ---------------------------------------------------------- pass driver --
+
+struct PassFlags {
+  bool geom_consistency = false;
+  bool prior_consistency = false;
+  bool mand_consistency = false;
+  bool multi_geometry = false;
+  int image_scale = 0;  // the texture{k}.dmb index
+};
+
+// ProcessProblem (main.cpp:237-634) at one pyramid level: one pass of one
+// reference view, reading and writing the reference's files under
+// <dense_folder>/HPM_MVS_plusplus/2333_%08d/.  Prior passes at distance 0.
+void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>& problems, int idx,
+                    const PassFlags& flags, int precision, uint64_t seed);
+// ConfidenceEvaluation (main.cpp:959-1109) -> confidence.dmb per view.
+void ConfidenceEvaluation(const std::string& dense_folder, const std::vector<Problem>& problems, bool geom);
+// RunFusion (main.cpp:823-957) -> HPM_MVS_plusplus.ply and fused.ply; returns
+// the fused points in the reference's order.
+std::vector<FusedPoint> RunFusion(const std::string& dense_folder, const std::vector<Problem>& problems, bool geom);
+// The single-level schedule of main.cpp:1249-1291: photometric pass, confidence,
+// prior pass, three geometric passes (the last two with mandatory
+// consistency after a confidence evaluation), fusion.
+std::vector<FusedPoint> RunPipeline(const std::string& dense_folder, int precision, uint64_t seed);
+
+std::string ResultFolder(const std::string& dense_folder, int ref_image_id);
+
+}  // namespace hpmhost

@@ -1,0 +1,55 @@
+/* hpm_pipeline.h -- C ABI of the host-side passes around PatchMatch: the dense
+ * folder pipeline (ProcessProblem, ConfidenceEvaluation, RunFusion and the
+ * single-level schedule of main.cpp) and the planar-prior construction steps
+ * of class HPM.  Library: hpm-mvs_plusplus_amd/lib/libhpm_host.so (links
+ * libhpm_mi355x.so).  Every function returns HPM_OK (0) or a negative HPM_E*
+ * status; hpm_host_last_error() gives the message of the last failure on the
+ * calling thread.
+ *
+ * Dense folder layout (the reference's): images/%08d.pgm|.ppm (binary
+ * Netpbm; the reference reads JPEG through OpenCV), cams/%08d_cam.txt,
+ * pair.txt; results in HPM_MVS_plusplus/2333_%08d/{depths,depths_geom,normals,
+ * costs,confidence,texture0}.dmb and HPM_MVS_plusplus/{HPM_MVS_plusplus,fused}.ply.
+ */
+#ifndef HPM_PIPELINE_H_
+#define HPM_PIPELINE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char* hpm_host_last_error(void);
+
+/* ProcessProblem (main.cpp:237-634) for problem `idx` of pair.txt at one
+ * pyramid level (prior passes at distance 0). */
+int hpm_host_process_problem(const char* dense_folder, int idx, int geom_consistency, int prior_consistency,
+                             int mand_consistency, int multi_geometry, int precision, uint64_t seed);
+/* ConfidenceEvaluation (main.cpp:959-1109): writes confidence.dmb per view. */
+int hpm_host_confidence(const char* dense_folder, int geom_consistency);
+/* RunFusion (main.cpp:823-957): writes the two PLY files; *num_points = fused points. */
+int hpm_host_fusion(const char* dense_folder, int geom_consistency, int64_t* num_points);
+/* The single-level schedule of main.cpp:1249-1291 followed by fusion. */
+int hpm_host_run_pipeline(const char* dense_folder, int precision, uint64_t seed, int64_t* num_points);
+
+/* GetSupportPoints_Classify_Check (HPM.cpp:1026-1073) at factor 1: costs and
+ * confidences w*h; out_xy receives up to `capacity` (x, y) pairs. */
+int hpm_host_support_points(const float* costs, const float* confidences, int w, int h, int32_t* out_xy,
+                            int capacity, int* count);
+/* DelaunayTriangulation (HPM.cpp:1075-1097): n points (x, y) inside w x h;
+ * out_tris receives up to `capacity` triangles (x1 y1 x2 y2 x3 y3). */
+int hpm_host_delaunay(int w, int h, const int32_t* xy, int n, int32_t* out_tris, int capacity, int* count);
+/* The barycentric raster of main.cpp:389-404: mask (w*h floats, zero-filled
+ * here) receives triangle index + 1. */
+int hpm_host_raster(const int32_t* tris, int n, int w, int h, float* mask);
+/* GetPriorPlaneParams' plane through three camera-frame points
+ * (HPM.cpp:1134-1167): plane = (n, w), |n| = 1, w >= 0. */
+int hpm_host_plane_through_points(const float* X1, const float* X2, const float* X3, float* plane);
+/* Edge map standing in for cv::Canny (main.cpp:287): 255 = edge. */
+int hpm_host_canny(const uint8_t* gray, int w, int h, float low, float high, uint8_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HPM_PIPELINE_H_ */

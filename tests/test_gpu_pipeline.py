@@ -1,0 +1,81 @@
+"""The dense-folder pipeline through libhpm_host.so on an MI355X: the
+single-level schedule of main.cpp:1249-1291 (photometric pass, confidence,
+planar-prior pass, three geometric passes with mandatory consistency, fusion)
+on a synthetic dense folder (images/*.pgm, cams/*_cam.txt, pair.txt).
+
+Checks: every pass writes the reference's files (.dmb with the reference
+layout, both PLY files); the final geometric depth maps agree with the
+ray-cast ground truth; fusion through the pipeline equals the oracle's
+consistency check on the maps it read (same points, same order).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="no HIP device")]
+
+
+@pytest.fixture(scope="module")
+def dense_folder(tmp_path_factory):
+    from hpm_mi355x import scenes
+    sc = scenes.make_plane_scene(seed=4, n_cams=5, width=160, height=120, f=131.25, n_src=4)
+    folder = str(tmp_path_factory.mktemp("dense"))
+    scenes.write_dense_folder(sc, folder)
+    return sc, folder
+
+
+def test_pipeline_runs_and_fuses(dense_folder):
+    from hpm_mi355x import pipeline as P
+    sc, folder = dense_folder
+    print("case: run_pipeline 5 views 160x120 fp32", flush=True)
+    n = P.run_pipeline(folder, precision="fp32", seed=3)
+    assert n > 5000
+    out = os.path.join(folder, "HPM_MVS_plusplus")
+    for i in range(5):
+        rf = os.path.join(out, f"2333_{i:08d}")
+        for f in ("depths.dmb", "depths_geom.dmb", "normals.dmb", "costs.dmb", "confidence.dmb", "texture0.dmb"):
+            assert os.path.exists(os.path.join(rf, f)), f
+        d = P.read_dmb(os.path.join(rf, "depths_geom.dmb"))
+        nm = P.read_dmb(os.path.join(rf, "normals.dmb"))
+        assert d.shape == (120, 160) and nm.shape == (120, 160, 3)
+        gt = sc.depths[i]
+        rel = np.abs(d - gt) / gt
+        print(f"view {i}: depth within 1% of GT {np.mean(rel <= 0.01):.3f}")
+        assert np.mean(rel <= 0.01) > 0.75
+    with open(os.path.join(out, "HPM_MVS_plusplus.ply"), "rb") as fh:
+        head = fh.read(400).split(b"end_header\n")[0].decode()
+    assert f"element vertex {n}" in head
+    with open(os.path.join(out, "fused.ply"), "rb") as fh:
+        data = fh.read()
+    hdr, body = data.split(b"end_header\n", 1)
+    k = int([l for l in hdr.decode().splitlines() if l.startswith("element vertex")][0].split()[-1])
+    assert len(body) == k * 27 and k <= n
+
+
+def test_pipeline_fusion_equals_oracle(dense_folder):
+    """Fusion of the pipeline's final maps against the oracle's RunFusion."""
+    from hpm_mi355x import pipeline as P
+    import hpm_mi355x as H
+    sc, folder = dense_folder
+    out = os.path.join(folder, "HPM_MVS_plusplus")
+    depths = [np.ascontiguousarray(P.read_dmb(os.path.join(out, f"2333_{i:08d}", "depths_geom.dmb"))) for i in range(5)]
+    normals = [np.ascontiguousarray(P.read_dmb(os.path.join(out, f"2333_{i:08d}", "normals.dmb"))) for i in range(5)]
+    sources = [[j for j, s in sc.pairs[i] if s > 0] for i in range(5)]
+    cams, hc = [], []
+    for i, c in enumerate(sc.cameras):  # the camera as the file holds it (decimal text -> float32)
+        tok = open(os.path.join(folder, "cams", f"{i:08d}_cam.txt")).read().split()
+        ext = np.array(tok[1:17], np.float32).reshape(4, 4)
+        K = np.array(tok[18:27], np.float32).reshape(3, 3)
+        dmin, dmax = np.float32(tok[27]), np.float32(tok[30])
+        cams.append(O.camera(K, ext[:3, :3], ext[:3, 3], c.width, c.height, dmin, dmax))
+        hc.append(H.make_camera(K, ext[:3, :3], ext[:3, 3], c.width, c.height, dmin, dmax))
+    rp, rx = O.consistency(0, cams, depths, normals, sources)
+    n = P.fusion(folder, geom=True)
+    assert n == len(rp)
+    gp, gx = H.consistency_check(H.CHECK_FUSION, hc, depths, normals, sources)
+    np.testing.assert_array_equal(gp, rp)
+    np.testing.assert_array_equal(gx, rx)
