@@ -169,6 +169,10 @@ int build_consts(hpm_ctx* ctx, RunConsts& C) {
 // Quad images of every view for the mixed-precision engine (rebuilt when the
 // storage precision changes).
 template <class S> int build_quads(hpm_ctx* ctx) {
+  for (int i = 0; i < ctx->nimg; ++i)  // tap addresses are formed exactly in float
+    if ((size_t)(ctx->cams[i].width + 1) * (ctx->cams[i].height + 1) >= ((size_t)1 << 24))
+      return fail(ctx, HPM_E_INVALID, "the mixed-precision path takes views below 2^24 texels (e.g. 4096 x 4095); "
+                                      "use HPM_PREC_FP32_EXACT for larger images");
   for (void* p : ctx->d_img_lp) hipFree(p);
   ctx->d_img_lp.assign(ctx->nimg, nullptr);
   for (int i = 0; i < ctx->nimg; ++i) {

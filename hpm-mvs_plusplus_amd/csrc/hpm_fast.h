@@ -11,9 +11,15 @@
 //    (1 + i g6 + j g7), in FP32: small magnitudes keep ~1e-6 px accuracy where
 //    absolute FP32 pixel coordinates carry ~6e-5 px of rounding at 1600 px.
 //  * Texels.  A "quad" image holds, for every texel (x, y) of the (w+1) x (h+1)
-//    grid starting at (-1, -1), the four texels of its bilinear cell with the
-//    reference's border clamp applied; a bilinear tap is one 8-byte load (FP16 /
-//    BF16) or one 16-byte load (FP32).  8-bit images are exact in FP16 and BF16.
+//    grid starting at (-1, -1), its bilinear cell with the reference's border
+//    clamp applied; a bilinear tap is one 8-byte load (FP16 / BF16) or one
+//    16-byte load (FP32).  FP16 and FP32 store the cell in difference form
+//    (t00, t10 - t00, t01 - t00, t11 - t10 - t01 + t00), so that the sample
+//    t00 + ax dx + ay (dy + ax dxy) is three fused multiply-adds; for 8-bit
+//    images every component is an integer of magnitude <= 510, exact in FP16.
+//    BF16 (8-bit mantissa) stores the four texels, exact for 8-bit images.
+//  * Tap addresses: the clamped cell offset (dy, dx) is combined as the float
+//    dy * (w + 1) + dx, exact below 2^24 texels per view (checked on the host).
 //  * Sums use shifted data: the reference window is centred on its centre
 //    texel, the source window on its first sample, so E[x^2] - E[x]^2 never
 //    cancels large magnitudes.  The reference-side statistics are accumulated
@@ -61,44 +67,34 @@ __device__ __forceinline__ float lerp_cell(float t00, float t10, float t01, floa
 struct QuadF16 {
   using Q = uint2;
   static constexpr int kind = 1;  // HPM_PREC_FP16
+  static constexpr int kShift = 3;  // log2 sizeof(Q)
   __device__ __forceinline__ static uint32_t h(float f) {
     const _Float16 x = (_Float16)f;  // round to nearest even
     uint16_t b;
     __builtin_memcpy(&b, &x, 2);
     return b;
   }
+  // difference form: (t00, t10 - t00) in x, (t01 - t00, t11 - t10 - t01 + t00) in y
   __device__ __forceinline__ static Q pack(float a, float b, float c, float d) {
-    return make_uint2(h(a) | (h(b) << 16), h(c) | (h(d) << 16));
+    return make_uint2(h(a) | (h(b - a) << 16), h(c - a) | (h(d - b - c + a) << 16));
   }
   __device__ __forceinline__ static Q load(__amdgpu_buffer_rsrc_t r, unsigned off) {
     const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
     return make_uint2(v[0], v[1]);
   }
-  __device__ __forceinline__ static void unpack(Q q, float& a, float& b, float& c, float& d) {
-    a = half_bits_to_float(q.x & 0xffffu);
-    b = half_bits_to_float(q.x >> 16);
-    c = half_bits_to_float(q.y & 0xffffu);
-    d = half_bits_to_float(q.y >> 16);
-  }
-  // top = t00 (1 - ax) + t10 ax as two fused multiply-adds that read the FP16
-  // texels directly (v_fma_mix_f32): fma(ax, t10, fma(-ax, t00, t00)).
+  // t00 + ax dx + ay (dy + ax dxy): three v_fma_mix_f32 reading the halves directly
   __device__ __forceinline__ static float bilinear(Q q, float ax, float ay) {
-#if HPM_MIX_LERP
     const _Float16* h0 = reinterpret_cast<const _Float16*>(&q.x);
     const _Float16* h1 = reinterpret_cast<const _Float16*>(&q.y);
-    const float top = ffma(ax, (float)h0[1], ffma(-ax, (float)h0[0], (float)h0[0]));
-    const float bot = ffma(ax, (float)h1[1], ffma(-ax, (float)h1[0], (float)h1[0]));
-    return ffma(ay, bot - top, top);
-#else
-    float a, b, c, d;
-    unpack(q, a, b, c, d);
-    return lerp_cell(a, b, c, d, ax, ay);
-#endif
+    const float b = ffma(ax, (float)h0[1], (float)h0[0]);
+    const float a = ffma(ax, (float)h1[1], (float)h1[0]);
+    return ffma(ay, a, b);
   }
 };
 struct QuadBF16 {
   using Q = uint2;
   static constexpr int kind = 2;  // HPM_PREC_BF16
+  static constexpr int kShift = 3;
   __device__ __forceinline__ static uint32_t h(float f) {  // round to nearest even (finite inputs)
     const uint32_t u = __float_as_uint(f);
     return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
@@ -125,15 +121,17 @@ struct QuadBF16 {
 struct QuadF32 {
   using Q = float4;
   static constexpr int kind = 3;  // HPM_PREC_FP32_FAST
-  __device__ __forceinline__ static Q pack(float a, float b, float c, float d) { return make_float4(a, b, c, d); }
+  static constexpr int kShift = 4;
+  __device__ __forceinline__ static Q pack(float a, float b, float c, float d) {
+    return make_float4(a, b - a, c - a, d - b - c + a);  // difference form, as QuadF16
+  }
   __device__ __forceinline__ static Q load(__amdgpu_buffer_rsrc_t r, unsigned off) {
     const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
     return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
   }
-  __device__ __forceinline__ static void unpack(Q q, float& a, float& b, float& c, float& d) {
-    a = q.x; b = q.y; c = q.z; d = q.w;
+  __device__ __forceinline__ static float bilinear(Q q, float ax, float ay) {
+    return ffma(ay, ffma(ax, q.w, q.z), ffma(ax, q.y, q.x));
   }
-  __device__ __forceinline__ static float bilinear(Q q, float ax, float ay) { return lerp_cell(q.x, q.y, q.z, q.w, ax, ay); }
 };
 
 struct FastWin {
@@ -206,27 +204,31 @@ struct FastEngine {
     const float fu = (float)(uc - flu), fv = (float)(vc - flv);
     // clamping the cell offset in float keeps the cell inside [0, w] x [0, h]
     // (the reference's border clamp); a NaN offset converts to 0, also inside.
-    // clamping the cell offset in float keeps the cell inside [0, w] x [0, h]
-    // (the reference's border clamp); a NaN offset converts to 0, also inside.
     const float lox = (float)(-iu), hix = (float)(sw - iu);
     const float loy = (float)(-iv), hiy = (float)(sh - iv);
     const int qp = sw + 1;
+    const float qpf = (float)qp;
+    const unsigned base = (unsigned)(iv * qp + iu);
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<void*>(C.quad[v]), (short)0, qp * (sh + 1) * (int)sizeof(typename S::Q), 0x00020000);
-    // Tap (ii, jj): cell offset and bilinear fractions.
+    // Tap (i, j) in the source cell frame: x = (fu + i cu + j bx) / D,
+    // y = (fv + i cv + j by) / D, D = 1 + i g6 + j g7 (the projective map of
+    // the window offset around the centre, with the centre's fraction folded in).
+    const float cu = ffma(fu, fg6, au), bx = ffma(fu, fg7, bu);
+    const float cv = ffma(fv, fg6, av), by = ffma(fv, fg7, bv);
     auto tap = [&](int ii, int jj, unsigned& off, float& ax, float& ay) {
       const float i = (float)(-5 + 2 * ii), j = (float)(-5 + 2 * jj);
-      const float den = ffma(i, fg6, ffma(j, fg7, 1.0f));
+      const float den = ffma(j, fg7, ffma(i, fg6, 1.0f));
       const float r = __builtin_amdgcn_rcpf(den);
-      const float du = ffma(i, au, j * bu) * r;
-      const float dv = ffma(i, av, j * bv) * r;
-      const float x = fu + du, y = fv + dv;
+      const float x = ffma(j, bx, ffma(i, cu, fu)) * r;
+      const float y = ffma(j, by, ffma(i, cv, fv)) * r;
       const float fx = floorf(x), fy = floorf(y);
       ax = x - fx;
       ay = y - fy;
-      const int tx = iu + (int)__builtin_amdgcn_fmed3f(fx, lox, hix);
-      const int ty = iv + (int)__builtin_amdgcn_fmed3f(fy, loy, hiy);
-      off = (__umul24((unsigned)ty, (unsigned)qp) + (unsigned)tx) * (unsigned)sizeof(typename S::Q);
+      const float cxl = __builtin_amdgcn_fmed3f(fx, lox, hix);
+      const float cyl = __builtin_amdgcn_fmed3f(fy, loy, hiy);
+      const int rel = (int)ffma(cyl, qpf, cxl);  // exact: |rel| < (w + 1) (h + 1) < 2^24
+      off = ((unsigned)rel + base) << S::kShift;
     };
     // Software pipeline over the six columns: the loads of columns
     // ii + 1 .. ii + HPM_PIPE_COLS are in flight while column ii is consumed.
