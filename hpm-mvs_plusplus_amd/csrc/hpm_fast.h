@@ -176,20 +176,29 @@ struct FastEngine {
     fw.var_r = (float)(srr / sw - m * m);
   }
 
+  // 1 / x in double: the hardware estimate refined by two Newton steps
+  // (each doubles the correct bits), in place of the IEEE division sequence.
+  __device__ __forceinline__ static double recip(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = dfma(r, dfma(-x, r, 1.0), r);
+    return dfma(r, dfma(-x, r, 1.0), r);
+  }
+
   HPM_NCC_INLINE static float ncc(const RunConsts& C, int v, int px, int py, float4 pl, const Win& fw) {
-    const double inv_d = 1.0 / (double)pl.w;
+    // m = K_r^-T n / d (per plane), H = A - b m^T (per view pair), in double
+    const double inv_d = recip((double)pl.w);
     const double m0 = (double)pl.x * C.kinv[0] * inv_d;
     const double m1 = (double)pl.y * C.kinv[1] * inv_d;
-    const double m2 = ((double)pl.z - (double)pl.x * C.kinv[2] - (double)pl.y * C.kinv[3]) * inv_d;
+    const double m2 = dfma(-(double)pl.y, C.kinv[3], dfma(-(double)pl.x, C.kinv[2], (double)pl.z)) * inv_d;
     const double* A = C.A[v - 1];
     const double* b = C.bvec[v - 1];
-    const double h0 = A[0] - b[0] * m0, h1 = A[1] - b[0] * m1, h2 = A[2] - b[0] * m2;
-    const double h3 = A[3] - b[1] * m0, h4 = A[4] - b[1] * m1, h5 = A[5] - b[1] * m2;
-    const double h6 = A[6] - b[2] * m0, h7 = A[7] - b[2] * m1, h8 = A[8] - b[2] * m2;
-    const double X = h0 * px + h1 * py + h2;
-    const double Y = h3 * px + h4 * py + h5;
-    const double Z = h6 * px + h7 * py + h8;
-    const double rz = 1.0 / Z;
+    const double h0 = dfma(-b[0], m0, A[0]), h1 = dfma(-b[0], m1, A[1]), h2 = dfma(-b[0], m2, A[2]);
+    const double h3 = dfma(-b[1], m0, A[3]), h4 = dfma(-b[1], m1, A[4]), h5 = dfma(-b[1], m2, A[5]);
+    const double h6 = dfma(-b[2], m0, A[6]), h7 = dfma(-b[2], m1, A[7]), h8 = dfma(-b[2], m2, A[8]);
+    const double X = dfma(h0, px, dfma(h1, py, h2));
+    const double Y = dfma(h3, px, dfma(h4, py, h5));
+    const double Z = dfma(h6, px, dfma(h7, py, h8));
+    const double rz = recip(Z);
     const double uc = X * rz, vc = Y * rz;
     const DevCam& sc = C.cams[v];
     const int sw = sc.width, sh = sc.height;
@@ -197,8 +206,8 @@ struct FastEngine {
     if (fw.var_r < 1e-5f) return 2.0f;
     const double g6 = h6 * rz, g7 = h7 * rz;
     const float fg6 = (float)g6, fg7 = (float)g7;
-    const float au = (float)(h0 * rz - uc * g6), bu = (float)(h1 * rz - uc * g7);
-    const float av = (float)(h3 * rz - vc * g6), bv = (float)(h4 * rz - vc * g7);
+    const float au = (float)dfma(-uc, g6, h0 * rz), bu = (float)dfma(-uc, g7, h1 * rz);
+    const float av = (float)dfma(-vc, g6, h3 * rz), bv = (float)dfma(-vc, g7, h4 * rz);
     const double flu = floor(uc), flv = floor(vc);
     const int iu = (int)flu + 1, iv = (int)flv + 1;  // +1: the quad grid starts at (-1, -1)
     const float fu = (float)(uc - flu), fv = (float)(vc - flv);
