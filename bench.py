@@ -5,11 +5,14 @@ plane -> depth/normal output), inputs resident in HBM.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp16|bf16|fp32fast|fp32]
 
-One "step" is one RunPatchMatch of one reference view.  The headline `value`
-is the mixed-precision path (FP16 texel storage, FP32 accumulation by
-default); the plain FP32 path (bit-exact with the FP32 oracle) is timed beside
-it on the same view and reported as `plain_fp32`, together with the fraction of
-pixels whose depth agrees with it within 1 %.  Prints one JSON line.
+One "step" is one batched RunPatchMatch of --batch reference views (default 4:
+views 3-6 of an 11-camera scene, each with its 10 source views), issued
+through hpm_run_patchmatch_batch with hipGraph replay.  The headline `value`
+is views/s of that batched mixed-precision path (FP16 texel storage, FP32
+accumulation by default).  Beside it: the same path one view at a time
+(`single_view`), and the plain FP32 path (bit-exact with the FP32 oracle, one
+view at a time, no graph) as `plain_fp32`, with the fraction of pixels whose
+depth agrees with it within 1 %.  Prints one JSON line.
 
 The roofline object prices the dominant kernel (the red/black propagation
 kernel) by its algorithmic bytes: 181 texels per NCC evaluation (37 reference
@@ -47,6 +50,15 @@ def make_workload(width=W_FULL, height=H_FULL, nsrc=NSRC):
     ref = nsrc // 2
     ids, cams, imgs = scenes.reference_view_problem(sc, ref, nsrc)
     return sc, ids, cams, imgs
+
+
+def batch_views(sc, batch, nsrc=NSRC):
+    """Reference views of the batched run: `batch` central cameras, each with
+    its nsrc best sources."""
+    from hpm_mi355x import scenes
+    n = len(sc.cameras)
+    first = max(0, n // 2 - batch // 2)
+    return [scenes.reference_view_problem(sc, r, nsrc) for r in range(first, first + batch)]
 
 
 def canny_like(img):
@@ -132,6 +144,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=4, help="reference views per batched step")
     ap.add_argument("--precision", default="fp16", choices=["fp16", "bf16", "fp32fast", "fp32"])
     ap.add_argument("--no-plain-fp32", action="store_true", help="skip timing the plain FP32 path beside it")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -157,18 +170,45 @@ def main():
     ctx.set_canny(canny)
     print(f"[bench] workload ready in {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
 
+    # single view, HIP-event timing of each kernel class (roofline, profile)
     r = run_views(ctx, args.steps, args.warmup, SEED)
     planes = ctx.planes()
     gt = sc.depths[ids[0]]
     within1 = float(np.mean(np.abs(planes[..., 3] - gt) / gt <= 0.01))
-    print(f"[bench] {args.precision}: {r['views_s']:.2f} views/s", file=sys.stderr, flush=True)
+    print(f"[bench] {args.precision} single view: {r['views_s']:.2f} views/s", file=sys.stderr, flush=True)
+
+    # the headline: batched views with hipGraph replay
+    batch = []
+    for (bids, bcams, bimgs) in batch_views(sc, args.batch):
+        c = Hm.Context(0)
+        c.set_params(Hm.default_params(precision=args.precision))
+        c.set_views([Hm.camera_from_scene(x) for x in bcams], bimgs)
+        c.set_canny(canny_like(bimgs[0]))
+        c.set_graph(True)
+        batch.append((bids, c))
+    ctxs = [c for _, c in batch]
+    for i in range(args.warmup):
+        Hm.run_batch(ctxs, [SEED + 10 * i + k for k in range(len(ctxs))])
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        Hm.run_batch(ctxs, [SEED + 1000 + 10 * k + j for j in range(len(ctxs))])
+    elapsed_b = time.perf_counter() - t0
+    views_s_b = args.steps * len(ctxs) / elapsed_b
+    within_b = []
+    for bids, c in batch:
+        d = c.planes()[..., 3]
+        g = sc.depths[bids[0]]
+        within_b.append(float(np.mean(np.abs(d - g) / g <= 0.01)))
+    for c in ctxs:
+        c.close()
+    print(f"[bench] {args.precision} batch of {len(ctxs)}: {views_s_b:.2f} views/s", file=sys.stderr, flush=True)
 
     plain = None
     if not args.no_plain_fp32 and args.precision != "fp32":
         ctx.set_params(Hm.default_params(precision="fp32"))
         rp = run_views(ctx, max(2, args.steps // 2), 1, SEED)
         pp = ctx.planes()
-        # same seed as the last timed step of the headline run
+        # same seed as the last timed step of the single-view run
         ctx.set_params(Hm.default_params(precision=args.precision))
         ctx.run(SEED + 100 + args.steps - 1)
         pf = ctx.planes()
@@ -179,7 +219,8 @@ def main():
         plain = {"views_s": round(rp["views_s"], 4), "ms_per_step": round(1e3 * rp["elapsed"] / max(2, args.steps // 2), 3),
                  "k_propagate_avg_ms": round(rp["prop_ms"] / max(rp["launches"], 1), 3),
                  "depth_within_1pct_of_gt": round(float(np.mean(np.abs(pp[..., 3] - gt) / gt <= 0.01)), 4),
-                 "speedup": round(r["views_s"] / rp["views_s"], 3),
+                 "speedup_batched": round(views_s_b / rp["views_s"], 3),
+                 "speedup_single_view": round(r["views_s"] / rp["views_s"], 3),
                  "depth_within_1pct_of_plain_fp32": round(agree, 4)}
         print(f"[bench] plain fp32: {rp['views_s']:.2f} views/s", file=sys.stderr, flush=True)
 
@@ -194,12 +235,12 @@ def main():
 
     out = {
         "metric": "views/s @1600x1200, 10 src: plain FP32 vs FP16+batched, at <=1% depth err",
-        "value": round(r["views_s"], 4),
+        "value": round(views_s_b, 4),
         "unit": "views/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * r["elapsed"] / args.steps, 3),
+        "ms_per_step": round(1e3 * elapsed_b / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -207,16 +248,18 @@ def main():
         "data": "synthetic (seeded ray-cast textured planes, 11 cameras on a 30 deg arc)",
         "config": {"workload": "RunPatchMatch photometric, 1600x1200 reference + 10 source views, 3 iterations",
                    "precision": args.precision, "width": W_FULL, "height": H_FULL, "sources": NSRC,
-                   "views_per_launch": 1},
+                   "views_per_step": len(ctxs), "batched": True, "hipgraph": True},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if traffic is None else traffic["bytes_per_launch"],
                      "kernel": "k_propagate", "avg_launch_ms": round(avg_launch_s * 1e3, 3),
                      "alg_bytes_per_launch": int(alg_bytes), "ncc_evals_per_launch": int(per_launch_evals)},
+        "single_view": {"views_s": round(r["views_s"], 4), "ms_per_step": round(1e3 * r["elapsed"] / args.steps, 3)},
         "plain_fp32": plain,
         "mpix_hyp_view_per_s": round(r["ncc_prop"] / (r["prop_ms"] / 1e3) / 1e6, 1) if r["prop_ms"] > 0 else None,
         "mpix_hyp_view_per_s_reference_count": round(r["ncc_ref"] / r["elapsed"] / 1e6, 1),
         "depth_within_1pct_of_gt": round(within1, 4),
+        "depth_within_1pct_of_gt_batch": [round(x, 4) for x in within_b],
     }
     ctx.close()
     if not args.no_cpu_baseline:

@@ -52,6 +52,12 @@ struct hpm_ctx {
   bool timing = false;
   hipEvent_t ev[8] = {};
   hpm_stats stats{};
+  // hipGraph of the kernel sequence of one RunPatchMatch, replayed while the
+  // kernel arguments it was captured with stay valid
+  bool use_graph = false;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<unsigned char> gkey;  // what the captured arguments depend on
+  int g_launches = 0;
 };
 
 namespace {
@@ -236,9 +242,6 @@ template <class E> int launch_schedule(hpm_ctx* ctx, int n_halfpasses, int do_ou
   hipStream_t st = ctx->stream;
   const int V = ctx->nimg - 1;
   const size_t lds_bytes = (size_t)8 * V * kRowStride * sizeof(float);
-  if (V <= kLdsMaxSrc)  // the rows need up to 160 KiB of dynamic LDS
-    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_propagate<E, true>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
   if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[0], st));
   hipLaunchKernelGGL(k_init<E>, gi, b256, 0, st, ctx->d_consts, S);
   HIP_OK(hipGetLastError());
@@ -262,6 +265,55 @@ template <class E> int launch_schedule(hpm_ctx* ctx, int n_halfpasses, int do_ou
   }
   if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[3], st));
   ctx->stats.propagate_launches = done;
+  return HPM_OK;
+}
+
+// Per-kernel attributes the schedule needs (set outside any stream capture).
+template <class E> int prepare_schedule(hpm_ctx* ctx) {
+  const int V = ctx->nimg - 1;
+  const size_t lds_bytes = (size_t)8 * V * kRowStride * sizeof(float);
+  if (V <= kLdsMaxSrc)  // the rows need up to 160 KiB of dynamic LDS
+    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_propagate<E, true>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
+  return HPM_OK;
+}
+
+// Runs the schedule directly, or through the context's hipGraph: captured on
+// first use (and whenever the kernel arguments would change: buffers, sizes,
+// modes, precision, partial schedule), then replayed.  The per-run values
+// (seed, thresholds) live in the RunConsts buffer in device memory, written on
+// the stream before the launch, so one graph serves every run.
+template <class E> int run_schedule(hpm_ctx* ctx, int n_halfpasses, int do_output) {
+  int rc = prepare_schedule<E>(ctx);
+  if (rc) return rc;
+  if (!ctx->use_graph || ctx->timing) return launch_schedule<E>(ctx, n_halfpasses, do_output);
+  struct Key {
+    DevState S;
+    int W, H, nimg, precision, geom, prior, hier, mand, iters, nh, out;
+  } k;
+  std::memset(&k, 0, sizeof(k));
+  k.S = make_state(ctx);
+  k.W = ctx->W; k.H = ctx->H; k.nimg = ctx->nimg; k.precision = ctx->params.precision;
+  k.geom = ctx->params.geom_consistency; k.prior = ctx->params.prior_consistency;
+  k.hier = ctx->params.hierarchy; k.mand = ctx->params.mand_consistency;
+  k.iters = ctx->params.num_iterations; k.nh = n_halfpasses; k.out = do_output;
+  const unsigned char* kb = reinterpret_cast<const unsigned char*>(&k);
+  if (!ctx->gexec || ctx->gkey.size() != sizeof(k) || std::memcmp(ctx->gkey.data(), kb, sizeof(k)) != 0) {
+    if (ctx->gexec) { HIP_OK(hipGraphExecDestroy(ctx->gexec)); ctx->gexec = nullptr; }
+    hipGraph_t g = nullptr;
+    HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    rc = launch_schedule<E>(ctx, n_halfpasses, do_output);
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) { if (g) hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fail(ctx, HPM_E_DEVICE, std::string("stream capture: ") + hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ei != hipSuccess) { ctx->gexec = nullptr; return fail(ctx, HPM_E_DEVICE, "hipGraphInstantiate failed"); }
+    ctx->gkey.assign(kb, kb + sizeof(k));
+    ctx->g_launches = ctx->stats.propagate_launches;
+  }
+  ctx->stats.propagate_launches = ctx->g_launches;
+  HIP_OK(hipGraphLaunch(ctx->gexec, ctx->stream));
   return HPM_OK;
 }
 
@@ -315,6 +367,7 @@ void hpm_destroy(hpm_ctx* ctx) {
   if (!ctx) return;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
+  if (ctx->gexec) hipGraphExecDestroy(ctx->gexec);
   free_views(ctx);
   hipFree(ctx->d_consts);
   hipFree(ctx->d_counters);
@@ -470,11 +523,32 @@ int hpm_run_patchmatch_partial(hpm_ctx* ctx, uint64_t seed, int n_halfpasses, in
   HIP_OK(hipMemcpyAsync(ctx->d_consts, &C, sizeof(C), hipMemcpyHostToDevice, ctx->stream));
   HIP_OK(hipMemsetAsync(ctx->d_counters, 0, 8 * sizeof(unsigned long long), ctx->stream));
   switch (P.precision) {
-    case HPM_PREC_FP16: return launch_schedule<FastFor<QuadF16>>(ctx, n_halfpasses, do_output);
-    case HPM_PREC_BF16: return launch_schedule<FastFor<QuadBF16>>(ctx, n_halfpasses, do_output);
-    case HPM_PREC_FP32_FAST: return launch_schedule<FastFor<QuadF32>>(ctx, n_halfpasses, do_output);
-    default: return launch_schedule<ExactEngine>(ctx, n_halfpasses, do_output);
+    case HPM_PREC_FP16: return run_schedule<FastFor<QuadF16>>(ctx, n_halfpasses, do_output);
+    case HPM_PREC_BF16: return run_schedule<FastFor<QuadBF16>>(ctx, n_halfpasses, do_output);
+    case HPM_PREC_FP32_FAST: return run_schedule<FastFor<QuadF32>>(ctx, n_halfpasses, do_output);
+    default: return run_schedule<ExactEngine>(ctx, n_halfpasses, do_output);
   }
+}
+
+int hpm_set_graph(hpm_ctx* ctx, int enable) {
+  if (!ctx) return HPM_E_INVALID;
+  ctx->use_graph = enable != 0;
+  return HPM_OK;
+}
+
+int hpm_run_patchmatch_batch(hpm_ctx* const* ctxs, int n, const uint64_t* seeds) {
+  if (!ctxs || !seeds || n < 0) return HPM_E_INVALID;
+  for (int i = 0; i < n; ++i)
+    if (!ctxs[i]) return HPM_E_INVALID;
+  for (int i = 0; i < n; ++i) {
+    const int rc = hpm_run_patchmatch_partial(ctxs[i], seeds[i], -1, 1);
+    if (rc) return rc;
+  }
+  for (int i = 0; i < n; ++i) {
+    const int rc = hpm_synchronize(ctxs[i]);
+    if (rc) return rc;
+  }
+  return HPM_OK;
 }
 
 int hpm_run_patchmatch(hpm_ctx* ctx, uint64_t seed) { return hpm_run_patchmatch_partial(ctx, seed, -1, 1); }

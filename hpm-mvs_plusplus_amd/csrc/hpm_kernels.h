@@ -44,14 +44,14 @@ template <class E> struct Tiles {
   static constexpr int px_per_block = 256 / E::kLanes;
   static constexpr int sq = E::kLanes == 1 ? 16 : 8;          // 2D tile side
 #ifndef HPM_COLOUR_TILE_W
-#define HPM_COLOUR_TILE_W 16
+#define HPM_COLOUR_TILE_W 32
 #endif
   static constexpr int cw = E::kLanes == 1 ? HPM_COLOUR_TILE_W : 8;  // colour tile width (same-colour columns)
   static constexpr int ch = px_per_block / cw;                // colour tile height
 };
 
 #ifndef HPM_XCD_STRIP
-#define HPM_XCD_STRIP 4
+#define HPM_XCD_STRIP 2
 #endif
 // Workgroup -> tile, XCD-aware.  The dispatcher deals workgroups round-robin to
 // the 8 XCDs (XCD = id % 8), each with its own 4 MB L2.  Every XCD gets one

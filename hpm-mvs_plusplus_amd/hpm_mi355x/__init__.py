@@ -57,7 +57,8 @@ EXPORTS = [
     "hpm_set_coarse", "hpm_run_patchmatch", "hpm_run_patchmatch_partial", "hpm_synchronize", "hpm_get_planes",
     "hpm_get_costs", "hpm_get_texture", "hpm_get_selected_views", "hpm_get_confidence", "hpm_get_size",
     "hpm_get_stats", "hpm_set_timing", "hpm_cost_fixed", "hpm_cost_fixed_device", "hpm_init_cost",
-    "hpm_geom_cost", "hpm_jbu_depth", "hpm_jbu_prior", "hpm_consistency_check",
+    "hpm_geom_cost", "hpm_jbu_depth", "hpm_jbu_prior", "hpm_consistency_check", "hpm_set_graph",
+    "hpm_run_patchmatch_batch",
 ]
 
 _lib = None
@@ -92,6 +93,8 @@ def lib():
     L.hpm_run_patchmatch.argtypes = [P, ctypes.c_uint64]
     L.hpm_run_patchmatch_partial.argtypes = [P, ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
     L.hpm_synchronize.argtypes = [P]
+    L.hpm_set_graph.argtypes = [P, ctypes.c_int]
+    L.hpm_run_patchmatch_batch.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
     for n in ("hpm_get_planes", "hpm_get_costs", "hpm_get_texture", "hpm_get_confidence"):
         getattr(L, n).argtypes = [P, fp]
     L.hpm_get_selected_views.argtypes = [P, up]
@@ -240,6 +243,9 @@ class Context:
     def synchronize(self):
         self._check(self._L.hpm_synchronize(self._h), "hpm_synchronize")
 
+    def set_graph(self, on=True):
+        self._check(self._L.hpm_set_graph(self._h, int(on)), "hpm_set_graph")
+
     def set_timing(self, on=True):
         self._check(self._L.hpm_set_timing(self._h, int(on)), "hpm_set_timing")
 
@@ -363,4 +369,15 @@ def consistency_check(mode, cameras, depths, normals, sources, capacity=None):
         raise HpmError(f"hpm_consistency_check failed ({rc})")
     k = min(count.value, cap)
     return pairs[:k], pts[:k]
+
+
+def run_batch(contexts, seeds):
+    """hpm_run_patchmatch_batch: RunPatchMatch on several contexts at once."""
+    n = len(contexts)
+    hs = (ctypes.c_void_p * n)(*[c.handle for c in contexts])
+    sd = (ctypes.c_uint64 * n)(*[int(s) for s in seeds])
+    rc = lib().hpm_run_patchmatch_batch(hs, n, sd)
+    if rc != HPM_OK:
+        msgs = [lib().hpm_last_error(c.handle) for c in contexts]
+        raise HpmError(f"hpm_run_patchmatch_batch failed ({rc}): " + "; ".join(m.decode() for m in msgs if m))
 

@@ -127,6 +127,19 @@ int hpm_run_patchmatch(hpm_ctx* ctx, uint64_t seed);
  * passes (-1 = all) and the output conversion if do_output != 0. */
 int hpm_run_patchmatch_partial(hpm_ctx* ctx, uint64_t seed, int n_halfpasses, int do_output);
 
+/* Enables (1) or disables (0) hipGraph replay: the kernel sequence of
+ * RunPatchMatch is captured into a graph on the first run and replayed on later
+ * runs until a buffer, size, mode or precision changes (then recaptured).
+ * No counterpart in the reference, which launches and synchronises each kernel
+ * (HPM.cu:3015-3027).  Timing (hpm_set_timing) runs without the graph. */
+int hpm_set_graph(hpm_ctx* ctx, int enable);
+
+/* Batched RunPatchMatch (HPM.cu:2987-3035 for n reference views): every
+ * context's schedule is issued on its own stream, so that views of the same or
+ * different sizes run concurrently on the GPU; returns when all are done.
+ * seeds[i] keys context i's random streams. */
+int hpm_run_patchmatch_batch(hpm_ctx* const* ctxs, int n, const uint64_t* seeds);
+
 /* Waits for the context stream. */
 int hpm_synchronize(hpm_ctx* ctx);
 

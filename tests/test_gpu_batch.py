@@ -1,0 +1,88 @@
+"""Batched RunPatchMatch and hipGraph replay (needs an MI355X).
+
+* hipGraph replay: a context with graphs enabled gives, run after run, the
+  same bits as a context launching the kernels directly -- with other work on
+  the GPU between replays, after a mode/precision change (recapture), and on
+  both the exact and the mixed-precision engine.
+* hpm_run_patchmatch_batch on contexts of different sizes and view counts
+  gives the same bits as running each context alone.
+"""
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="no HIP device")]
+
+
+def _ctx(cams, imgs, graph=False, **params):
+    import hpm_mi355x as H
+    ctx = H.Context(0)
+    ctx.set_params(H.default_params(**params))
+    ctx.set_views([H.camera_from_scene(c) for c in cams], imgs)
+    ctx.set_graph(graph)
+    return ctx
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16"])
+def test_graph_replay_matches_direct_launches(small_scene, prec):
+    import hpm_mi355x as H
+    from hpm_mi355x import scenes
+    sc, ids, cams, imgs = small_scene
+    print(f"case: graph replay {prec} 160x120 V=4", flush=True)
+    direct = _ctx(cams, imgs, graph=False, precision=prec)
+    graphed = _ctx(cams, imgs, graph=True, precision=prec)
+    other = scenes.random_planes_per_pixel(cams[0], seed=9)
+    for k, seed in enumerate([3, 4, 5, 6]):
+        direct.run(seed)
+        graphed.run(seed)
+        np.testing.assert_array_equal(graphed.planes(), direct.planes())
+        np.testing.assert_array_equal(graphed.costs(), direct.costs())
+        # unrelated GPU work between replays
+        graphed.cost_fixed(other, precision=H.PRECISIONS[prec])
+        if k == 1:  # partial schedule and back: recapture twice
+            direct.run(seed, n_halfpasses=3)
+            graphed.run(seed, n_halfpasses=3)
+            np.testing.assert_array_equal(graphed.planes(), direct.planes())
+
+
+def test_graph_recaptures_on_mode_change(small_scene):
+    import hpm_mi355x as H
+    sc, ids, cams, imgs = small_scene
+    direct = _ctx(cams, imgs, graph=False)
+    graphed = _ctx(cams, imgs, graph=True)
+    direct.run(1)
+    graphed.run(1)
+    base_p, base_c = direct.planes(), direct.costs()
+    for prec, geom in [("fp16", 0), ("fp32", 1), ("fp32", 0)]:
+        for c in (direct, graphed):
+            c.set_params(H.default_params(precision=prec, geom_consistency=geom))
+            if geom:
+                c.set_src_depths([None] + [sc.depths[i] for i in ids[1:]])
+                c.reload_hypotheses(base_p, base_c)
+            c.run(7)
+        np.testing.assert_array_equal(graphed.planes(), direct.planes())
+
+
+def test_batch_matches_single_runs():
+    import hpm_mi355x as H
+    from hpm_mi355x import scenes
+    print("case: batch of 3 contexts (160x120 V=4, 128x96 V=6, 96x72 V=3)", flush=True)
+    probs = []
+    for (w, h, f, n, ref, seed) in [(160, 120, 131.25, 5, 2, 1), (128, 96, 105.0, 7, 3, 3), (96, 72, 78.75, 4, 1, 2)]:
+        sc = scenes.make_plane_scene(seed=seed, n_cams=n, width=w, height=h, f=f, n_src=n - 1)
+        probs.append(scenes.reference_view_problem(sc, ref, n - 1))
+    batch = [_ctx(c, i, precision="fp16") for (_, c, i) in probs]
+    single = [_ctx(c, i, precision="fp16") for (_, c, i) in probs]
+    seeds = [11, 12, 13]
+    H.run_batch(batch, seeds)
+    for s, c, sd in zip(single, batch, seeds):
+        s.run(sd)
+        np.testing.assert_array_equal(c.planes(), s.planes())
+        np.testing.assert_array_equal(c.costs(), s.costs())
+    for c in batch:
+        c.set_graph(True)
+    H.run_batch(batch, seeds)
+    H.run_batch(batch, seeds)
+    for s, c in zip(single, batch):
+        np.testing.assert_array_equal(c.planes(), s.planes())
