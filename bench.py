@@ -6,8 +6,9 @@ plane -> depth/normal output), inputs resident in HBM.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp16|bf16|fp32fast|fp32]
 
 One "step" is one batched RunPatchMatch of --batch reference views (default 4:
-views 3-6 of an 11-camera scene, each with its 10 source views), issued
-through hpm_run_patchmatch_batch with hipGraph replay.  The headline `value`
+views 3-6 of an 11-camera scene, each with its 10 source views, all reading
+one shared image pool), issued through hpm_run_patchmatch_batch with hipGraph
+replay.  The headline `value`
 is views/s of that batched mixed-precision path (FP16 texel storage, FP32
 accumulation by default).  Beside it: the same path one view at a time
 (`single_view`), and the plain FP32 path (bit-exact with the FP32 oracle, one
@@ -178,11 +179,12 @@ def main():
     print(f"[bench] {args.precision} single view: {r['views_s']:.2f} views/s", file=sys.stderr, flush=True)
 
     # the headline: batched views with hipGraph replay
+    pool = Hm.ImagePool([Hm.camera_from_scene(c) for c in sc.cameras], [im.astype(np.float32) for im in sc.images])
     batch = []
     for (bids, bcams, bimgs) in batch_views(sc, args.batch):
         c = Hm.Context(0)
         c.set_params(Hm.default_params(precision=args.precision))
-        c.set_views([Hm.camera_from_scene(x) for x in bcams], bimgs)
+        c.set_views_pooled(pool, bids)
         c.set_canny(canny_like(bimgs[0]))
         c.set_graph(True)
         batch.append((bids, c))
@@ -201,6 +203,7 @@ def main():
         within_b.append(float(np.mean(np.abs(d - g) / g <= 0.01)))
     for c in ctxs:
         c.close()
+    pool.close()
     print(f"[bench] {args.precision} batch of {len(ctxs)}: {views_s_b:.2f} views/s", file=sys.stderr, flush=True)
 
     plain = None

@@ -24,8 +24,20 @@ using namespace hpm;
 #endif
 template <class S> using FastFor = typename std::conditional<HPM_FAST_LANES == 4, QuadLaneEngine<S>, FastEngine<S>>::type;
 
+// Images of a whole scene on one device, shared by the contexts of its
+// reference views: FP32 images plus, per storage precision, the quad images of
+// the mixed-precision engine (built on first use).
+struct hpm_pool {
+  int device = 0;
+  std::vector<hpm_camera> cams;
+  std::vector<float*> d_img;
+  std::vector<void*> quads[4];  // indexed by HPM_PREC_* (0 unused)
+};
+
 struct hpm_ctx {
   int device = 0;
+  hpm_pool* pool = nullptr;        // images borrowed from a pool (not owned)
+  std::vector<int> pool_ids;
   hipStream_t stream = nullptr;
   std::string err;
   hpm_params params{};
@@ -82,9 +94,13 @@ template <class T> int dalloc(hpm_ctx* ctx, T** p, size_t n) {
 }
 
 void free_views(hpm_ctx* c) {
-  for (float* p : c->d_img) hipFree(p);
+  if (!c->pool) {
+    for (float* p : c->d_img) hipFree(p);
+    for (void* p : c->d_img_lp) hipFree(p);
+  }
+  c->pool = nullptr;
+  c->pool_ids.clear();
   for (float* p : c->d_dimg) hipFree(p);
-  for (void* p : c->d_img_lp) hipFree(p);
   c->d_img.clear(); c->d_dimg.clear(); c->d_img_lp.clear(); c->lp_kind = -1;
   void* bufs[] = {c->d_planes, c->d_costs, c->d_pre_costs, c->d_conf, c->d_texture, c->d_sel,
                   c->d_masks,  c->d_canny, c->d_prior,     c->d_scaled};
@@ -194,8 +210,43 @@ template <class S> int build_quads(hpm_ctx* ctx) {
   return HPM_OK;
 }
 
+template <class S> int build_pool_quads(hpm_ctx* ctx) {
+  hpm_pool* pool = ctx->pool;
+  std::vector<void*>& q = pool->quads[S::kind];
+  if (q.size() != pool->d_img.size()) {
+    for (size_t i = 0; i < pool->cams.size(); ++i)
+      if ((size_t)(pool->cams[i].width + 1) * (pool->cams[i].height + 1) >= ((size_t)1 << 24))
+        return fail(ctx, HPM_E_INVALID, "the mixed-precision path takes views below 2^24 texels");
+    q.assign(pool->d_img.size(), nullptr);
+    for (size_t i = 0; i < pool->d_img.size(); ++i) {
+      const int w = pool->cams[i].width, h = pool->cams[i].height;
+      if (hipMalloc(&q[i], (size_t)(w + 1) * (h + 1) * sizeof(typename S::Q)) != hipSuccess) {
+        q[i] = nullptr;
+        return fail(ctx, HPM_E_NOMEM, "quad image allocation failed");
+      }
+      const dim3 b(16, 16), g((w + 1 + 15) / 16, (h + 1 + 15) / 16);
+      hipLaunchKernelGGL(k_build_quad<S>, g, b, 0, ctx->stream, pool->d_img[i], w, h, q[i]);
+      if (hipGetLastError() != hipSuccess) return fail(ctx, HPM_E_DEVICE, "k_build_quad launch failed");
+    }
+    // other contexts of the pool may run on other streams
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+  }
+  ctx->d_img_lp.assign(ctx->nimg, nullptr);
+  for (int i = 0; i < ctx->nimg; ++i) ctx->d_img_lp[i] = q[ctx->pool_ids[i]];
+  ctx->lp_kind = S::kind;
+  return HPM_OK;
+}
+
 int ensure_quads(hpm_ctx* ctx, int precision) {
   if (ctx->lp_kind == precision && (int)ctx->d_img_lp.size() == ctx->nimg) return HPM_OK;
+  if (ctx->pool) {
+    switch (precision) {
+      case HPM_PREC_FP16: return build_pool_quads<QuadF16>(ctx);
+      case HPM_PREC_BF16: return build_pool_quads<QuadBF16>(ctx);
+      case HPM_PREC_FP32_FAST: return build_pool_quads<QuadF32>(ctx);
+      default: return fail(ctx, HPM_E_INVALID, "unknown precision");
+    }
+  }
   switch (precision) {
     case HPM_PREC_FP16: return build_quads<QuadF16>(ctx);
     case HPM_PREC_BF16: return build_quads<QuadBF16>(ctx);
@@ -386,6 +437,8 @@ int hpm_set_params(hpm_ctx* ctx, const hpm_params* params) {
   return HPM_OK;
 }
 
+static int alloc_pixel_state(hpm_ctx* ctx);
+
 int hpm_set_views(hpm_ctx* ctx, int num_images, const hpm_camera* cameras, const float* const* images) {
   if (!ctx) return HPM_E_INVALID;
   if (!cameras || !images) return fail(ctx, HPM_E_INVALID, "null cameras or images");
@@ -412,6 +465,76 @@ int hpm_set_views(hpm_ctx* ctx, int num_images, const hpm_camera* cameras, const
     if ((rc = dalloc(ctx, &ctx->d_img[i], n))) return rc;
     HIP_OK(hipMemcpyAsync(ctx->d_img[i], images[i], n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
   }
+  return alloc_pixel_state(ctx);
+}
+
+int hpm_pool_create(hpm_pool** out, int device, int num_images, const hpm_camera* cameras,
+                    const float* const* images) {
+  if (!out || !cameras || !images || num_images < 1) return HPM_E_INVALID;
+  *out = nullptr;
+  for (int i = 0; i < num_images; ++i)
+    if (!images[i] || cameras[i].width <= 0 || cameras[i].height <= 0 ||
+        (size_t)cameras[i].width * cameras[i].height > (1u << 28))
+      return HPM_E_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return HPM_E_DEVICE;
+  hpm_pool* pool = new hpm_pool();
+  pool->device = device;
+  pool->cams.assign(cameras, cameras + num_images);
+  pool->d_img.assign(num_images, nullptr);
+  for (int i = 0; i < num_images; ++i) {
+    const size_t n = (size_t)cameras[i].width * cameras[i].height;
+    if (hipMalloc((void**)&pool->d_img[i], n * sizeof(float)) != hipSuccess) {
+      hpm_pool_destroy(pool);
+      return HPM_E_NOMEM;
+    }
+    if (hipMemcpy(pool->d_img[i], images[i], n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+      hpm_pool_destroy(pool);
+      return HPM_E_DEVICE;
+    }
+  }
+  *out = pool;
+  return HPM_OK;
+}
+
+void hpm_pool_destroy(hpm_pool* pool) {
+  if (!pool) return;
+  hipSetDevice(pool->device);
+  hipDeviceSynchronize();
+  for (float* p : pool->d_img) hipFree(p);
+  for (auto& q : pool->quads)
+    for (void* p : q) hipFree(p);
+  delete pool;
+}
+
+int hpm_set_views_pooled(hpm_ctx* ctx, hpm_pool* pool, int num_images, const int32_t* ids) {
+  if (!ctx) return HPM_E_INVALID;
+  if (!pool || !ids) return fail(ctx, HPM_E_INVALID, "null pool or ids");
+  if (pool->device != ctx->device) return fail(ctx, HPM_E_INVALID, "pool and context are on different devices");
+  if (num_images < 2 || num_images > HPM_MAX_IMAGES)
+    return fail(ctx, HPM_E_INVALID, "num_images must be in [2, 33] (view masks are 32-bit)");
+  for (int i = 0; i < num_images; ++i)
+    if (ids[i] < 0 || ids[i] >= (int)pool->d_img.size()) return fail(ctx, HPM_E_INVALID, "image id outside the pool");
+  const hpm_camera& rc0 = pool->cams[ids[0]];
+  if (rc0.width < 16 || rc0.height < 16) return fail(ctx, HPM_E_INVALID, "reference image must be at least 16x16");
+  HIP_OK(hipSetDevice(ctx->device));
+  HIP_OK(hipStreamSynchronize(ctx->stream));
+  free_views(ctx);
+  ctx->pool = pool;
+  ctx->pool_ids.assign(ids, ids + num_images);
+  ctx->nimg = num_images;
+  ctx->cams.clear();
+  ctx->d_img.clear();
+  for (int i = 0; i < num_images; ++i) {
+    ctx->cams.push_back(pool->cams[ids[i]]);
+    ctx->d_img.push_back(pool->d_img[ids[i]]);
+  }
+  ctx->W = rc0.width;
+  ctx->H = rc0.height;
+  return alloc_pixel_state(ctx);
+}
+
+static int alloc_pixel_state(hpm_ctx* ctx) {
+  int rc;
   const size_t P = npix(ctx);
   if ((rc = dalloc(ctx, &ctx->d_planes, P)) || (rc = dalloc(ctx, &ctx->d_costs, P)) ||
       (rc = dalloc(ctx, &ctx->d_pre_costs, P)) || (rc = dalloc(ctx, &ctx->d_conf, P)) ||

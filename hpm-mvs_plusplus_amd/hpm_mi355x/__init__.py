@@ -58,7 +58,7 @@ EXPORTS = [
     "hpm_get_costs", "hpm_get_texture", "hpm_get_selected_views", "hpm_get_confidence", "hpm_get_size",
     "hpm_get_stats", "hpm_set_timing", "hpm_cost_fixed", "hpm_cost_fixed_device", "hpm_init_cost",
     "hpm_geom_cost", "hpm_jbu_depth", "hpm_jbu_prior", "hpm_consistency_check", "hpm_set_graph",
-    "hpm_run_patchmatch_batch",
+    "hpm_run_patchmatch_batch", "hpm_pool_create", "hpm_pool_destroy", "hpm_set_views_pooled",
 ]
 
 _lib = None
@@ -94,6 +94,11 @@ def lib():
     L.hpm_run_patchmatch_partial.argtypes = [P, ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
     L.hpm_synchronize.argtypes = [P]
     L.hpm_set_graph.argtypes = [P, ctypes.c_int]
+    L.hpm_pool_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.c_int, ctypes.POINTER(hpm_camera),
+                                  ctypes.POINTER(fp)]
+    L.hpm_pool_destroy.argtypes = [P]
+    L.hpm_pool_destroy.restype = None
+    L.hpm_set_views_pooled.argtypes = [P, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]
     L.hpm_run_patchmatch_batch.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
     for n in ("hpm_get_planes", "hpm_get_costs", "hpm_get_texture", "hpm_get_confidence"):
         getattr(L, n).argtypes = [P, fp]
@@ -243,6 +248,14 @@ class Context:
     def synchronize(self):
         self._check(self._L.hpm_synchronize(self._h), "hpm_synchronize")
 
+    def set_views_pooled(self, pool, ids):
+        """hpm_set_views on images of an ImagePool (ids[0] = reference)."""
+        arr = (ctypes.c_int32 * len(ids))(*[int(i) for i in ids])
+        self._check(self._L.hpm_set_views_pooled(self._h, pool.handle, len(ids), arr), "hpm_set_views_pooled")
+        self._pool = pool  # keeps the pool alive while this context uses it
+        self.width, self.height = pool.sizes[ids[0]]
+        self.num_images = len(ids)
+
     def set_graph(self, on=True):
         self._check(self._L.hpm_set_graph(self._h, int(on)), "hpm_set_graph")
 
@@ -380,4 +393,37 @@ def run_batch(contexts, seeds):
     if rc != HPM_OK:
         msgs = [lib().hpm_last_error(c.handle) for c in contexts]
         raise HpmError(f"hpm_run_patchmatch_batch failed ({rc}): " + "; ".join(m.decode() for m in msgs if m))
+
+
+class ImagePool:
+    """hpm_pool: a scene's images on one device, shared by several contexts."""
+
+    def __init__(self, cameras, images, device=0):
+        L = lib()
+        n = len(cameras)
+        cams = (hpm_camera * n)(*cameras)
+        self._imgs = [np.ascontiguousarray(i, np.float32) for i in images]
+        ptrs = (ctypes.POINTER(ctypes.c_float) * n)(*[_fptr(i) for i in self._imgs])
+        h = ctypes.c_void_p()
+        rc = L.hpm_pool_create(ctypes.byref(h), device, n, cams, ptrs)
+        if rc != HPM_OK:
+            raise HpmError(f"hpm_pool_create failed ({rc})")
+        self._h = h
+        self.sizes = [(c.width, c.height) for c in cameras]
+        self._imgs = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hpm_pool_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 

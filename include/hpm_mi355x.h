@@ -91,6 +91,21 @@ int hpm_set_params(hpm_ctx* ctx, const hpm_params* params);
  * allocates the per-pixel state. */
 int hpm_set_views(hpm_ctx* ctx, int num_images, const hpm_camera* cameras, const float* const* images);
 
+/* Image pool: the images and cameras of a whole scene, uploaded once to
+ * `device` and shared by the contexts of its reference views (hpm_set_views
+ * would copy the reference and source images into every context).  Quad images
+ * of the mixed-precision engine are built once per precision on first use.  No
+ * counterpart in the reference, which uploads the images of every problem
+ * into every HPM object (HPM.cpp:741-772).  The pool must outlive the
+ * contexts that use it; contexts may run concurrently on it. */
+typedef struct hpm_pool hpm_pool;
+int hpm_pool_create(hpm_pool** out, int device, int num_images, const hpm_camera* cameras,
+                    const float* const* images);
+void hpm_pool_destroy(hpm_pool* pool);
+/* hpm_set_views with pool images ids[0] (reference), ids[1..num_images-1]
+ * (sources), without copying them. */
+int hpm_set_views_pooled(hpm_ctx* ctx, hpm_pool* pool, int num_images, const int32_t* ids);
+
 /* Geometric-consistency depth textures (HPM.cpp:693-732, 786-811): one depth
  * map per image, index 0 (the reference's own) is not read by the cost. */
 int hpm_set_src_depths(hpm_ctx* ctx, const float* const* depths);

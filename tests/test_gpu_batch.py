@@ -86,3 +86,30 @@ def test_batch_matches_single_runs():
     H.run_batch(batch, seeds)
     for s, c in zip(single, batch):
         np.testing.assert_array_equal(c.planes(), s.planes())
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16"])
+def test_pooled_views_match_copied_views(prec):
+    """Contexts on a shared image pool give the bits of contexts holding their
+    own copies, alone and batched."""
+    import hpm_mi355x as H
+    from hpm_mi355x import scenes
+    print(f"case: image pool {prec} 128x96, 7 cameras, 3 reference views", flush=True)
+    sc = scenes.make_plane_scene(seed=3, n_cams=7, width=128, height=96, f=105.0, n_src=4)
+    pool = H.ImagePool([H.camera_from_scene(c) for c in sc.cameras], [im.astype(np.float32) for im in sc.images])
+    pooled, copied, seeds = [], [], []
+    for ref in (2, 3, 4):
+        ids, cams, imgs = scenes.reference_view_problem(sc, ref, 4)
+        c = H.Context(0)
+        c.set_params(H.default_params(precision=prec))
+        c.set_views_pooled(pool, ids)
+        pooled.append(c)
+        copied.append(_ctx(cams, imgs, precision=prec))
+        seeds.append(40 + ref)
+    H.run_batch(pooled, seeds)
+    for c, d, sd in zip(pooled, copied, seeds):
+        d.run(sd)
+        np.testing.assert_array_equal(c.planes(), d.planes())
+        np.testing.assert_array_equal(c.costs(), d.costs())
+    pooled[1].run(seeds[1])  # alone, after the batch
+    np.testing.assert_array_equal(pooled[1].planes(), copied[1].planes())
