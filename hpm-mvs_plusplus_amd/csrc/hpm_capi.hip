@@ -12,6 +12,7 @@
 #include "../../include/hpm_mi355x.h"
 #include "hpm_fast.h"
 #include "hpm_fusion.h"
+#include "hpm_grad.h"
 #include "hpm_jbu.h"
 #include "hpm_kernels.h"
 
@@ -813,5 +814,128 @@ int hpm_geom_cost(hpm_ctx* ctx, const float* planes, float* out) {
   return rc;
 }
 
-}  // extern "C"
 
+// ------------------------------------------------------------------ gradient
+
+static int grad_setup(hpm_ctx* ctx) {
+  if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
+  HIP_OK(hipSetDevice(ctx->device));
+  int rc = ensure_quads(ctx, HPM_PREC_FP32_FAST);
+  if (rc) return rc;
+  RunConsts C;
+  build_consts(ctx, C);
+  HIP_OK(hipMemcpyAsync(ctx->d_consts, &C, sizeof(C), hipMemcpyHostToDevice, ctx->stream));
+  return HPM_OK;
+}
+
+static int grad_launch(hpm_ctx* ctx, const void* params, const void* weights, void* cost, void* grad, void* gn) {
+  const dim3 b(16, 16), g((ctx->W + 15) / 16, (ctx->H + 15) / 16);
+  if (grad)
+    hipLaunchKernelGGL(k_cost_grad<true>, g, b, 0, ctx->stream, ctx->d_consts, (const float4*)params,
+                       (const float*)weights, (float*)cost, (float*)grad, (float*)gn);
+  else
+    hipLaunchKernelGGL(k_cost_grad<false>, g, b, 0, ctx->stream, ctx->d_consts, (const float4*)params,
+                       (const float*)weights, (float*)cost, nullptr, nullptr);
+  HIP_OK(hipGetLastError());
+  return HPM_OK;
+}
+
+int hpm_cost_grad_device(hpm_ctx* ctx, const void* params_dev, const void* weights_dev, void* out_cost_dev,
+                         void* out_grad_dev, void* out_gn_dev) {
+  if (!ctx || !params_dev || !weights_dev || !out_cost_dev) return HPM_E_INVALID;
+  int rc = grad_setup(ctx);
+  if (rc) return rc;
+  return grad_launch(ctx, params_dev, weights_dev, out_cost_dev, out_grad_dev, out_grad_dev ? out_gn_dev : nullptr);
+}
+
+// Device buffers of the host-pointer gradient entry points.
+struct GradBuffers {
+  float4* params = nullptr;
+  float* weights = nullptr;
+  float* cost = nullptr;
+  float* grad = nullptr;
+  float* gn = nullptr;
+  float* lambda = nullptr;
+  float4* cand = nullptr;
+  float* cand_cost = nullptr;
+  ~GradBuffers() {
+    hipFree(params); hipFree(weights); hipFree(cost); hipFree(grad); hipFree(gn); hipFree(lambda);
+    hipFree(cand); hipFree(cand_cost);
+  }
+};
+
+static int grad_alloc(hpm_ctx* ctx, GradBuffers& B, const float* params, const float* weights, bool refine) {
+  const size_t P = npix(ctx), V = ctx->nimg - 1;
+  int rc;
+  if ((rc = dalloc(ctx, &B.params, P)) || (rc = dalloc(ctx, &B.weights, P * V)) || (rc = dalloc(ctx, &B.cost, P)) ||
+      (rc = dalloc(ctx, &B.grad, 3 * P)) || (rc = dalloc(ctx, &B.gn, 6 * P)))
+    return rc;
+  if (refine && ((rc = dalloc(ctx, &B.lambda, P)) || (rc = dalloc(ctx, &B.cand, P)) ||
+                 (rc = dalloc(ctx, &B.cand_cost, P))))
+    return rc;
+  if ((rc = upload(ctx, B.params, params, P * sizeof(float4))) || (rc = upload(ctx, B.weights, weights, P * V * 4)))
+    return rc;
+  return HPM_OK;
+}
+
+int hpm_cost_grad(hpm_ctx* ctx, const float* params, const float* weights, float* out_cost, float* out_grad,
+                  float* out_gn) {
+  if (!ctx || !params || !weights || !out_cost) return HPM_E_INVALID;
+  int rc = grad_setup(ctx);
+  if (rc) return rc;
+  GradBuffers B;
+  const size_t P = npix(ctx);
+  if ((rc = grad_alloc(ctx, B, params, weights, false))) return rc;
+  if ((rc = grad_launch(ctx, B.params, B.weights, B.cost, out_grad ? B.grad : nullptr, out_gn ? B.gn : nullptr)))
+    return rc;
+  if ((rc = download(ctx, out_cost, B.cost, P * 4))) return rc;
+  if (out_grad && (rc = download(ctx, out_grad, B.grad, 3 * P * 4))) return rc;
+  if (out_grad && out_gn && (rc = download(ctx, out_gn, B.gn, 6 * P * 4))) return rc;
+  return HPM_OK;
+}
+
+static int refine_common(hpm_ctx* ctx, float* params, const float* weights, int iterations, float* out_cost,
+                         bool gauss_newton, uint64_t seed) {
+  if (!ctx || !params || !weights || iterations < 0) return HPM_E_INVALID;
+  int rc = grad_setup(ctx);
+  if (rc) return rc;
+  GradBuffers B;
+  const int P = (int)npix(ctx);
+  if ((rc = grad_alloc(ctx, B, params, weights, true))) return rc;
+  const float dmin = ctx->cams[0].depth_min * 0.6f, dmax = ctx->cams[0].depth_max * 1.2f;
+  const dim3 b1(256), g1((P + 255) / 256);
+  hipLaunchKernelGGL(k_fill_f32, g1, b1, 0, ctx->stream, B.lambda, P, 1e-2f);
+  HIP_OK(hipGetLastError());
+  if ((rc = grad_launch(ctx, B.params, B.weights, B.cost, gauss_newton ? B.grad : nullptr,
+                        gauss_newton ? B.gn : nullptr)))
+    return rc;
+  for (int it = 0; it < iterations; ++it) {
+    if (gauss_newton) {
+      if (it > 0 && (rc = grad_launch(ctx, B.params, B.weights, B.cost, B.grad, B.gn))) return rc;
+      hipLaunchKernelGGL(k_gn_step, g1, b1, 0, ctx->stream, P, B.params, B.grad, B.gn, B.lambda, dmin, dmax, B.cand);
+    } else {
+      const float decay = powf(0.8f, (float)it);
+      hipLaunchKernelGGL(k_random_step, g1, b1, 0, ctx->stream, P, seed, (uint32_t)(1000 + it), B.params,
+                         0.02f * decay, 0.0872665f * decay, dmin, dmax, B.cand);
+    }
+    HIP_OK(hipGetLastError());
+    if ((rc = grad_launch(ctx, B.cand, B.weights, B.cand_cost, nullptr, nullptr))) return rc;
+    hipLaunchKernelGGL(k_accept, g1, b1, 0, ctx->stream, P, B.params, B.cost, B.cand, B.cand_cost,
+                       gauss_newton ? B.lambda : nullptr);
+    HIP_OK(hipGetLastError());
+  }
+  if ((rc = download(ctx, params, B.params, (size_t)P * sizeof(float4)))) return rc;
+  if (out_cost && (rc = download(ctx, out_cost, B.cost, (size_t)P * 4))) return rc;
+  return HPM_OK;
+}
+
+int hpm_refine_gn(hpm_ctx* ctx, float* params, const float* weights, int iterations, float* out_cost) {
+  return refine_common(ctx, params, weights, iterations, out_cost, true, 0);
+}
+
+int hpm_refine_random(hpm_ctx* ctx, float* params, const float* weights, int iterations, uint64_t seed,
+                      float* out_cost) {
+  return refine_common(ctx, params, weights, iterations, out_cost, false, seed);
+}
+
+}  // extern "C"

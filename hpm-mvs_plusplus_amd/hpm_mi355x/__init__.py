@@ -59,6 +59,7 @@ EXPORTS = [
     "hpm_get_stats", "hpm_set_timing", "hpm_cost_fixed", "hpm_cost_fixed_device", "hpm_init_cost",
     "hpm_geom_cost", "hpm_jbu_depth", "hpm_jbu_prior", "hpm_consistency_check", "hpm_set_graph",
     "hpm_run_patchmatch_batch", "hpm_pool_create", "hpm_pool_destroy", "hpm_set_views_pooled",
+    "hpm_cost_grad", "hpm_cost_grad_device", "hpm_refine_gn", "hpm_refine_random",
 ]
 
 _lib = None
@@ -94,6 +95,10 @@ def lib():
     L.hpm_run_patchmatch_partial.argtypes = [P, ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
     L.hpm_synchronize.argtypes = [P]
     L.hpm_set_graph.argtypes = [P, ctypes.c_int]
+    L.hpm_cost_grad.argtypes = [P, fp, fp, fp, fp, fp]
+    L.hpm_cost_grad_device.argtypes = [P, P, P, P, P, P]
+    L.hpm_refine_gn.argtypes = [P, fp, fp, ctypes.c_int, fp]
+    L.hpm_refine_random.argtypes = [P, fp, fp, ctypes.c_int, ctypes.c_uint64, fp]
     L.hpm_pool_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.c_int, ctypes.POINTER(hpm_camera),
                                   ctypes.POINTER(fp)]
     L.hpm_pool_destroy.argtypes = [P]
@@ -255,6 +260,39 @@ class Context:
         self._pool = pool  # keeps the pool alive while this context uses it
         self.width, self.height = pool.sizes[ids[0]]
         self.num_images = len(ids)
+
+    def cost_grad(self, params, weights, gn=False):
+        """hpm_cost_grad: (cost (h, w), grad (h, w, 3)[, gn (h, w, 6)])."""
+        h, w = self.height, self.width
+        pp = np.zeros((h, w, 4), np.float32)
+        pp[..., :3] = np.asarray(params, np.float32)[..., :3]
+        wt = np.ascontiguousarray(weights, np.float32)
+        cost = np.empty((h, w), np.float32)
+        grad = np.empty((h, w, 3), np.float32)
+        g6 = np.empty((h, w, 6), np.float32) if gn else None
+        self._check(self._L.hpm_cost_grad(self._h, _fptr(pp), _fptr(wt), _fptr(cost), _fptr(grad),
+                                          _fptr(g6) if gn else None), "hpm_cost_grad")
+        return (cost, grad, g6) if gn else (cost, grad)
+
+    def cost_grad_device(self, params_ptr, weights_ptr, cost_ptr, grad_ptr, gn_ptr=None):
+        """hpm_cost_grad_device on raw device pointers (ints); asynchronous."""
+        self._check(self._L.hpm_cost_grad_device(self._h, params_ptr, weights_ptr, cost_ptr, grad_ptr, gn_ptr),
+                    "hpm_cost_grad_device")
+
+    def refine(self, params, weights, iterations, method="gn", seed=1):
+        """hpm_refine_gn / hpm_refine_random: returns (params (h, w, 4), cost (h, w))."""
+        h, w = self.height, self.width
+        pp = np.zeros((h, w, 4), np.float32)
+        pp[..., :3] = np.asarray(params, np.float32)[..., :3]
+        wt = np.ascontiguousarray(weights, np.float32)
+        cost = np.empty((h, w), np.float32)
+        if method == "gn":
+            rc = self._L.hpm_refine_gn(self._h, _fptr(pp), _fptr(wt), iterations, _fptr(cost))
+        else:
+            rc = self._L.hpm_refine_random(self._h, _fptr(pp), _fptr(wt), iterations, ctypes.c_uint64(seed),
+                                           _fptr(cost))
+        self._check(rc, "hpm_refine_" + method)
+        return pp, cost
 
     def set_graph(self, on=True):
         self._check(self._L.hpm_set_graph(self._h, int(on)), "hpm_set_graph")

@@ -209,6 +209,36 @@ int hpm_jbu_depth(const float* fine_gray, int w, int h, const float* coarse_dept
 int hpm_jbu_prior(const float* fine_gray, int w, int h, const float* coarse_depth, const float* coarse_normals, int cw,
                   int ch, float* out_depth, float* out_normals);
 
+/* Backward pass of the multi-view bilateral NCC (no counterpart in the
+ * reference, which has no gradient).  Per pixel, params = (d, theta, phi, -):
+ * plane depth d and normal n = (sin theta cos phi, sin theta sin phi,
+ * -cos theta) in the reference camera frame; weights = num_images-1 frozen view
+ * weights per pixel (views with weight <= 0 are skipped).  Outputs
+ * out_cost = sum_v a_v C_v / sum_v a_v (C_v the NCC cost of HPM.cu:375-448 on
+ * FP32 texels), out_grad = dC/d(d, theta, phi) (3 per pixel; may be NULL), and
+ * out_gn (6 per pixel, upper triangle of the Gauss-Newton matrix
+ * sum_v a_v J_v^T J_v / sum_v a_v of the residuals sqrt(w_k / W) (r^_k - s^_k)
+ * between the normalised reference and source windows, whose squared norm / 2
+ * is C_v; may be NULL).  The gradient is 0 on the clamp and small-variance
+ * branches. */
+int hpm_cost_grad(hpm_ctx* ctx, const float* params, const float* weights, float* out_cost, float* out_grad,
+                  float* out_gn);
+/* Same on device pointers already in HBM (e.g. torch tensors), asynchronous on
+ * the context stream. */
+int hpm_cost_grad_device(hpm_ctx* ctx, const void* params_dev, const void* weights_dev, void* out_cost_dev,
+                         void* out_grad_dev, void* out_gn_dev);
+/* Gauss-Newton (Levenberg-Marquardt) refinement of every pixel's plane:
+ * `iterations` steps on the normalised-window residuals, each kept only where
+ * it lowers the cost.  params (host, 4 floats per pixel) are updated in place;
+ * out_cost (may be NULL) receives the final cost.  Beside the reference's
+ * random-perturbation refinement (PlaneHypothesisRefinement, HPM.cu:765-841). */
+int hpm_refine_gn(hpm_ctx* ctx, float* params, const float* weights, int iterations, float* out_cost);
+/* The random-perturbation counterpart on the same parameters and cost: per
+ * step a candidate d (1 + 0.02 s u0), theta + 5 deg s u1, phi + 5 deg s u2
+ * (u uniform in [-1, 1], s = 0.8^step), kept where it lowers the cost. */
+int hpm_refine_random(hpm_ctx* ctx, float* params, const float* weights, int iterations, uint64_t seed,
+                      float* out_cost);
+
 /* Multi-view consistency check of RunFusion (mode HPM_CHECK_FUSION,
  * main.cpp:823-957) and ConfidenceEvaluation (mode HPM_CHECK_CONFIDENCE,
  * main.cpp:959-1109), with the reference's greedy masks: images in order,

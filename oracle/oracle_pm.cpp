@@ -1135,6 +1135,128 @@ template <class T> int RunImpl(const orc_camera* cams, int num_images, const flo
   return RunPatchMatch(c, n_halfpasses, do_output);
 }
 
+// ------------------------------------------------------------ plane gradient
+// The aggregated cost of a pixel's plane and its gradient with respect to the
+// plane's depth and two normal angles.  The reference has no gradient; this is
+// the definition the GPU backward kernel is checked against:
+//   C = sum_v w_v C_v / sum_v w_v over the views with w_v > 0 (weights frozen),
+//   C_v = ComputeBilateralNCC (HPM.cu:375-448) of the plane
+//   n = (sin t cos p, sin t sin p, -cos t), w = -n . X(d), X(d) = Get3DPoint at
+//   depth d (HPM.cu:114-140),
+// differentiated through H = K_s (R_rel - t_rel n^T / w) K_r^-1 (HPM.cu:251-311):
+// dC_v/dH accumulated over the 36 taps (bilinear sample derivative times the
+// projective map's derivative), then dH/d(d, t, p).  Zero where C_v is 2
+// (centre outside, small variance) or clamped.  FP64 throughout.
+namespace grad {
+
+// K_s M K_r^-1 with the operations of ComputeHomography.
+void MapHomography(const orc_camera& rc, const orc_camera& sc, const double M[9], double H[9]) {
+  const double rK0 = rc.K[0], rK2 = rc.K[2], rK4 = rc.K[4], rK5 = rc.K[5];
+  double tmp[9];
+  for (int a = 0; a < 3; ++a) {
+    tmp[3 * a + 0] = M[3 * a + 0] / rK0;
+    tmp[3 * a + 1] = M[3 * a + 1] / rK4;
+    tmp[3 * a + 2] = -M[3 * a + 0] * rK2 / rK0 - M[3 * a + 1] * rK5 / rK4 + M[3 * a + 2];
+  }
+  const double sK0 = sc.K[0], sK2 = sc.K[2], sK4 = sc.K[4], sK5 = sc.K[5], sK8 = sc.K[8];
+  H[0] = sK0 * tmp[0] + sK2 * tmp[6];
+  H[1] = sK0 * tmp[1] + sK2 * tmp[7];
+  H[2] = sK0 * tmp[2] + sK2 * tmp[8];
+  H[3] = sK4 * tmp[3] + sK5 * tmp[6];
+  H[4] = sK4 * tmp[4] + sK5 * tmp[7];
+  H[5] = sK4 * tmp[5] + sK5 * tmp[8];
+  H[6] = sK8 * tmp[6];
+  H[7] = sK8 * tmp[7];
+  H[8] = sK8 * tmp[8];
+}
+
+void RelativePose(const orc_camera& rc, const orc_camera& sc, double R_rel[9], double t_rel[3]) {
+  double ref_C[3], src_C[3], C_rel[3];
+  for (int r = 0; r < 3; ++r) {
+    ref_C[r] = -((double)rc.R[0 + r] * rc.t[0] + (double)rc.R[3 + r] * rc.t[1] + (double)rc.R[6 + r] * rc.t[2]);
+    src_C[r] = -((double)sc.R[0 + r] * sc.t[0] + (double)sc.R[3 + r] * sc.t[1] + (double)sc.R[6 + r] * sc.t[2]);
+  }
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      R_rel[3 * a + b] = (double)sc.R[3 * a] * rc.R[3 * b] + (double)sc.R[3 * a + 1] * rc.R[3 * b + 1] +
+                         (double)sc.R[3 * a + 2] * rc.R[3 * b + 2];
+  for (int r = 0; r < 3; ++r) C_rel[r] = ref_C[r] - src_C[r];
+  for (int a = 0; a < 3; ++a)
+    t_rel[a] = (double)sc.R[3 * a] * C_rel[0] + (double)sc.R[3 * a + 1] * C_rel[1] + (double)sc.R[3 * a + 2] * C_rel[2];
+}
+
+// bilinear sample and its derivatives along u and v (Bilinear's cell and clamp)
+double BilinearGrad(const Image& im, double u, double v, double& su, double& sv) {
+  const double fu = std::floor(u), fv = std::floor(v);
+  const double a = u - fu, b = v - fv;
+  const double cu = std::min(std::max(fu, -1.0), (double)im.w), cv = std::min(std::max(fv, -1.0), (double)im.h);
+  const int x0 = (int)cu, y0 = (int)cv;
+  const double i00 = im.at(x0, y0), i10 = im.at(x0 + 1, y0), i01 = im.at(x0, y0 + 1), i11 = im.at(x0 + 1, y0 + 1);
+  const double top = i00 + a * (i10 - i00), bot = i01 + a * (i11 - i01);
+  su = (1.0 - b) * (i10 - i00) + b * (i11 - i01);
+  sv = bot - top;
+  return top + b * (bot - top);
+}
+
+// C_v and dC_v/dH for one view (returns false on the constant-cost branches).
+bool CostAndDH(const Ctx<double>& c, int v, int px, int py, const double H[9], double& cost, double dH[9]) {
+  const orc_camera& sc = c.cams[v];
+  for (int k = 0; k < 9; ++k) dH[k] = 0;
+  cost = 2.0;
+  {
+    const double X = H[0] * px + H[1] * py + H[2], Y = H[3] * px + H[4] * py + H[5], Z = H[6] * px + H[7] * py + H[8];
+    const double u = X / Z, vv = Y / Z;
+    if (u >= sc.width || u < 0 || vv >= sc.height || vv < 0) return false;
+  }
+  const Image& ref = c.imgs[0];
+  const Image& src = c.imgs[v];
+  const double ss = c.prm.sigma_spatial, scol = c.prm.sigma_color;
+  const double rc0 = ref.at(px, py);
+  double w[36], r[36], s[36], su[36], sv[36], uu[36], vvv[36], zz[36];
+  int qx[36], qy[36];
+  int k = 0;
+  for (int i = -5; i <= 5; i += 2)
+    for (int j = -5; j <= 5; j += 2, ++k) {
+      qx[k] = px + i;
+      qy[k] = py + j;
+      r[k] = ref.at(qx[k], qy[k]);
+      w[k] = std::exp(-std::sqrt((double)(i * i + j * j)) / (2 * ss * ss) - std::fabs(r[k] - rc0) / (2 * scol * scol));
+      const double X = H[0] * qx[k] + H[1] * qy[k] + H[2], Y = H[3] * qx[k] + H[4] * qy[k] + H[5];
+      zz[k] = H[6] * qx[k] + H[7] * qy[k] + H[8];
+      uu[k] = X / zz[k];
+      vvv[k] = Y / zz[k];
+      s[k] = BilinearGrad(src, uu[k], vvv[k], su[k], sv[k]);
+    }
+  double W = 0, mr = 0, ms = 0;
+  for (k = 0; k < 36; ++k) { W += w[k]; mr += w[k] * r[k]; ms += w[k] * s[k]; }
+  mr /= W; ms /= W;
+  double vr = 0, vs = 0, cv = 0;
+  for (k = 0; k < 36; ++k) {
+    vr += w[k] * (r[k] - mr) * (r[k] - mr);
+    vs += w[k] * (s[k] - ms) * (s[k] - ms);
+    cv += w[k] * (r[k] - mr) * (s[k] - ms);
+  }
+  vr /= W; vs /= W; cv /= W;
+  if (vr < 1e-5 || vs < 1e-5) return false;
+  const double den = std::sqrt(vr * vs);
+  const double raw = 1.0 - cv / den;
+  cost = std::max(0.0, std::min(2.0, raw));
+  if (raw <= 0.0 || raw >= 2.0) return false;
+  for (k = 0; k < 36; ++k) {
+    const double dc_ds = -(w[k] / W) / den * ((r[k] - mr) - (cv / vs) * (s[k] - ms));
+    const double gu = dc_ds * su[k] / zz[k], gv = dc_ds * sv[k] / zz[k];
+    const double p[3] = {(double)qx[k], (double)qy[k], 1.0};
+    for (int b = 0; b < 3; ++b) {
+      dH[b] += gu * p[b];
+      dH[3 + b] += gv * p[b];
+      dH[6 + b] -= (gu * uu[k] + gv * vvv[k]) * p[b];
+    }
+  }
+  return true;
+}
+
+}  // namespace grad
+
 template <class T> int CostFixedImpl(const orc_camera* cams, int num_images, const float* const* images,
                                      const orc_params* params, const T* planes, T* out_costs, int row_begin, int row_end) {
   Ctx<T> c;
@@ -1225,6 +1347,101 @@ int orc_geom_cost_f32(const orc_camera* cams, int num_images, const float* const
 }
 
 // Scalar primitives, exported for the pinning tests.
+// Aggregated cost and gradient per pixel (see namespace grad): params holds
+// (d, theta, phi, unused) per pixel, weights (num_images - 1) per pixel;
+// out_cost one double per pixel, out_grad (dC/dd, dC/dtheta, dC/dphi).
+int orc_cost_grad_f64(const orc_camera* cams, int num_images, const float* const* images, const orc_params* params,
+                      const float* pparams, const float* weights, double* out_cost, double* out_grad) {
+  Ctx<double> c;
+  c.W = cams[0].width; c.H = cams[0].height; c.nv = num_images; c.cams = cams; c.prm = *params;
+  for (int i = 0; i < num_images; ++i) c.imgs.push_back(Image{images[i], cams[i].width, cams[i].height});
+  const int nvs = num_images - 1;
+  std::vector<double> Rr(9 * nvs), tr(3 * nvs);
+  for (int v = 1; v <= nvs; ++v) grad::RelativePose(cams[0], cams[v], &Rr[9 * (v - 1)], &tr[3 * (v - 1)]);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int y = 0; y < c.H; ++y)
+    for (int x = 0; x < c.W; ++x) {
+      const size_t p = (size_t)y * c.W + x;
+      const double d = pparams[4 * p], th = pparams[4 * p + 1], ph = pparams[4 * p + 2];
+      const double n[3] = {std::sin(th) * std::cos(ph), std::sin(th) * std::sin(ph), -std::cos(th)};
+      const double dn_dt[3] = {std::cos(th) * std::cos(ph), std::cos(th) * std::sin(ph), std::sin(th)};
+      const double dn_dp[3] = {-std::sin(th) * std::sin(ph), std::sin(th) * std::cos(ph), 0.0};
+      double X[3];
+      Get3DPoint<double>(cams[0], x, y, d, X);
+      const double wpl = -(n[0] * X[0] + n[1] * X[1] + n[2] * X[2]);
+      const double q[3] = {X[0] / d, X[1] / d, X[2] / d};
+      double wsum = 0, C = 0, g[3] = {0, 0, 0};
+      for (int v = 1; v <= nvs; ++v) {
+        const double wv = weights[p * nvs + (v - 1)];
+        if (!(wv > 0)) continue;
+        const double* R = &Rr[9 * (v - 1)];
+        const double* t = &tr[3 * (v - 1)];
+        double M[9], H[9], dMn[3][9], dMw[9], dHn[3][9], dHw[9];
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) {
+            M[3 * a + b] = R[3 * a + b] - t[a] * n[b] / wpl;
+            dMw[3 * a + b] = t[a] * n[b] / (wpl * wpl);
+            for (int e = 0; e < 3; ++e) dMn[e][3 * a + b] = (b == e) ? -t[a] / wpl : 0.0;
+          }
+        grad::MapHomography(cams[0], cams[v], M, H);
+        grad::MapHomography(cams[0], cams[v], dMw, dHw);
+        for (int e = 0; e < 3; ++e) {
+          // MapHomography is linear in M; the constant part of the K maps is 0
+          grad::MapHomography(cams[0], cams[v], dMn[e], dHn[e]);
+        }
+        double cv, dH[9];
+        const bool live = grad::CostAndDH(c, v, x, y, H, cv, dH);
+        wsum += wv;
+        C += wv * cv;
+        if (!live) continue;
+        // dC_v/dd = dH . dH/dw * dw/dd, dw/dd = -n . q
+        double dHdw = 0, dHdn[3] = {0, 0, 0};
+        for (int k = 0; k < 9; ++k) {
+          dHdw += dH[k] * dHw[k];
+          for (int e = 0; e < 3; ++e) dHdn[e] += dH[k] * dHn[e][k];
+        }
+        // total derivative with respect to n_e includes w(n): dw/dn_e = -X_e
+        double dCdn[3];
+        for (int e = 0; e < 3; ++e) dCdn[e] = dHdn[e] + dHdw * (-X[e]);
+        const double nq = n[0] * q[0] + n[1] * q[1] + n[2] * q[2];
+        g[0] += wv * dHdw * (-nq);
+        g[1] += wv * (dCdn[0] * dn_dt[0] + dCdn[1] * dn_dt[1] + dCdn[2] * dn_dt[2]);
+        g[2] += wv * (dCdn[0] * dn_dp[0] + dCdn[1] * dn_dp[1] + dCdn[2] * dn_dp[2]);
+      }
+      out_cost[p] = wsum > 0 ? C / wsum : 2.0;
+      for (int e = 0; e < 3; ++e) out_grad[3 * p + e] = wsum > 0 ? g[e] / wsum : 0.0;
+    }
+  return 0;
+}
+
+// Aggregated cost only, at the same parameterisation (for finite differences).
+int orc_cost_params_f64(const orc_camera* cams, int num_images, const float* const* images,
+                        const orc_params* params, const double* pparams, const float* weights, double* out_cost) {
+  Ctx<double> c;
+  c.W = cams[0].width; c.H = cams[0].height; c.nv = num_images; c.cams = cams; c.prm = *params;
+  for (int i = 0; i < num_images; ++i) c.imgs.push_back(Image{images[i], cams[i].width, cams[i].height});
+  const int nvs = num_images - 1;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int y = 0; y < c.H; ++y)
+    for (int x = 0; x < c.W; ++x) {
+      const size_t p = (size_t)y * c.W + x;
+      const double d = pparams[4 * p], th = pparams[4 * p + 1], ph = pparams[4 * p + 2];
+      const double n[3] = {std::sin(th) * std::cos(ph), std::sin(th) * std::sin(ph), -std::cos(th)};
+      double X[3];
+      Get3DPoint<double>(cams[0], x, y, d, X);
+      const V4<double> pl{n[0], n[1], n[2], -(n[0] * X[0] + n[1] * X[1] + n[2] * X[2])};
+      double wsum = 0, C = 0;
+      for (int v = 1; v <= nvs; ++v) {
+        const double wv = weights[p * nvs + (v - 1)];
+        if (!(wv > 0)) continue;
+        wsum += wv;
+        C += wv * ComputeBilateralNCC(c, v, x, y, pl);
+      }
+      out_cost[p] = wsum > 0 ? C / wsum : 2.0;
+    }
+  return 0;
+}
+
 float orc_expf(float x) { return orc::expf_hpm(x); }
 double orc_exp(double x) { return orc::exp_hpm(x); }
 float orc_sinf(float x) { return orc::sinf_hpm(x); }

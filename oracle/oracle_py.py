@@ -48,6 +48,10 @@ def lib():
         f32 = ctypes.c_float
         L.orc_expf.argtypes = [f32]; L.orc_expf.restype = f32
         L.orc_acosf.argtypes = [f32]; L.orc_acosf.restype = f32
+        L.orc_cost_grad_f64.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_cost_params_f64.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_jbu_depth.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_void_p]
         L.orc_jbu_prior.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
@@ -252,4 +256,30 @@ def consistency(mode, cams, depths, normals, sources):
     pts = np.zeros((cap, 6), np.float32)
     lib().orc_consistency(0, n, ca, dptr, nptr, _p(off), _p(ids), None, _p(pairs), _p(pts), cap, ctypes.byref(count))
     return pairs[:count.value], pts[:count.value]
+
+
+def cost_grad(cams, images, prm, params, weights):
+    """FP64 aggregated cost and its gradient w.r.t. (depth, theta, phi).
+    params: (h, w, 3 or 4) float32; weights: (h, w, V) float32."""
+    h, w = params.shape[:2]
+    pp = np.zeros((h, w, 4), np.float32)
+    pp[..., :3] = params[..., :3]
+    wt = np.ascontiguousarray(weights, np.float32)
+    imgs, arr = _img_ptrs(images)
+    cost = np.zeros((h, w), np.float64)
+    grad = np.zeros((h, w, 3), np.float64)
+    lib().orc_cost_grad_f64(_cams_array(cams), len(cams), arr, ctypes.byref(prm), _p(pp), _p(wt), _p(cost), _p(grad))
+    return cost, grad
+
+
+def cost_params(cams, images, prm, params, weights):
+    """FP64 aggregated cost at (depth, theta, phi) given in double."""
+    h, w = params.shape[:2]
+    pp = np.zeros((h, w, 4), np.float64)
+    pp[..., :3] = params[..., :3]
+    wt = np.ascontiguousarray(weights, np.float32)
+    imgs, arr = _img_ptrs(images)
+    cost = np.zeros((h, w), np.float64)
+    lib().orc_cost_params_f64(_cams_array(cams), len(cams), arr, ctypes.byref(prm), _p(pp), _p(wt), _p(cost))
+    return cost
 
