@@ -80,50 +80,22 @@ def test_gradient_vs_central_differences_of_gpu_cost():
 
 
 def test_torch_autograd_function():
-    torch = pytest.importorskip("torch")
-    if not torch.cuda.is_available():
-        pytest.skip("torch without a GPU")
-    sc, ids, cams, imgs, params, weights, gt = _problem(noise_seed=2)
-    ctx = _ctx(cams, imgs)
-    dev = torch.device("cuda:0")
-
-    class PlaneCost(torch.autograd.Function):
-        """Per-pixel aggregated NCC cost of (depth, theta, phi); backward through
-        the hand-written HIP kernel."""
-
-        @staticmethod
-        def forward(fctx, p4, wts):
-            cost = torch.empty(p4.shape[:2], dtype=torch.float32, device=p4.device)
-            grad = torch.empty(p4.shape[:2] + (3,), dtype=torch.float32, device=p4.device)
-            torch.cuda.synchronize()
-            ctx.cost_grad_device(p4.data_ptr(), wts.data_ptr(), cost.data_ptr(), grad.data_ptr())
-            ctx.synchronize()
-            fctx.save_for_backward(grad)
-            return cost
-
-        @staticmethod
-        def backward(fctx, gout):
-            (grad,) = fctx.saved_tensors
-            g4 = torch.zeros(grad.shape[:2] + (4,), dtype=torch.float32, device=grad.device)
-            g4[..., :3] = gout[..., None] * grad
-            return g4, None
-
-    p4 = torch.zeros(params.shape[:2] + (4,), dtype=torch.float32)
-    p4[..., :3] = torch.from_numpy(params)
-    p4 = p4.to(dev).requires_grad_(True)
-    wts = torch.from_numpy(weights).to(dev)
-    cost = PlaneCost.apply(p4, wts)
-    cost.sum().backward()
-    ref_cost, ref_grad = ctx.cost_grad(params, weights)
-    np.testing.assert_array_equal(cost.detach().cpu().numpy(), ref_cost)
-    np.testing.assert_array_equal(p4.grad[..., :3].cpu().numpy(), ref_grad)
-    # one small gradient step on the depth lowers the summed cost
-    with torch.no_grad():
-        step = p4.clone()
-        live = (cost > 0.02) & (cost < 1.5)
-        step[..., 0] -= 1e-4 * torch.sign(p4.grad[..., 0]) * step[..., 0] * live
-    new = PlaneCost.apply(step, wts)
-    assert float(new[live].sum()) < float(cost[live].sum())
+    """tests/torch_plane_cost.py in a fresh process (torch's HIP runtime has to
+    initialise the GPU before this library's): forward/backward of a
+    torch.autograd.Function through hpm_cost_grad_device give the kernel's exact
+    cost and gradient, and gradient steps through autograd lower the cost."""
+    import json
+    import os
+    import subprocess
+    import sys
+    script = os.path.join(os.path.dirname(os.path.abspath(__file__)), "torch_plane_cost.py")
+    r = subprocess.run([sys.executable, script], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    if "skipped" in out:
+        pytest.skip(out["skipped"])
+    assert r.returncode == 0, out
+    assert out["cost_equal"] and out["grad_equal"] and out["descends"]
 
 
 def test_gauss_newton_vs_random_refinement():
