@@ -219,12 +219,17 @@ def main():
         ctx.run(SEED + 100 + args.steps - 1)
         pe = ctx.planes()
         agree = float(np.mean(np.abs(pf[..., 3] - pe[..., 3]) / np.abs(pe[..., 3]) <= 0.01))
+        # pixels the plain FP32 run gets right (within 1 % of ground truth): share the FP16 run also gets right
+        ok_e = np.abs(pe[..., 3] - gt) / gt <= 0.01
+        ok_f = np.abs(pf[..., 3] - gt) / gt <= 0.01
+        agree_ok = float(np.mean(ok_f[ok_e])) if ok_e.any() else None
         plain = {"views_s": round(rp["views_s"], 4), "ms_per_step": round(1e3 * rp["elapsed"] / max(2, args.steps // 2), 3),
                  "k_propagate_avg_ms": round(rp["prop_ms"] / max(rp["launches"], 1), 3),
                  "depth_within_1pct_of_gt": round(float(np.mean(np.abs(pp[..., 3] - gt) / gt <= 0.01)), 4),
                  "speedup_batched": round(views_s_b / rp["views_s"], 3),
                  "speedup_single_view": round(r["views_s"] / rp["views_s"], 3),
-                 "depth_within_1pct_of_plain_fp32": round(agree, 4)}
+                 "depth_within_1pct_of_plain_fp32": round(agree, 4),
+                 "fp16_right_where_fp32_right": None if agree_ok is None else round(agree_ok, 4)}
         print(f"[bench] plain fp32: {rp['views_s']:.2f} views/s", file=sys.stderr, flush=True)
 
     bytes_per_texel = {"fp32": 4, "fp32fast": 4, "fp16": 2, "bf16": 2}[args.precision]
