@@ -26,7 +26,12 @@ profiles/hbm_traffic.json when that file holds the precision (else null).
 
     python bench.py --calibration-copy-mib 4096
 
-runs only that copy kernel (4 GiB read + 4 GiB written), for the counter runs.
+runs only that copy kernel (4 GiB read + 4 GiB written), for the counter runs;
+
+    python bench.py --workload cost
+
+prints the cost-at-fixed-hypotheses report (FP32 exact / FP32 / FP16 / BF16
+image storage: ms, algorithmic GB/s, max and mean |cost - cost_FP64|).
 """
 import argparse
 import json
@@ -140,8 +145,63 @@ def run_views(ctx, steps, warmup, seed0):
             "ncc_prop": ncc_prop, "ncc_ref": ncc_ref}
 
 
+def cost_microbench(reps=20):
+    """Cost at fixed hypotheses (hpm_cost_fixed_device): 1600x1200 reference +
+    10 sources, one random plane per pixel (depth U[1, 5], normal within 60 deg
+    of the ray), FP32 exact / FP32 / FP16 / BF16 image storage with FP32
+    accumulation; time per launch (wall clock over `reps` launches), algorithmic
+    GB/s and |cost - cost_FP64| over the (pixel, view) pairs whose validity
+    agrees with the FP64 oracle."""
+    import ctypes
+    import hpm_mi355x as Hm
+    from hpm_mi355x import scenes
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+    sc, ids, cams, imgs = make_workload()
+    planes = scenes.random_planes_per_pixel(cams[0], seed=4)
+    ctx = Hm.Context(0)
+    ctx.set_params(Hm.default_params())
+    ctx.set_views([Hm.camera_from_scene(c) for c in cams], imgs)
+    P, V = W_FULL * H_FULL, NSRC
+    hip = ctypes.CDLL("libamdhip64.so")
+    dp, dout = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(dp), ctypes.c_size_t(P * 16)) == 0
+    assert hip.hipMalloc(ctypes.byref(dout), ctypes.c_size_t(P * V * 4)) == 0
+    pl = np.ascontiguousarray(planes, np.float32)
+    assert hip.hipMemcpy(dp, pl.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(P * 16), 1) == 0
+    print("[bench] FP64 oracle costs ...", file=sys.stderr, flush=True)
+    oc = [O.camera_from_scene(c) for c in cams]
+    ref64 = O.cost_fixed(oc, imgs, O.params(len(oc), oc[0]), planes.astype(np.float64), np.float64)
+    res = {}
+    for name, prec, bpt in (("fp32_exact", 0, 4), ("fp32", 3, 4), ("fp16", 1, 2), ("bf16", 2, 2)):
+        ctx.cost_fixed_device(dp.value, dout.value, prec)  # builds the quad images once
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.cost_fixed_device(dp.value, dout.value, prec)
+        ctx.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / reps
+        got = np.empty((H_FULL, W_FULL, V), np.float32)
+        assert hip.hipMemcpy(got.ctypes.data_as(ctypes.c_void_p), dout, ctypes.c_size_t(P * V * 4), 2) == 0
+        same = (got == 2.0) == (ref64 == 2.0)
+        err = np.abs(got.astype(np.float64) - ref64)[same]
+        alg = P * V * 181 * bpt + P * (16 + 4 * V)
+        res[name] = {"ms": round(ms, 3), "alg_GBps": round(alg / (ms * 1e-3) / 1e9, 1),
+                     "frac_of_8TBps": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3),
+                     "max_abs_err": float(err.max()), "mean_abs_err": float(err.mean()),
+                     "validity_agrees": round(float(same.mean()), 6)}
+        print(f"[bench] cost {name}: {ms:.3f} ms", file=sys.stderr, flush=True)
+    hip.hipFree(dp)
+    hip.hipFree(dout)
+    ctx.close()
+    return {"workload": "cost at fixed hypotheses, 1600x1200, 10 sources, 1 random plane per pixel",
+            "evals": P * V, "results": res}
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="patchmatch", choices=["patchmatch", "cost"],
+                    help="patchmatch: the RunPatchMatch throughput line; cost: the cost microbenchmark report")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -159,6 +219,9 @@ def main():
         sys.exit(2)
     if args.calibration_copy_mib:
         print(json.dumps(calibration_copy(args.calibration_copy_mib)), flush=True)
+        return
+    if args.workload == "cost":
+        print(json.dumps(cost_microbench()), flush=True)
         return
 
     import hpm_mi355x as Hm
