@@ -131,6 +131,48 @@ MapF ToGrayFloat(const MapU8& img) {
   return g;
 }
 
+template <class T>
+static void ResizeLinearImpl(const Map<T>& src, Map<T>& dst, bool round_u8) {
+  const double sx = (double)src.width / dst.width, sy = (double)src.height / dst.height;
+  std::vector<int> x0(dst.width), y0(dst.height);
+  std::vector<float> ax(dst.width), ay(dst.height);
+  auto axis = [](int n, int sn, double s, std::vector<int>& i0, std::vector<float>& a) {
+    for (int d = 0; d < n; ++d) {
+      double f = (d + 0.5) * s - 0.5;
+      int i = (int)std::floor(f);
+      float w = (float)(f - i);
+      if (i < 0) { i = 0; w = 0.0f; }
+      if (i >= sn - 1) { i = sn - 1; w = 0.0f; }
+      i0[d] = i;
+      a[d] = w;
+    }
+  };
+  axis(dst.width, src.width, sx, x0, ax);
+  axis(dst.height, src.height, sy, y0, ay);
+  for (int y = 0; y < dst.height; ++y)
+    for (int x = 0; x < dst.width; ++x)
+      for (int c = 0; c < src.channels; ++c) {
+        const int xa = x0[x], xb = std::min(xa + 1, src.width - 1);
+        const int ya = y0[y], yb = std::min(ya + 1, src.height - 1);
+        const float top = (float)src.at(xa, ya, c) + ax[x] * ((float)src.at(xb, ya, c) - (float)src.at(xa, ya, c));
+        const float bot = (float)src.at(xa, yb, c) + ax[x] * ((float)src.at(xb, yb, c) - (float)src.at(xa, yb, c));
+        const float v = top + ay[y] * (bot - top);
+        dst.at(x, y, c) = round_u8 ? (T)std::min(255.0f, std::max(0.0f, std::floor(v + 0.5f))) : (T)v;
+      }
+}
+
+MapU8 ResizeLinear(const MapU8& src, int w, int h) {
+  MapU8 d(w, h, src.channels);
+  ResizeLinearImpl(src, d, true);
+  return d;
+}
+
+MapF ResizeLinear(const MapF& src, int w, int h) {
+  MapF d(w, h, src.channels);
+  ResizeLinearImpl(src, d, false);
+  return d;
+}
+
 bool ExportPointCloud(const std::string& path, const std::vector<FusedPoint>& pts) {
   FILE* fp = std::fopen(path.c_str(), "wb");
   if (!fp) return false;
@@ -394,22 +436,32 @@ void PlaneThroughPoints(const float X1[3], const float X2[3], const float X3[3],
 
 MapU8 CannyEdges(const MapU8& gray, float low, float high) {
   const int W = gray.width, H = gray.height;
-  auto px = [&](int x, int y) {
+  const int C = gray.channels;
+  if (C != 1 && C != 3) throw Error("Canny: 1 or 3 channels expected");
+  auto px = [&](int x, int y, int c) {
     x = std::min(std::max(x, 0), W - 1);
     y = std::min(std::max(y, 0), H - 1);
-    return (int)gray.data[(size_t)y * W + x];
+    return (int)gray.data[((size_t)y * W + x) * C + c];
   };
-  std::vector<int> gx((size_t)W * H), gy((size_t)W * H), mag((size_t)W * H);
+  std::vector<int> gx((size_t)W * H), gy((size_t)W * H), mag((size_t)W * H, -1);
   for (int y = 0; y < H; ++y)
     for (int x = 0; x < W; ++x) {
-      const int dx = (px(x + 1, y - 1) + 2 * px(x + 1, y) + px(x + 1, y + 1)) -
-                     (px(x - 1, y - 1) + 2 * px(x - 1, y) + px(x - 1, y + 1));
-      const int dy = (px(x - 1, y + 1) + 2 * px(x, y + 1) + px(x + 1, y + 1)) -
-                     (px(x - 1, y - 1) + 2 * px(x, y - 1) + px(x + 1, y - 1));
       const size_t i = (size_t)y * W + x;
-      gx[i] = dx;
-      gy[i] = dy;
-      mag[i] = std::abs(dx) + std::abs(dy);
+      // a colour image keeps, per pixel, the channel of largest |dx| + |dy|;
+      // channels are visited B, G, R as in an OpenCV BGR image, first wins ties
+      for (int k = 0; k < C; ++k) {
+        const int c = C - 1 - k;
+        const int dx = (px(x + 1, y - 1, c) + 2 * px(x + 1, y, c) + px(x + 1, y + 1, c)) -
+                       (px(x - 1, y - 1, c) + 2 * px(x - 1, y, c) + px(x - 1, y + 1, c));
+        const int dy = (px(x - 1, y + 1, c) + 2 * px(x, y + 1, c) + px(x + 1, y + 1, c)) -
+                       (px(x - 1, y - 1, c) + 2 * px(x, y - 1, c) + px(x + 1, y - 1, c));
+        const int v = std::abs(dx) + std::abs(dy);
+        if (v > mag[i]) {
+          gx[i] = dx;
+          gy[i] = dy;
+          mag[i] = v;
+        }
+      }
     }
   auto m = [&](int x, int y) { return (x < 0 || y < 0 || x >= W || y >= H) ? 0 : mag[(size_t)y * W + x]; };
   // 0 none, 1 weak, 2 strong
@@ -468,6 +520,43 @@ static MapU8 LoadImage(const std::string& dense_folder, int id) {
   throw Error("cannot read image " + ImagePath(dense_folder, id, ".pgm") + " (binary PGM/PPM)");
 }
 
+void SceneImages::ensure(int id) {
+  if (id < 0) throw Error("negative image id");
+  if ((size_t)id >= orig_.size()) {
+    orig_.resize(id + 1);
+    cur_.resize(id + 1);
+    have_orig_.resize(id + 1, false);
+    have_cur_.resize(id + 1, false);
+  }
+}
+
+const MapU8& SceneImages::Original(int id) {
+  ensure(id);
+  if (!have_orig_[id]) {
+    orig_[id] = LoadImage(folder_, id);
+    have_orig_[id] = true;
+  }
+  return orig_[id];
+}
+
+void SceneImages::PrepareLevel(int id, int target_max_side) {  // main.cpp:123-145
+  const MapU8& img = Original(id);
+  if (std::max(img.width, img.height) > target_max_side) {
+    const float factor = std::min((float)target_max_side / img.width, (float)target_max_side / img.height);
+    const int nw = std::max(1, (int)std::round(img.width * factor));
+    const int nh = std::max(1, (int)std::round(img.height * factor));
+    cur_[id] = ResizeLinear(img, nw, nh);
+  } else {
+    cur_[id] = img;
+  }
+  have_cur_[id] = true;
+}
+
+const MapU8& SceneImages::Current(int id) {
+  ensure(id);
+  return have_cur_[id] ? cur_[id] : Original(id);
+}
+
 static MapF LoadDmb(const std::string& path) {
   MapF m;
   if (!ReadDmb(path, m)) throw Error("cannot read " + path);
@@ -494,7 +583,8 @@ void HPM::SetHierarchyParams() { params_.hierarchy = 1; }
 void HPM::SetMandConsistencyParams(bool flag) { params_.mand_consistency = flag ? 1 : 0; }
 void HPM::SetPrecision(int precision) { params_.precision = precision; }
 
-void HPM::InuputInitialization(const std::string& dense_folder, const std::vector<Problem>& problems, int idx) {
+void HPM::InuputInitialization(const std::string& dense_folder, const std::vector<Problem>& problems, int idx,
+                               SceneImages* images) {
   images_.clear();
   cams_.clear();
   depths_.clear();
@@ -502,14 +592,27 @@ void HPM::InuputInitialization(const std::string& dense_folder, const std::vecto
   std::vector<int> ids{problem.ref_image_id};
   ids.insert(ids.end(), problem.src_image_ids.begin(), problem.src_image_ids.end());
   for (int id : ids) {
-    MapU8 img = LoadImage(dense_folder, id);
     std::ostringstream cp;
     cp << dense_folder << "/cams/" << std::setw(8) << std::setfill('0') << id << "_cam.txt";
     hpm_camera c = ReadCamera(cp.str());
-    c.width = img.width;
-    c.height = img.height;
+    MapU8 loaded;
+    const MapU8* img;
+    if (images) {
+      img = &images->Current(id);
+      // K scaled by the prepared / original size, HPM.cpp:668-674
+      const MapU8& orig = images->Original(id);
+      const float scale_x = img->width / static_cast<float>(orig.width);
+      const float scale_y = img->height / static_cast<float>(orig.height);
+      c.K[0] *= scale_x; c.K[2] *= scale_x;
+      c.K[4] *= scale_y; c.K[5] *= scale_y;
+    } else {
+      loaded = LoadImage(dense_folder, id);
+      img = &loaded;
+    }
+    c.width = img->width;
+    c.height = img->height;
     cams_.push_back(c);
-    images_.push_back(ToGrayFloat(img));
+    images_.push_back(ToGrayFloat(*img));
   }
   if (params_.geom_consistency) {  // HPM.cpp:693-732
     const char* suffix = multi_geometry_ ? "/depths_geom.dmb" : "/depths.dmb";
@@ -529,6 +632,25 @@ void HPM::CudaSpaceInitialization(const std::string& dense_folder, const Problem
     // reload of the reference's own maps, HPM.cpp:813-846
     const std::string rf = ResultFolder(dense_folder, problem.ref_image_id);
     CudaHypothesesReload(depths_[0], LoadDmb(rf + "/costs.dmb"), LoadDmb(rf + "/normals.dmb"));
+  }
+  if (params_.hierarchy) {  // HPM.cpp:849-905
+    const std::string rf = ResultFolder(dense_folder, problem.ref_image_id);
+    const MapF fine_depth = LoadDmb(rf + "/depths.dmb");  // the JBU-upsampled depth
+    const MapF cn = LoadDmb(rf + "/normals.dmb");         // the coarser level's maps
+    const MapF cc = LoadDmb(rf + "/costs.dmb");
+    const int cw = cn.width, ch = cn.height;
+    if (fine_depth.width != GetReferenceImageWidth() || fine_depth.height != GetReferenceImageHeight())
+      throw Error("hierarchy pass: depths.dmb does not have the reference image size");
+    // the reference compares the coarse width with the fine rows (HPM.cpp:869)
+    const bool upsample = cw != GetReferenceImageHeight() || ch != GetReferenceImageWidth();
+    std::vector<float> coarse((size_t)4 * cw * ch);
+    for (size_t i = 0; i < (size_t)cw * ch; ++i) {
+      coarse[4 * i + 0] = cn.data[3 * i + 0];
+      coarse[4 * i + 1] = cn.data[3 * i + 1];
+      coarse[4 * i + 2] = cn.data[3 * i + 2];
+      coarse[4 * i + 3] = upsample ? cc.data[i] : (i < fine_depth.data.size() ? fine_depth.data[i] : 0.0f);
+    }
+    check(hpm_set_coarse(ctx_, coarse.data(), cw, ch, fine_depth.data.data()), "hpm_set_coarse");
   }
 }
 
@@ -613,6 +735,27 @@ void HPM::GetPriorPlaneParams(const Triangle& t, const MapF& depths, float plane
   PlaneThroughPoints(X1, X2, X3, plane);
 }
 
+// Get3DPointonRefCam_factor, HPM.cpp:247-256.
+static void PointOnRefCamFactor(int x, int y, float depth, const hpm_camera& c, float factor, float X[3]) {
+  X[0] = depth * ((float)x - c.K[2] * factor) / (c.K[0] * factor);
+  X[1] = depth * ((float)y - c.K[5] * factor) / (c.K[4] * factor);
+  X[2] = depth;
+}
+
+void HPM::GetPriorPlaneParamsFactor(const Triangle& t, const MapF& depths, float factor, float plane[4]) const {
+  float X1[3], X2[3], X3[3];
+  PointOnRefCamFactor(t.pt1.x, t.pt1.y, depths.at(t.pt1.x, t.pt1.y), cams_[0], factor, X1);
+  PointOnRefCamFactor(t.pt2.x, t.pt2.y, depths.at(t.pt2.x, t.pt2.y), cams_[0], factor, X2);
+  PointOnRefCamFactor(t.pt3.x, t.pt3.y, depths.at(t.pt3.x, t.pt3.y), cams_[0], factor, X3);
+  PlaneThroughPoints(X1, X2, X3, plane);
+}
+
+float HPM::GetDepthFromPlaneParamFactor(const float pl[4], int x, int y, float factor) const {  // HPM.cpp:1174-1177
+  const float* K = cams_[0].K;
+  return -pl[3] * (K[0] * factor) /
+         (((float)x - K[2] * factor) * pl[0] + (K[0] / K[4]) * ((float)y - K[5] * factor) * pl[1] + K[0] * factor * pl[2]);
+}
+
 float HPM::GetDepthFromPlaneParam(const float pl[4], int x, int y) const {  // HPM.cpp:1169-1172
   const float* K = cams_[0].K;
   return -pl[3] * K[0] / (((float)x - K[2]) * pl[0] + (K[0] / K[4]) * ((float)y - K[5]) * pl[1] + K[0] * pl[2]);
@@ -644,8 +787,103 @@ float HPM::GetDistance2Origin(int x, int y, float depth, const float n[4]) const
 
 static void MakeDir(const std::string& path) { ::mkdir(path.c_str(), 0755); }
 
+// Planar prior at distance 0 (main.cpp:326-441): support points, Delaunay,
+// per-triangle planes, prior depth range test.
+static void PriorAtLevel(HPM& hpm, const MapF& depths, const MapF& costs, const MapF& confidences) {
+  const int width = hpm.GetReferenceImageWidth(), height = hpm.GetReferenceImageHeight();
+  const std::vector<Point2i> support = GetSupportPoints(costs, confidences, width, height);
+  const std::vector<Triangle> tris = DelaunayTriangulation(width, height, support);
+  MapF mask(width, height);
+  std::vector<float> planes;
+  for (const Triangle& t : tris) {
+    float pl[4];
+    hpm.GetPriorPlaneParams(t, depths, pl);
+    planes.insert(planes.end(), pl, pl + 4);
+  }
+  RasterTriangles(tris, mask);
+  for (int i = 0; i < width; ++i)
+    for (int j = 0; j < height; ++j) {
+      if (mask.at(i, j) > 0) {
+        const float d = hpm.GetDepthFromPlaneParam(&planes[4 * ((size_t)mask.at(i, j) - 1)], i, j);
+        if (!(d <= hpm.GetMaxDepth() && d >= hpm.GetMinDepth())) mask.at(i, j) = 0;
+      }
+    }
+  hpm.CudaPlanarPriorInitialization(planes, mask);
+}
+
+// Planar prior 1 or 2 levels coarser (main.cpp:443-597): the maps are
+// downsampled by 1 / (2 d), the prior is built there, upsampled with the
+// depth+normal JBU and reloaded per pixel.
+static void PriorFromCoarser(HPM& hpm, const MapF& depths, const MapF& costs, const MapF& confidences, int distance) {
+  const int width = hpm.GetReferenceImageWidth(), height = hpm.GetReferenceImageHeight();
+  const float factor = (float)(1.0 / (distance * 2));
+  const int hw = (int)std::round(width * factor), hh = (int)std::round(height * factor);
+  const MapF depths_ds = ResizeLinear(depths, hw, hh);
+  const MapF costs_ds = ResizeLinear(costs, hw, hh);
+  const MapF conf_ds = ResizeLinear(confidences, hw, hh);
+  // GetSupportPoints_Classify_Check scans (int)(width * factor) columns (HPM.cpp:1030-1031)
+  const int sw = (int)(width * factor), sh = (int)(height * factor);
+  const std::vector<Point2i> support = GetSupportPoints(costs_ds, conf_ds, std::min(sw, hw), std::min(sh, hh));
+  const std::vector<Triangle> tris = DelaunayTriangulation(hw, hh, support);
+  MapF mask(hw, hh);
+  std::vector<float> planes;
+  for (const Triangle& t : tris) {
+    float pl[4];
+    hpm.GetPriorPlaneParamsFactor(t, depths_ds, factor, pl);
+    planes.insert(planes.end(), pl, pl + 4);
+  }
+  RasterTriangles(tris, mask);
+  MapF prior_d(hw, hh);
+  std::vector<float> prior_n((size_t)4 * hw * hh, 0.0f);  // world normal xyz, depth
+  for (int i = 0; i < hw; ++i)
+    for (int j = 0; j < hh; ++j) {
+      if (mask.at(i, j) > 0) {
+        const float* pl = &planes[4 * ((size_t)mask.at(i, j) - 1)];
+        const float d = hpm.GetDepthFromPlaneParamFactor(pl, i, j, factor);
+        if (d <= hpm.GetMaxDepth() * 1.2f && d >= hpm.GetMinDepth() * 0.6f) {
+          prior_d.at(i, j) = d;
+          float nw[4];
+          hpm.TransformNormal(pl, nw);
+          const size_t c = (size_t)j * hw + i;
+          prior_n[4 * c + 0] = nw[0];
+          prior_n[4 * c + 1] = nw[1];
+          prior_n[4 * c + 2] = nw[2];
+          prior_n[4 * c + 3] = d;
+        } else {
+          mask.at(i, j) = 0;
+        }
+      }
+    }
+  // JointBilateralUpsampling_prior (HPM.cpp:1323-1392): nothing is upsampled
+  // when the integer scale is 1; its NaN patch is overwritten at once
+  MapF up_d(width, height);
+  std::vector<float> up_n((size_t)4 * width * height, 0.0f);
+  const int scale = std::max(height / hh, width / hw);
+  if (scale != 1) {
+    const int rc = hpm_jbu_prior(hpm.GetReferenceImage().data.data(), width, height, prior_d.data.data(),
+                                 prior_n.data(), hw, hh, up_d.data.data(), up_n.data());
+    if (rc != HPM_OK) throw Error("hpm_jbu_prior failed");
+  }
+  MapF mask_new(width, height);
+  std::vector<float> prior_planes((size_t)4 * width * height, 0.0f);
+  for (int i = 0; i < width; ++i)
+    for (int j = 0; j < height; ++j) {
+      const float d = up_d.at(i, j);
+      if (d <= hpm.GetMaxDepth() && d >= hpm.GetMinDepth()) {
+        mask_new.at(i, j) = 1;
+        const size_t c = (size_t)j * width + i;
+        const float in[4] = {up_n[4 * c], up_n[4 * c + 1], up_n[4 * c + 2], d};
+        float pl[4];
+        hpm.TransformNormal2RefCam(in, pl);
+        pl[3] = hpm.GetDistance2Origin(i, j, d, pl);
+        std::memcpy(&prior_planes[4 * c], pl, 16);
+      }
+    }
+  hpm.ReloadPlanarPriorInitialization(mask_new, prior_planes);
+}
+
 void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>& problems, int idx,
-                    const PassFlags& flags, int precision, uint64_t seed) {
+                    const PassFlags& flags, int precision, uint64_t seed, SceneImages* images) {
   const Problem& problem = problems.at(idx);
   MakeDir(dense_folder + "/HPM_MVS_plusplus");
   const std::string rf = ResultFolder(dense_folder, problem.ref_image_id);
@@ -653,55 +891,58 @@ void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>&
   HPM hpm;
   hpm.SetPrecision(precision);
   if (flags.geom_consistency) hpm.SetGeomConsistencyParams(flags.multi_geometry);
-  hpm.InuputInitialization(dense_folder, problems, idx);
+  if (flags.hierarchy) hpm.SetHierarchyParams();
+  hpm.InuputInitialization(dense_folder, problems, idx, images);
   hpm.SetMandConsistencyParams(flags.mand_consistency);
   if (flags.prior_consistency) hpm.SetPlanarPriorParams();
   hpm.CudaSpaceInitialization(dense_folder, problem);
   const int width = hpm.GetReferenceImageWidth(), height = hpm.GetReferenceImageHeight();
 
-  // cv::Canny(image, 50, 150), main.cpp:287
-  MapU8 gray8(width, height);
-  for (size_t i = 0; i < gray8.data.size(); ++i) gray8.data[i] = (uint8_t)hpm.GetReferenceImage().data[i];
-  hpm.CudaCannyInitialization(CannyEdges(gray8, 50.0f, 150.0f));
+  // cv::Canny on the level's colour image, main.cpp:280-289
+  {
+    const MapU8 loaded = images ? MapU8() : LoadImage(dense_folder, problem.ref_image_id);
+    const MapU8& colour = images ? images->Current(problem.ref_image_id) : loaded;
+    if (colour.width != width || colour.height != height)
+      throw Error("prepared reference image size does not match the camera size");
+    hpm.CudaCannyInitialization(CannyEdges(colour, 50.0f, 150.0f));
+  }
   if (flags.mand_consistency || flags.prior_consistency) hpm.CudaConfidenceInitialization(dense_folder, problems, idx);
 
   MapF depths(width, height), normals(width, height, 3), costs(width, height), texture(width, height);
-  if (flags.prior_consistency) {  // main.cpp:324-441, distance 0
+  bool run = true;
+  if (flags.prior_consistency) {  // main.cpp:324-597
     const char* suffix = flags.multi_geometry ? "/depths_geom.dmb" : "/depths.dmb";
     depths = LoadDmb(rf + suffix);
     normals = LoadDmb(rf + "/normals.dmb");
     const MapF confidences = LoadDmb(rf + "/confidence.dmb");
     costs = LoadDmb(rf + "/costs.dmb");
-    const std::vector<Point2i> support = GetSupportPoints(costs, confidences, width, height);
-    const std::vector<Triangle> tris = DelaunayTriangulation(width, height, support);
-    MapF mask(width, height);
+    if (depths.width != width || depths.height != height) throw Error("prior pass: maps of another size in " + rf);
+    // the prior texture map must exist (main.cpp:353-355, 458-460); its
+    // texture flag is never used by the support-point selection
+    const int tex_level = flags.image_scale + flags.hpm_scale_distance;
+    if (flags.hpm_scale_distance >= 0 && flags.hpm_scale_distance <= 2)
+      (void)LoadDmb(rf + "/texture" + std::to_string(tex_level) + ".dmb");
+    if (flags.hpm_scale_distance == 0)
+      PriorAtLevel(hpm, depths, costs, confidences);
+    else if (flags.hpm_scale_distance == 1 || flags.hpm_scale_distance == 2)
+      PriorFromCoarser(hpm, depths, costs, confidences, flags.hpm_scale_distance);
+    // any other distance runs nothing: the loaded maps are written back
+    // unchanged, as main.cpp:349-597 does when neither branch is taken
+    run = flags.hpm_scale_distance >= 0 && flags.hpm_scale_distance <= 2;
+    if (run) hpm.CudaHypothesesReload(depths, costs, normals);
+  }
+  if (run) {
+    hpm.RunPatchMatch(seed);
     std::vector<float> planes;
-    for (const Triangle& t : tris) {
-      float pl[4];
-      hpm.GetPriorPlaneParams(t, depths, pl);
-      planes.insert(planes.end(), pl, pl + 4);
+    hpm.GetPlanes(planes);
+    for (size_t i = 0; i < (size_t)width * height; ++i) {
+      depths.data[i] = planes[4 * i + 3];
+      normals.data[3 * i + 0] = planes[4 * i + 0];
+      normals.data[3 * i + 1] = planes[4 * i + 1];
+      normals.data[3 * i + 2] = planes[4 * i + 2];
     }
-    RasterTriangles(tris, mask);
-    for (int i = 0; i < width; ++i)
-      for (int j = 0; j < height; ++j) {
-        if (mask.at(i, j) > 0) {
-          const float d = hpm.GetDepthFromPlaneParam(&planes[4 * ((size_t)mask.at(i, j) - 1)], i, j);
-          if (!(d <= hpm.GetMaxDepth() && d >= hpm.GetMinDepth())) mask.at(i, j) = 0;
-        }
-      }
-    hpm.CudaPlanarPriorInitialization(planes, mask);
-    hpm.CudaHypothesesReload(depths, costs, normals);
+    hpm.GetCosts(costs);
   }
-  hpm.RunPatchMatch(seed);
-  std::vector<float> planes;
-  hpm.GetPlanes(planes);
-  for (size_t i = 0; i < (size_t)width * height; ++i) {
-    depths.data[i] = planes[4 * i + 3];
-    normals.data[3 * i + 0] = planes[4 * i + 0];
-    normals.data[3 * i + 1] = planes[4 * i + 1];
-    normals.data[3 * i + 2] = planes[4 * i + 2];
-  }
-  hpm.GetCosts(costs);
   const std::string dpath = rf + (flags.geom_consistency ? "/depths_geom.dmb" : "/depths.dmb");
   if (!WriteDmb(dpath, depths) || !WriteDmb(rf + "/normals.dmb", normals) || !WriteDmb(rf + "/costs.dmb", costs))
     throw Error("cannot write results under " + rf);
@@ -710,6 +951,23 @@ void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>&
     if (!WriteDmb(rf + "/texture" + std::to_string(flags.image_scale) + ".dmb", texture))
       throw Error("cannot write texture under " + rf);
   }
+}
+
+void JointBilateralUpsampling(const std::string& dense_folder, const Problem& problem, SceneImages& images) {
+  const std::string rf = ResultFolder(dense_folder, problem.ref_image_id);
+  const MapF coarse = LoadDmb(rf + "/depths_geom.dmb");
+  const MapF fine = ToGrayFloat(images.Current(problem.ref_image_id));
+  const int scale = std::max(fine.height / coarse.height, fine.width / coarse.width);  // HPM.cpp:1270
+  if (scale == 1) return;  // HPM.cpp:1272-1275
+  MapF out(fine.width, fine.height);
+  if (hpm_jbu_depth(fine.data.data(), fine.width, fine.height, coarse.data.data(), coarse.width, coarse.height,
+                    out.data.data()) != HPM_OK)
+    throw Error("hpm_jbu_depth failed");
+  for (int i = 0; i < fine.width; ++i)  // NaN patch, HPM.cpp:1297-1300
+    for (int j = 0; j < fine.height; ++j)
+      if (out.at(i, j) != out.at(i, j))
+        out.at(i, j) = coarse.at(std::min(i / 2, coarse.width - 1), std::min(j / 2, coarse.height - 1));
+  if (!WriteDmb(rf + "/depths.dmb", out)) throw Error("cannot write " + rf + "/depths.dmb");
 }
 
 struct ViewMaps {
@@ -797,8 +1055,15 @@ std::vector<FusedPoint> RunFusion(const std::string& dense_folder, const std::ve
   const int rc = hpm_consistency_check(HPM_CHECK_FUSION, n, v.cams.data(), dp.data(), np.data(), v.offsets.data(),
                                        v.ids.data(), nullptr, pairs.data(), xyzn.data(), (int64_t)cap, &count);
   if (rc != HPM_OK) throw Error("hpm_consistency_check (fusion) failed");
+  // colours from the image resized to its depth map (RescaleImageAndCamera,
+  // HPM.cpp:186-207)
   std::vector<MapU8> imgs;
-  for (const Problem& p : problems) imgs.push_back(LoadImage(dense_folder, p.ref_image_id));
+  for (int i = 0; i < n; ++i) {
+    MapU8 img = LoadImage(dense_folder, problems[i].ref_image_id);
+    if (img.width != v.depths[i].width || img.height != v.depths[i].height)
+      img = ResizeLinear(img, v.depths[i].width, v.depths[i].height);
+    imgs.push_back(std::move(img));
+  }
   std::vector<FusedPoint> pts((size_t)count);
   for (int64_t k = 0; k < count; ++k) {
     FusedPoint& fp = pts[k];
@@ -833,6 +1098,86 @@ std::vector<FusedPoint> RunPipeline(const std::string& dense_folder, int precisi
     }
     f.geom_consistency = true;
     for (int i = 0; i < n; ++i) ProcessProblem(dense_folder, problems, i, f, precision, seed + 2000 * (it + 2) + i);
+  }
+  return RunFusion(dense_folder, problems, true);
+}
+
+std::vector<FusedPoint> RunPyramidPipeline(const std::string& dense_folder, int initial_image_scale, int size_bound,
+                                           int precision, uint64_t seed) {
+  if (initial_image_scale < 1) throw Error("initial image scale must be >= 1");
+  if (size_bound < 1) throw Error("size bound must be >= 1");
+  const std::vector<Problem> problems = ReadPairFile(dense_folder + "/pair.txt");
+  const int n = (int)problems.size();
+  MakeDir(dense_folder + "/HPM_MVS_plusplus");
+  SceneImages images(dense_folder);
+  // ComputeMultiScaleSettings, main.cpp:202-235
+  std::vector<int> initial_max(n), num_down(n), cur_max(n, 0);
+  int max_down = -1;
+  for (int i = 0; i < n; ++i) {
+    const MapU8& img = images.Original(problems[i].ref_image_id);
+    const int side = std::max(img.width, img.height);
+    initial_max[i] = std::max(1, (side + initial_image_scale / 2) / initial_image_scale);
+    int k = 0;
+    for (int m = initial_max[i]; m > size_bound; m /= 2) ++k;
+    num_down[i] = k;
+    max_down = std::max(max_down, k);
+  }
+  // every image a problem reads takes the level size of the problem whose
+  // reference it is (PyramidMaxSideForImage, main.cpp:93-99)
+  std::vector<int> ids;
+  for (const Problem& p : problems) {
+    ids.push_back(p.ref_image_id);
+    ids.insert(ids.end(), p.src_image_ids.begin(), p.src_image_ids.end());
+  }
+  std::sort(ids.begin(), ids.end());
+  ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+  auto level_side = [&](int id) {
+    for (int i = 0; i < n; ++i)
+      if (problems[i].ref_image_id == id) return cur_max[i];
+    throw Error("no pyramid size for source image " + std::to_string(id));
+  };
+
+  uint64_t pass = 0;
+  auto each_view = [&](PassFlags f, const std::vector<int>* distances) {
+    for (int i = 0; i < n; ++i) {
+      if (distances) f.hpm_scale_distance = (*distances)[i];
+      ProcessProblem(dense_folder, problems, i, f, precision, seed + 10007 * pass + i, &images);
+    }
+    ++pass;
+  };
+  const int max_hpm_scale = max_down;
+  for (int level = max_down; level >= 0; --level) {  // main.cpp:1235-1352
+    for (int i = 0; i < n; ++i)
+      if (num_down[i] >= 0) {
+        cur_max[i] = (int)(initial_max[i] / std::pow(2.0, num_down[i]));
+        --num_down[i];
+      }
+    for (int id : ids) images.PrepareLevel(id, level_side(id));
+
+    PassFlags f;
+    f.image_scale = level;
+    if (level != max_down) {  // upsample the coarser level, then a hierarchy pass
+      for (int i = 0; i < n; ++i) JointBilateralUpsampling(dense_folder, problems[i], images);
+      f.hierarchy = true;
+    }
+    each_view(f, nullptr);  // photometric or hierarchy pass
+    f.hierarchy = false;
+    f.prior_consistency = true;
+    for (int hs = max_hpm_scale; hs >= level; --hs) {  // prior passes, coarsest prior first
+      ConfidenceEvaluation(dense_folder, problems, false);
+      std::vector<int> dist(n);
+      for (int i = 0; i < n; ++i) dist[i] = hs - num_down[i] - 1;
+      each_view(f, &dist);
+    }
+    f.prior_consistency = false;
+    f.hpm_scale_distance = 0;
+    for (int it = 0; it < 3; ++it) {  // geometric passes
+      f.multi_geometry = it > 0;
+      f.mand_consistency = it > 0;
+      if (it > 0) ConfidenceEvaluation(dense_folder, problems, true);  // over the previous depths_geom
+      f.geom_consistency = true;
+      each_view(f, nullptr);
+    }
   }
   return RunFusion(dense_folder, problems, true);
 }

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/hpm_mi355x.h"
@@ -50,6 +51,32 @@ bool ReadNetpbm(const std::string& path, MapU8& out);
 // Grey float image of an 8-bit image: P5 as is; P6 with the ITU-R BT.601
 // weights 0.299 R + 0.587 G + 0.114 B rounded to the nearest integer.
 MapF ToGrayFloat(const MapU8& img);
+
+// cv::resize(src, dst, size, 0, 0, INTER_LINEAR): each destination pixel
+// samples the source at ((x + 0.5) sx - 0.5, (y + 0.5) sy - 0.5), sx = sw / dw,
+// bilinearly, with coordinates clamped to the border.  8-bit results are
+// rounded to the nearest integer (OpenCV interpolates 8-bit images in fixed
+// point; results can differ by one level).
+MapU8 ResizeLinear(const MapU8& src, int w, int h);
+MapF ResizeLinear(const MapF& src, int w, int h);
+
+// The dense folder's images at their original size and the scaled copies of
+// the current pyramid level (PrepareScaleImages, main.cpp:107-161, kept in
+// memory instead of temporary PNG files).
+class SceneImages {
+ public:
+  explicit SceneImages(std::string dense_folder) : folder_(std::move(dense_folder)) {}
+  const MapU8& Original(int id);
+  // scales image `id` so that its larger side is at most target_max_side
+  void PrepareLevel(int id, int target_max_side);
+  // the current level's image (the original when no level was prepared)
+  const MapU8& Current(int id);
+ private:
+  std::string folder_;
+  std::vector<MapU8> orig_, cur_;
+  std::vector<bool> have_orig_, have_cur_;
+  void ensure(int id);
+};
 
 struct FusedPoint {  // PointList, main.h:92-96
   float coord[3];
@@ -119,9 +146,11 @@ class HPM {
   // InuputInitialization (HPM.cpp:644-733): grey images and cameras of the
   // problem's reference and sources from `images/%08d.{pgm,ppm}` and
   // `cams/%08d_cam.txt`; source depth maps in geometric mode.
-  void InuputInitialization(const std::string& dense_folder, const std::vector<Problem>& problems, int idx);
+  void InuputInitialization(const std::string& dense_folder, const std::vector<Problem>& problems, int idx,
+                            SceneImages* images = nullptr);
   // CudaSpaceInitialization (HPM.cpp:741-906): uploads; reloads the
-  // reference's own previous maps in geometric mode.
+  // reference's own previous maps in geometric mode and the coarse level's
+  // maps in hierarchy mode.
   void CudaSpaceInitialization(const std::string& dense_folder, const Problem& problem);
   void CudaCannyInitialization(const MapU8& canny);
   void CudaConfidenceInitialization(const std::string& dense_folder, const std::vector<Problem>& problems, int idx);
@@ -145,6 +174,10 @@ class HPM {
   // GetPriorPlaneParams (HPM.cpp:1134-1167): plane of a triangle's corners in
   // the reference camera frame.
   void GetPriorPlaneParams(const Triangle& t, const MapF& depths, float plane[4]) const;
+  // the _factor variants (HPM.cpp:1099-1132, 1174-1177): the reference camera
+  // scaled by `factor`
+  void GetPriorPlaneParamsFactor(const Triangle& t, const MapF& depths, float factor, float plane[4]) const;
+  float GetDepthFromPlaneParamFactor(const float plane[4], int x, int y, float factor) const;
   float GetDepthFromPlaneParam(const float plane[4], int x, int y) const;
   void TransformNormal(const float in[4], float out[4]) const;
   void TransformNormal2RefCam(const float in[4], float out[4]) const;
@@ -166,15 +199,22 @@ struct PassFlags {
   bool geom_consistency = false;
   bool prior_consistency = false;
   bool mand_consistency = false;
+  bool hierarchy = false;
   bool multi_geometry = false;
-  int image_scale = 0;  // the texture{k}.dmb index
+  int image_scale = 0;          // the pyramid level (texture{k}.dmb index)
+  int hpm_scale_distance = 0;   // prior passes: 0 at the level, 1 or 2 levels coarser
 };
 
-// ProcessProblem (main.cpp:237-634) at one pyramid level: one pass of one
-// reference view, reading and writing the reference's files under
-// <dense_folder>/HPM_MVS_plusplus/2333_%08d/.  Prior passes at distance 0.
+// ProcessProblem (main.cpp:237-634): one pass of one reference view at the
+// current pyramid level, reading and writing the reference's files under
+// <dense_folder>/HPM_MVS_plusplus/2333_%08d/.  `images` holds the level's
+// images (nullptr: the originals from disk).
 void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>& problems, int idx,
-                    const PassFlags& flags, int precision, uint64_t seed);
+                    const PassFlags& flags, int precision, uint64_t seed, SceneImages* images = nullptr);
+// JointBilateralUpsampling (main.cpp:636-657) + RunJBU (HPM.cpp:1266-1319): the
+// coarser level's depths_geom.dmb upsampled to the current level's size,
+// guided by the reference image, written as depths.dmb.
+void JointBilateralUpsampling(const std::string& dense_folder, const Problem& problem, SceneImages& images);
 // ConfidenceEvaluation (main.cpp:959-1109) -> confidence.dmb per view.
 void ConfidenceEvaluation(const std::string& dense_folder, const std::vector<Problem>& problems, bool geom);
 // RunFusion (main.cpp:823-957) -> HPM_MVS_plusplus.ply and fused.ply; returns
@@ -184,6 +224,13 @@ std::vector<FusedPoint> RunFusion(const std::string& dense_folder, const std::ve
 // prior pass, three geometric passes (the last two with mandatory
 // consistency after a confidence evaluation), fusion.
 std::vector<FusedPoint> RunPipeline(const std::string& dense_folder, int precision, uint64_t seed);
+// The pyramid schedule of main.cpp:1183-1352: images scaled by
+// 1 / initial_image_scale, halved until the larger side is <= size_bound
+// (ComputeMultiScaleSettings, main.cpp:202-235; the reference's bound is 1000);
+// coarsest level as RunPipeline, finer levels JBU -> hierarchy pass -> prior
+// passes at distances max..0 -> three geometric passes; then fusion.
+std::vector<FusedPoint> RunPyramidPipeline(const std::string& dense_folder, int initial_image_scale, int size_bound,
+                                           int precision, uint64_t seed);
 
 std::string ResultFolder(const std::string& dense_folder, int ref_image_id);
 

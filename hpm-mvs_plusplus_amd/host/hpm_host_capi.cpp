@@ -65,6 +65,15 @@ int hpm_host_run_pipeline(const char* dense_folder, int precision, uint64_t seed
   return guarded([&] { *num_points = (int64_t)hpmhost::RunPipeline(dense_folder, precision, seed).size(); });
 }
 
+int hpm_host_run_pyramid_pipeline(const char* dense_folder, int initial_image_scale, int size_bound, int precision,
+                                  uint64_t seed, int64_t* num_points) {
+  if (!dense_folder || !num_points || initial_image_scale < 1 || size_bound < 1) return HPM_E_INVALID;
+  return guarded([&] {
+    *num_points = (int64_t)hpmhost::RunPyramidPipeline(dense_folder, initial_image_scale, size_bound, precision, seed)
+                      .size();
+  });
+}
+
 int hpm_host_support_points(const float* costs, const float* confidences, int w, int h, int32_t* out_xy,
                             int capacity, int* count) {
   if (!costs || !confidences || w <= 0 || h <= 0 || !count || (capacity > 0 && !out_xy)) return HPM_E_INVALID;
@@ -114,11 +123,11 @@ int hpm_host_plane_through_points(const float* X1, const float* X2, const float*
   return guarded([&] { hpmhost::PlaneThroughPoints(X1, X2, X3, plane); });
 }
 
-int hpm_host_canny(const uint8_t* gray, int w, int h, float low, float high, uint8_t* out) {
-  if (!gray || !out || w <= 0 || h <= 0) return HPM_E_INVALID;
+int hpm_host_canny(const uint8_t* image, int w, int h, int channels, float low, float high, uint8_t* out) {
+  if (!image || !out || w <= 0 || h <= 0 || (channels != 1 && channels != 3)) return HPM_E_INVALID;
   return guarded([&] {
-    hpmhost::MapU8 g(w, h);
-    std::memcpy(g.data.data(), gray, (size_t)w * h);
+    hpmhost::MapU8 g(w, h, channels);
+    std::memcpy(g.data.data(), image, (size_t)w * h * channels);
     const auto e = hpmhost::CannyEdges(g, low, high);
     std::memcpy(out, e.data.data(), (size_t)w * h);
   });

@@ -31,13 +31,15 @@ def host_lib():
     L.hpm_host_confidence.argtypes = [ctypes.c_char_p, ctypes.c_int]
     L.hpm_host_fusion.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
     L.hpm_host_run_pipeline.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64)]
+    L.hpm_host_run_pyramid_pipeline.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64)]
     L.hpm_host_support_points.argtypes = [fp, fp, ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_int)]
     L.hpm_host_delaunay.argtypes = [ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int, i32p, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_int)]
     L.hpm_host_raster.argtypes = [i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, fp]
     L.hpm_host_plane_through_points.argtypes = [fp, fp, fp, fp]
-    L.hpm_host_canny.argtypes = [u8p, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, u8p]
+    L.hpm_host_canny.argtypes = [u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, u8p]
     _lib = L
     return L
 
@@ -74,6 +76,16 @@ def run_pipeline(folder, precision="fp32", seed=1):
     n = ctypes.c_int64(0)
     _check(host_lib().hpm_host_run_pipeline(folder.encode(), PRECISIONS[precision], seed, ctypes.byref(n)),
            "hpm_host_run_pipeline")
+    return n.value
+
+
+def run_pyramid_pipeline(folder, initial_image_scale=4, size_bound=1000, precision="fp32", seed=1):
+    """main.cpp's pyramid schedule and fusion over a dense folder; returns the
+    number of fused points (the PLY files are written under HPM_MVS_plusplus/)."""
+    n = ctypes.c_int64(0)
+    _check(host_lib().hpm_host_run_pyramid_pipeline(folder.encode(), initial_image_scale, size_bound,
+                                                    PRECISIONS[precision], seed, ctypes.byref(n)),
+           "hpm_host_run_pyramid_pipeline")
     return n.value
 
 
@@ -114,10 +126,12 @@ def plane_through_points(X1, X2, X3):
     return out
 
 
-def canny(gray, low=50.0, high=150.0):
-    g = np.ascontiguousarray(gray, np.uint8)
-    out = np.zeros_like(g)
-    _check(host_lib().hpm_host_canny(_ptr(g, ctypes.c_uint8), g.shape[1], g.shape[0], low, high,
+def canny(image, low=50.0, high=150.0):
+    """Edge map of an (h, w) grey or (h, w, 3) R G B uint8 image; 255 = edge."""
+    g = np.ascontiguousarray(image, np.uint8)
+    channels = 1 if g.ndim == 2 else g.shape[2]
+    out = np.zeros(g.shape[:2], np.uint8)
+    _check(host_lib().hpm_host_canny(_ptr(g, ctypes.c_uint8), g.shape[1], g.shape[0], channels, low, high,
                                      _ptr(out, ctypes.c_uint8)), "canny")
     return out
 

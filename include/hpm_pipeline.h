@@ -32,6 +32,16 @@ int hpm_host_confidence(const char* dense_folder, int geom_consistency);
 int hpm_host_fusion(const char* dense_folder, int geom_consistency, int64_t* num_points);
 /* The single-level schedule of main.cpp:1249-1291 followed by fusion. */
 int hpm_host_run_pipeline(const char* dense_folder, int precision, uint64_t seed, int64_t* num_points);
+/* The pyramid schedule of main.cpp:1183-1352 followed by fusion
+ * (main.cpp:1353-1359 without --mask): images scaled by 1/initial_image_scale
+ * (the reference's --scale, default 4), halved until the larger side is
+ * <= size_bound (ComputeMultiScaleSettings, main.cpp:202-235, where the bound
+ * is fixed at 1000).  Coarsest level: photometric, prior and geometric
+ * passes; each finer level: JBU of the coarser depths_geom, a hierarchy pass,
+ * prior passes from 2 and 1 levels coarser down to the level itself, and three
+ * geometric passes. */
+int hpm_host_run_pyramid_pipeline(const char* dense_folder, int initial_image_scale, int size_bound, int precision,
+                                  uint64_t seed, int64_t* num_points);
 
 /* GetSupportPoints_Classify_Check (HPM.cpp:1026-1073) at factor 1: costs and
  * confidences w*h; out_xy receives up to `capacity` (x, y) pairs. */
@@ -46,8 +56,10 @@ int hpm_host_raster(const int32_t* tris, int n, int w, int h, float* mask);
 /* GetPriorPlaneParams' plane through three camera-frame points
  * (HPM.cpp:1134-1167): plane = (n, w), |n| = 1, w >= 0. */
 int hpm_host_plane_through_points(const float* X1, const float* X2, const float* X3, float* plane);
-/* Edge map standing in for cv::Canny (main.cpp:287): 255 = edge. */
-int hpm_host_canny(const uint8_t* gray, int w, int h, float low, float high, uint8_t* out);
+/* Edge map standing in for cv::Canny (main.cpp:287): 255 = edge.  `image` is
+ * w*h*channels bytes, channels 1 (grey) or 3 (R G B, as a PPM holds it); a
+ * colour pixel takes the gradient of its channel of largest |dx| + |dy|. */
+int hpm_host_canny(const uint8_t* image, int w, int h, int channels, float low, float high, uint8_t* out);
 
 #ifdef __cplusplus
 }

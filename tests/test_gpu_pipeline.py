@@ -7,6 +7,11 @@ Checks: every pass writes the reference's files (.dmb with the reference
 layout, both PLY files); the final geometric depth maps agree with the
 ray-cast ground truth; fusion through the pipeline equals the oracle's
 consistency check on the maps it read (same points, same order).
+
+The pyramid schedule (main.cpp:1183-1352) runs on a 320x240 folder with a
+size bound of 100: levels of 80x60, 160x120 and 320x240, so JBU, the
+hierarchy pass and prior passes from 2 and 1 levels coarser all run; the
+finest level's depth maps must reach the same ground-truth accuracy.
 """
 import os
 
@@ -79,3 +84,30 @@ def test_pipeline_fusion_equals_oracle(dense_folder):
     gp, gx = H.consistency_check(H.CHECK_FUSION, hc, depths, normals, sources)
     np.testing.assert_array_equal(gp, rp)
     np.testing.assert_array_equal(gx, rx)
+
+
+def test_pyramid_pipeline_three_levels(tmp_path):
+    from hpm_mi355x import pipeline as P
+    from hpm_mi355x import scenes
+    sc = scenes.make_plane_scene(seed=4, n_cams=5, width=320, height=240, f=262.5, n_src=4)
+    folder = str(tmp_path)
+    scenes.write_dense_folder(sc, folder)
+    print("case: run_pyramid_pipeline 5 views 320x240 bound 100 fp16", flush=True)
+    n = P.run_pyramid_pipeline(folder, initial_image_scale=1, size_bound=100, precision="fp16", seed=9)
+    assert n > 4 * 5000
+    out = os.path.join(folder, "HPM_MVS_plusplus")
+    for i in range(5):
+        rf = os.path.join(out, f"2333_{i:08d}")
+        # the photometric / hierarchy pass of each level writes its texture map at the level's size
+        for level, shape in ((2, (60, 80)), (1, (120, 160)), (0, (240, 320))):
+            assert P.read_dmb(os.path.join(rf, f"texture{level}.dmb")).shape == shape
+        d = P.read_dmb(os.path.join(rf, "depths_geom.dmb"))
+        assert d.shape == (240, 320)
+        gt = sc.depths[i]
+        rel = np.abs(d - gt) / gt
+        print(f"view {i}: depth within 1% of GT {np.mean(rel <= 0.01):.3f}")
+        assert np.mean(rel <= 0.01) > 0.75
+    with open(os.path.join(out, "HPM_MVS_plusplus.ply"), "rb") as fh:
+        head = fh.read(400).split(b"end_header\n")[0].decode()
+    assert f"element vertex {n}" in head
+

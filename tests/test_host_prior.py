@@ -131,3 +131,25 @@ def test_edge_map_step_and_flat():
     assert len(cols) == 1 and cols[0] in (14, 15)
     assert np.all(e[:, cols[0]] == 255)
     assert not P.canny(np.full((20, 30), 77, np.uint8)).any()
+
+
+def test_edge_map_colour_takes_the_strongest_channel():
+    """A colour image: equal channels give the grey map; an edge present in one
+    channel only is found; the channel of largest |dx| + |dy| decides, so a
+    weak edge in another channel does not hide it."""
+    g = np.zeros((20, 30), np.uint8)
+    g[:, 15:] = 200
+    grey = P.canny(g)
+    np.testing.assert_array_equal(P.canny(np.stack([g, g, g], -1)), grey)
+    for c in range(3):
+        rgb = np.zeros((20, 30, 3), np.uint8)
+        rgb[..., c] = g
+        rgb[..., (c + 1) % 3] = g // 20  # weak step (10 levels) below the low threshold on its own
+        np.testing.assert_array_equal(P.canny(rgb), grey)
+    # a horizontal step in one channel and a vertical one in another: both found
+    rgb = np.zeros((40, 40, 3), np.uint8)
+    rgb[:, 20:, 0] = 200
+    rgb[20:, :, 2] = 200
+    e = P.canny(rgb)
+    assert e[5, 19:21].any() and e[19:21, 5].any()
+
