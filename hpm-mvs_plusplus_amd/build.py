@@ -14,8 +14,9 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "lib", "libhpm_mi355x.so")
-SOURCES = [os.path.join(HERE, "csrc", "hpm_capi.hip")]
-HEADERS = ["hpm_math.h", "hpm_device.h", "hpm_kernels.h", "hpm_jbu.h", "hpm_fast.h", "hpm_fusion.h"]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("hpm_capi.hip", "hpm_batch.hip", "hpm_batch_lp.hip")]
+HEADERS = ["hpm_math.h", "hpm_device.h", "hpm_kernels.h", "hpm_jbu.h", "hpm_fast.h", "hpm_fusion.h", "hpm_grad.h",
+           "hpm_internal.h", "hpm_multi.h", "hpm_batch_impl.h"]
 HOST_LIB = os.path.join(HERE, "lib", "libhpm_host.so")
 HOST_SOURCES = [os.path.join(HERE, "host", "hpm_host.cpp"), os.path.join(HERE, "host", "hpm_host_capi.cpp")]
 GXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wextra"]
@@ -36,12 +37,26 @@ def hipcc():
 
 def build(verbose=False, extra=()):
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
+    # the translation units compile in parallel, then link into one library
+    objs, procs = [], []
+    for src in SOURCES:
+        obj = os.path.join(os.path.dirname(LIB), os.path.basename(src) + ".o")
+        cmd = [hipcc()] + [f for f in HIPCC_FLAGS if f != "-shared"] + list(extra) + ["-c", src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((subprocess.Popen(cmd, cwd=HERE), cmd))
+        objs.append(obj)
+    for p, cmd in procs:
+        if p.wait() != 0:
+            raise subprocess.CalledProcessError(p.returncode, cmd)
     tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + list(extra) + SOURCES + ["-o", tmp]
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True, cwd=HERE)
     os.replace(tmp, LIB)
+    for obj in objs:
+        os.remove(obj)
     tmp = HOST_LIB + ".tmp"
     cmd = (["g++"] + GXX_FLAGS + HOST_SOURCES + ["-o", tmp, "-L" + os.path.dirname(LIB), "-lhpm_mi355x",
            "-Wl,-rpath,$ORIGIN"])

@@ -1,0 +1,219 @@
+// hpm_batch.hip -- fused multi-view RunPatchMatch: the schedules of many
+// contexts (reference views of any size and source count) run as one launch
+// per stage -- init, every colour pass, output -- over the concatenated tile
+// grids of all views, optionally replayed as one hipGraph.  The launch shape
+// (colour tile width, XCD strip height) is a run-time choice that
+// hpm_batch_autotune picks by timing.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "hpm_batch_impl.h"
+
+using namespace hpm;
+using namespace hpm_batch_detail;
+
+int hpm_batch_detail::bfail(hpm_batch* b, int code, const std::string& msg) {
+  if (b) b->err = msg;
+  return code;
+}
+
+namespace {
+
+int colour_tiles(int tile_w, int W, int H2) {
+  const int ch = 256 / tile_w;
+  return (((W + 1) / 2 + tile_w - 1) / tile_w) * ((H2 + ch - 1) / ch);
+}
+
+int round8(int n) { return (n + 7) & ~7; }
+
+// (Re)builds the slots and workgroup tables when a view's buffers, size or
+// source count, or the colour tile width, changed since the last run.
+int ensure_layout(hpm_batch* b) {
+  std::vector<unsigned char> key;
+  append(key, b->tile_w);
+  for (hpm_ctx* c : b->ctxs) {
+    append(key, hpm_detail::state_of(c));
+    append(key, c->d_consts);
+    append(key, c->W);
+    append(key, c->H);
+    append(key, c->nimg);
+  }
+  if (key == b->layout_key && b->d_slots) return HPM_OK;
+  const int n = (int)b->ctxs.size();
+  std::vector<MultiSlot> slots(n);
+  std::vector<uint16_t> wg[3];
+  b->max_v = 0;
+  for (int i = 0; i < n; ++i) {
+    hpm_ctx* c = b->ctxs[i];
+    MultiSlot& m = slots[i];
+    std::memset(&m, 0, sizeof(m));
+    m.Cp = c->d_consts;
+    m.S = hpm_detail::state_of(c);
+    m.H2 = (c->H / 2) * 2;
+    m.count[kStageInit] = tiles_2d<ExactEngine>(c->W, c->H);  // 16 x 16 tiles for every one-lane engine
+    m.count[kStageColour] = colour_tiles(b->tile_w, c->W, m.H2);
+    m.count[kStageOutput] = (int)(((size_t)c->W * c->H + 255) / 256);
+    for (int s = 0; s < 3; ++s) {
+      m.first[s] = (int)wg[s].size();
+      wg[s].insert(wg[s].end(), (size_t)round8(m.count[s]), (uint16_t)i);
+    }
+    if (c->nimg - 1 > b->max_v) b->max_v = c->nimg - 1;
+  }
+  if (b->d_slots) { hipFree(b->d_slots); b->d_slots = nullptr; }
+  for (auto& p : b->d_wg) if (p) { hipFree(p); p = nullptr; }
+  BHIP_OK(hipMalloc((void**)&b->d_slots, sizeof(MultiSlot) * n));
+  BHIP_OK(hipMemcpy(b->d_slots, slots.data(), sizeof(MultiSlot) * n, hipMemcpyHostToDevice));
+  for (int s = 0; s < 3; ++s) {
+    if (wg[s].size() > (size_t)0x7fffffff) return bfail(b, HPM_E_INVALID, "batch grid too large");
+    b->grid[s] = (int)wg[s].size();
+    BHIP_OK(hipMalloc((void**)&b->d_wg[s], sizeof(uint16_t) * wg[s].size()));
+    BHIP_OK(hipMemcpy(b->d_wg[s], wg[s].data(), sizeof(uint16_t) * wg[s].size(), hipMemcpyHostToDevice));
+  }
+  b->layout_key = key;
+  return HPM_OK;
+}
+
+bool widths_for(int precision, int tile_w) {
+  return tile_w == 32 || (precision != HPM_PREC_FP32_EXACT && (tile_w == 16 || tile_w == 64));
+}
+
+int run_once(hpm_batch* b, const uint64_t* seeds) {
+  const int n = (int)b->ctxs.size();
+  b->consts.resize(n);
+  int precision = -1, iters = -1;
+  for (int i = 0; i < n; ++i) {
+    hpm_ctx* c = b->ctxs[i];
+    if (c->device != b->device) return bfail(b, HPM_E_INVALID, "contexts on different devices");
+    if (i == 0) { precision = c->params.precision; iters = c->params.num_iterations; }
+    if (c->params.precision != precision || c->params.num_iterations != iters)
+      return bfail(b, HPM_E_INVALID, "a fused batch needs one precision and one iteration count");
+    const int rc = hpm_detail::prepare_run(c, seeds[i], b->consts[i]);
+    if (rc) return bfail(b, rc, "view " + std::to_string(i) + ": " + c->err);
+  }
+  BHIP_OK(hipSetDevice(b->device));
+  for (hpm_ctx* c : b->ctxs) BHIP_OK(hipStreamSynchronize(c->stream));  // quad builds, uploads
+  if (!widths_for(precision, b->tile_w)) return bfail(b, HPM_E_INVALID, "colour tile width not available for this precision");
+  int rc = ensure_layout(b);
+  if (rc) return rc;
+  for (int i = 0; i < n; ++i) {
+    BHIP_OK(hipMemcpyAsync(b->ctxs[i]->d_consts, &b->consts[i], sizeof(RunConsts), hipMemcpyHostToDevice, b->stream));
+    BHIP_OK(hipMemsetAsync(b->ctxs[i]->d_counters, 0, 8 * sizeof(unsigned long long), b->stream));
+  }
+  BHIP_OK(hipEventRecord(b->ev[0], b->stream));
+  switch (precision) {
+    case HPM_PREC_FP16: rc = run_engine<FastFor<QuadF16>, true>(b, precision, iters); break;
+    case HPM_PREC_BF16: rc = run_engine<FastFor<QuadBF16>, true>(b, precision, iters); break;
+    case HPM_PREC_FP32_FAST: rc = run_engine<FastFor<QuadF32>, true>(b, precision, iters); break;
+    default: rc = run_engine<ExactEngine, false>(b, precision, iters); break;
+  }
+  if (rc) return rc;
+  BHIP_OK(hipEventRecord(b->ev[1], b->stream));
+  BHIP_OK(hipStreamSynchronize(b->stream));
+  BHIP_OK(hipEventElapsedTime(&b->last_ms, b->ev[0], b->ev[1]));
+  for (hpm_ctx* c : b->ctxs) c->stats.propagate_launches = 2 * iters;
+  return HPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hpm_batch_create(hpm_batch** out, hpm_ctx* const* ctxs, int n) {
+  if (!out) return HPM_E_INVALID;
+  *out = nullptr;
+  if (!ctxs || n < 1 || n > 65535) return HPM_E_INVALID;
+  for (int i = 0; i < n; ++i)
+    if (!ctxs[i] || ctxs[i]->device != ctxs[0]->device) return HPM_E_INVALID;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < i; ++j)
+      if (ctxs[i] == ctxs[j]) return HPM_E_INVALID;  // one view's state cannot be two slots
+  hpm_batch* b = new hpm_batch();
+  b->device = ctxs[0]->device;
+  b->ctxs.assign(ctxs, ctxs + n);
+  if (hipSetDevice(b->device) != hipSuccess || hipStreamCreate(&b->stream) != hipSuccess ||
+      hipEventCreate(&b->ev[0]) != hipSuccess || hipEventCreate(&b->ev[1]) != hipSuccess) {
+    hpm_batch_destroy(b);
+    return HPM_E_DEVICE;
+  }
+  *out = b;
+  return HPM_OK;
+}
+
+void hpm_batch_destroy(hpm_batch* b) {
+  if (!b) return;
+  hipSetDevice(b->device);
+  if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->gexec) hipGraphExecDestroy(b->gexec);
+  if (b->d_slots) hipFree(b->d_slots);
+  for (auto p : b->d_wg) if (p) hipFree(p);
+  for (auto e : b->ev) if (e) hipEventDestroy(e);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+}
+
+const char* hpm_batch_last_error(const hpm_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int hpm_batch_set_graph(hpm_batch* b, int enable) {
+  if (!b) return HPM_E_INVALID;
+  b->use_graph = enable != 0;
+  return HPM_OK;
+}
+
+int hpm_batch_set_shape(hpm_batch* b, int tile_w, int strip) {
+  if (!b) return HPM_E_INVALID;
+  if (tile_w == 0) tile_w = HPM_COLOUR_TILE_W;
+  if (tile_w != 16 && tile_w != 32 && tile_w != 64) return bfail(b, HPM_E_INVALID, "tile width must be 16, 32 or 64");
+  if (strip < 1 || strip > 8) return bfail(b, HPM_E_INVALID, "strip must be in [1, 8]");
+  b->tile_w = tile_w;
+  b->strip = strip;
+  return HPM_OK;
+}
+
+int hpm_batch_get_shape(const hpm_batch* b, int* tile_w, int* strip) {
+  if (!b || !tile_w || !strip) return HPM_E_INVALID;
+  *tile_w = b->tile_w;
+  *strip = b->strip;
+  return HPM_OK;
+}
+
+int hpm_batch_run(hpm_batch* b, const uint64_t* seeds) {
+  if (!b || !seeds) return HPM_E_INVALID;
+  return run_once(b, seeds);
+}
+
+float hpm_batch_last_ms(const hpm_batch* b) { return b ? b->last_ms : 0.0f; }
+
+int hpm_batch_autotune(hpm_batch* b, const uint64_t* seeds, float* best_ms) {
+  if (!b || !seeds) return HPM_E_INVALID;
+  for (hpm_ctx* c : b->ctxs) {
+    const hpm_params& P = c->params;
+    if (P.geom_consistency || P.prior_consistency || P.hierarchy || P.mand_consistency)
+      return bfail(b, HPM_E_STATE, "autotuning reruns the schedule: photometric passes only (init overwrites the state)");
+  }
+  const int precision = b->ctxs[0]->params.precision;
+  const int widths[3] = {16, 32, 64}, strips[3] = {1, 2, 4};
+  int best_w = b->tile_w, best_s = b->strip;
+  float best = -1.0f;
+  for (int w : widths) {
+    if (!widths_for(precision, w)) continue;
+    for (int s : strips) {
+      b->tile_w = w;
+      b->strip = s;
+      const int rc = run_once(b, seeds);
+      if (rc) return rc;
+      if (best < 0.0f || b->last_ms < best) { best = b->last_ms; best_w = w; best_s = s; }
+    }
+  }
+  b->tile_w = best_w;
+  b->strip = best_s;
+  // leave the views with the result of the chosen shape
+  const int rc = run_once(b, seeds);
+  if (rc) return rc;
+  if (best_ms) *best_ms = best;
+  return HPM_OK;
+}
+
+}  // extern "C"

@@ -12,66 +12,12 @@
 #include "../../include/hpm_mi355x.h"
 #include "hpm_fast.h"
 #include "hpm_fusion.h"
+#include "hpm_internal.h"
 #include "hpm_grad.h"
 #include "hpm_jbu.h"
 #include "hpm_kernels.h"
 
 using namespace hpm;
-
-// Mixed-precision engine: four lanes per pixel (QuadLaneEngine) or one
-// (FastEngine); both give the same costs to within FP32 rounding.
-#ifndef HPM_FAST_LANES
-#define HPM_FAST_LANES 1
-#endif
-template <class S> using FastFor = typename std::conditional<HPM_FAST_LANES == 4, QuadLaneEngine<S>, FastEngine<S>>::type;
-
-// Images of a whole scene on one device, shared by the contexts of its
-// reference views: FP32 images plus, per storage precision, the quad images of
-// the mixed-precision engine (built on first use).
-struct hpm_pool {
-  int device = 0;
-  std::vector<hpm_camera> cams;
-  std::vector<float*> d_img;
-  std::vector<void*> quads[4];  // indexed by HPM_PREC_* (0 unused)
-};
-
-struct hpm_ctx {
-  int device = 0;
-  hpm_pool* pool = nullptr;        // images borrowed from a pool (not owned)
-  std::vector<int> pool_ids;
-  hipStream_t stream = nullptr;
-  std::string err;
-  hpm_params params{};
-  int nimg = 0, W = 0, H = 0;
-  std::vector<hpm_camera> cams;
-  std::vector<float*> d_img;
-  std::vector<float*> d_dimg;
-  std::vector<void*> d_img_lp;     // FP16 / BF16 copies (fast paths)
-  int lp_kind = -1;                // precision of d_img_lp
-  float4* d_planes = nullptr;
-  float* d_costs = nullptr;
-  float* d_pre_costs = nullptr;
-  float* d_conf = nullptr;
-  float* d_texture = nullptr;
-  uint32_t* d_sel = nullptr;
-  uint32_t* d_masks = nullptr;
-  uint32_t* d_canny = nullptr;
-  float4* d_prior = nullptr;
-  float4* d_scaled = nullptr;
-  int scaled_w = 0, scaled_h = 0;
-  bool upsample = false;
-  RunConsts* d_consts = nullptr;
-  unsigned long long* d_counters = nullptr;
-  bool timing = false;
-  hipEvent_t ev[8] = {};
-  hpm_stats stats{};
-  // hipGraph of the kernel sequence of one RunPatchMatch, replayed while the
-  // kernel arguments it was captured with stay valid
-  bool use_graph = false;
-  hipGraphExec_t gexec = nullptr;
-  std::vector<unsigned char> gkey;  // what the captured arguments depend on
-  int g_launches = 0;
-};
 
 namespace {
 
@@ -629,8 +575,11 @@ int hpm_set_timing(hpm_ctx* ctx, int enable) {
   return HPM_OK;
 }
 
-int hpm_run_patchmatch_partial(hpm_ctx* ctx, uint64_t seed, int n_halfpasses, int do_output) {
-  if (!ctx) return HPM_E_INVALID;
+}  // extern "C"
+
+namespace hpm_detail {
+
+int prepare_run(hpm_ctx* ctx, uint64_t seed, RunConsts& C) {
   if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
   const hpm_params& P = ctx->params;
   if (P.geom_consistency && (int)ctx->d_dimg.size() != ctx->nimg)
@@ -641,9 +590,25 @@ int hpm_run_patchmatch_partial(hpm_ctx* ctx, uint64_t seed, int n_halfpasses, in
   HIP_OK(hipSetDevice(ctx->device));
   int rc;
   if (P.precision != HPM_PREC_FP32_EXACT && (rc = ensure_quads(ctx, P.precision))) return rc;
-  RunConsts C;
   build_consts(ctx, C);
   C.seed = seed;
+  return HPM_OK;
+}
+
+DevState state_of(hpm_ctx* ctx) { return make_state(ctx); }
+
+int record_error(hpm_ctx* ctx, int code, const std::string& msg) { return fail(ctx, code, msg); }
+
+}  // namespace hpm_detail
+
+extern "C" {
+
+int hpm_run_patchmatch_partial(hpm_ctx* ctx, uint64_t seed, int n_halfpasses, int do_output) {
+  if (!ctx) return HPM_E_INVALID;
+  RunConsts C;
+  int rc = hpm_detail::prepare_run(ctx, seed, C);
+  if (rc) return rc;
+  const hpm_params& P = ctx->params;
   HIP_OK(hipMemcpyAsync(ctx->d_consts, &C, sizeof(C), hipMemcpyHostToDevice, ctx->stream));
   HIP_OK(hipMemsetAsync(ctx->d_counters, 0, 8 * sizeof(unsigned long long), ctx->stream));
   switch (P.precision) {

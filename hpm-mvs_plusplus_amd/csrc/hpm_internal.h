@@ -1,0 +1,76 @@
+// hpm_internal.h -- the context and image-pool structures behind the opaque
+// handles of include/hpm_mi355x.h, shared by the translation units of the
+// library (single-view schedule and C ABI; fused multi-view schedule).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/hpm_mi355x.h"
+#include "hpm_fast.h"
+
+// Mixed-precision engine: four lanes per pixel (QuadLaneEngine) or one
+// (FastEngine); both give the same costs to within FP32 rounding.
+#ifndef HPM_FAST_LANES
+#define HPM_FAST_LANES 1
+#endif
+template <class S> using FastFor = typename std::conditional<HPM_FAST_LANES == 4, hpm::QuadLaneEngine<S>, hpm::FastEngine<S>>::type;
+
+// Images of a whole scene on one device, shared by the contexts of its
+// reference views: FP32 images plus, per storage precision, the quad images of
+// the mixed-precision engine (built on first use).
+struct hpm_pool {
+  int device = 0;
+  std::vector<hpm_camera> cams;
+  std::vector<float*> d_img;
+  std::vector<void*> quads[4];  // indexed by HPM_PREC_* (0 unused)
+};
+
+struct hpm_ctx {
+  int device = 0;
+  hpm_pool* pool = nullptr;        // images borrowed from a pool (not owned)
+  std::vector<int> pool_ids;
+  hipStream_t stream = nullptr;
+  std::string err;
+  hpm_params params{};
+  int nimg = 0, W = 0, H = 0;
+  std::vector<hpm_camera> cams;
+  std::vector<float*> d_img;
+  std::vector<float*> d_dimg;
+  std::vector<void*> d_img_lp;     // FP16 / BF16 copies (fast paths)
+  int lp_kind = -1;                // precision of d_img_lp
+  float4* d_planes = nullptr;
+  float* d_costs = nullptr;
+  float* d_pre_costs = nullptr;
+  float* d_conf = nullptr;
+  float* d_texture = nullptr;
+  uint32_t* d_sel = nullptr;
+  uint32_t* d_masks = nullptr;
+  uint32_t* d_canny = nullptr;
+  float4* d_prior = nullptr;
+  float4* d_scaled = nullptr;
+  int scaled_w = 0, scaled_h = 0;
+  bool upsample = false;
+  hpm::RunConsts* d_consts = nullptr;
+  unsigned long long* d_counters = nullptr;
+  bool timing = false;
+  hipEvent_t ev[8] = {};
+  hpm_stats stats{};
+  // hipGraph of the kernel sequence of one RunPatchMatch, replayed while the
+  // kernel arguments it was captured with stay valid
+  bool use_graph = false;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<unsigned char> gkey;  // what the captured arguments depend on
+  int g_launches = 0;
+};
+
+namespace hpm_detail {
+// Validates the context for a run, builds the quad images of its storage
+// precision when needed and fills the run's constant block (seed included);
+// nothing is launched.  Returns HPM_OK or an error recorded on the context.
+int prepare_run(hpm_ctx* ctx, uint64_t seed, hpm::RunConsts& C);
+hpm::DevState state_of(hpm_ctx* ctx);
+int record_error(hpm_ctx* ctx, int code, const std::string& msg);
+}  // namespace hpm_detail

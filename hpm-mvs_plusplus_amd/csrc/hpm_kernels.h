@@ -38,16 +38,18 @@ struct Sample {
 };
 
 // Pixel tiles of a 256-thread (1D) block.  Engines with one lane per pixel
-// cover 16 x 16 pixels (2D kernels) or 16 same-colour columns x 16 rows
-// (colour passes); four-lane engines cover 8 x 8 and 8 x 8.
-template <class E> struct Tiles {
-  static constexpr int px_per_block = 256 / E::kLanes;
-  static constexpr int sq = E::kLanes == 1 ? 16 : 8;          // 2D tile side
+// cover 16 x 16 pixels (2D kernels) or CW same-colour columns x 256 / CW rows
+// (colour passes, CW = HPM_COLOUR_TILE_W unless a launch picks another
+// width); four-lane engines cover 8 x 8 and 8 x 8.
 #ifndef HPM_COLOUR_TILE_W
 #define HPM_COLOUR_TILE_W 32
 #endif
-  static constexpr int cw = E::kLanes == 1 ? HPM_COLOUR_TILE_W : 8;  // colour tile width (same-colour columns)
-  static constexpr int ch = px_per_block / cw;                // colour tile height
+template <class E, int CW = (E::kLanes == 1 ? HPM_COLOUR_TILE_W : 8)> struct Tiles {
+  static constexpr int px_per_block = 256 / E::kLanes;
+  static constexpr int sq = E::kLanes == 1 ? 16 : 8;  // 2D tile side
+  static constexpr int cw = CW;                       // colour tile width (same-colour columns)
+  static constexpr int ch = px_per_block / cw;        // colour tile height
+  static_assert(px_per_block % CW == 0, "colour tile width must divide the pixels of a block");
 };
 
 #ifndef HPM_XCD_STRIP
@@ -56,35 +58,43 @@ template <class E> struct Tiles {
 // Workgroup -> tile, XCD-aware.  The dispatcher deals workgroups round-robin to
 // the 8 XCDs (XCD = id % 8), each with its own 4 MB L2.  Every XCD gets one
 // contiguous range of tiles, walked column-major inside horizontal strips of
-// HPM_XCD_STRIP tiles, so the workgroups an XCD runs at the same time cover a
-// compact area and the source texels they gather stay in its L2.
-__device__ __forceinline__ void xcd_tile(int nx, int ny, int& tx, int& ty) {
-  const int id = (int)blockIdx.x;
+// `strip` tiles, so the workgroups an XCD runs at the same time cover a
+// compact area and the source texels they gather stay in its L2.  `id` is the
+// workgroup's index among the nx * ny of its view; a multi-view launch starts
+// every view at a multiple of 8 so that id % 8 is still the XCD.
+__device__ __forceinline__ void xcd_tile(int id, int nx, int ny, int strip, int& tx, int& ty) {
   const int N = nx * ny;
   const int xcd = id & 7, k = id >> 3;
   const int base = N >> 3, rem = N & 7;
   const int t = xcd * base + (xcd < rem ? xcd : rem) + k;  // XCD x owns base + (x < rem) tiles
-  const int S = HPM_XCD_STRIP;
-  const int strip = t / (nx * S);
-  const int h = (ny - strip * S) < S ? (ny - strip * S) : S;
-  const int r = t - strip * nx * S;
+  const int S = strip;
+  const int band = t / (nx * S);
+  const int h = (ny - band * S) < S ? (ny - band * S) : S;
+  const int r = t - band * nx * S;
   tx = r / h;
-  ty = strip * S + r % h;
+  ty = band * S + r % h;
 }
 
-template <class E> __device__ __forceinline__ void tile_pixel(int& px, int& py, int W, int H) {
+template <class E> __host__ __device__ constexpr int tiles_2d(int W, int H) {
+  return ((W + Tiles<E>::sq - 1) / Tiles<E>::sq) * ((H + Tiles<E>::sq - 1) / Tiles<E>::sq);
+}
+template <class E, int CW> __host__ __device__ constexpr int tiles_colour(int W, int H2) {
+  return (((W + 1) / 2 + Tiles<E, CW>::cw - 1) / Tiles<E, CW>::cw) * ((H2 + Tiles<E, CW>::ch - 1) / Tiles<E, CW>::ch);
+}
+template <class E> __device__ __forceinline__ void tile_pixel(int id, int strip, int& px, int& py, int W, int H) {
   const int pix = (int)threadIdx.x / E::kLanes;
   int tx, ty;
-  xcd_tile((W + Tiles<E>::sq - 1) / Tiles<E>::sq, (H + Tiles<E>::sq - 1) / Tiles<E>::sq, tx, ty);
+  xcd_tile(id, (W + Tiles<E>::sq - 1) / Tiles<E>::sq, (H + Tiles<E>::sq - 1) / Tiles<E>::sq, strip, tx, ty);
   px = tx * Tiles<E>::sq + pix % Tiles<E>::sq;
   py = ty * Tiles<E>::sq + pix / Tiles<E>::sq;
 }
-template <class E> __device__ __forceinline__ void colour_pixel(int& cx, int& py, int W, int H2) {
+template <class E, int CW> __device__ __forceinline__ void colour_pixel(int id, int strip, int& cx, int& py, int W, int H2) {
+  using T = Tiles<E, CW>;
   const int pix = (int)threadIdx.x / E::kLanes;
   int tx, ty;
-  xcd_tile(((W + 1) / 2 + Tiles<E>::cw - 1) / Tiles<E>::cw, (H2 + Tiles<E>::ch - 1) / Tiles<E>::ch, tx, ty);
-  cx = tx * Tiles<E>::cw + pix % Tiles<E>::cw;
-  py = ty * Tiles<E>::ch + pix / Tiles<E>::cw;
+  xcd_tile(id, ((W + 1) / 2 + T::cw - 1) / T::cw, (H2 + T::ch - 1) / T::ch, strip, tx, ty);
+  cx = tx * T::cw + pix % T::cw;
+  py = ty * T::ch + pix / T::cw;
 }
 // counts once per pixel: only the first lane of a multi-lane pixel contributes
 template <class E> __device__ __forceinline__ unsigned first_lane(unsigned x) {
@@ -197,10 +207,9 @@ __device__ __forceinline__ void upsample_init(const RunConsts& C, const DevState
 }
 
 template <class E>
-__global__ HPM_PROP_ATTR void k_init(const RunConsts* __restrict__ Cp, DevState S) {
-  const RunConsts& C = *Cp;
+__device__ __forceinline__ void init_body(const RunConsts& C, const DevState& S, int id, int strip) {
   int px, py;
-  tile_pixel<E>(px, py, C.W, C.H);
+  tile_pixel<E>(id, strip, px, py, C.W, C.H);
   unsigned nncc = 0;
   if (px < C.W && py < C.H) {
     const int center = py * C.W + px;
@@ -271,6 +280,11 @@ __global__ HPM_PROP_ATTR void k_init(const RunConsts* __restrict__ Cp, DevState 
     }
   }
   wave_count_add(&S.counters[0], first_lane<E>(nncc));
+}
+
+template <class E>
+__global__ HPM_PROP_ATTR void k_init(const RunConsts* __restrict__ Cp, DevState S) {
+  init_body<E>(*Cp, S, (int)blockIdx.x, HPM_XCD_STRIP);
 }
 
 // ------------------------------------------------------------------ propagation
@@ -736,13 +750,13 @@ __device__ __forceinline__ void propagate_pixel_mand(const RunConsts& C, const D
 
 // One colour pass.  Thread (tx, y) of a 32 x 8 block handles the tx-th pixel of
 // the pass's colour in row y: x = 2 tx + ((y + colour) & 1).
-template <class E, bool kLds>
-__global__ HPM_PROP_ATTR void k_propagate(const RunConsts* __restrict__ Cp, DevState S, int colour, int iter, int H2) {
-  const RunConsts& C = *Cp;
+template <class E, bool kLds, int CW>
+__device__ __forceinline__ void propagate_body(const RunConsts& C, const DevState& S, int colour, int iter, int H2, int id,
+                                               int strip) {
   extern __shared__ float lds_rows[];
   float private_rows[kLds ? 1 : 8 * kMaxSrc];
   int tx, py;
-  colour_pixel<E>(tx, py, C.W, H2);
+  colour_pixel<E, CW>(id, strip, tx, py, C.W, H2);
   const int px = 2 * tx + ((py + colour) & 1);
   unsigned nncc = 0, n_ext = 0;
   bool active = px < C.W && py < H2;
@@ -765,15 +779,25 @@ __global__ HPM_PROP_ATTR void k_propagate(const RunConsts* __restrict__ Cp, DevS
   wave_count_add(&S.counters[2], first_lane<E>(ref_count));
 }
 
-__global__ __launch_bounds__(256) void k_output(const RunConsts* __restrict__ Cp, float4* planes) {  // HPM.cu:2829-2842
-  const RunConsts& C = *Cp;
-  const int px = blockIdx.x * blockDim.x + threadIdx.x;
-  const int py = blockIdx.y * blockDim.y + threadIdx.y;
-  if (px >= C.W || py >= C.H) return;
+template <class E, bool kLds>
+__global__ HPM_PROP_ATTR void k_propagate(const RunConsts* __restrict__ Cp, DevState S, int colour, int iter, int H2) {
+  propagate_body<E, kLds, Tiles<E>::cw>(*Cp, S, colour, iter, H2, (int)blockIdx.x, HPM_XCD_STRIP);
+}
+
+// GetDepthandNormal, HPM.cu:2829-2842
+__device__ __forceinline__ void output_pixel(const RunConsts& C, float4* planes, int px, int py) {
   const int center = py * C.W + px;
   float4 pl = planes[center];
   pl.w = depth_from_plane(C.cams[0], pl, px, py);
   planes[center] = to_world(C.cams[0], pl);
+}
+
+static __global__ __launch_bounds__(256) void k_output(const RunConsts* __restrict__ Cp, float4* planes) {
+  const RunConsts& C = *Cp;
+  const int px = blockIdx.x * blockDim.x + threadIdx.x;
+  const int py = blockIdx.y * blockDim.y + threadIdx.y;
+  if (px >= C.W || py >= C.H) return;
+  output_pixel(C, planes, px, py);
 }
 
 // ComputeMultiViewCostVector over all pixels at fixed planes (cost microbench).
@@ -782,7 +806,7 @@ __global__ __launch_bounds__(256) void k_cost_fixed(const RunConsts* __restrict_
                                                     float* __restrict__ out) {
   const RunConsts& C = *Cp;
   int px, py;
-  tile_pixel<E>(px, py, C.W, C.H);
+  tile_pixel<E>((int)blockIdx.x, HPM_XCD_STRIP, px, py, C.W, C.H);
   if (px >= C.W || py >= C.H) return;
   const int p = py * C.W + px;
   typename E::Win rw;
@@ -810,7 +834,7 @@ __global__ __launch_bounds__(256) void k_init_cost(const RunConsts* __restrict__
   sel[p] = m;
 }
 
-__global__ __launch_bounds__(256) void k_geom_cost(const RunConsts* __restrict__ Cp, const float4* __restrict__ planes,
+static __global__ __launch_bounds__(256) void k_geom_cost(const RunConsts* __restrict__ Cp, const float4* __restrict__ planes,
                                                    float* __restrict__ out) {
   const RunConsts& C = *Cp;
   const int px = blockIdx.x * blockDim.x + threadIdx.x;

@@ -60,6 +60,8 @@ EXPORTS = [
     "hpm_geom_cost", "hpm_jbu_depth", "hpm_jbu_prior", "hpm_consistency_check", "hpm_set_graph",
     "hpm_run_patchmatch_batch", "hpm_pool_create", "hpm_pool_destroy", "hpm_set_views_pooled",
     "hpm_cost_grad", "hpm_cost_grad_device", "hpm_refine_gn", "hpm_refine_random",
+    "hpm_batch_create", "hpm_batch_destroy", "hpm_batch_last_error", "hpm_batch_run", "hpm_batch_set_graph",
+    "hpm_batch_set_shape", "hpm_batch_get_shape", "hpm_batch_last_ms", "hpm_batch_autotune",
 ]
 
 _lib = None
@@ -105,6 +107,18 @@ def lib():
     L.hpm_pool_destroy.restype = None
     L.hpm_set_views_pooled.argtypes = [P, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]
     L.hpm_run_patchmatch_batch.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
+    L.hpm_batch_create.argtypes = [ctypes.POINTER(P), ctypes.POINTER(P), ctypes.c_int]
+    L.hpm_batch_destroy.argtypes = [P]
+    L.hpm_batch_destroy.restype = None
+    L.hpm_batch_last_error.argtypes = [P]
+    L.hpm_batch_last_error.restype = ctypes.c_char_p
+    L.hpm_batch_run.argtypes = [P, ctypes.POINTER(ctypes.c_uint64)]
+    L.hpm_batch_set_graph.argtypes = [P, ctypes.c_int]
+    L.hpm_batch_set_shape.argtypes = [P, ctypes.c_int, ctypes.c_int]
+    L.hpm_batch_get_shape.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.hpm_batch_last_ms.argtypes = [P]
+    L.hpm_batch_last_ms.restype = ctypes.c_float
+    L.hpm_batch_autotune.argtypes = [P, ctypes.POINTER(ctypes.c_uint64), fp]
     for n in ("hpm_get_planes", "hpm_get_costs", "hpm_get_texture", "hpm_get_confidence"):
         getattr(L, n).argtypes = [P, fp]
     L.hpm_get_selected_views.argtypes = [P, up]
@@ -431,6 +445,65 @@ def run_batch(contexts, seeds):
     if rc != HPM_OK:
         msgs = [lib().hpm_last_error(c.handle) for c in contexts]
         raise HpmError(f"hpm_run_patchmatch_batch failed ({rc}): " + "; ".join(m.decode() for m in msgs if m))
+
+
+class Batch:
+    """hpm_batch: the RunPatchMatch schedules of several contexts fused into
+    one launch per stage (views of any size; one precision)."""
+
+    def __init__(self, contexts):
+        L = lib()
+        n = len(contexts)
+        self._ctxs = list(contexts)  # the batch holds their handles
+        hs = (ctypes.c_void_p * n)(*[c.handle for c in contexts])
+        h = ctypes.c_void_p()
+        rc = L.hpm_batch_create(ctypes.byref(h), hs, n)
+        if rc != HPM_OK:
+            raise HpmError(f"hpm_batch_create failed ({rc})")
+        self._h = h
+
+    def _check(self, rc, what):
+        if rc != HPM_OK:
+            raise HpmError(f"{what} failed ({rc}): {lib().hpm_batch_last_error(self._h).decode()}")
+
+    def _seeds(self, seeds):
+        if len(seeds) != len(self._ctxs):
+            raise ValueError("one seed per context")
+        return (ctypes.c_uint64 * len(seeds))(*[int(s) for s in seeds])
+
+    def run(self, seeds):
+        self._check(lib().hpm_batch_run(self._h, self._seeds(seeds)), "hpm_batch_run")
+
+    def set_graph(self, on=True):
+        self._check(lib().hpm_batch_set_graph(self._h, int(on)), "hpm_batch_set_graph")
+
+    def set_shape(self, tile_w=32, strip=2):
+        self._check(lib().hpm_batch_set_shape(self._h, int(tile_w), int(strip)), "hpm_batch_set_shape")
+
+    def shape(self):
+        w, s = ctypes.c_int(), ctypes.c_int()
+        self._check(lib().hpm_batch_get_shape(self._h, ctypes.byref(w), ctypes.byref(s)), "hpm_batch_get_shape")
+        return w.value, s.value
+
+    def last_ms(self):
+        return float(lib().hpm_batch_last_ms(self._h))
+
+    def autotune(self, seeds):
+        """Times every launch shape, keeps the fastest; returns (tile_w, strip, ms)."""
+        ms = ctypes.c_float()
+        self._check(lib().hpm_batch_autotune(self._h, self._seeds(seeds), ctypes.byref(ms)), "hpm_batch_autotune")
+        return self.shape() + (ms.value,)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hpm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class ImagePool:

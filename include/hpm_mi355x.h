@@ -155,6 +155,37 @@ int hpm_set_graph(hpm_ctx* ctx, int enable);
  * seeds[i] keys context i's random streams. */
 int hpm_run_patchmatch_batch(hpm_ctx* const* ctxs, int n, const uint64_t* seeds);
 
+/* Fused multi-view RunPatchMatch.  The reference runs one view per
+ * ProcessProblem (main.cpp:237-634, RunPatchMatch HPM.cu:2987-3035); a batch
+ * runs the schedules of n contexts -- views of any size and source count, e.g.
+ * all reference views of a pass or several pyramid levels -- as one launch per
+ * stage (init, each colour pass, output) over the concatenated tile grids of
+ * all views, optionally replayed as one hipGraph.  Every context keeps its own
+ * parameters, modes and buffers; all must share one device, one precision and
+ * one iteration count.  A context's results equal hpm_run_patchmatch's with the
+ * same seed.  The batch holds the context pointers: keep the contexts alive
+ * while it exists (their views and modes may change between runs). */
+typedef struct hpm_batch hpm_batch;
+int hpm_batch_create(hpm_batch** out, hpm_ctx* const* ctxs, int n);
+void hpm_batch_destroy(hpm_batch* batch);
+const char* hpm_batch_last_error(const hpm_batch* batch);
+/* Runs every context's RunPatchMatch; seeds[i] keys context i's random streams.
+ * Returns when all views are done. */
+int hpm_batch_run(hpm_batch* batch, const uint64_t* seeds);
+int hpm_batch_set_graph(hpm_batch* batch, int enable);
+/* Launch shape: colour tile width in same-colour columns (16, 32 or 64; 0 =
+ * 32; the exact FP32 path takes 32 only) and the height, in tiles, of the
+ * strips each XCD walks (1..8).  Results do not depend on it. */
+int hpm_batch_set_shape(hpm_batch* batch, int tile_w, int strip);
+int hpm_batch_get_shape(const hpm_batch* batch, int* tile_w, int* strip);
+/* Device time of the last run's schedule, milliseconds (HIP events). */
+float hpm_batch_last_ms(const hpm_batch* batch);
+/* Runs the batch once per launch shape (tile widths x strips 1, 2, 4), keeps
+ * the fastest and runs it once more, so the views hold its result; *best_ms =
+ * its schedule time.  Photometric passes only (their init overwrites all
+ * state, so reruns are identical); otherwise HPM_E_STATE. */
+int hpm_batch_autotune(hpm_batch* batch, const uint64_t* seeds, float* best_ms);
+
 /* Waits for the context stream. */
 int hpm_synchronize(hpm_ctx* ctx);
 

@@ -6,6 +6,10 @@
   both the exact and the mixed-precision engine.
 * hpm_run_patchmatch_batch on contexts of different sizes and view counts
   gives the same bits as running each context alone.
+* The fused batch (hpm_batch: one launch per stage over all views) gives the
+  same bits, counters and texture maps as single runs -- for every launch
+  shape, replayed from its hipGraph, in geometric-consistency mode, with a
+  view beyond the LDS row limit (25 sources), and after autotuning.
 """
 import numpy as np
 import pytest
@@ -113,3 +117,105 @@ def test_pooled_views_match_copied_views(prec):
         np.testing.assert_array_equal(c.costs(), d.costs())
     pooled[1].run(seeds[1])  # alone, after the batch
     np.testing.assert_array_equal(pooled[1].planes(), copied[1].planes())
+
+
+def _uneven_problems(n_views=4, extra=()):
+    from hpm_mi355x import scenes
+    specs = [(160, 120, 131.25, 5, 2, 1), (128, 96, 105.0, 7, 3, 3), (96, 72, 78.75, 4, 1, 2),
+             (200, 150, 164.0, 6, 2, 5)][:n_views] + list(extra)
+    out = []
+    for (w, h, f, n, ref, seed) in specs:
+        sc = scenes.make_plane_scene(seed=seed, n_cams=n, width=w, height=h, f=f, n_src=n - 1)
+        out.append((sc,) + tuple(scenes.reference_view_problem(sc, ref, n - 1)))
+    return out
+
+
+def _same(a, b):
+    np.testing.assert_array_equal(a.planes(), b.planes())
+    np.testing.assert_array_equal(a.costs(), b.costs())
+    np.testing.assert_array_equal(a.texture(), b.texture())
+    sa, sb = a.stats(), b.stats()
+    assert (sa.ncc_init, sa.ncc_propagate, sa.ncc_reference) == (sb.ncc_init, sb.ncc_propagate, sb.ncc_reference)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16", "bf16"])
+def test_fused_batch_matches_single_runs(prec):
+    import hpm_mi355x as H
+    print(f"case: fused batch {prec}, 4 views 96x72..200x150, V 3..6", flush=True)
+    probs = _uneven_problems()
+    fused = [_ctx(c, i, precision=prec) for (_, _, c, i) in probs]
+    single = [_ctx(c, i, precision=prec) for (_, _, c, i) in probs]
+    seeds = [21, 22, 23, 24]
+    for s, sd in zip(single, seeds):
+        s.run(sd)
+    b = H.Batch(fused)
+    shapes = [(32, 2), (32, 1), (32, 4)] + ([(16, 1), (64, 2), (16, 4)] if prec != "fp32" else [])
+    for graph in (False, True):
+        b.set_graph(graph)
+        for (w, st) in shapes:
+            b.set_shape(w, st)
+            for _ in range(2 if graph else 1):  # capture, then replay
+                b.run(seeds)
+                for f, s in zip(fused, single):
+                    _same(f, s)
+    assert b.last_ms() > 0
+    if prec == "fp32":
+        with pytest.raises(H.HpmError):
+            b.set_shape(16, 2)
+            b.run(seeds)
+
+
+def test_fused_batch_geometric_mode():
+    """A geometric-consistency pass (source depth maps, reloaded hypotheses)
+    through the fused batch equals the single-context pass."""
+    import hpm_mi355x as H
+    print("case: fused batch geom fp32 + fp16, 3 views", flush=True)
+    probs = _uneven_problems(3)
+    for prec in ("fp32", "fp16"):
+        ctxs = {k: [_ctx(c, i, precision=prec) for (_, _, c, i) in probs] for k in ("fused", "single")}
+        base = []
+        for (sc, ids, c, i), s in zip(probs, ctxs["single"]):
+            s.run(5)
+            base.append((s.planes(), s.costs()))
+        for k in ("fused", "single"):
+            for (sc, ids, c, i), ctx, (p, cost) in zip(probs, ctxs[k], base):
+                ctx.set_params(H.default_params(precision=prec, geom_consistency=1))
+                ctx.set_src_depths([None] + [sc.depths[j] for j in ids[1:]])
+                ctx.reload_hypotheses(p, cost)
+        for s, sd in zip(ctxs["single"], (7, 8, 9)):
+            s.run(sd)
+        b = H.Batch(ctxs["fused"])
+        b.set_graph(True)
+        b.run([7, 8, 9])
+        for f, s in zip(ctxs["fused"], ctxs["single"]):
+            np.testing.assert_array_equal(f.planes(), s.planes())
+            np.testing.assert_array_equal(f.costs(), s.costs())
+        with pytest.raises(H.HpmError):  # reruns would not start from the same state
+            b.autotune([7, 8, 9])
+
+
+def test_fused_batch_private_rows_and_autotune():
+    """A view with 25 sources (beyond the LDS rows) in the batch switches all
+    views to private rows; results still equal single runs, and autotuning
+    keeps a valid shape whose result equals them too."""
+    import hpm_mi355x as H
+    print("case: fused batch fp16 with a 25-source view + autotune", flush=True)
+    probs = _uneven_problems(2, extra=[(64, 48, 52.5, 26, 12, 7)])
+    fused = [_ctx(c, i, precision="fp16") for (_, _, c, i) in probs]
+    single = [_ctx(c, i, precision="fp16") for (_, _, c, i) in probs]
+    seeds = [31, 32, 33]
+    for s, sd in zip(single, seeds):
+        s.run(sd)
+    b = H.Batch(fused)
+    b.run(seeds)
+    for f, s in zip(fused, single):
+        _same(f, s)
+    w, st, ms = b.autotune(seeds)
+    assert w in (16, 32, 64) and st in (1, 2, 4) and ms > 0
+    for f, s in zip(fused, single):
+        _same(f, s)
+    # mixed precisions are refused
+    fused[0].set_params(H.default_params(precision="fp32"))
+    with pytest.raises(H.HpmError):
+        b.run(seeds)
+
