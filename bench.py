@@ -7,8 +7,9 @@ plane -> depth/normal output), inputs resident in HBM.
 
 One "step" is one batched RunPatchMatch of --batch reference views (default 4:
 views 3-6 of an 11-camera scene, each with its 10 source views, all reading
-one shared image pool), issued through hpm_run_patchmatch_batch with hipGraph
-replay.  The headline `value`
+one shared image pool), issued as one fused batch (hpm_batch: one launch per
+stage over all views' tiles, replayed from a hipGraph; --batch-mode streams
+issues one stream per view instead).  The headline `value`
 is views/s of that batched mixed-precision path (FP16 texel storage, FP32
 accumulation by default).  Beside it: the same path one view at a time
 (`single_view`), and the plain FP32 path (bit-exact with the FP32 oracle, one
@@ -31,7 +32,13 @@ runs only that copy kernel (4 GiB read + 4 GiB written), for the counter runs;
     python bench.py --workload cost
 
 prints the cost-at-fixed-hypotheses report (FP32 exact / FP32 / FP16 / BF16
-image storage: ms, algorithmic GB/s, max and mean |cost - cost_FP64|).
+image storage: ms, algorithmic GB/s, max and mean |cost - cost_FP64|);
+
+    python bench.py --workload uneven
+
+prints the uneven-size batch report: 64 reference views drawn from 640x480,
+1024x768, 1600x1200 and 2048x1536 with 6-10 sources each, one at a time vs
+multi-stream vs fused (default and autotuned launch shape).
 """
 import argparse
 import json
@@ -145,6 +152,87 @@ def run_views(ctx, steps, warmup, seed0):
             "ncc_prop": ncc_prop, "ncc_ref": ncc_ref}
 
 
+def uneven_batch_bench(precision="fp16", n_views=64, reps=3):
+    """64 reference views of uneven sizes (640x480 .. 2048x1536, drawn
+    uniformly), 6-10 sources each, over one 40-plane scene: aggregate views/s
+    of the one-view-at-a-time loop (one RunPatchMatch per view, as the
+    reference's ProcessProblem loop runs them), of the multi-stream batch, and
+    of the fused batch (one launch per stage over all views, hipGraph) with
+    the default and the autotuned launch shape.  The fused results are checked
+    bit for bit against the loop's."""
+    import hashlib
+    import hpm_mi355x as Hm
+    from hpm_mi355x import scenes
+    t0 = time.perf_counter()
+    sc = scenes.make_batch_scene(seed=11, n_views=n_views)
+    print(f"[bench] uneven scene rendered in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    pool = Hm.ImagePool([Hm.camera_from_scene(c) for c in sc.cameras], [im.astype(np.float32) for im in sc.images])
+    ctxs = []
+    for i in range(n_views):
+        ids = [i] + [j for j, _ in sc.pairs[i]]
+        c = Hm.Context(0)
+        c.set_params(Hm.default_params(precision=precision))
+        c.set_views_pooled(pool, ids)
+        c.set_canny(canny_like(sc.images[i].astype(np.float32)))
+        ctxs.append(c)
+    seeds = [SEED + i for i in range(n_views)]
+    mpix = sum(c.width * c.height for c in ctxs) / 1e6
+
+    def digest():
+        return [hashlib.sha1(c.planes().tobytes()).hexdigest() for c in ctxs]
+
+    def timed(fn):
+        fn()  # warm-up (quad images, graph capture)
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return reps * n_views / (time.perf_counter() - t)
+
+    def loop():
+        for c, s in zip(ctxs, seeds):
+            c.run(s)
+
+    res = {"one_at_a_time": timed(loop)}
+    ref = digest()
+    print(f"[bench] one at a time: {res['one_at_a_time']:.2f} views/s", file=sys.stderr, flush=True)
+    res["streams"] = timed(lambda: Hm.run_batch(ctxs, seeds))
+    same_streams = digest() == ref
+    print(f"[bench] streams: {res['streams']:.2f} views/s", file=sys.stderr, flush=True)
+    b = Hm.Batch(ctxs)
+    b.set_graph(True)
+    res["fused"] = timed(lambda: b.run(seeds))
+    same_fused = digest() == ref
+    print(f"[bench] fused: {res['fused']:.2f} views/s", file=sys.stderr, flush=True)
+    w, st, ms = b.autotune(seeds)
+    res["fused_autotuned"] = timed(lambda: b.run(seeds))
+    same_tuned = digest() == ref
+    print(f"[bench] fused, autotuned ({w}, {st}): {res['fused_autotuned']:.2f} views/s", file=sys.stderr, flush=True)
+    within = []
+    for i, c in enumerate(ctxs):
+        d = c.planes()[..., 3]
+        g = sc.depths[i]
+        within.append(float(np.mean(np.abs(d - g) / g <= 0.01)))
+    sizes = {}
+    for c in sc.cameras:
+        sizes[f"{c.width}x{c.height}"] = sizes.get(f"{c.width}x{c.height}", 0) + 1
+    b.close()
+    for c in ctxs:
+        c.close()
+    pool.close()
+    best = max(res["fused"], res["fused_autotuned"])
+    return {"metric": "aggregate views/s, 64 reference views of uneven size (640x480..2048x1536), 6-10 sources",
+            "value": round(best, 3), "unit": "views/s", "higher_is_better": True, "precision": precision,
+            "views": n_views, "mpix_total": round(mpix, 2), "sizes": sizes,
+            "sources": [len(p) for p in sc.pairs],
+            "views_s": {k: round(v, 3) for k, v in res.items()},
+            "speedup_fused_vs_one_at_a_time": round(best / res["one_at_a_time"], 3),
+            "autotuned_shape": {"colour_tile_w": w, "xcd_strip": st, "schedule_ms": round(ms, 3)},
+            "bit_exact_vs_one_at_a_time": {"streams": same_streams, "fused": same_fused, "fused_autotuned": same_tuned},
+            "depth_within_1pct_of_gt_mean": round(float(np.mean(within)), 4),
+            "data": "synthetic (seeded ray-cast: 40 textured planes + background, 64 cameras on two rings)",
+            "reps": reps}
+
+
 def cost_microbench(reps=20):
     """Cost at fixed hypotheses (hpm_cost_fixed_device): 1600x1200 reference +
     10 sources, one random plane per pixel (depth U[1, 5], normal within 60 deg
@@ -200,8 +288,11 @@ def cost_microbench(reps=20):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="patchmatch", choices=["patchmatch", "cost"],
-                    help="patchmatch: the RunPatchMatch throughput line; cost: the cost microbenchmark report")
+    ap.add_argument("--workload", default="patchmatch", choices=["patchmatch", "cost", "uneven"],
+                    help="patchmatch: the RunPatchMatch throughput line; cost: the cost microbenchmark report; "
+                         "uneven: 64 reference views of uneven size, one-at-a-time vs batched")
+    ap.add_argument("--batch-mode", default="fused", choices=["fused", "streams"],
+                    help="fused: one launch per stage over the batch (hpm_batch); streams: one stream per view")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -222,6 +313,9 @@ def main():
         return
     if args.workload == "cost":
         print(json.dumps(cost_microbench()), flush=True)
+        return
+    if args.workload == "uneven":
+        print(json.dumps(uneven_batch_bench(args.precision)), flush=True)
         return
 
     import hpm_mi355x as Hm
@@ -252,13 +346,22 @@ def main():
         c.set_graph(True)
         batch.append((bids, c))
     ctxs = [c for _, c in batch]
-    for i in range(args.warmup):
-        Hm.run_batch(ctxs, [SEED + 10 * i + k for k in range(len(ctxs))])
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        Hm.run_batch(ctxs, [SEED + 1000 + 10 * k + j for j in range(len(ctxs))])
-    elapsed_b = time.perf_counter() - t0
-    views_s_b = args.steps * len(ctxs) / elapsed_b
+    fused = Hm.Batch(ctxs)
+    fused.set_graph(True)
+    runners = {"fused": lambda seeds: fused.run(seeds), "streams": lambda seeds: Hm.run_batch(ctxs, seeds)}
+    batch_rates = {}
+    for mode in [args.batch_mode] + [m for m in runners if m != args.batch_mode]:
+        for i in range(args.warmup):
+            runners[mode]([SEED + 10 * i + k for k in range(len(ctxs))])
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            runners[mode]([SEED + 1000 + 10 * k + j for j in range(len(ctxs))])
+        el = time.perf_counter() - t0
+        batch_rates[mode] = (args.steps * len(ctxs) / el, el)
+        print(f"[bench] {args.precision} batch of {len(ctxs)} ({mode}): {batch_rates[mode][0]:.2f} views/s",
+              file=sys.stderr, flush=True)
+    views_s_b, elapsed_b = batch_rates[args.batch_mode]
+    fused.close()
     within_b = []
     for bids, c in batch:
         d = c.planes()[..., 3]
@@ -267,7 +370,6 @@ def main():
     for c in ctxs:
         c.close()
     pool.close()
-    print(f"[bench] {args.precision} batch of {len(ctxs)}: {views_s_b:.2f} views/s", file=sys.stderr, flush=True)
 
     plain = None
     if not args.no_plain_fp32 and args.precision != "fp32":
@@ -319,13 +421,14 @@ def main():
         "data": "synthetic (seeded ray-cast textured planes, 11 cameras on a 30 deg arc)",
         "config": {"workload": "RunPatchMatch photometric, 1600x1200 reference + 10 source views, 3 iterations",
                    "precision": args.precision, "width": W_FULL, "height": H_FULL, "sources": NSRC,
-                   "views_per_step": len(ctxs), "batched": True, "hipgraph": True},
+                   "views_per_step": len(ctxs), "batched": args.batch_mode, "hipgraph": True},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if traffic is None else traffic["bytes_per_launch"],
                      "kernel": "k_propagate", "avg_launch_ms": round(avg_launch_s * 1e3, 3),
                      "alg_bytes_per_launch": int(alg_bytes), "ncc_evals_per_launch": int(per_launch_evals)},
         "single_view": {"views_s": round(r["views_s"], 4), "ms_per_step": round(1e3 * r["elapsed"] / args.steps, 3)},
+        "batch_modes_views_s": {k: round(v[0], 4) for k, v in batch_rates.items()},
         "plain_fp32": plain,
         "mpix_hyp_view_per_s": round(r["ncc_prop"] / (r["prop_ms"] / 1e3) / 1e6, 1) if r["prop_ms"] > 0 else None,
         "mpix_hyp_view_per_s_reference_count": round(r["ncc_ref"] / r["elapsed"] / 1e6, 1),

@@ -22,6 +22,16 @@ int hpm_batch_detail::bfail(hpm_batch* b, int code, const std::string& msg) {
 
 namespace {
 
+void free_layout(hpm_batch* b) {
+  void* bufs[] = {b->d_consts, b->d_states, b->d_views, b->d_wg[0], b->d_wg[1], b->d_wg[2]};
+  for (void* p : bufs)
+    if (p) hipFree(p);
+  b->d_consts = nullptr;
+  b->d_states = nullptr;
+  b->d_views = nullptr;
+  for (auto& p : b->d_wg) p = nullptr;
+}
+
 int colour_tiles(int tile_w, int W, int H2) {
   const int ch = 256 / tile_w;
   return (((W + 1) / 2 + tile_w - 1) / tile_w) * ((H2 + ch - 1) / ch);
@@ -36,22 +46,21 @@ int ensure_layout(hpm_batch* b) {
   append(key, b->tile_w);
   for (hpm_ctx* c : b->ctxs) {
     append(key, hpm_detail::state_of(c));
-    append(key, c->d_consts);
     append(key, c->W);
     append(key, c->H);
     append(key, c->nimg);
   }
-  if (key == b->layout_key && b->d_slots) return HPM_OK;
+  if (key == b->layout_key && b->d_views) return HPM_OK;
   const int n = (int)b->ctxs.size();
-  std::vector<MultiSlot> slots(n);
+  std::vector<MultiView> views(n);
+  std::vector<DevState> states(n);
   std::vector<uint16_t> wg[3];
   b->max_v = 0;
   for (int i = 0; i < n; ++i) {
     hpm_ctx* c = b->ctxs[i];
-    MultiSlot& m = slots[i];
+    MultiView& m = views[i];
     std::memset(&m, 0, sizeof(m));
-    m.Cp = c->d_consts;
-    m.S = hpm_detail::state_of(c);
+    states[i] = hpm_detail::state_of(c);
     m.H2 = (c->H / 2) * 2;
     m.count[kStageInit] = tiles_2d<ExactEngine>(c->W, c->H);  // 16 x 16 tiles for every one-lane engine
     m.count[kStageColour] = colour_tiles(b->tile_w, c->W, m.H2);
@@ -62,10 +71,12 @@ int ensure_layout(hpm_batch* b) {
     }
     if (c->nimg - 1 > b->max_v) b->max_v = c->nimg - 1;
   }
-  if (b->d_slots) { hipFree(b->d_slots); b->d_slots = nullptr; }
-  for (auto& p : b->d_wg) if (p) { hipFree(p); p = nullptr; }
-  BHIP_OK(hipMalloc((void**)&b->d_slots, sizeof(MultiSlot) * n));
-  BHIP_OK(hipMemcpy(b->d_slots, slots.data(), sizeof(MultiSlot) * n, hipMemcpyHostToDevice));
+  free_layout(b);
+  BHIP_OK(hipMalloc((void**)&b->d_consts, sizeof(RunConsts) * n));
+  BHIP_OK(hipMalloc((void**)&b->d_states, sizeof(DevState) * n));
+  BHIP_OK(hipMalloc((void**)&b->d_views, sizeof(MultiView) * n));
+  BHIP_OK(hipMemcpy(b->d_states, states.data(), sizeof(DevState) * n, hipMemcpyHostToDevice));
+  BHIP_OK(hipMemcpy(b->d_views, views.data(), sizeof(MultiView) * n, hipMemcpyHostToDevice));
   for (int s = 0; s < 3; ++s) {
     if (wg[s].size() > (size_t)0x7fffffff) return bfail(b, HPM_E_INVALID, "batch grid too large");
     b->grid[s] = (int)wg[s].size();
@@ -98,10 +109,9 @@ int run_once(hpm_batch* b, const uint64_t* seeds) {
   if (!widths_for(precision, b->tile_w)) return bfail(b, HPM_E_INVALID, "colour tile width not available for this precision");
   int rc = ensure_layout(b);
   if (rc) return rc;
-  for (int i = 0; i < n; ++i) {
-    BHIP_OK(hipMemcpyAsync(b->ctxs[i]->d_consts, &b->consts[i], sizeof(RunConsts), hipMemcpyHostToDevice, b->stream));
+  BHIP_OK(hipMemcpyAsync(b->d_consts, b->consts.data(), sizeof(RunConsts) * n, hipMemcpyHostToDevice, b->stream));
+  for (int i = 0; i < n; ++i)
     BHIP_OK(hipMemsetAsync(b->ctxs[i]->d_counters, 0, 8 * sizeof(unsigned long long), b->stream));
-  }
   BHIP_OK(hipEventRecord(b->ev[0], b->stream));
   switch (precision) {
     case HPM_PREC_FP16: rc = run_engine<FastFor<QuadF16>, true>(b, precision, iters); break;
@@ -147,8 +157,7 @@ void hpm_batch_destroy(hpm_batch* b) {
   hipSetDevice(b->device);
   if (b->stream) hipStreamSynchronize(b->stream);
   if (b->gexec) hipGraphExecDestroy(b->gexec);
-  if (b->d_slots) hipFree(b->d_slots);
-  for (auto p : b->d_wg) if (p) hipFree(p);
+  free_layout(b);
   for (auto e : b->ev) if (e) hipEventDestroy(e);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;

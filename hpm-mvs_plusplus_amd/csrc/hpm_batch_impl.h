@@ -19,7 +19,9 @@ struct hpm_batch {
   std::vector<hpm::RunConsts> consts;  // host sources of the per-run constant uploads
   // layout: per-view slots and the per-workgroup view tables of the three grids
   std::vector<unsigned char> layout_key;
-  hpm::MultiSlot* d_slots = nullptr;
+  hpm::RunConsts* d_consts = nullptr;  // one constant block per view, uploaded per run
+  hpm::DevState* d_states = nullptr;
+  hpm::MultiView* d_views = nullptr;
   uint16_t* d_wg[3] = {nullptr, nullptr, nullptr};
   int grid[3] = {0, 0, 0};
   int max_v = 0;
@@ -63,19 +65,21 @@ template <class E, int CW> int set_lds(hpm_batch* b) {
 template <class E, int CW> int launch_stages(hpm_batch* b, int iters) {
   hipStream_t st = b->stream;
   const dim3 blk(256);
-  hipLaunchKernelGGL(k_init_multi<E>, dim3(b->grid[kStageInit]), blk, 0, st, b->d_slots, b->d_wg[kStageInit], b->strip);
+  hipLaunchKernelGGL(k_init_multi<E>, dim3(b->grid[kStageInit]), blk, 0, st, b->d_consts, b->d_states, b->d_views,
+                     b->d_wg[kStageInit], b->strip);
   BHIP_OK(hipGetLastError());
   for (int it = 0; it < iters; ++it)
     for (int colour = 0; colour < 2; ++colour) {
       if (b->max_v <= kLdsMaxSrc)
         hipLaunchKernelGGL((k_propagate_multi<E, true, CW>), dim3(b->grid[kStageColour]), blk, lds_bytes(b), st,
-                           b->d_slots, b->d_wg[kStageColour], colour, it, b->strip);
+                           b->d_consts, b->d_states, b->d_views, b->d_wg[kStageColour], colour, it, b->strip);
       else
-        hipLaunchKernelGGL((k_propagate_multi<E, false, CW>), dim3(b->grid[kStageColour]), blk, 0, st, b->d_slots,
-                           b->d_wg[kStageColour], colour, it, b->strip);
+        hipLaunchKernelGGL((k_propagate_multi<E, false, CW>), dim3(b->grid[kStageColour]), blk, 0, st, b->d_consts,
+                           b->d_states, b->d_views, b->d_wg[kStageColour], colour, it, b->strip);
       BHIP_OK(hipGetLastError());
     }
-  hipLaunchKernelGGL(k_output_multi, dim3(b->grid[kStageOutput]), blk, 0, st, b->d_slots, b->d_wg[kStageOutput]);
+  hipLaunchKernelGGL(k_output_multi, dim3(b->grid[kStageOutput]), blk, 0, st, b->d_consts, b->d_states, b->d_views,
+                     b->d_wg[kStageOutput]);
   BHIP_OK(hipGetLastError());
   return HPM_OK;
 }

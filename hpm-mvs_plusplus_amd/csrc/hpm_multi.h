@@ -6,6 +6,13 @@
 // view through a per-workgroup table (one 16-bit load) and runs exactly the
 // single-view body on that view's tile, so results are those of the
 // single-view kernels.
+//
+// The views' constant blocks and states are arrays passed as read-only,
+// non-aliasing kernel arguments: indexed by the wave-uniform view number,
+// their fields load through the scalar cache as the single-view kernels'
+// arguments do (a constant block reached through a pointer read from memory
+// would load through the vector path and leave the image pointers, which
+// feed the buffer resources, in vector registers).
 #pragma once
 #include "hpm_kernels.h"
 
@@ -13,45 +20,47 @@ namespace hpm {
 
 enum { kStageInit = 0, kStageColour = 1, kStageOutput = 2 };
 
-struct MultiSlot {
-  const RunConsts* Cp;  // the view's constant block (device memory)
-  DevState S;           // the view's pixel state
-  int H2;               // rows the colour passes visit, (H / 2) * 2
-  int first[3];         // first workgroup of the view in the init / colour / output grids
-  int count[3];         // the view's workgroups in each grid
+struct MultiView {
+  int H2;        // rows the colour passes visit, (H / 2) * 2
+  int first[3];  // first workgroup of the view in the init / colour / output grids
+  int count[3];  // the view's workgroups in each grid
 };
 
 // The host sizes every view's init grid in 16 x 16 tiles: one-lane engines.
 template <class E>
-__global__ HPM_PROP_ATTR void k_init_multi(const MultiSlot* __restrict__ slots, const uint16_t* __restrict__ wg, int strip) {
+__global__ HPM_PROP_ATTR void k_init_multi(const RunConsts* __restrict__ Cs, const DevState* __restrict__ Ss,
+                                           const MultiView* __restrict__ views, const uint16_t* __restrict__ wg,
+                                           int strip) {
   static_assert(E::kLanes == 1, "multi-view launches take one-lane engines");
-  const MultiSlot& m = slots[wg[blockIdx.x]];
-  const int id = (int)blockIdx.x - m.first[kStageInit];
-  if (id >= m.count[kStageInit]) return;  // padding up to the next multiple of 8
-  const DevState S = m.S;
-  init_body<E>(*m.Cp, S, id, strip);
+  const int v = __builtin_amdgcn_readfirstlane((int)wg[blockIdx.x]);
+  const int id = (int)blockIdx.x - views[v].first[kStageInit];
+  if (id >= views[v].count[kStageInit]) return;  // padding up to the next multiple of 8
+  init_body<E>(Cs[v], Ss[v], id, strip);
 }
 
 template <class E, bool kLds, int CW>
-__global__ HPM_PROP_ATTR void k_propagate_multi(const MultiSlot* __restrict__ slots, const uint16_t* __restrict__ wg,
+__global__ HPM_PROP_ATTR void k_propagate_multi(const RunConsts* __restrict__ Cs, const DevState* __restrict__ Ss,
+                                                const MultiView* __restrict__ views, const uint16_t* __restrict__ wg,
                                                 int colour, int iter, int strip) {
   static_assert(E::kLanes == 1, "multi-view launches take one-lane engines");
-  const MultiSlot& m = slots[wg[blockIdx.x]];
-  const int id = (int)blockIdx.x - m.first[kStageColour];
-  if (id >= m.count[kStageColour]) return;
-  const DevState S = m.S;
-  propagate_body<E, kLds, CW>(*m.Cp, S, colour, iter, m.H2, id, strip);
+  const int v = __builtin_amdgcn_readfirstlane((int)wg[blockIdx.x]);
+  const int id = (int)blockIdx.x - views[v].first[kStageColour];
+  if (id >= views[v].count[kStageColour]) return;
+  propagate_body<E, kLds, CW>(Cs[v], Ss[v], colour, iter, views[v].H2, id, strip);
 }
 
 // 256 consecutive pixels (row-major) per workgroup
-static __global__ __launch_bounds__(256) void k_output_multi(const MultiSlot* __restrict__ slots, const uint16_t* __restrict__ wg) {
-  const MultiSlot& m = slots[wg[blockIdx.x]];
-  const int id = (int)blockIdx.x - m.first[kStageOutput];
-  if (id >= m.count[kStageOutput]) return;
-  const RunConsts& C = *m.Cp;
+static __global__ __launch_bounds__(256) void k_output_multi(const RunConsts* __restrict__ Cs,
+                                                             const DevState* __restrict__ Ss,
+                                                             const MultiView* __restrict__ views,
+                                                             const uint16_t* __restrict__ wg) {
+  const int v = __builtin_amdgcn_readfirstlane((int)wg[blockIdx.x]);
+  const int id = (int)blockIdx.x - views[v].first[kStageOutput];
+  if (id >= views[v].count[kStageOutput]) return;
+  const RunConsts& C = Cs[v];
   const int p = id * 256 + (int)threadIdx.x;
   if (p >= C.W * C.H) return;
-  output_pixel(C, m.S.planes, p % C.W, p / C.W);
+  output_pixel(C, Ss[v].planes, p % C.W, p / C.W);
 }
 
 }  // namespace hpm

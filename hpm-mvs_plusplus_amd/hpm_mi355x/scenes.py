@@ -233,6 +233,50 @@ def make_plane_scene(seed=1, n_planes=8, n_cams=10, width=640, height=480, f=525
                  pairs=pair_scores(cams, n_src))
 
 
+def _render_job(args):
+    cam, planes, noise_sigma, seed = args
+    return render(cam, planes, noise_sigma, np.random.default_rng(seed))
+
+
+def render_all(cams, planes, noise_sigma, seed, workers=None):
+    """Renders every camera (camera k's pixel noise seeded by (seed, k), so the
+    result does not depend on the number of worker processes)."""
+    jobs = [(c, planes, noise_sigma, [seed, k]) for k, c in enumerate(cams)]
+    workers = workers if workers is not None else min(16, os.cpu_count() or 1, len(jobs))
+    if workers <= 1:
+        return [_render_job(j) for j in jobs]
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(workers) as pool:
+        return pool.map(_render_job, jobs, chunksize=1)
+
+
+def make_batch_scene(seed=1, n_views=64, sizes=((640, 480), (1024, 768), (1600, 1200), (2048, 1536)),
+                     n_planes=40, src_range=(6, 10), radius=3.0, elevations_deg=(10.0, 25.0), span_deg=60.0,
+                     noise_sigma=2.0, workers=None) -> Scene:
+    """Reference views of uneven sizes over one scene: n_views cameras on two
+    rings aimed at the origin, each with a size drawn uniformly from `sizes`
+    (focal length 525 px per 640 px of width, principal point at the centre)
+    and a source count drawn uniformly from src_range (the top-scored
+    neighbours).  Scene.pairs[i] holds exactly camera i's sources."""
+    rng = np.random.default_rng(seed)
+    pick = rng.integers(0, len(sizes), n_views)
+    n_src = rng.integers(src_range[0], src_range[1] + 1, n_views)
+    base = ring_cameras(n_views, radius, elevations_deg, 1.0, 0.0, 0.0, 1, 1, span_deg=span_deg)
+    cams = []
+    for k, c in enumerate(base):
+        w, h = sizes[pick[k]]
+        f = 525.0 * w / 640.0
+        cams.append(Camera(K=np.array([[f, 0.0, w / 2], [0.0, f, h / 2], [0.0, 0.0, 1.0]]), R=c.R, t=c.t,
+                           width=int(w), height=int(h)))
+    planes = random_planes(rng, n_planes, (0.0, 0.0, -radius), 2.0, 4.0, 50.0)
+    planes.append(Plane(normal=np.array([0.0, 0.0, -1.0]), point=np.array([0.0, 0.0, 2.0]),
+                        tex_seed=int(rng.integers(1 << 30))))
+    out = render_all(cams, planes, noise_sigma, seed + 2, workers)
+    pairs = [p[:int(n)] for p, n in zip(pair_scores(cams, int(src_range[1])), n_src)]
+    return Scene(planes=planes, cameras=cams, images=[o[0] for o in out], depths=[o[1] for o in out],
+                 normals=[o[2] for o in out], pairs=pairs)
+
+
 def reference_view_problem(scene: Scene, ref: int, n_src: int):
     """(ref camera, source cameras, images as float32) for one RunPatchMatch."""
     ids = [ref] + [j for j, s in scene.pairs[ref][:n_src] if s > 0]
