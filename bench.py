@@ -293,6 +293,8 @@ def main():
                          "uneven: 64 reference views of uneven size, one-at-a-time vs batched")
     ap.add_argument("--batch-mode", default="fused", choices=["fused", "streams"],
                     help="fused: one launch per stage over the batch (hpm_batch); streams: one stream per view")
+    ap.add_argument("--compare-batch-modes", action="store_true",
+                    help="also time the other batch mode (its kernels then appear in a profile of the run)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -348,9 +350,15 @@ def main():
     ctxs = [c for _, c in batch]
     fused = Hm.Batch(ctxs)
     fused.set_graph(True)
+    tuned = None
+    if args.batch_mode == "fused":  # pick the launch shape (untimed; the runs are ordinary warm-up work)
+        tw, ts, tms = fused.autotune([SEED + k for k in range(len(ctxs))])
+        tuned = {"colour_tile_w": tw, "xcd_strip": ts, "schedule_ms": round(tms, 3)}
+        print(f"[bench] autotuned launch shape: tile {tw}, strip {ts}", file=sys.stderr, flush=True)
     runners = {"fused": lambda seeds: fused.run(seeds), "streams": lambda seeds: Hm.run_batch(ctxs, seeds)}
     batch_rates = {}
-    for mode in [args.batch_mode] + [m for m in runners if m != args.batch_mode]:
+    modes = [args.batch_mode] + ([m for m in runners if m != args.batch_mode] if args.compare_batch_modes else [])
+    for mode in modes:
         for i in range(args.warmup):
             runners[mode]([SEED + 10 * i + k for k in range(len(ctxs))])
         t0 = time.perf_counter()
@@ -421,7 +429,8 @@ def main():
         "data": "synthetic (seeded ray-cast textured planes, 11 cameras on a 30 deg arc)",
         "config": {"workload": "RunPatchMatch photometric, 1600x1200 reference + 10 source views, 3 iterations",
                    "precision": args.precision, "width": W_FULL, "height": H_FULL, "sources": NSRC,
-                   "views_per_step": len(ctxs), "batched": args.batch_mode, "hipgraph": True},
+                   "views_per_step": len(ctxs), "batched": args.batch_mode, "hipgraph": True,
+                   "launch_shape": tuned},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if traffic is None else traffic["bytes_per_launch"],
