@@ -38,7 +38,14 @@ image storage: ms, algorithmic GB/s, max and mean |cost - cost_FP64|);
 
 prints the uneven-size batch report: 64 reference views drawn from 640x480,
 1024x768, 1600x1200 and 2048x1536 with 6-10 sources each, one at a time vs
-multi-stream vs fused (default and autotuned launch shape).
+multi-stream vs fused (default and autotuned launch shape);
+
+    python bench.py --workload pipeline --config c1|c3 [--scale N]
+
+runs the full pyramid pipeline with fusion over a synthetic dense folder
+(c1: 10 views of 640x480; c3: 30 views of 3072x2048, scale 4 or 1) and
+reports seconds per depth map, the host stage breakdown and the accuracy
+against ground truth.
 """
 import argparse
 import json
@@ -233,6 +240,61 @@ def uneven_batch_bench(precision="fp16", n_views=64, reps=3):
             "reps": reps}
 
 
+def pipeline_bench(config="c1", precision="fp16", scale=None, keep_folder=False):
+    """The full pipeline (main.cpp's pyramid schedule + fusion, through
+    libhpm_host.so) over a synthetic dense folder.  c1: 10 cameras of 640x480
+    on a 30 deg arc, 8 planes, top-9 sources, --scale 1 (one level).  c3: 30
+    cameras of 3072x2048 on two rings, 24 plane patches + 6 spheres, 10
+    sources, --scale 4 (one level, 768x512) or 1 (three levels).  Reports the
+    wall time, the time per depth map, the host stage breakdown and the depth
+    accuracy of the final geometric maps against ray-cast ground truth."""
+    import shutil
+    import tempfile
+    from hpm_mi355x import pipeline as P
+    from hpm_mi355x import scenes
+    t0 = time.perf_counter()
+    if config == "c1":
+        sc = scenes.make_plane_scene(seed=1, n_planes=8, n_cams=10, width=640, height=480, f=525.0, n_src=9)
+        depth_hi = 5.0
+        scale = 1 if scale is None else scale
+    else:
+        sc = scenes.make_orbit_scene(seed=3)
+        depth_hi = 7.5
+        scale = 4 if scale is None else scale
+    folder = tempfile.mkdtemp(prefix=f"hpm_{config}_")
+    scenes.write_dense_folder(sc, folder, depth_lo=1.0, depth_hi=depth_hi)
+    print(f"[bench] {config} dense folder ready in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    P.set_verbose(1)
+    P.stage_times(reset=True)
+    t1 = time.perf_counter()
+    n_points = P.run_pyramid_pipeline(folder, initial_image_scale=scale, size_bound=1000, precision=precision, seed=SEED)
+    wall = time.perf_counter() - t1
+    P.set_verbose(0)
+    st = P.stage_times()
+    within, sizes = [], set()
+    for i, cam in enumerate(sc.cameras):
+        d = P.read_dmb(os.path.join(folder, "HPM_MVS_plusplus", f"2333_{i:08d}", "depths_geom.dmb"))
+        h, w = d.shape
+        sizes.add(f"{w}x{h}")
+        gt = sc.depths[i] if (w, h) == (cam.width, cam.height) else \
+            scenes.ground_truth_depth(sc, scenes.scaled_camera(cam, w, h))
+        ok = gt > 0
+        within.append(float(np.mean(np.abs(d[ok] - gt[ok]) / gt[ok] <= 0.01)))
+    if not keep_folder:
+        shutil.rmtree(folder, ignore_errors=True)
+    n = len(sc.cameras)
+    passes = st["patchmatch"][1]
+    return {"metric": f"full pipeline ({config}), seconds per depth map", "value": round(wall / n, 4), "unit": "s",
+            "higher_is_better": False, "config": config, "precision": precision, "initial_image_scale": scale,
+            "views": n, "final_size": sorted(sizes), "wall_s": round(wall, 3), "fused_points": int(n_points),
+            "patchmatch_runs": int(passes),
+            "patchmatch_ms_avg": round(1e3 * st["patchmatch"][0] / max(passes, 1), 3),
+            "stage_s": {k: round(v[0], 3) for k, v in st.items()},
+            "depth_within_1pct_of_gt_mean": round(float(np.mean(within)), 4),
+            "depth_within_1pct_of_gt_min": round(float(np.min(within)), 4),
+            "data": "synthetic (seeded ray-cast scene written as images/*.pgm, cams/*_cam.txt, pair.txt)"}
+
+
 def cost_microbench(reps=20):
     """Cost at fixed hypotheses (hpm_cost_fixed_device): 1600x1200 reference +
     10 sources, one random plane per pixel (depth U[1, 5], normal within 60 deg
@@ -288,9 +350,12 @@ def cost_microbench(reps=20):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="patchmatch", choices=["patchmatch", "cost", "uneven"],
+    ap.add_argument("--workload", default="patchmatch", choices=["patchmatch", "cost", "uneven", "pipeline"],
                     help="patchmatch: the RunPatchMatch throughput line; cost: the cost microbenchmark report; "
-                         "uneven: 64 reference views of uneven size, one-at-a-time vs batched")
+                         "uneven: 64 reference views of uneven size, one-at-a-time vs batched; "
+                         "pipeline: the full pyramid pipeline over a dense folder (--config c1|c3)")
+    ap.add_argument("--config", default="c1", choices=["c1", "c3"], help="pipeline workload: scene")
+    ap.add_argument("--scale", type=int, default=None, help="pipeline workload: initial image scale")
     ap.add_argument("--batch-mode", default="fused", choices=["fused", "streams"],
                     help="fused: one launch per stage over the batch (hpm_batch); streams: one stream per view")
     ap.add_argument("--compare-batch-modes", action="store_true",
@@ -318,6 +383,9 @@ def main():
         return
     if args.workload == "uneven":
         print(json.dumps(uneven_batch_bench(args.precision)), flush=True)
+        return
+    if args.workload == "pipeline":
+        print(json.dumps(pipeline_bench(args.config, args.precision, args.scale)), flush=True)
         return
 
     import hpm_mi355x as Hm

@@ -19,7 +19,8 @@ HEADERS = ["hpm_math.h", "hpm_device.h", "hpm_kernels.h", "hpm_jbu.h", "hpm_fast
            "hpm_internal.h", "hpm_multi.h", "hpm_batch_impl.h"]
 HOST_LIB = os.path.join(HERE, "lib", "libhpm_host.so")
 HOST_SOURCES = [os.path.join(HERE, "host", "hpm_host.cpp"), os.path.join(HERE, "host", "hpm_host_capi.cpp")]
-GXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wextra"]
+GXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-ffp-contract=off", "-fno-fast-math", "-Wall",
+             "-Wextra"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

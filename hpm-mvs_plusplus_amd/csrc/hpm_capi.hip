@@ -159,6 +159,7 @@ template <class S> int build_quads(hpm_ctx* ctx) {
 
 template <class S> int build_pool_quads(hpm_ctx* ctx) {
   hpm_pool* pool = ctx->pool;
+  std::lock_guard<std::mutex> lock(pool->quads_mu);
   std::vector<void*>& q = pool->quads[S::kind];
   if (q.size() != pool->d_img.size()) {
     for (size_t i = 0; i < pool->cams.size(); ++i)

@@ -69,16 +69,24 @@ struct FusionImage {
 
 constexpr int kFusionMaxImages = 1024;
 
-// One thread per pixel of reference image `ref`; writes match[j * P + p] and
-// term[j * P + p] for its num_ngb source slots.
+// One (matched source pixel, score) pair per (pixel, source slot); m = -1
+// when the source does not confirm the pixel.
+struct MatchTerm {
+  int m;
+  float term;
+};
+
+// One thread per pixel p0 + i (i < count) of reference image `ref`; writes
+// out[i * num_ngb + j] for its num_ngb source slots (pixel-major, so the
+// host's ordered pass reads each pixel's slots contiguously).
 __global__ __launch_bounds__(256) void k_consistency(const DevCam* __restrict__ cams, const FusionImage* __restrict__ imgs,
                                                      const int* __restrict__ srcs, int num_ngb, int ref, int mode,
-                                                     int* __restrict__ match, float* __restrict__ term) {
+                                                     int p0, int count, MatchTerm* __restrict__ out) {
   const DevCam& rc = cams[ref];
-  const int cols = rc.width, rows = rc.height;
-  const int P = cols * rows;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P) return;
+  const int cols = rc.width;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int p = p0 + i;
   const int r = p / cols, c = p - r * cols;
   const float ref_depth = imgs[ref].depth[p];
   const float* rn = imgs[ref].normal + 3 * (size_t)p;
@@ -121,8 +129,7 @@ __global__ __launch_bounds__(256) void k_consistency(const DevCam* __restrict__ 
         }
       }
     }
-    match[(size_t)j * P + p] = m;
-    term[(size_t)j * P + p] = score;
+    out[(size_t)i * num_ngb + j] = MatchTerm{m, score};
   }
 }
 
@@ -163,8 +170,6 @@ int hpm_consistency_check(int mode, int num_images, const hpm_camera* cameras, c
   DevCam* d_cams = nullptr;
   FusionImage* d_imgs = nullptr;
   int* d_srcs = nullptr;
-  int* d_match = nullptr;
-  float* d_term = nullptr;
   std::vector<float*> d_buf(2 * num_images, nullptr);
   int rc = HPM_OK;
   auto ok = [&](hipError_t e) {
@@ -173,59 +178,94 @@ int hpm_consistency_check(int mode, int num_images, const hpm_camera* cameras, c
   };
   std::vector<FusionImage> himgs(num_images);
   const int total_src = src_offsets[num_images];
+  // The per-pixel tests run on the GPU in chunks of rows, in the order the
+  // ordered pass consumes them; each chunk is copied into one of two pinned
+  // host buffers while the host resolves the previous one.
+  const size_t chunk_px = std::min<size_t>(max_p, (size_t)1 << 20);
+  const size_t chunk_elems = chunk_px * (size_t)std::max(max_ngb, 1);
+  MatchTerm* d_out = nullptr;
+  MatchTerm* h_out[2] = {nullptr, nullptr};
+  hipStream_t st = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
   if (ok(hipMalloc((void**)&d_cams, sizeof(DevCam) * num_images)) &&
       ok(hipMalloc((void**)&d_imgs, sizeof(FusionImage) * num_images)) &&
       ok(hipMalloc((void**)&d_srcs, sizeof(int) * std::max(total_src, 1))) &&
-      ok(hipMalloc((void**)&d_match, sizeof(int) * max_p * std::max(max_ngb, 1))) &&
-      ok(hipMalloc((void**)&d_term, sizeof(float) * max_p * std::max(max_ngb, 1)))) {
+      ok(hipMalloc((void**)&d_out, sizeof(MatchTerm) * chunk_elems)) &&
+      ok(hipHostMalloc((void**)&h_out[0], sizeof(MatchTerm) * chunk_elems, hipHostMallocDefault)) &&
+      ok(hipHostMalloc((void**)&h_out[1], sizeof(MatchTerm) * chunk_elems, hipHostMallocDefault)) &&
+      ok(hipStreamCreate(&st)) && ok(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming)) &&
+      ok(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming))) {
     for (int i = 0; i < num_images && rc == HPM_OK; ++i) {
       const size_t P = (size_t)cameras[i].width * cameras[i].height;
       if (ok(hipMalloc((void**)&d_buf[2 * i], P * 4)) && ok(hipMalloc((void**)&d_buf[2 * i + 1], P * 12)) &&
-          ok(hipMemcpy(d_buf[2 * i], depths[i], P * 4, hipMemcpyHostToDevice)) &&
-          ok(hipMemcpy(d_buf[2 * i + 1], normals[i], P * 12, hipMemcpyHostToDevice)))
+          ok(hipMemcpyAsync(d_buf[2 * i], depths[i], P * 4, hipMemcpyHostToDevice, st)) &&
+          ok(hipMemcpyAsync(d_buf[2 * i + 1], normals[i], P * 12, hipMemcpyHostToDevice, st)))
         himgs[i] = FusionImage{d_buf[2 * i], d_buf[2 * i + 1]};
     }
   }
-  if (rc == HPM_OK && ok(hipMemcpy(d_cams, hc.data(), sizeof(DevCam) * num_images, hipMemcpyHostToDevice)) &&
-      ok(hipMemcpy(d_imgs, himgs.data(), sizeof(FusionImage) * num_images, hipMemcpyHostToDevice)) && total_src > 0)
-    ok(hipMemcpy(d_srcs, src_ids, sizeof(int) * total_src, hipMemcpyHostToDevice));
+  if (rc == HPM_OK && ok(hipMemcpyAsync(d_cams, hc.data(), sizeof(DevCam) * num_images, hipMemcpyHostToDevice, st)) &&
+      ok(hipMemcpyAsync(d_imgs, himgs.data(), sizeof(FusionImage) * num_images, hipMemcpyHostToDevice, st)) &&
+      total_src > 0)
+    ok(hipMemcpyAsync(d_srcs, src_ids, sizeof(int) * total_src, hipMemcpyHostToDevice, st));
+  if (rc == HPM_OK) ok(hipStreamSynchronize(st));  // the pageable sources are only read synchronously
+
+  // chunks in consumption order: (image, first pixel, pixels)
+  struct Chunk { int img; size_t p0, n; };
+  std::vector<Chunk> chunks;
+  for (int i = 0; i < num_images; ++i) {
+    if (src_offsets[i + 1] - src_offsets[i] <= 0) continue;
+    const size_t P = (size_t)cameras[i].width * cameras[i].height;
+    for (size_t p0 = 0; p0 < P; p0 += chunk_px) chunks.push_back({i, p0, std::min(chunk_px, P - p0)});
+  }
+  auto issue = [&](size_t k) {
+    const Chunk& c = chunks[k];
+    const int num_ngb = src_offsets[c.img + 1] - src_offsets[c.img];
+    hipLaunchKernelGGL(k_consistency, dim3((unsigned)((c.n + 255) / 256)), dim3(256), 0, st, d_cams, d_imgs,
+                       d_srcs + src_offsets[c.img], num_ngb, c.img, mode, (int)c.p0, (int)c.n, d_out);
+    return ok(hipGetLastError()) &&
+           ok(hipMemcpyAsync(h_out[k & 1], d_out, sizeof(MatchTerm) * c.n * num_ngb, hipMemcpyDeviceToHost, st)) &&
+           ok(hipEventRecord(ev[k & 1], st));
+  };
 
   std::vector<std::vector<uint8_t>> masks(num_images);
-  std::vector<int> match;
-  std::vector<float> term;
   int64_t npts = 0;
   for (int i = 0; i < num_images && rc == HPM_OK; ++i) {
     masks[i].assign((size_t)cameras[i].width * cameras[i].height, 0);
     if (mode == HPM_CHECK_CONFIDENCE) std::memset(out_confidence[i], 0, masks[i].size() * sizeof(float));
   }
+  for (size_t k = 0; k < chunks.size() && k < 2 && rc == HPM_OK; ++k) issue(k);
+  size_t k = 0;
+  std::vector<int> used;
   for (int i = 0; i < num_images && rc == HPM_OK; ++i) {
     const int cols = cameras[i].width, rows = cameras[i].height;
     const size_t P = (size_t)cols * rows;
     const int num_ngb = src_offsets[i + 1] - src_offsets[i];
     const int* srcs = src_ids + src_offsets[i];
-    if (num_ngb > 0) {
-      hipLaunchKernelGGL(k_consistency, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, 0, d_cams, d_imgs,
-                         d_srcs + src_offsets[i], num_ngb, i, mode, d_match, d_term);
-      if (!ok(hipGetLastError())) break;
-      match.resize(P * num_ngb);
-      term.resize(P * num_ngb);
-      if (!ok(hipMemcpy(match.data(), d_match, P * num_ngb * sizeof(int), hipMemcpyDeviceToHost)) ||
-          !ok(hipMemcpy(term.data(), d_term, P * num_ngb * sizeof(float), hipMemcpyDeviceToHost)))
-        break;
-    }
     // The ordered greedy pass (main.cpp:884-952 / 1019-1088); used_list lives
     // for the whole image, as in the reference.
-    std::vector<int> used(num_ngb, -1);
+    used.assign(num_ngb, -1);
+    size_t p_begin = 0, p_end = 0;
+    const MatchTerm* mt = nullptr;
+    // chunk k consumed: its pinned buffer takes chunk k + 2
+    auto release = [&]() { return !(k + 2 < chunks.size() && !issue(k + 2)) && (++k, true); };
     for (size_t p = 0; p < P; ++p) {
+      if (num_ngb > 0 && p == p_end) {  // next chunk of this image
+        if (mt && !release()) break;
+        if (!ok(hipEventSynchronize(ev[k & 1]))) break;
+        mt = h_out[k & 1];
+        p_begin = chunks[k].p0;
+        p_end = p_begin + chunks[k].n;
+      }
       if (masks[i][p] == 1) continue;
       int num_consistent = 0;
       float dynamic_consistency = 0;
+      const MatchTerm* row = mt + (p - p_begin) * num_ngb;  // num_ngb == 0: not read
       for (int j = 0; j < num_ngb; ++j) {
-        const int m = match[(size_t)j * P + p];
+        const int m = row[j].m;
         if (m < 0) continue;
         if (masks[srcs[j]][m] == 1) continue;
         used[j] = m;
-        dynamic_consistency += term[(size_t)j * P + p];
+        dynamic_consistency += row[j].term;
         num_consistent++;
       }
       if (num_consistent >= 1 && ((double)dynamic_consistency > 0.3 * num_consistent)) {
@@ -253,9 +293,17 @@ int hpm_consistency_check(int mode, int num_images, const hpm_camera* cameras, c
         }
       }
     }
+    if (mt && rc == HPM_OK) release();
   }
+  if (st) hipStreamSynchronize(st);
+  for (MatchTerm* h : h_out)
+    if (h) hipHostFree(h);
+  for (hipEvent_t e : ev)
+    if (e) hipEventDestroy(e);
+  if (st) hipStreamDestroy(st);
+  hipFree(d_out);
   for (float* b : d_buf) hipFree(b);
-  hipFree(d_cams); hipFree(d_imgs); hipFree(d_srcs); hipFree(d_match); hipFree(d_term);
+  hipFree(d_cams); hipFree(d_imgs); hipFree(d_srcs);
   *out_count = npts;
   return rc;
 }

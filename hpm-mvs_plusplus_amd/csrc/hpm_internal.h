@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -26,6 +27,7 @@ struct hpm_pool {
   std::vector<hpm_camera> cams;
   std::vector<float*> d_img;
   std::vector<void*> quads[4];  // indexed by HPM_PREC_* (0 unused)
+  std::mutex quads_mu;          // contexts on several host threads may build them first
 };
 
 struct hpm_ctx {

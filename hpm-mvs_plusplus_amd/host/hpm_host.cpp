@@ -9,7 +9,12 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <exception>
+#include <thread>
 #include <cmath>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -262,6 +267,12 @@ struct Delaunay {
   std::vector<V> vs;
   std::vector<T> ts;
   int last = 0;
+  // per-insert scratch kept across inserts (reset by stamp or by the entries
+  // touched), so that an insert costs the size of its cavity, not of the mesh
+  std::vector<uint32_t> mark;  // mark[t] == stamp: t is in the current cavity
+  uint32_t stamp = 0;
+  std::vector<int> vstart, vend;  // per vertex: new triangle starting / ending there
+  std::vector<int> cavity, stack, created;
 
   static int64_t orient(const V& a, const V& b, const V& c) {
     return (b.x - a.x) * (c.y - a.y) - (b.y - a.y) * (c.x - a.x);
@@ -284,6 +295,8 @@ struct Delaunay {
     const int64_t big = 3 * std::max(w, h), cx = w, cy = h;  // doubled centre
     vs = {{cx + 2 * big, cy}, {cx, cy + 2 * big}, {cx - 2 * big, cy - 2 * big}};
     ts.push_back(T{{0, 1, 2}, {-1, -1, -1}, true});
+    vstart.assign(3, -1);
+    vend.assign(3, -1);
   }
 
   int locate(const V& p) {
@@ -317,19 +330,26 @@ struct Delaunay {
       if (vs[ts[t0].v[k]].x == x && vs[ts[t0].v[k]].y == y) return;  // duplicate point
     const int pi = (int)vs.size();
     vs.push_back(p);
+    vstart.push_back(-1);
+    vend.push_back(-1);
+    if (mark.size() < ts.size()) mark.resize(ts.size() + ts.size() / 2 + 16, 0);
+    if (++stamp == 0) {  // wrapped: clear the marks once
+      std::fill(mark.begin(), mark.end(), 0);
+      stamp = 1;
+    }
     // cavity: triangles whose circumcircle strictly contains p, grown from t0
-    std::vector<int> cavity{t0}, stack{t0};
-    std::vector<char> in(ts.size(), 0);
-    in[t0] = 1;
+    cavity.assign(1, t0);
+    stack.assign(1, t0);
+    mark[t0] = stamp;
     while (!stack.empty()) {
       const int t = stack.back();
       stack.pop_back();
       for (int k = 0; k < 3; ++k) {
         const int n = ts[t].nb[k];
-        if (n < 0 || in[n]) continue;
+        if (n < 0 || mark[n] == stamp) continue;
         const T& tn = ts[n];
         if (incircle(vs[tn.v[0]], vs[tn.v[1]], vs[tn.v[2]], p) > 0) {
-          in[n] = 1;
+          mark[n] = stamp;
           cavity.push_back(n);
           stack.push_back(n);
         }
@@ -342,17 +362,16 @@ struct Delaunay {
     for (int t : cavity)
       for (int k = 0; k < 3; ++k) {
         const int n = ts[t].nb[k];
-        if (n < 0 || !in[n]) boundary.push_back({ts[t].v[(k + 1) % 3], ts[t].v[(k + 2) % 3], n});
+        if (n < 0 || mark[n] != stamp) boundary.push_back({ts[t].v[(k + 1) % 3], ts[t].v[(k + 2) % 3], n});
       }
     for (int t : cavity) ts[t].alive = false;
-    std::vector<int> starts(vs.size(), -1), ends(vs.size(), -1);
-    std::vector<int> created;
+    created.clear();
     for (const E& e : boundary) {
       const int id = (int)ts.size();
       ts.push_back(T{{e.a, e.b, pi}, {-1, -1, e.out}, true});
       created.push_back(id);
-      starts[e.a] = id;
-      ends[e.b] = id;
+      vstart[e.a] = id;
+      vend[e.b] = id;
       if (e.out >= 0)
         for (int k = 0; k < 3; ++k) {
           T& o = ts[e.out];
@@ -362,9 +381,10 @@ struct Delaunay {
     }
     for (int id : created) {
       T& t = ts[id];
-      t.nb[0] = starts[t.v[1]];  // edge (b, p) is shared with the triangle that starts at b
-      t.nb[1] = ends[t.v[0]];    // edge (p, a) with the triangle that ends at a
+      t.nb[0] = vstart[t.v[1]];  // edge (b, p) is shared with the triangle that starts at b
+      t.nb[1] = vend[t.v[0]];    // edge (p, a) with the triangle that ends at a
     }
+    for (const E& e : boundary) vstart[e.a] = vend[e.b] = -1;
     last = created.empty() ? 0 : created.back();
   }
 };
@@ -513,6 +533,19 @@ static std::string ImagePath(const std::string& dense_folder, int id, const char
   return s.str();
 }
 
+// The size from the Netpbm header only.
+static void ImageSize(const std::string& dense_folder, int id, int& w, int& h) {
+  for (const char* ext : {".pgm", ".ppm"}) {
+    FILE* fp = std::fopen(ImagePath(dense_folder, id, ext).c_str(), "rb");
+    if (!fp) continue;
+    std::string magic, sw, sh;
+    const bool ok = next_token(fp, magic) && (magic == "P5" || magic == "P6") && next_token(fp, sw) && next_token(fp, sh);
+    std::fclose(fp);
+    if (ok && (w = std::atoi(sw.c_str())) > 0 && (h = std::atoi(sh.c_str())) > 0) return;
+  }
+  throw Error("cannot read the size of image " + ImagePath(dense_folder, id, ".pgm"));
+}
+
 static MapU8 LoadImage(const std::string& dense_folder, int id) {
   MapU8 img;
   if (ReadNetpbm(ImagePath(dense_folder, id, ".pgm"), img) || ReadNetpbm(ImagePath(dense_folder, id, ".ppm"), img))
@@ -525,9 +558,78 @@ void SceneImages::ensure(int id) {
   if ((size_t)id >= orig_.size()) {
     orig_.resize(id + 1);
     cur_.resize(id + 1);
+    gray_.resize(id + 1);
+    edges_.resize(id + 1);
     have_orig_.resize(id + 1, false);
     have_cur_.resize(id + 1, false);
+    have_gray_.resize(id + 1, false);
+    have_edges_.resize(id + 1, false);
   }
+}
+
+SceneImages::~SceneImages() { drop_pool(); }
+
+void SceneImages::drop_pool() {
+  if (pool_) hpm_pool_destroy(pool_);
+  pool_ = nullptr;
+  pool_index_.clear();
+}
+
+const MapF& SceneImages::CurrentGray(int id) {
+  ensure(id);
+  if (!have_gray_[id]) {
+    gray_[id] = ToGrayFloat(Current(id));
+    have_gray_[id] = true;
+  }
+  return gray_[id];
+}
+
+const MapU8& SceneImages::CurrentEdges(int id) {
+  ensure(id);
+  if (!have_edges_[id]) {
+    edges_[id] = CannyEdges(Current(id), 50.0f, 150.0f);
+    have_edges_[id] = true;
+  }
+  return edges_[id];
+}
+
+hpm_camera SceneImages::CurrentCamera(int id) {
+  std::ostringstream cp;
+  cp << folder_ << "/cams/" << std::setw(8) << std::setfill('0') << id << "_cam.txt";
+  hpm_camera c = ReadCamera(cp.str());
+  const MapU8& img = Current(id);
+  // K scaled by the prepared / original size, HPM.cpp:668-674
+  const MapU8& orig = Original(id);
+  const float scale_x = img.width / static_cast<float>(orig.width);
+  const float scale_y = img.height / static_cast<float>(orig.height);
+  c.K[0] *= scale_x; c.K[2] *= scale_x;
+  c.K[4] *= scale_y; c.K[5] *= scale_y;
+  c.width = img.width;
+  c.height = img.height;
+  return c;
+}
+
+void SceneImages::BuildPool(const std::vector<int>& ids, int device) {
+  drop_pool();
+  std::vector<hpm_camera> cams;
+  std::vector<const float*> ptrs;
+  for (int id : ids) {
+    cams.push_back(CurrentCamera(id));
+    ptrs.push_back(CurrentGray(id).data.data());
+    CurrentEdges(id);  // derived here, so that the passes' threads only read the cache
+  }
+  if (hpm_pool_create(&pool_, device, (int)ids.size(), cams.data(), ptrs.data()) != HPM_OK) {
+    pool_ = nullptr;
+    throw Error("hpm_pool_create failed");
+  }
+  for (size_t k = 0; k < ids.size(); ++k) {
+    if ((size_t)ids[k] >= pool_index_.size()) pool_index_.resize(ids[k] + 1, -1);
+    pool_index_[ids[k]] = (int)k;
+  }
+}
+
+int SceneImages::PoolIndex(int id) const {
+  return (pool_ && id >= 0 && (size_t)id < pool_index_.size()) ? pool_index_[id] : -1;
 }
 
 const MapU8& SceneImages::Original(int id) {
@@ -540,7 +642,10 @@ const MapU8& SceneImages::Original(int id) {
 }
 
 void SceneImages::PrepareLevel(int id, int target_max_side) {  // main.cpp:123-145
+  drop_pool();
   const MapU8& img = Original(id);
+  have_gray_[id] = false;
+  have_edges_[id] = false;
   if (std::max(img.width, img.height) > target_max_side) {
     const float factor = std::min((float)target_max_side / img.width, (float)target_max_side / img.height);
     const int nw = std::max(1, (int)std::round(img.width * factor));
@@ -586,33 +691,35 @@ void HPM::SetPrecision(int precision) { params_.precision = precision; }
 void HPM::InuputInitialization(const std::string& dense_folder, const std::vector<Problem>& problems, int idx,
                                SceneImages* images) {
   images_.clear();
+  owned_images_.clear();
   cams_.clear();
   depths_.clear();
+  pool_ = nullptr;
+  pool_ids_.clear();
   const Problem& problem = problems.at(idx);
   std::vector<int> ids{problem.ref_image_id};
   ids.insert(ids.end(), problem.src_image_ids.begin(), problem.src_image_ids.end());
-  for (int id : ids) {
-    std::ostringstream cp;
-    cp << dense_folder << "/cams/" << std::setw(8) << std::setfill('0') << id << "_cam.txt";
-    hpm_camera c = ReadCamera(cp.str());
-    MapU8 loaded;
-    const MapU8* img;
-    if (images) {
-      img = &images->Current(id);
-      // K scaled by the prepared / original size, HPM.cpp:668-674
-      const MapU8& orig = images->Original(id);
-      const float scale_x = img->width / static_cast<float>(orig.width);
-      const float scale_y = img->height / static_cast<float>(orig.height);
-      c.K[0] *= scale_x; c.K[2] *= scale_x;
-      c.K[4] *= scale_y; c.K[5] *= scale_y;
-    } else {
-      loaded = LoadImage(dense_folder, id);
-      img = &loaded;
+  if (images) {
+    for (int id : ids) {
+      cams_.push_back(images->CurrentCamera(id));
+      images_.push_back(&images->CurrentGray(id));
+      pool_ids_.push_back(images->PoolIndex(id));
     }
-    c.width = img->width;
-    c.height = img->height;
-    cams_.push_back(c);
-    images_.push_back(ToGrayFloat(*img));
+    const bool all = std::all_of(pool_ids_.begin(), pool_ids_.end(), [](int k) { return k >= 0; });
+    pool_ = all ? images->Pool() : nullptr;
+  } else {
+    owned_images_.reserve(ids.size());
+    for (int id : ids) {
+      std::ostringstream cp;
+      cp << dense_folder << "/cams/" << std::setw(8) << std::setfill('0') << id << "_cam.txt";
+      hpm_camera c = ReadCamera(cp.str());
+      const MapU8 img = LoadImage(dense_folder, id);
+      c.width = img.width;
+      c.height = img.height;
+      cams_.push_back(c);
+      owned_images_.push_back(ToGrayFloat(img));
+    }
+    for (const MapF& m : owned_images_) images_.push_back(&m);
   }
   if (params_.geom_consistency) {  // HPM.cpp:693-732
     const char* suffix = multi_geometry_ ? "/depths_geom.dmb" : "/depths.dmb";
@@ -622,9 +729,13 @@ void HPM::InuputInitialization(const std::string& dense_folder, const std::vecto
 
 void HPM::CudaSpaceInitialization(const std::string& dense_folder, const Problem& problem) {
   check(hpm_set_params(ctx_, &params_), "hpm_set_params");
-  std::vector<const float*> ptrs;
-  for (const MapF& m : images_) ptrs.push_back(m.data.data());
-  check(hpm_set_views(ctx_, (int)cams_.size(), cams_.data(), ptrs.data()), "hpm_set_views");
+  if (pool_) {  // the level's images are on the device already
+    check(hpm_set_views_pooled(ctx_, pool_, (int)pool_ids_.size(), pool_ids_.data()), "hpm_set_views_pooled");
+  } else {
+    std::vector<const float*> ptrs;
+    for (const MapF* m : images_) ptrs.push_back(m->data.data());
+    check(hpm_set_views(ctx_, (int)cams_.size(), cams_.data(), ptrs.data()), "hpm_set_views");
+  }
   if (params_.geom_consistency) {
     std::vector<const float*> dp;
     for (const MapF& m : depths_) dp.push_back(m.data.data());
@@ -882,8 +993,95 @@ static void PriorFromCoarser(HPM& hpm, const MapF& depths, const MapF& costs, co
   hpm.ReloadPlanarPriorInitialization(mask_new, prior_planes);
 }
 
+namespace {
+std::mutex g_stage_mu;
+StageTimes g_stages{};
+int g_verbose = 0;
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Adds the time since construction (or the last lap) to a stage.
+class StageClock {
+ public:
+  StageClock() : t_(now_s()) {}
+  double lap(Stage s) {
+    const double t = now_s(), d = t - t_;
+    t_ = t;
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    g_stages.seconds[s] += d;
+    g_stages.calls[s] += 1;
+    return d;
+  }
+ private:
+  double t_;
+};
+}  // namespace
+
+StageTimes GetStageTimes() {
+  std::lock_guard<std::mutex> lk(g_stage_mu);
+  return g_stages;
+}
+
+void ResetStageTimes() {
+  std::lock_guard<std::mutex> lk(g_stage_mu);
+  g_stages = StageTimes{};
+}
+
+void SetVerbose(int level) { g_verbose = level; }
+
+namespace {
+int g_pass_threads = 0;  // 0: min(8, hardware threads)
+
+int pass_threads(int n) {
+  int t = g_pass_threads;
+  if (t <= 0) t = std::min(8, (int)std::max(1u, std::thread::hardware_concurrency()));
+  return std::max(1, std::min(t, n));
+}
+
+// Runs fn(i) for i in [0, n) on up to pass_threads(n) host threads; the first
+// exception is rethrown after every thread has finished.
+template <class F> void parallel_views(int n, F fn) {
+  const int t = pass_threads(n);
+  if (t <= 1) {
+    for (int i = 0; i < n; ++i) fn(i);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::exception_ptr err;
+  std::mutex err_mu;
+  std::vector<std::thread> pool;
+  for (int k = 0; k < t; ++k)
+    pool.emplace_back([&] {
+      for (int i; (i = next.fetch_add(1)) < n;) {
+        try {
+          fn(i);
+        } catch (...) {
+          std::lock_guard<std::mutex> lk(err_mu);
+          if (!err) err = std::current_exception();
+          next.store(n);
+        }
+      }
+    });
+  for (auto& th : pool) th.join();
+  if (err) std::rethrow_exception(err);
+}
+
+// Whether the views of a pass may run in any order: a pass reads, besides
+// its own view's files, only files that no view of the pass writes.  Every
+// pass qualifies except the geometric passes over the previous pass's
+// depths_geom.dmb (multi_geometry), which the reference runs in view order,
+// each view reading the sources already updated before it.
+bool order_free(const PassFlags& f) { return !(f.geom_consistency && f.multi_geometry); }
+}  // namespace
+
+void SetPassThreads(int n) { g_pass_threads = n; }
+
 void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>& problems, int idx,
                     const PassFlags& flags, int precision, uint64_t seed, SceneImages* images) {
+  StageClock clk;
+  double t_setup = 0, t_edges = 0, t_prior = 0, t_pm = 0, t_write = 0;
   const Problem& problem = problems.at(idx);
   MakeDir(dense_folder + "/HPM_MVS_plusplus");
   const std::string rf = ResultFolder(dense_folder, problem.ref_image_id);
@@ -897,15 +1095,21 @@ void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>&
   if (flags.prior_consistency) hpm.SetPlanarPriorParams();
   hpm.CudaSpaceInitialization(dense_folder, problem);
   const int width = hpm.GetReferenceImageWidth(), height = hpm.GetReferenceImageHeight();
+  t_setup += clk.lap(kSetup);
 
   // cv::Canny on the level's colour image, main.cpp:280-289
-  {
-    const MapU8 loaded = images ? MapU8() : LoadImage(dense_folder, problem.ref_image_id);
-    const MapU8& colour = images ? images->Current(problem.ref_image_id) : loaded;
-    if (colour.width != width || colour.height != height)
+  if (images) {
+    const MapU8& edges = images->CurrentEdges(problem.ref_image_id);
+    if (edges.width != width || edges.height != height)
       throw Error("prepared reference image size does not match the camera size");
+    hpm.CudaCannyInitialization(edges);
+  } else {
+    const MapU8 colour = LoadImage(dense_folder, problem.ref_image_id);
+    if (colour.width != width || colour.height != height)
+      throw Error("reference image size does not match the camera size");
     hpm.CudaCannyInitialization(CannyEdges(colour, 50.0f, 150.0f));
   }
+  t_edges += clk.lap(kEdges);
   if (flags.mand_consistency || flags.prior_consistency) hpm.CudaConfidenceInitialization(dense_folder, problems, idx);
 
   MapF depths(width, height), normals(width, height, 3), costs(width, height), texture(width, height);
@@ -922,6 +1126,7 @@ void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>&
     const int tex_level = flags.image_scale + flags.hpm_scale_distance;
     if (flags.hpm_scale_distance >= 0 && flags.hpm_scale_distance <= 2)
       (void)LoadDmb(rf + "/texture" + std::to_string(tex_level) + ".dmb");
+    t_setup += clk.lap(kSetup);
     if (flags.hpm_scale_distance == 0)
       PriorAtLevel(hpm, depths, costs, confidences);
     else if (flags.hpm_scale_distance == 1 || flags.hpm_scale_distance == 2)
@@ -930,6 +1135,9 @@ void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>&
     // unchanged, as main.cpp:349-597 does when neither branch is taken
     run = flags.hpm_scale_distance >= 0 && flags.hpm_scale_distance <= 2;
     if (run) hpm.CudaHypothesesReload(depths, costs, normals);
+    t_prior += clk.lap(kPrior);
+  } else {
+    t_setup += clk.lap(kSetup);
   }
   if (run) {
     hpm.RunPatchMatch(seed);
@@ -942,6 +1150,7 @@ void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>&
       normals.data[3 * i + 2] = planes[4 * i + 2];
     }
     hpm.GetCosts(costs);
+    t_pm += clk.lap(kPatchMatch);
   }
   const std::string dpath = rf + (flags.geom_consistency ? "/depths_geom.dmb" : "/depths.dmb");
   if (!WriteDmb(dpath, depths) || !WriteDmb(rf + "/normals.dmb", normals) || !WriteDmb(rf + "/costs.dmb", costs))
@@ -951,12 +1160,31 @@ void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>&
     if (!WriteDmb(rf + "/texture" + std::to_string(flags.image_scale) + ".dmb", texture))
       throw Error("cannot write texture under " + rf);
   }
+  t_write += clk.lap(kWrite);
+  if (g_verbose > 0) {
+    const char* mode = flags.prior_consistency ? "prior"
+                       : flags.hierarchy       ? "hierarchy"
+                       : flags.mand_consistency ? "geometric+mandatory"
+                       : flags.geom_consistency ? "geometric"
+                                                : "photometric";
+    std::fprintf(stderr,
+                 "[hpm] view %08d %dx%d %s%s: setup %.1f ms, edges %.1f, prior %.1f, patchmatch %.1f, write %.1f\n",
+                 problem.ref_image_id, width, height, mode,
+                 flags.prior_consistency ? (" d=" + std::to_string(flags.hpm_scale_distance)).c_str() : "",
+                 1e3 * t_setup, 1e3 * t_edges, 1e3 * t_prior, 1e3 * t_pm, 1e3 * t_write);
+    std::fflush(stderr);
+  }
 }
 
 void JointBilateralUpsampling(const std::string& dense_folder, const Problem& problem, SceneImages& images) {
+  StageClock clk;
+  struct Lap {
+    StageClock& c;
+    ~Lap() { c.lap(kJbu); }
+  } lap{clk};
   const std::string rf = ResultFolder(dense_folder, problem.ref_image_id);
   const MapF coarse = LoadDmb(rf + "/depths_geom.dmb");
-  const MapF fine = ToGrayFloat(images.Current(problem.ref_image_id));
+  const MapF& fine = images.CurrentGray(problem.ref_image_id);
   const int scale = std::max(fine.height / coarse.height, fine.width / coarse.width);  // HPM.cpp:1270
   if (scale == 1) return;  // HPM.cpp:1272-1275
   MapF out(fine.width, fine.height);
@@ -987,10 +1215,11 @@ static ViewMaps LoadViewMaps(const std::string& dense_folder, const std::vector<
     std::ostringstream cp;
     cp << dense_folder << "/cams/" << std::setw(8) << std::setfill('0') << p.ref_image_id << "_cam.txt";
     hpm_camera c = ReadCamera(cp.str());
-    MapU8 img = LoadImage(dense_folder, p.ref_image_id);
+    int iw = 0, ih = 0;
+    ImageSize(dense_folder, p.ref_image_id, iw, ih);
     const MapF& d = v.depths.back();
-    if (img.width != d.width || img.height != d.height) {
-      const float sx = d.width / (float)img.width, sy = d.height / (float)img.height;
+    if (iw != d.width || ih != d.height) {
+      const float sx = d.width / (float)iw, sy = d.height / (float)ih;
       c.K[0] *= sx; c.K[2] *= sx; c.K[4] *= sy; c.K[5] *= sy;
     }
     c.width = d.width;
@@ -1019,7 +1248,14 @@ static ViewMaps LoadViewMaps(const std::string& dense_folder, const std::vector<
 }
 
 void ConfidenceEvaluation(const std::string& dense_folder, const std::vector<Problem>& problems, bool geom) {
+  StageClock clk;
+  struct Lap {
+    StageClock& c;
+    ~Lap() { c.lap(kConfidence); }
+  } lap{clk};
+  const double t0 = now_s();
   ViewMaps v = LoadViewMaps(dense_folder, problems, geom);
+  const double t1 = now_s();
   const int n = (int)problems.size();
   std::vector<MapF> conf;
   std::vector<const float*> dp, np;
@@ -1034,12 +1270,23 @@ void ConfidenceEvaluation(const std::string& dense_folder, const std::vector<Pro
   const int rc = hpm_consistency_check(HPM_CHECK_CONFIDENCE, n, v.cams.data(), dp.data(), np.data(), v.offsets.data(),
                                        v.ids.data(), cp.data(), nullptr, nullptr, 0, &count);
   if (rc != HPM_OK) throw Error("hpm_consistency_check (confidence) failed");
+  const double t2 = now_s();
   for (int i = 0; i < n; ++i)
     if (!WriteDmb(ResultFolder(dense_folder, problems[i].ref_image_id) + "/confidence.dmb", conf[i]))
       throw Error("cannot write confidence.dmb");
+  if (g_verbose > 0) {
+    std::fprintf(stderr, "[hpm] confidence (%d views): load %.1f ms, check %.1f ms, write %.1f ms\n", n,
+                 1e3 * (t1 - t0), 1e3 * (t2 - t1), 1e3 * (now_s() - t2));
+    std::fflush(stderr);
+  }
 }
 
 std::vector<FusedPoint> RunFusion(const std::string& dense_folder, const std::vector<Problem>& problems, bool geom) {
+  StageClock clk;
+  struct Lap {
+    StageClock& c;
+    ~Lap() { c.lap(kFusion); }
+  } lap{clk};
   ViewMaps v = LoadViewMaps(dense_folder, problems, geom);
   const int n = (int)problems.size();
   std::vector<const float*> dp, np;
@@ -1085,10 +1332,10 @@ std::vector<FusedPoint> RunPipeline(const std::string& dense_folder, int precisi
   const std::vector<Problem> problems = ReadPairFile(dense_folder + "/pair.txt");
   const int n = (int)problems.size();
   PassFlags f;
-  for (int i = 0; i < n; ++i) ProcessProblem(dense_folder, problems, i, f, precision, seed + i);  // photometric
+  parallel_views(n, [&](int i) { ProcessProblem(dense_folder, problems, i, f, precision, seed + i); });  // photometric
   ConfidenceEvaluation(dense_folder, problems, false);
   f.prior_consistency = true;
-  for (int i = 0; i < n; ++i) ProcessProblem(dense_folder, problems, i, f, precision, seed + 1000 + i);
+  parallel_views(n, [&](int i) { ProcessProblem(dense_folder, problems, i, f, precision, seed + 1000 + i); });
   f = PassFlags();
   for (int it = 0; it < 3; ++it) {  // main.cpp:1269-1290
     f.multi_geometry = it > 0;
@@ -1097,7 +1344,9 @@ std::vector<FusedPoint> RunPipeline(const std::string& dense_folder, int precisi
       ConfidenceEvaluation(dense_folder, problems, f.geom_consistency);
     }
     f.geom_consistency = true;
-    for (int i = 0; i < n; ++i) ProcessProblem(dense_folder, problems, i, f, precision, seed + 2000 * (it + 2) + i);
+    auto one = [&](int i) { ProcessProblem(dense_folder, problems, i, f, precision, seed + 2000 * (it + 2) + i); };
+    if (order_free(f)) parallel_views(n, one);
+    else for (int i = 0; i < n; ++i) one(i);
   }
   return RunFusion(dense_folder, problems, true);
 }
@@ -1138,11 +1387,14 @@ std::vector<FusedPoint> RunPyramidPipeline(const std::string& dense_folder, int 
   };
 
   uint64_t pass = 0;
-  auto each_view = [&](PassFlags f, const std::vector<int>* distances) {
-    for (int i = 0; i < n; ++i) {
-      if (distances) f.hpm_scale_distance = (*distances)[i];
-      ProcessProblem(dense_folder, problems, i, f, precision, seed + 10007 * pass + i, &images);
-    }
+  auto each_view = [&](const PassFlags& f, const std::vector<int>* distances) {
+    auto one = [&](int i) {
+      PassFlags fi = f;
+      if (distances) fi.hpm_scale_distance = (*distances)[i];
+      ProcessProblem(dense_folder, problems, i, fi, precision, seed + 10007 * pass + i, &images);
+    };
+    if (order_free(f)) parallel_views(n, one);
+    else for (int i = 0; i < n; ++i) one(i);
     ++pass;
   };
   const int max_hpm_scale = max_down;
@@ -1153,11 +1405,12 @@ std::vector<FusedPoint> RunPyramidPipeline(const std::string& dense_folder, int 
         --num_down[i];
       }
     for (int id : ids) images.PrepareLevel(id, level_side(id));
+    images.BuildPool(ids);
 
     PassFlags f;
     f.image_scale = level;
     if (level != max_down) {  // upsample the coarser level, then a hierarchy pass
-      for (int i = 0; i < n; ++i) JointBilateralUpsampling(dense_folder, problems[i], images);
+      parallel_views(n, [&](int i) { JointBilateralUpsampling(dense_folder, problems[i], images); });
       f.hierarchy = true;
     }
     each_view(f, nullptr);  // photometric or hierarchy pass

@@ -63,19 +63,44 @@ MapF ResizeLinear(const MapF& src, int w, int h);
 // The dense folder's images at their original size and the scaled copies of
 // the current pyramid level (PrepareScaleImages, main.cpp:107-161, kept in
 // memory instead of temporary PNG files).
+//
+// What every pass of a level derives from its images is derived once per
+// level: the grey float image, the edge map, the camera scaled to the level
+// size and, after BuildPool, one device copy of every image shared by the
+// passes' contexts.  The results are those of deriving them in every pass.
 class SceneImages {
  public:
   explicit SceneImages(std::string dense_folder) : folder_(std::move(dense_folder)) {}
+  ~SceneImages();
+  SceneImages(const SceneImages&) = delete;
+  SceneImages& operator=(const SceneImages&) = delete;
   const MapU8& Original(int id);
   // scales image `id` so that its larger side is at most target_max_side
   void PrepareLevel(int id, int target_max_side);
   // the current level's image (the original when no level was prepared)
   const MapU8& Current(int id);
+  // its grey float image (cv::cvtColor + convertTo, main.cpp:645-648)
+  const MapF& CurrentGray(int id);
+  // its edge map (cv::Canny 50 / 150, main.cpp:287)
+  const MapU8& CurrentEdges(int id);
+  // camera `id` of cams/%08d_cam.txt with K scaled to the current size
+  // (HPM.cpp:668-674)
+  hpm_camera CurrentCamera(int id);
+  // uploads the current grey images of `ids` to one device pool (replacing
+  // the previous level's); Pool() is null until then
+  void BuildPool(const std::vector<int>& ids, int device = 0);
+  hpm_pool* Pool() const { return pool_; }
+  // the image's index in the pool, -1 when it is not there
+  int PoolIndex(int id) const;
  private:
   std::string folder_;
-  std::vector<MapU8> orig_, cur_;
-  std::vector<bool> have_orig_, have_cur_;
+  std::vector<MapU8> orig_, cur_, edges_;
+  std::vector<MapF> gray_;
+  std::vector<bool> have_orig_, have_cur_, have_gray_, have_edges_;
+  hpm_pool* pool_ = nullptr;
+  std::vector<int> pool_index_;
   void ensure(int id);
+  void drop_pool();
 };
 
 struct FusedPoint {  // PointList, main.h:92-96
@@ -162,7 +187,7 @@ class HPM {
 
   int GetReferenceImageWidth() const { return cams_.empty() ? 0 : cams_[0].width; }
   int GetReferenceImageHeight() const { return cams_.empty() ? 0 : cams_[0].height; }
-  const MapF& GetReferenceImage() const { return images_.at(0); }
+  const MapF& GetReferenceImage() const { return *images_.at(0); }
   const hpm_camera& GetCamera(int i) const { return cams_.at(i); }
   float GetMinDepth() const;
   float GetMaxDepth() const;
@@ -189,8 +214,11 @@ class HPM {
   hpm_params params_{};
   bool multi_geometry_ = false;
   std::vector<hpm_camera> cams_;
-  std::vector<MapF> images_;
+  std::vector<const MapF*> images_;  // into owned_images_ or a SceneImages cache
+  std::vector<MapF> owned_images_;
   std::vector<MapF> depths_;
+  hpm_pool* pool_ = nullptr;  // the level's device images, when every view is there
+  std::vector<int32_t> pool_ids_;
 };
 
 // ---------------------------------------------------------- pass driver --
@@ -233,5 +261,26 @@ std::vector<FusedPoint> RunPyramidPipeline(const std::string& dense_folder, int 
                                            int precision, uint64_t seed);
 
 std::string ResultFolder(const std::string& dense_folder, int ref_image_id);
+
+// Wall time spent in each stage of the passes above, summed since the last
+// reset (process-wide).  Stages: setup (image and camera loading, device
+// upload, map loading), edges (Canny), prior (support points, Delaunay,
+// planes, prior JBU), patchmatch (RunPatchMatch and read-back), write (.dmb
+// output), jbu (JointBilateralUpsampling), confidence, fusion.
+enum Stage { kSetup, kEdges, kPrior, kPatchMatch, kWrite, kJbu, kConfidence, kFusion, kStageCount };
+struct StageTimes {
+  double seconds[kStageCount];
+  long long calls[kStageCount];
+};
+StageTimes GetStageTimes();
+void ResetStageTimes();
+// 0: quiet; 1: one line per ProcessProblem on stderr (view, pass, stage times)
+void SetVerbose(int level);
+// Host threads that run the views of an order-free pass concurrently (each
+// with its own context and stream; 0: min(8, hardware threads), 1: in view
+// order).  Every pass except the geometric passes over depths_geom.dmb is
+// order-free: its views read no file that another view of the pass writes,
+// so the files it produces do not depend on the thread count.
+void SetPassThreads(int n);
 
 }  // namespace hpmhost

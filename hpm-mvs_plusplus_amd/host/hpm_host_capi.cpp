@@ -74,6 +74,21 @@ int hpm_host_run_pyramid_pipeline(const char* dense_folder, int initial_image_sc
   });
 }
 
+int hpm_host_stage_times(double* seconds, int64_t* calls, int capacity) {
+  const hpmhost::StageTimes t = hpmhost::GetStageTimes();
+  for (int i = 0; i < capacity && i < hpmhost::kStageCount; ++i) {
+    if (seconds) seconds[i] = t.seconds[i];
+    if (calls) calls[i] = t.calls[i];
+  }
+  return hpmhost::kStageCount;
+}
+
+void hpm_host_reset_stage_times(void) { hpmhost::ResetStageTimes(); }
+
+void hpm_host_set_verbose(int level) { hpmhost::SetVerbose(level); }
+
+void hpm_host_set_threads(int n) { hpmhost::SetPassThreads(n); }
+
 int hpm_host_support_points(const float* costs, const float* confidences, int w, int h, int32_t* out_xy,
                             int capacity, int* count) {
   if (!costs || !confidences || w <= 0 || h <= 0 || !count || (capacity > 0 && !out_xy)) return HPM_E_INVALID;

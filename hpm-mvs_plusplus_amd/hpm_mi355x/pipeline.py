@@ -33,6 +33,13 @@ def host_lib():
     L.hpm_host_run_pipeline.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64)]
     L.hpm_host_run_pyramid_pipeline.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                 ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64)]
+    L.hpm_host_stage_times.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64), ctypes.c_int]
+    L.hpm_host_reset_stage_times.argtypes = []
+    L.hpm_host_reset_stage_times.restype = None
+    L.hpm_host_set_verbose.argtypes = [ctypes.c_int]
+    L.hpm_host_set_verbose.restype = None
+    L.hpm_host_set_threads.argtypes = [ctypes.c_int]
+    L.hpm_host_set_threads.restype = None
     L.hpm_host_support_points.argtypes = [fp, fp, ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_int)]
     L.hpm_host_delaunay.argtypes = [ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int, i32p, ctypes.c_int,
@@ -87,6 +94,30 @@ def run_pyramid_pipeline(folder, initial_image_scale=4, size_bound=1000, precisi
                                                     PRECISIONS[precision], seed, ctypes.byref(n)),
            "hpm_host_run_pyramid_pipeline")
     return n.value
+
+
+STAGES = ("setup", "edges", "prior", "patchmatch", "write", "jbu", "confidence", "fusion")
+
+
+def stage_times(reset=False):
+    """{stage: (seconds, calls)} summed since the last reset."""
+    n = len(STAGES)
+    sec = (ctypes.c_double * n)()
+    calls = (ctypes.c_int64 * n)()
+    host_lib().hpm_host_stage_times(sec, calls, n)
+    out = {k: (sec[i], calls[i]) for i, k in enumerate(STAGES)}
+    if reset:
+        host_lib().hpm_host_reset_stage_times()
+    return out
+
+
+def set_verbose(level=1):
+    host_lib().hpm_host_set_verbose(int(level))
+
+
+def set_threads(n=0):
+    """Host threads for the order-free passes (0: default, 1: view order)."""
+    host_lib().hpm_host_set_threads(int(n))
 
 
 def support_points(costs, confidences):

@@ -44,12 +44,57 @@ class Plane:
     tex_seed: int
     e1: np.ndarray = field(default=None)
     e2: np.ndarray = field(default=None)
+    half_extent: float = math.inf  # a square patch of this half side around `point` (inf: the whole plane)
 
     def __post_init__(self):
         a = np.array([1.0, 0.0, 0.0]) if abs(self.normal[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
         e1 = a - self.normal * (a @ self.normal)
         self.e1 = e1 / np.linalg.norm(e1)
         self.e2 = np.cross(self.normal, self.e1)
+
+
+    def intersect(self, C, rays):
+        """Ray parameter t of C + t * ray (inf where parallel)."""
+        denom = rays @ self.normal
+        num = (self.point - C) @ self.normal
+        with np.errstate(divide="ignore", invalid="ignore"):
+            t = num / denom
+        if math.isfinite(self.half_extent):
+            with np.errstate(invalid="ignore"):
+                d = C[None, None, :] + t[..., None] * rays - self.point
+                inside = (np.abs(d @ self.e1) <= self.half_extent) & (np.abs(d @ self.e2) <= self.half_extent)
+            t = np.where(inside, t, np.inf)
+        return t
+
+    def surface(self, X):
+        """Texture coordinates (m) and world normals at the points X (n x 3)."""
+        d = X - self.point
+        return d @ self.e1, d @ self.e2, np.broadcast_to(self.normal, X.shape)
+
+
+@dataclass
+class Sphere:
+    center: np.ndarray
+    radius: float
+    tex_seed: int
+
+    def intersect(self, C, rays):
+        """Nearest ray parameter t >= 0 of C + t * ray on the sphere (inf on a miss)."""
+        oc = C - self.center
+        a = np.einsum("...i,...i->...", rays, rays)
+        b = 2.0 * (rays @ oc)
+        c = oc @ oc - self.radius ** 2
+        disc = b * b - 4 * a * c
+        with np.errstate(invalid="ignore", divide="ignore"):
+            t = (-b - np.sqrt(disc)) / (2 * a)
+        return np.where(disc >= 0, t, np.inf)
+
+    def surface(self, X):
+        """Texture coordinates (arc lengths in m) and outward normals."""
+        d = (X - self.center) / self.radius
+        az = np.arctan2(d[:, 0], -d[:, 2])
+        el = np.arcsin(np.clip(d[:, 1], -1.0, 1.0))
+        return self.radius * az, self.radius * el, d
 
 
 @dataclass
@@ -144,7 +189,8 @@ def value_noise(u, v, seed, octaves=4, base_freq=6.0):
 
 
 def render(cam: Camera, planes, noise_sigma, rng, rows_per_chunk=256):
-    """Ray-cast the nearest plane per pixel: uint8 image, GT depth, GT world normal."""
+    """Ray-cast the nearest surface (planes and spheres) per pixel: uint8
+    image, GT depth, GT world normal."""
     H, W = cam.height, cam.width
     img = np.zeros((H, W), np.float64)
     depth = np.zeros((H, W), np.float32)
@@ -163,10 +209,7 @@ def render(cam: Camera, planes, noise_sigma, rng, rows_per_chunk=256):
         best_t = np.full(u.shape, np.inf)
         best_k = np.full(u.shape, -1, np.int64)
         for k, pl in enumerate(all_planes):
-            denom = rays_w @ pl.normal
-            num = (pl.point - C) @ pl.normal
-            with np.errstate(divide="ignore", invalid="ignore"):
-                t = num / denom
+            t = pl.intersect(C, rays_w)
             ok = (t > 0.5) & (t < 8.0) & np.isfinite(t)
             better = ok & (t < best_t)
             best_t = np.where(better, t, best_t)
@@ -179,11 +222,9 @@ def render(cam: Camera, planes, noise_sigma, rng, rows_per_chunk=256):
             m = best_k == k
             if not np.any(m):
                 continue
-            d = X[m] - pl.point
-            pu = d @ pl.e1
-            pv = d @ pl.e2
+            pu, pv, nrm = pl.surface(X[m])
             shade[m] = value_noise(pu, pv, pl.tex_seed)
-            normal[r0:r1][m] = pl.normal.astype(np.float32)
+            normal[r0:r1][m] = nrm.astype(np.float32)
         img[r0:r1] = 20.0 + 215.0 * shade
     img += rng.normal(0.0, noise_sigma, img.shape)
     img8 = np.clip(np.rint(img), 0, 255).astype(np.uint8)
@@ -248,6 +289,68 @@ def render_all(cams, planes, noise_sigma, seed, workers=None):
     import multiprocessing as mp
     with mp.get_context("fork").Pool(workers) as pool:
         return pool.map(_render_job, jobs, chunksize=1)
+
+
+def random_patches(rng, n, lo=(-1.6, -1.2, -1.4), hi=(1.6, 1.2, 1.4), ext=(0.4, 1.2), max_tilt_deg=50.0):
+    """Square textured plane patches: centres uniform in a box, half sides
+    U[ext], normals within max_tilt of -z (facing the cameras)."""
+    out = []
+    cos_max = math.cos(math.radians(max_tilt_deg))
+    for _ in range(n):
+        cz = rng.uniform(cos_max, 1.0)
+        phi = rng.uniform(0, 2 * math.pi)
+        sz = math.sqrt(max(0.0, 1 - cz * cz))
+        nrm = np.array([sz * math.cos(phi), sz * math.sin(phi), -cz])
+        out.append(Plane(normal=nrm, point=rng.uniform(lo, hi), tex_seed=int(rng.integers(1 << 30)),
+                         half_extent=float(rng.uniform(*ext))))
+    return out
+
+
+def random_spheres(rng, n, lo=(-1.0, -0.8, -1.6), hi=(1.0, 0.8, -0.4), r_lo=0.2, r_hi=0.6):
+    return [Sphere(center=rng.uniform(lo, hi), radius=float(rng.uniform(r_lo, r_hi)), tex_seed=int(rng.integers(1 << 30)))
+            for _ in range(n)]
+
+
+def make_orbit_scene(seed=3, n_views=30, width=3072, height=2048, n_planes=24, n_spheres=6, radius=4.0,
+                     elevations_deg=(15.0, 35.0), span_deg=60.0, n_src=10, noise_sigma=2.0, workers=None) -> Scene:
+    """Full-resolution scene: n_planes textured square plane patches (half
+    sides U[0.4, 1.2] m, normals within 50 deg of the scene axis) plus
+    n_spheres textured spheres (radius U[0.2, 0.6] m) in front of a background
+    plane, seen by n_views
+    cameras on two rings (radius 4 m, elevations 15 and 35 deg) over a 60 deg
+    arc, focal length 525 px per 640 px of width, each with its n_src
+    top-scored sources.  Rays that leave the scene within 8 m (a few per cent
+    near the top corners of the high ring) hit nothing: depth 0 there."""
+    rng = np.random.default_rng(seed)
+    f = 525.0 * width / 640.0
+    cams = ring_cameras(n_views, radius, elevations_deg, f, width / 2, height / 2, width, height, span_deg=span_deg)
+    surfaces = random_patches(rng, n_planes)
+    surfaces += random_spheres(rng, n_spheres)
+    surfaces.append(Plane(normal=np.array([0.0, 0.0, -1.0]), point=np.array([0.0, 0.0, 2.5]),
+                          tex_seed=int(rng.integers(1 << 30))))
+    for c in cams:
+        c.depth_min, c.depth_max = 1.0, 7.5
+    out = render_all(cams, surfaces, noise_sigma, seed + 2, workers)
+    return Scene(planes=surfaces, cameras=cams, images=[o[0] for o in out], depths=[o[1] for o in out],
+                 normals=[o[2] for o in out], pairs=pair_scores(cams, n_src))
+
+
+def scaled_camera(cam: Camera, width: int, height: int) -> Camera:
+    """The camera of an image resized to width x height: K scaled per axis by
+    the size ratio, as RescaleImageAndCamera (HPM.cpp:186-207) and
+    InuputInitialization (HPM.cpp:668-674) scale it."""
+    sx, sy = width / cam.width, height / cam.height
+    K = cam.K.copy()
+    K[0, 0] *= sx
+    K[0, 2] *= sx
+    K[1, 1] *= sy
+    K[1, 2] *= sy
+    return Camera(K=K, R=cam.R, t=cam.t, width=width, height=height, depth_min=cam.depth_min, depth_max=cam.depth_max)
+
+
+def ground_truth_depth(scene: Scene, cam: Camera):
+    """Ray-cast depth of the scene's surfaces for any camera (no texture)."""
+    return render(cam, scene.planes, 0.0, np.random.default_rng(0))[1]
 
 
 def make_batch_scene(seed=1, n_views=64, sizes=((640, 480), (1024, 768), (1600, 1200), (2048, 1536)),

@@ -43,6 +43,23 @@ int hpm_host_run_pipeline(const char* dense_folder, int precision, uint64_t seed
 int hpm_host_run_pyramid_pipeline(const char* dense_folder, int initial_image_scale, int size_bound, int precision,
                                   uint64_t seed, int64_t* num_points);
 
+/* Wall time of the pipeline stages, summed process-wide since the last reset
+ * (no counterpart in the reference, which prints progress lines).  Stage order:
+ * 0 setup (image/camera loading, device upload, map loading), 1 edges (Canny),
+ * 2 prior (support points, Delaunay, planes, prior JBU), 3 patchmatch
+ * (RunPatchMatch and read-back), 4 write (.dmb output), 5 jbu, 6 confidence,
+ * 7 fusion.  Fills up to `capacity` entries; returns the number of stages. */
+int hpm_host_stage_times(double* seconds, int64_t* calls, int capacity);
+void hpm_host_reset_stage_times(void);
+/* 0 quiet; 1: one line per ProcessProblem on stderr (view, pass, stage times). */
+void hpm_host_set_verbose(int level);
+/* Host threads running the views of a pass concurrently, each view with its
+ * own context and stream (0: min(8, hardware threads); 1: view order).  Only
+ * passes whose views read no file another view of the pass writes run
+ * concurrently (all but the geometric passes over depths_geom.dmb, which the
+ * reference runs in view order); the output files do not depend on it. */
+void hpm_host_set_threads(int n);
+
 /* GetSupportPoints_Classify_Check (HPM.cpp:1026-1073) at factor 1: costs and
  * confidences w*h; out_xy receives up to `capacity` (x, y) pairs. */
 int hpm_host_support_points(const float* costs, const float* confidences, int w, int h, int32_t* out_xy,

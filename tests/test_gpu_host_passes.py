@@ -43,11 +43,12 @@ def test_jbu_prior_bitexact(h, w, s):
     np.testing.assert_array_equal(gn, rn)
 
 
-def _noisy_maps(seed=11):
+def _noisy_maps(seed=11, n_cams=5, width=96, height=72):
     """Ground-truth depth / normal maps of 5 views with 0.3 % depth noise, 5 %
     outliers and a few invalid pixels, so that every branch of the test fires."""
     from hpm_mi355x import scenes
-    sc = scenes.make_plane_scene(seed=2, n_cams=5, width=96, height=72, f=78.75, n_src=4)
+    sc = scenes.make_plane_scene(seed=2, n_cams=n_cams, width=width, height=height, f=78.75 * width / 96,
+                                 n_src=n_cams - 1)
     rng = np.random.default_rng(seed)
     depths, normals = [], []
     for d, n in zip(sc.depths, sc.normals):
@@ -87,6 +88,29 @@ def test_confidence_bitexact():
     assert sum(int((x > 0).sum()) for x in r) > 1000
     for a, b in zip(g, r):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_consistency_chunked_bitexact(mode):
+    """Views of 1280 x 960 (1.23 Mpix) are resolved in two chunks of rows each,
+    and a view without sources sits between them: fusion and confidence still
+    equal the oracle's bit for bit."""
+    import hpm_mi355x as H
+    print(f"case: consistency mode {mode} 3 x 1280x960, chunked", flush=True)
+    sc, depths, normals, sources = _noisy_maps(13 + mode, n_cams=3, width=1280, height=960)
+    sources[1] = []
+    hc = [H.camera_from_scene(c) for c in sc.cameras]
+    oc = [O.camera_from_scene(c) for c in sc.cameras]
+    g = H.consistency_check(H.CHECK_FUSION if mode == 0 else H.CHECK_CONFIDENCE, hc, depths, normals, sources)
+    r = O.consistency(mode, oc, depths, normals, sources)
+    if mode == 0:
+        assert len(r[0]) > 100000
+        np.testing.assert_array_equal(g[0], r[0])
+        np.testing.assert_array_equal(g[1], r[1])
+    else:
+        assert sum(int((x > 0).sum()) for x in r) > 100000
+        for a, b in zip(g, r):
+            np.testing.assert_array_equal(a, b)
 
 
 def test_fusion_two_view_closed_form_on_gpu():

@@ -111,3 +111,30 @@ def test_pyramid_pipeline_three_levels(tmp_path):
         head = fh.read(400).split(b"end_header\n")[0].decode()
     assert f"element vertex {n}" in head
 
+
+def test_pipeline_files_do_not_depend_on_host_threads(tmp_path):
+    """The order-free passes run their views on several host threads; every
+    file the pipeline writes (.dmb maps, both PLY files) is byte-identical to
+    a run in view order."""
+    import filecmp
+    from hpm_mi355x import pipeline as P
+    from hpm_mi355x import scenes
+    sc = scenes.make_plane_scene(seed=6, n_cams=5, width=160, height=120, f=131.25, n_src=4)
+    runs = {}
+    for threads in (1, 4):
+        folder = str(tmp_path / f"t{threads}")
+        scenes.write_dense_folder(sc, folder)
+        P.set_threads(threads)
+        print(f"case: pyramid pipeline, 5 views 160x120, bound 100, {threads} host threads", flush=True)
+        runs[threads] = (folder, P.run_pyramid_pipeline(folder, initial_image_scale=1, size_bound=100,
+                                                        precision="fp16", seed=4))
+    P.set_threads(0)
+    assert runs[1][1] == runs[4][1]
+    names = []
+    for root, _, files in os.walk(os.path.join(runs[1][0], "HPM_MVS_plusplus")):
+        for f in files:
+            names.append(os.path.relpath(os.path.join(root, f), runs[1][0]))
+    assert len(names) > 5 * 6
+    for rel in names:
+        assert filecmp.cmp(os.path.join(runs[1][0], rel), os.path.join(runs[4][0], rel), shallow=False), rel
+
