@@ -860,8 +860,13 @@ int hpm_cost_grad(hpm_ctx* ctx, const float* params, const float* weights, float
   return HPM_OK;
 }
 
+// Refinement loop: per iteration a random-perturbation proposal (kRandom),
+// a Gauss-Newton / Levenberg-Marquardt step (kGaussNewton), or both in turn
+// (kHybrid: the random proposal reaches the basin, the GN step converges in
+// it); every proposal is kept only where it lowers the cost.
+enum RefineMode { kRandom = 0, kGaussNewton = 1, kHybrid = 2 };
 static int refine_common(hpm_ctx* ctx, float* params, const float* weights, int iterations, float* out_cost,
-                         bool gauss_newton, uint64_t seed) {
+                         int mode, uint64_t seed) {
   if (!ctx || !params || !weights || iterations < 0) return HPM_E_INVALID;
   int rc = grad_setup(ctx);
   if (rc) return rc;
@@ -872,23 +877,30 @@ static int refine_common(hpm_ctx* ctx, float* params, const float* weights, int 
   const dim3 b1(256), g1((P + 255) / 256);
   hipLaunchKernelGGL(k_fill_f32, g1, b1, 0, ctx->stream, B.lambda, P, 1e-2f);
   HIP_OK(hipGetLastError());
-  if ((rc = grad_launch(ctx, B.params, B.weights, B.cost, gauss_newton ? B.grad : nullptr,
-                        gauss_newton ? B.gn : nullptr)))
+  const bool gn_first = mode == kGaussNewton;
+  if ((rc = grad_launch(ctx, B.params, B.weights, B.cost, gn_first ? B.grad : nullptr, gn_first ? B.gn : nullptr)))
     return rc;
+  auto propose_and_accept = [&](bool gn_step) -> int {
+    if ((rc = grad_launch(ctx, B.cand, B.weights, B.cand_cost, nullptr, nullptr))) return rc;
+    hipLaunchKernelGGL(k_accept, g1, b1, 0, ctx->stream, P, B.params, B.cost, B.cand, B.cand_cost,
+                       gn_step ? B.lambda : nullptr);
+    HIP_OK(hipGetLastError());
+    return HPM_OK;
+  };
   for (int it = 0; it < iterations; ++it) {
-    if (gauss_newton) {
-      if (it > 0 && (rc = grad_launch(ctx, B.params, B.weights, B.cost, B.grad, B.gn))) return rc;
-      hipLaunchKernelGGL(k_gn_step, g1, b1, 0, ctx->stream, P, B.params, B.grad, B.gn, B.lambda, dmin, dmax, B.cand);
-    } else {
+    if (mode != kGaussNewton) {
       const float decay = powf(0.8f, (float)it);
       hipLaunchKernelGGL(k_random_step, g1, b1, 0, ctx->stream, P, seed, (uint32_t)(1000 + it), B.params,
                          0.02f * decay, 0.0872665f * decay, dmin, dmax, B.cand);
+      HIP_OK(hipGetLastError());
+      if ((rc = propose_and_accept(false))) return rc;
     }
-    HIP_OK(hipGetLastError());
-    if ((rc = grad_launch(ctx, B.cand, B.weights, B.cand_cost, nullptr, nullptr))) return rc;
-    hipLaunchKernelGGL(k_accept, g1, b1, 0, ctx->stream, P, B.params, B.cost, B.cand, B.cand_cost,
-                       gauss_newton ? B.lambda : nullptr);
-    HIP_OK(hipGetLastError());
+    if (mode != kRandom) {
+      if ((it > 0 || mode == kHybrid) && (rc = grad_launch(ctx, B.params, B.weights, B.cost, B.grad, B.gn))) return rc;
+      hipLaunchKernelGGL(k_gn_step, g1, b1, 0, ctx->stream, P, B.params, B.grad, B.gn, B.lambda, dmin, dmax, B.cand);
+      HIP_OK(hipGetLastError());
+      if ((rc = propose_and_accept(true))) return rc;
+    }
   }
   if ((rc = download(ctx, params, B.params, (size_t)P * sizeof(float4)))) return rc;
   if (out_cost && (rc = download(ctx, out_cost, B.cost, (size_t)P * 4))) return rc;
@@ -896,12 +908,17 @@ static int refine_common(hpm_ctx* ctx, float* params, const float* weights, int 
 }
 
 int hpm_refine_gn(hpm_ctx* ctx, float* params, const float* weights, int iterations, float* out_cost) {
-  return refine_common(ctx, params, weights, iterations, out_cost, true, 0);
+  return refine_common(ctx, params, weights, iterations, out_cost, kGaussNewton, 0);
 }
 
 int hpm_refine_random(hpm_ctx* ctx, float* params, const float* weights, int iterations, uint64_t seed,
                       float* out_cost) {
-  return refine_common(ctx, params, weights, iterations, out_cost, false, seed);
+  return refine_common(ctx, params, weights, iterations, out_cost, kRandom, seed);
+}
+
+int hpm_refine_hybrid(hpm_ctx* ctx, float* params, const float* weights, int iterations, uint64_t seed,
+                      float* out_cost) {
+  return refine_common(ctx, params, weights, iterations, out_cost, kHybrid, seed);
 }
 
 }  // extern "C"

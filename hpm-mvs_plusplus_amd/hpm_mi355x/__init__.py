@@ -59,7 +59,7 @@ EXPORTS = [
     "hpm_get_stats", "hpm_set_timing", "hpm_cost_fixed", "hpm_cost_fixed_device", "hpm_init_cost",
     "hpm_geom_cost", "hpm_jbu_depth", "hpm_jbu_prior", "hpm_consistency_check", "hpm_set_graph",
     "hpm_run_patchmatch_batch", "hpm_pool_create", "hpm_pool_destroy", "hpm_set_views_pooled",
-    "hpm_cost_grad", "hpm_cost_grad_device", "hpm_refine_gn", "hpm_refine_random",
+    "hpm_cost_grad", "hpm_cost_grad_device", "hpm_refine_gn", "hpm_refine_random", "hpm_refine_hybrid",
     "hpm_batch_create", "hpm_batch_destroy", "hpm_batch_last_error", "hpm_batch_run", "hpm_batch_set_graph",
     "hpm_batch_set_shape", "hpm_batch_get_shape", "hpm_batch_last_ms", "hpm_batch_autotune",
 ]
@@ -101,6 +101,7 @@ def lib():
     L.hpm_cost_grad_device.argtypes = [P, P, P, P, P, P]
     L.hpm_refine_gn.argtypes = [P, fp, fp, ctypes.c_int, fp]
     L.hpm_refine_random.argtypes = [P, fp, fp, ctypes.c_int, ctypes.c_uint64, fp]
+    L.hpm_refine_hybrid.argtypes = [P, fp, fp, ctypes.c_int, ctypes.c_uint64, fp]
     L.hpm_pool_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.c_int, ctypes.POINTER(hpm_camera),
                                   ctypes.POINTER(fp)]
     L.hpm_pool_destroy.argtypes = [P]
@@ -294,7 +295,7 @@ class Context:
                     "hpm_cost_grad_device")
 
     def refine(self, params, weights, iterations, method="gn", seed=1):
-        """hpm_refine_gn / hpm_refine_random: returns (params (h, w, 4), cost (h, w))."""
+        """hpm_refine_gn / _random / _hybrid: returns (params (h, w, 4), cost (h, w))."""
         h, w = self.height, self.width
         pp = np.zeros((h, w, 4), np.float32)
         pp[..., :3] = np.asarray(params, np.float32)[..., :3]
@@ -302,9 +303,14 @@ class Context:
         cost = np.empty((h, w), np.float32)
         if method == "gn":
             rc = self._L.hpm_refine_gn(self._h, _fptr(pp), _fptr(wt), iterations, _fptr(cost))
-        else:
+        elif method == "random":
             rc = self._L.hpm_refine_random(self._h, _fptr(pp), _fptr(wt), iterations, ctypes.c_uint64(seed),
                                            _fptr(cost))
+        elif method == "hybrid":
+            rc = self._L.hpm_refine_hybrid(self._h, _fptr(pp), _fptr(wt), iterations, ctypes.c_uint64(seed),
+                                           _fptr(cost))
+        else:
+            raise ValueError(f"unknown refinement method {method!r}")
         self._check(rc, "hpm_refine_" + method)
         return pp, cost
 

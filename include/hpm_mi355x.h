@@ -269,6 +269,12 @@ int hpm_refine_gn(hpm_ctx* ctx, float* params, const float* weights, int iterati
  * (u uniform in [-1, 1], s = 0.8^step), kept where it lowers the cost. */
 int hpm_refine_random(hpm_ctx* ctx, float* params, const float* weights, int iterations, uint64_t seed,
                       float* out_cost);
+/* Both in turn: per step the random candidate above, then a Gauss-Newton step
+ * from the result, each kept where it lowers the cost (the random candidate
+ * reaches the basin of the optimum when the start lies outside the few
+ * pixels of disparity where the GN model holds; GN then converges in it). */
+int hpm_refine_hybrid(hpm_ctx* ctx, float* params, const float* weights, int iterations, uint64_t seed,
+                      float* out_cost);
 
 /* Multi-view consistency check of RunFusion (mode HPM_CHECK_FUSION,
  * main.cpp:823-957) and ConfidenceEvaluation (mode HPM_CHECK_CONFIDENCE,
