@@ -794,16 +794,78 @@ static int grad_setup(hpm_ctx* ctx) {
   return HPM_OK;
 }
 
-static int grad_launch(hpm_ctx* ctx, const void* params, const void* weights, void* cost, void* grad, void* gn) {
+static int grad_launch(hpm_ctx* ctx, const void* params, const void* weights, void* cost, void* grad, void* gn,
+                       const RunConsts* consts = nullptr) {
   const dim3 b(16, 16), g((ctx->W + 15) / 16, (ctx->H + 15) / 16);
+  const RunConsts* C = consts ? consts : ctx->d_consts;
   if (grad)
-    hipLaunchKernelGGL(k_cost_grad<true>, g, b, 0, ctx->stream, ctx->d_consts, (const float4*)params,
-                       (const float*)weights, (float*)cost, (float*)grad, (float*)gn);
+    hipLaunchKernelGGL(k_cost_grad<true>, g, b, 0, ctx->stream, C, (const float4*)params, (const float*)weights,
+                       (float*)cost, (float*)grad, (float*)gn);
   else
-    hipLaunchKernelGGL(k_cost_grad<false>, g, b, 0, ctx->stream, ctx->d_consts, (const float4*)params,
-                       (const float*)weights, (float*)cost, nullptr, nullptr);
+    hipLaunchKernelGGL(k_cost_grad<false>, g, b, 0, ctx->stream, C, (const float4*)params, (const float*)weights,
+                       (float*)cost, nullptr, nullptr);
   HIP_OK(hipGetLastError());
   return HPM_OK;
+}
+
+// The views smoothed by a Gaussian of `sigma` px (radius ceil(3 sigma), at
+// most 16), as FP32 images and FP32 quads, and a constant block reading them:
+// the cost of the smoothed images has a basin of convergence as wide as the
+// blur, where the sharp images' is about a texel of their finest texture.
+struct SmoothedViews {
+  std::vector<float*> img;
+  std::vector<void*> quad;
+  RunConsts* d_consts = nullptr;
+  ~SmoothedViews() {
+    for (float* p : img) hipFree(p);
+    for (void* p : quad) hipFree(p);
+    hipFree(d_consts);
+  }
+};
+
+static int build_smoothed(hpm_ctx* ctx, float sigma, SmoothedViews& S) {
+  BlurTaps t{};
+  t.r = std::min(kBlurMaxRadius, std::max(1, (int)std::ceil(3.0f * sigma)));
+  double sum = 0.0;
+  for (int k = -t.r; k <= t.r; ++k) sum += std::exp(-0.5 * (double)k * k / ((double)sigma * sigma));
+  for (int k = -t.r; k <= t.r; ++k) t.w[k + t.r] = (float)(std::exp(-0.5 * (double)k * k / ((double)sigma * sigma)) / sum);
+  float* tmp = nullptr;
+  size_t max_px = 0;
+  for (int i = 0; i < ctx->nimg; ++i) max_px = std::max(max_px, (size_t)ctx->cams[i].width * ctx->cams[i].height);
+  int rc = dalloc(ctx, &tmp, max_px);
+  if (rc) return rc;
+  S.img.assign(ctx->nimg, nullptr);
+  S.quad.assign(ctx->nimg, nullptr);
+  for (int i = 0; i < ctx->nimg && rc == HPM_OK; ++i) {
+    const int w = ctx->cams[i].width, h = ctx->cams[i].height;
+    if ((rc = dalloc(ctx, &S.img[i], (size_t)w * h))) break;
+    if (hipMalloc(&S.quad[i], (size_t)(w + 1) * (h + 1) * sizeof(QuadF32::Q)) != hipSuccess) {
+      S.quad[i] = nullptr;
+      rc = fail(ctx, HPM_E_NOMEM, "smoothed quad allocation failed");
+      break;
+    }
+    const dim3 b(16, 16), g((w + 15) / 16, (h + 15) / 16);
+    hipLaunchKernelGGL(k_blur<true>, g, b, 0, ctx->stream, ctx->d_img[i], tmp, w, h, t);
+    hipLaunchKernelGGL(k_blur<false>, g, b, 0, ctx->stream, tmp, S.img[i], w, h, t);
+    const dim3 gq((w + 1 + 15) / 16, (h + 1 + 15) / 16);
+    hipLaunchKernelGGL(k_build_quad<QuadF32>, gq, b, 0, ctx->stream, S.img[i], w, h, S.quad[i]);
+    if (hipGetLastError() != hipSuccess) rc = fail(ctx, HPM_E_DEVICE, "smoothing launch failed");
+  }
+  if (rc == HPM_OK) {
+    RunConsts C;
+    build_consts(ctx, C);
+    for (int i = 0; i < ctx->nimg; ++i) {
+      C.img[i] = S.img[i];
+      C.quad[i] = S.quad[i];
+    }
+    C.storage = HPM_PREC_FP32_FAST;
+    if ((rc = dalloc(ctx, &S.d_consts, 1)) == HPM_OK &&
+        hipMemcpyAsync(S.d_consts, &C, sizeof(C), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+      rc = fail(ctx, HPM_E_DEVICE, "constant upload failed");
+    if (rc == HPM_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(ctx, HPM_E_DEVICE, "smoothing failed");
+  }
+  hipFree(tmp);
+  return rc;
 }
 
 int hpm_cost_grad_device(hpm_ctx* ctx, const void* params_dev, const void* weights_dev, void* out_cost_dev,
@@ -866,7 +928,7 @@ int hpm_cost_grad(hpm_ctx* ctx, const float* params, const float* weights, float
 // it); every proposal is kept only where it lowers the cost.
 enum RefineMode { kRandom = 0, kGaussNewton = 1, kHybrid = 2 };
 static int refine_common(hpm_ctx* ctx, float* params, const float* weights, int iterations, float* out_cost,
-                         int mode, uint64_t seed) {
+                         int mode, uint64_t seed, int smoothed_iterations = 0, const RunConsts* smoothed = nullptr) {
   if (!ctx || !params || !weights || iterations < 0) return HPM_E_INVALID;
   int rc = grad_setup(ctx);
   if (rc) return rc;
@@ -877,17 +939,28 @@ static int refine_common(hpm_ctx* ctx, float* params, const float* weights, int 
   const dim3 b1(256), g1((P + 255) / 256);
   hipLaunchKernelGGL(k_fill_f32, g1, b1, 0, ctx->stream, B.lambda, P, 1e-2f);
   HIP_OK(hipGetLastError());
+  // the first smoothed_iterations steps run on the smoothed views' cost
+  const RunConsts* cur = (smoothed && smoothed_iterations > 0) ? smoothed : nullptr;
   const bool gn_first = mode == kGaussNewton;
-  if ((rc = grad_launch(ctx, B.params, B.weights, B.cost, gn_first ? B.grad : nullptr, gn_first ? B.gn : nullptr)))
+  if ((rc = grad_launch(ctx, B.params, B.weights, B.cost, gn_first ? B.grad : nullptr, gn_first ? B.gn : nullptr,
+                        cur)))
     return rc;
   auto propose_and_accept = [&](bool gn_step) -> int {
-    if ((rc = grad_launch(ctx, B.cand, B.weights, B.cand_cost, nullptr, nullptr))) return rc;
+    if ((rc = grad_launch(ctx, B.cand, B.weights, B.cand_cost, nullptr, nullptr, cur))) return rc;
     hipLaunchKernelGGL(k_accept, g1, b1, 0, ctx->stream, P, B.params, B.cost, B.cand, B.cand_cost,
                        gn_step ? B.lambda : nullptr);
     HIP_OK(hipGetLastError());
     return HPM_OK;
   };
   for (int it = 0; it < iterations; ++it) {
+    if (cur && it == smoothed_iterations) {  // to the sharp views: their cost, their gradient, a fresh damping
+      cur = nullptr;
+      hipLaunchKernelGGL(k_fill_f32, g1, b1, 0, ctx->stream, B.lambda, P, 1e-2f);
+      HIP_OK(hipGetLastError());
+      if ((rc = grad_launch(ctx, B.params, B.weights, B.cost, gn_first ? B.grad : nullptr, gn_first ? B.gn : nullptr)))
+        return rc;
+    }
+    const bool fresh = cur == nullptr && smoothed && it == smoothed_iterations;
     if (mode != kGaussNewton) {
       const float decay = powf(0.8f, (float)it);
       hipLaunchKernelGGL(k_random_step, g1, b1, 0, ctx->stream, P, seed, (uint32_t)(1000 + it), B.params,
@@ -896,11 +969,15 @@ static int refine_common(hpm_ctx* ctx, float* params, const float* weights, int 
       if ((rc = propose_and_accept(false))) return rc;
     }
     if (mode != kRandom) {
-      if ((it > 0 || mode == kHybrid) && (rc = grad_launch(ctx, B.params, B.weights, B.cost, B.grad, B.gn))) return rc;
+      if ((it > 0 || mode == kHybrid) && !fresh && (rc = grad_launch(ctx, B.params, B.weights, B.cost, B.grad, B.gn, cur)))
+        return rc;
       hipLaunchKernelGGL(k_gn_step, g1, b1, 0, ctx->stream, P, B.params, B.grad, B.gn, B.lambda, dmin, dmax, B.cand);
       HIP_OK(hipGetLastError());
       if ((rc = propose_and_accept(true))) return rc;
     }
+  }
+  if (cur) {  // every step ran on the smoothed views: report the sharp views' cost
+    if ((rc = grad_launch(ctx, B.params, B.weights, B.cost, nullptr, nullptr))) return rc;
   }
   if ((rc = download(ctx, params, B.params, (size_t)P * sizeof(float4)))) return rc;
   if (out_cost && (rc = download(ctx, out_cost, B.cost, (size_t)P * 4))) return rc;
@@ -919,6 +996,19 @@ int hpm_refine_random(hpm_ctx* ctx, float* params, const float* weights, int ite
 int hpm_refine_hybrid(hpm_ctx* ctx, float* params, const float* weights, int iterations, uint64_t seed,
                       float* out_cost) {
   return refine_common(ctx, params, weights, iterations, out_cost, kHybrid, seed);
+}
+
+int hpm_refine_gn_smoothed(hpm_ctx* ctx, float* params, const float* weights, int iterations,
+                           int smoothed_iterations, float sigma, float* out_cost) {
+  if (!ctx || !params || !weights || iterations < 0 || smoothed_iterations < 0 || !(sigma > 0.0f) || sigma > 5.0f)
+    return HPM_E_INVALID;
+  if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
+  HIP_OK(hipSetDevice(ctx->device));
+  SmoothedViews S;
+  int rc = build_smoothed(ctx, sigma, S);
+  if (rc) return rc;
+  return refine_common(ctx, params, weights, iterations, out_cost, kGaussNewton, 0,
+                       std::min(smoothed_iterations, iterations), S.d_consts);
 }
 
 }  // extern "C"

@@ -142,7 +142,8 @@ int hpm_consistency_check(int mode, int num_images, const hpm_camera* cameras, c
                           float* const* out_confidence, int32_t* out_pairs, float* out_points, int64_t capacity,
                           int64_t* out_count) {
   using namespace hpm;
-  if ((mode != HPM_CHECK_FUSION && mode != HPM_CHECK_CONFIDENCE) || num_images < 1 || num_images > kFusionMaxImages ||
+  if ((mode != HPM_CHECK_FUSION && mode != HPM_CHECK_CONFIDENCE && mode != HPM_CHECK_FUSION_MASK) || num_images < 1 ||
+      num_images > kFusionMaxImages ||
       !cameras || !depths || !normals || !src_offsets || !src_ids || !out_count)
     return HPM_E_INVALID;
   if (mode == HPM_CHECK_CONFIDENCE) {
@@ -150,7 +151,8 @@ int hpm_consistency_check(int mode, int num_images, const hpm_camera* cameras, c
     for (int i = 0; i < num_images; ++i)
       if (!out_confidence[i]) return HPM_E_INVALID;
   }
-  if (mode == HPM_CHECK_FUSION && capacity > 0 && !out_pairs) return HPM_E_INVALID;
+  const bool fusion = mode != HPM_CHECK_CONFIDENCE;
+  if (fusion && capacity > 0 && !out_pairs) return HPM_E_INVALID;
   if (src_offsets[0] != 0) return HPM_E_INVALID;
   size_t max_p = 0;
   int max_ngb = 0;
@@ -268,8 +270,12 @@ int hpm_consistency_check(int mode, int num_images, const hpm_camera* cameras, c
         dynamic_consistency += row[j].term;
         num_consistent++;
       }
-      if (num_consistent >= 1 && ((double)dynamic_consistency > 0.3 * num_consistent)) {
-        if (mode == HPM_CHECK_FUSION) {
+      // RunFusion compares with the double 0.3 * n; RunFusion_Sky_Strict
+      // (main.cpp:793-797) with `float factor = 0.3`, i.e. in float
+      const bool pass = mode == HPM_CHECK_FUSION_MASK ? dynamic_consistency > 0.3f * (float)num_consistent
+                                                      : (double)dynamic_consistency > 0.3 * num_consistent;
+      if (num_consistent >= 1 && pass) {
+        if (fusion) {
           if (npts < capacity) {
             out_pairs[2 * npts] = i;
             out_pairs[2 * npts + 1] = (int32_t)p;

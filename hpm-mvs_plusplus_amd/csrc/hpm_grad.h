@@ -321,4 +321,28 @@ __global__ __launch_bounds__(256) void k_random_step(int P, uint64_t seed, uint3
   cand[p] = x;
 }
 
+// Separable Gaussian smoothing of an FP32 image (border replicated), for the
+// smoothed first stage of the coarse-to-fine Gauss-Newton refinement.
+constexpr int kBlurMaxRadius = 16;
+struct BlurTaps {
+  float w[2 * kBlurMaxRadius + 1];
+  int r;
+};
+
+template <bool kRows>
+__global__ __launch_bounds__(256) void k_blur(const float* __restrict__ in, float* __restrict__ out, int w, int h,
+                                              BlurTaps t) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  if (x >= w || y >= h) return;
+  float acc = 0.0f;
+  for (int k = -t.r; k <= t.r; ++k) {
+    int xx = kRows ? x + k : x, yy = kRows ? y : y + k;
+    xx = xx < 0 ? 0 : (xx >= w ? w - 1 : xx);
+    yy = yy < 0 ? 0 : (yy >= h ? h - 1 : yy);
+    acc = fmaf(t.w[k + t.r], in[(size_t)yy * w + xx], acc);
+  }
+  out[(size_t)y * w + x] = acc;
+}
+
 }  // namespace hpm

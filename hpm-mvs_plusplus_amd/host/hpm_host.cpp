@@ -19,6 +19,7 @@
 #include <cstring>
 #include <fstream>
 #include <iomanip>
+#include <memory>
 #include <sstream>
 
 namespace hpmhost {
@@ -1281,12 +1282,16 @@ void ConfidenceEvaluation(const std::string& dense_folder, const std::vector<Pro
   }
 }
 
-std::vector<FusedPoint> RunFusion(const std::string& dense_folder, const std::vector<Problem>& problems, bool geom) {
-  StageClock clk;
-  struct Lap {
-    StageClock& c;
-    ~Lap() { c.lap(kFusion); }
-  } lap{clk};
+namespace {
+// The fused points of RunFusion's consistency pass (main.cpp:823-957), with
+// the view and pixel each one comes from.
+struct Fused {
+  std::vector<FusedPoint> pts;
+  std::vector<int32_t> view, pixel;
+  std::vector<int> width, height;  // per view: the depth map size
+};
+
+Fused FuseViews(const std::string& dense_folder, const std::vector<Problem>& problems, bool geom, int mode) {
   ViewMaps v = LoadViewMaps(dense_folder, problems, geom);
   const int n = (int)problems.size();
   std::vector<const float*> dp, np;
@@ -1296,36 +1301,97 @@ std::vector<FusedPoint> RunFusion(const std::string& dense_folder, const std::ve
     np.push_back(v.normals[i].data.data());
     cap += v.depths[i].data.size();
   }
-  std::vector<int32_t> pairs(2 * cap);
-  std::vector<float> xyzn(6 * cap);
+  // room for every pixel, left uninitialised: only the fused ones are written
+  std::unique_ptr<int32_t[]> pairs(new int32_t[2 * std::max<size_t>(cap, 1)]);
+  std::unique_ptr<float[]> xyzn(new float[6 * std::max<size_t>(cap, 1)]);
   int64_t count = 0;
-  const int rc = hpm_consistency_check(HPM_CHECK_FUSION, n, v.cams.data(), dp.data(), np.data(), v.offsets.data(),
-                                       v.ids.data(), nullptr, pairs.data(), xyzn.data(), (int64_t)cap, &count);
+  const int rc = hpm_consistency_check(mode, n, v.cams.data(), dp.data(), np.data(), v.offsets.data(),
+                                       v.ids.data(), nullptr, pairs.get(), xyzn.get(), (int64_t)cap, &count);
   if (rc != HPM_OK) throw Error("hpm_consistency_check (fusion) failed");
   // colours from the image resized to its depth map (RescaleImageAndCamera,
   // HPM.cpp:186-207)
   std::vector<MapU8> imgs;
+  Fused F;
   for (int i = 0; i < n; ++i) {
     MapU8 img = LoadImage(dense_folder, problems[i].ref_image_id);
     if (img.width != v.depths[i].width || img.height != v.depths[i].height)
       img = ResizeLinear(img, v.depths[i].width, v.depths[i].height);
     imgs.push_back(std::move(img));
+    F.width.push_back(v.depths[i].width);
+    F.height.push_back(v.depths[i].height);
   }
-  std::vector<FusedPoint> pts((size_t)count);
+  F.pts.resize((size_t)count);
+  F.view.resize((size_t)count);
+  F.pixel.resize((size_t)count);
   for (int64_t k = 0; k < count; ++k) {
-    FusedPoint& fp = pts[k];
+    FusedPoint& fp = F.pts[k];
     std::memcpy(fp.coord, &xyzn[6 * k], 12);
     std::memcpy(fp.normal, &xyzn[6 * k + 3], 12);
     const MapU8& img = imgs[pairs[2 * k]];
     const size_t p = (size_t)pairs[2 * k + 1];
     for (int c = 0; c < 3; ++c)  // B, G, R (a grey image repeats its value)
       fp.color[c] = img.channels == 1 ? img.data[p] : img.data[3 * p + (2 - c)];
+    F.view[k] = pairs[2 * k];
+    F.pixel[k] = pairs[2 * k + 1];
   }
+  return F;
+}
+
+void ExportBoth(const std::string& dense_folder, const char* name, const std::vector<FusedPoint>& pts) {
   const std::string out = dense_folder + "/HPM_MVS_plusplus";
   MakeDir(out);
-  if (!ExportPointCloud(out + "/HPM_MVS_plusplus.ply", pts) || !ExportFusedPointCloud(out + "/fused.ply", pts))
+  if (!ExportPointCloud(out + "/" + name, pts) || !ExportFusedPointCloud(out + "/fused.ply", pts))
     throw Error("cannot write the point clouds under " + out);
-  return pts;
+}
+}  // namespace
+
+std::vector<FusedPoint> RunFusion(const std::string& dense_folder, const std::vector<Problem>& problems, bool geom) {
+  StageClock clk;
+  struct Lap {
+    StageClock& c;
+    ~Lap() { c.lap(kFusion); }
+  } lap{clk};
+  Fused F = FuseViews(dense_folder, problems, geom, HPM_CHECK_FUSION);
+  ExportBoth(dense_folder, "HPM_MVS_plusplus.ply", F.pts);
+  return std::move(F.pts);
+}
+
+std::vector<FusedPoint> RunFusionSkyStrict(const std::string& dense_folder, const std::vector<Problem>& problems,
+                                           bool geom) {
+  StageClock clk;
+  struct Lap {
+    StageClock& c;
+    ~Lap() { c.lap(kFusion); }
+  } lap{clk};
+  Fused F = FuseViews(dense_folder, problems, geom, HPM_CHECK_FUSION_MASK);
+  // masks/%08d resized to the depth map (RescaleMask, HPM.cpp:209-219), B G R
+  const int n = (int)problems.size();
+  std::vector<MapU8> masks;
+  for (int i = 0; i < n; ++i) {
+    MapU8 m;
+    const std::string base = dense_folder + "/masks/" + [&] {
+      std::ostringstream os;
+      os << std::setw(8) << std::setfill('0') << problems[i].ref_image_id;
+      return os.str();
+    }();
+    if (!ReadNetpbm(base + ".ppm", m) && !ReadNetpbm(base + ".pgm", m))
+      throw Error("cannot read mask " + base + ".ppm (binary PPM/PGM)");
+    if (m.width != F.width[i] || m.height != F.height[i]) m = ResizeLinear(m, F.width[i], F.height[i]);
+    masks.push_back(std::move(m));
+  }
+  std::vector<FusedPoint> kept;
+  kept.reserve(F.pts.size());
+  for (size_t k = 0; k < F.pts.size(); ++k) {
+    const MapU8& m = masks[F.view[k]];
+    const size_t p = (size_t)F.pixel[k];
+    const int b = m.channels == 1 ? m.data[p] : m.data[3 * p + 2];
+    const int g = m.channels == 1 ? m.data[p] : m.data[3 * p + 1];
+    const int r = m.channels == 1 ? m.data[p] : m.data[3 * p + 0];
+    // main.cpp:805: any channel at its sky value rejects the point
+    if (b != 234 && g != 235 && r != 55) kept.push_back(F.pts[k]);
+  }
+  ExportBoth(dense_folder, "HPM_MVS_plusplus_mask.ply", kept);
+  return kept;
 }
 
 std::vector<FusedPoint> RunPipeline(const std::string& dense_folder, int precision, uint64_t seed) {
@@ -1352,7 +1418,7 @@ std::vector<FusedPoint> RunPipeline(const std::string& dense_folder, int precisi
 }
 
 std::vector<FusedPoint> RunPyramidPipeline(const std::string& dense_folder, int initial_image_scale, int size_bound,
-                                           int precision, uint64_t seed) {
+                                           int precision, uint64_t seed, bool mask_fusion) {
   if (initial_image_scale < 1) throw Error("initial image scale must be >= 1");
   if (size_bound < 1) throw Error("size bound must be >= 1");
   const std::vector<Problem> problems = ReadPairFile(dense_folder + "/pair.txt");
@@ -1432,7 +1498,7 @@ std::vector<FusedPoint> RunPyramidPipeline(const std::string& dense_folder, int 
       each_view(f, nullptr);
     }
   }
-  return RunFusion(dense_folder, problems, true);
+  return mask_fusion ? RunFusionSkyStrict(dense_folder, problems, true) : RunFusion(dense_folder, problems, true);
 }
 
 }  // namespace hpmhost

@@ -248,6 +248,12 @@ void ConfidenceEvaluation(const std::string& dense_folder, const std::vector<Pro
 // RunFusion (main.cpp:823-957) -> HPM_MVS_plusplus.ply and fused.ply; returns
 // the fused points in the reference's order.
 std::vector<FusedPoint> RunFusion(const std::string& dense_folder, const std::vector<Problem>& problems, bool geom);
+// RunFusion_Sky_Strict (main.cpp:659-821, --mask): the same consistency pass,
+// then a fused point is dropped when its pixel in masks/%08d.{ppm,pgm}
+// (resized to the depth map) has B == 234 or G == 235 or R == 55; the sources
+// are masked either way.  Writes HPM_MVS_plusplus_mask.ply and fused.ply.
+std::vector<FusedPoint> RunFusionSkyStrict(const std::string& dense_folder, const std::vector<Problem>& problems,
+                                           bool geom);
 // The single-level schedule of main.cpp:1249-1291: photometric pass, confidence,
 // prior pass, three geometric passes (the last two with mandatory
 // consistency after a confidence evaluation), fusion.
@@ -258,7 +264,7 @@ std::vector<FusedPoint> RunPipeline(const std::string& dense_folder, int precisi
 // coarsest level as RunPipeline, finer levels JBU -> hierarchy pass -> prior
 // passes at distances max..0 -> three geometric passes; then fusion.
 std::vector<FusedPoint> RunPyramidPipeline(const std::string& dense_folder, int initial_image_scale, int size_bound,
-                                           int precision, uint64_t seed);
+                                           int precision, uint64_t seed, bool mask_fusion = false);
 
 std::string ResultFolder(const std::string& dense_folder, int ref_image_id);
 

@@ -66,10 +66,20 @@ int hpm_host_run_pipeline(const char* dense_folder, int precision, uint64_t seed
 }
 
 int hpm_host_run_pyramid_pipeline(const char* dense_folder, int initial_image_scale, int size_bound, int precision,
-                                  uint64_t seed, int64_t* num_points) {
+                                  uint64_t seed, int mask_fusion, int64_t* num_points) {
   if (!dense_folder || !num_points || initial_image_scale < 1 || size_bound < 1) return HPM_E_INVALID;
   return guarded([&] {
-    *num_points = (int64_t)hpmhost::RunPyramidPipeline(dense_folder, initial_image_scale, size_bound, precision, seed)
+    *num_points = (int64_t)hpmhost::RunPyramidPipeline(dense_folder, initial_image_scale, size_bound, precision, seed,
+                                                       mask_fusion != 0)
+                      .size();
+  });
+}
+
+int hpm_host_fusion_mask(const char* dense_folder, int geom, int64_t* num_points) {
+  if (!dense_folder || !num_points) return HPM_E_INVALID;
+  return guarded([&] {
+    *num_points = (int64_t)hpmhost::RunFusionSkyStrict(
+                      dense_folder, hpmhost::ReadPairFile(std::string(dense_folder) + "/pair.txt"), geom != 0)
                       .size();
   });
 }

@@ -60,6 +60,7 @@ EXPORTS = [
     "hpm_geom_cost", "hpm_jbu_depth", "hpm_jbu_prior", "hpm_consistency_check", "hpm_set_graph",
     "hpm_run_patchmatch_batch", "hpm_pool_create", "hpm_pool_destroy", "hpm_set_views_pooled",
     "hpm_cost_grad", "hpm_cost_grad_device", "hpm_refine_gn", "hpm_refine_random", "hpm_refine_hybrid",
+    "hpm_refine_gn_smoothed",
     "hpm_batch_create", "hpm_batch_destroy", "hpm_batch_last_error", "hpm_batch_run", "hpm_batch_set_graph",
     "hpm_batch_set_shape", "hpm_batch_get_shape", "hpm_batch_last_ms", "hpm_batch_autotune",
 ]
@@ -102,6 +103,7 @@ def lib():
     L.hpm_refine_gn.argtypes = [P, fp, fp, ctypes.c_int, fp]
     L.hpm_refine_random.argtypes = [P, fp, fp, ctypes.c_int, ctypes.c_uint64, fp]
     L.hpm_refine_hybrid.argtypes = [P, fp, fp, ctypes.c_int, ctypes.c_uint64, fp]
+    L.hpm_refine_gn_smoothed.argtypes = [P, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_float, fp]
     L.hpm_pool_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.c_int, ctypes.POINTER(hpm_camera),
                                   ctypes.POINTER(fp)]
     L.hpm_pool_destroy.argtypes = [P]
@@ -294,8 +296,8 @@ class Context:
         self._check(self._L.hpm_cost_grad_device(self._h, params_ptr, weights_ptr, cost_ptr, grad_ptr, gn_ptr),
                     "hpm_cost_grad_device")
 
-    def refine(self, params, weights, iterations, method="gn", seed=1):
-        """hpm_refine_gn / _random / _hybrid: returns (params (h, w, 4), cost (h, w))."""
+    def refine(self, params, weights, iterations, method="gn", seed=1, smoothed_iterations=None, sigma=2.0):
+        """hpm_refine_gn / _random / _hybrid / _gn_smoothed: returns (params (h, w, 4), cost (h, w))."""
         h, w = self.height, self.width
         pp = np.zeros((h, w, 4), np.float32)
         pp[..., :3] = np.asarray(params, np.float32)[..., :3]
@@ -309,6 +311,9 @@ class Context:
         elif method == "hybrid":
             rc = self._L.hpm_refine_hybrid(self._h, _fptr(pp), _fptr(wt), iterations, ctypes.c_uint64(seed),
                                            _fptr(cost))
+        elif method == "gn_smoothed":  # the first half of the steps (by default) on the smoothed views
+            k = (iterations + 1) // 2 if smoothed_iterations is None else smoothed_iterations
+            rc = self._L.hpm_refine_gn_smoothed(self._h, _fptr(pp), _fptr(wt), iterations, k, sigma, _fptr(cost))
         else:
             raise ValueError(f"unknown refinement method {method!r}")
         self._check(rc, "hpm_refine_" + method)
@@ -400,6 +405,7 @@ def jbu_prior(fine_gray, coarse_depth, coarse_normals):
 
 CHECK_FUSION = 0
 CHECK_CONFIDENCE = 1
+CHECK_FUSION_MASK = 2
 
 
 def _pack_views(cameras, depths, normals, sources):

@@ -32,7 +32,8 @@ def host_lib():
     L.hpm_host_fusion.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
     L.hpm_host_run_pipeline.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64)]
     L.hpm_host_run_pyramid_pipeline.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                                ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64)]
+                                                ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+    L.hpm_host_fusion_mask.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
     L.hpm_host_stage_times.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64), ctypes.c_int]
     L.hpm_host_reset_stage_times.argtypes = []
     L.hpm_host_reset_stage_times.restype = None
@@ -86,13 +87,22 @@ def run_pipeline(folder, precision="fp32", seed=1):
     return n.value
 
 
-def run_pyramid_pipeline(folder, initial_image_scale=4, size_bound=1000, precision="fp32", seed=1):
-    """main.cpp's pyramid schedule and fusion over a dense folder; returns the
-    number of fused points (the PLY files are written under HPM_MVS_plusplus/)."""
+def run_pyramid_pipeline(folder, initial_image_scale=4, size_bound=1000, precision="fp32", seed=1, mask=False):
+    """main.cpp's pyramid schedule and fusion (strict sky-mask fusion with
+    mask=True) over a dense folder; returns the number of fused points (the
+    PLY files are written under HPM_MVS_plusplus/)."""
     n = ctypes.c_int64(0)
     _check(host_lib().hpm_host_run_pyramid_pipeline(folder.encode(), initial_image_scale, size_bound,
-                                                    PRECISIONS[precision], seed, ctypes.byref(n)),
+                                                    PRECISIONS[precision], seed, int(mask), ctypes.byref(n)),
            "hpm_host_run_pyramid_pipeline")
+    return n.value
+
+
+def fusion_mask(folder, geom=True):
+    """RunFusion_Sky_Strict over the folder's maps and masks/%08d.ppm; returns
+    the number of points kept."""
+    n = ctypes.c_int64(0)
+    _check(host_lib().hpm_host_fusion_mask(folder.encode(), int(geom), ctypes.byref(n)), "hpm_host_fusion_mask")
     return n.value
 
 

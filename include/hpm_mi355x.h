@@ -275,6 +275,14 @@ int hpm_refine_random(hpm_ctx* ctx, float* params, const float* weights, int ite
  * pixels of disparity where the GN model holds; GN then converges in it). */
 int hpm_refine_hybrid(hpm_ctx* ctx, float* params, const float* weights, int iterations, uint64_t seed,
                       float* out_cost);
+/* Coarse-to-fine Gauss-Newton: the first smoothed_iterations steps minimise
+ * the cost of the views smoothed by a Gaussian of `sigma` px (0 < sigma <= 5,
+ * radius ceil(3 sigma)), whose basin of convergence is about as wide as the
+ * blur, the remaining steps the views' own cost (the sharp cost's basin is
+ * about a texel of the finest texture: at full resolution a 2 % depth error
+ * lies outside it).  out_cost receives the sharp views' cost. */
+int hpm_refine_gn_smoothed(hpm_ctx* ctx, float* params, const float* weights, int iterations,
+                           int smoothed_iterations, float sigma, float* out_cost);
 
 /* Multi-view consistency check of RunFusion (mode HPM_CHECK_FUSION,
  * main.cpp:823-957) and ConfidenceEvaluation (mode HPM_CHECK_CONFIDENCE,
@@ -297,6 +305,10 @@ int hpm_refine_hybrid(hpm_ctx* ctx, float* params, const float* weights, int ite
  * Runs on the current HIP device; blocking. */
 #define HPM_CHECK_FUSION 0
 #define HPM_CHECK_CONFIDENCE 1
+/* RunFusion_Sky_Strict (main.cpp:659-821, --mask): as HPM_CHECK_FUSION with
+ * that function's float acceptance threshold (`float factor = 0.3`); the
+ * mask colour test on the fused points is the caller's. */
+#define HPM_CHECK_FUSION_MASK 2
 int hpm_consistency_check(int mode, int num_images, const hpm_camera* cameras, const float* const* depths,
                           const float* const* normals, const int32_t* src_offsets, const int32_t* src_ids,
                           float* const* out_confidence, int32_t* out_pairs, float* out_points, int64_t capacity,

@@ -30,6 +30,12 @@ int hpm_host_process_problem(const char* dense_folder, int idx, int geom_consist
 int hpm_host_confidence(const char* dense_folder, int geom_consistency);
 /* RunFusion (main.cpp:823-957): writes the two PLY files; *num_points = fused points. */
 int hpm_host_fusion(const char* dense_folder, int geom_consistency, int64_t* num_points);
+/* RunFusion_Sky_Strict (main.cpp:659-821, the --mask option): reads
+ * masks/%08d.ppm (or .pgm) per view, drops fused points whose mask pixel has
+ * B == 234 or G == 235 or R == 55; writes HPM_MVS_plusplus_mask.ply and
+ * fused.ply.  Also the reference's --fusion-only with --mask; hpm_host_fusion
+ * is --fusion-only without it (main.cpp:1196-1206). */
+int hpm_host_fusion_mask(const char* dense_folder, int geom_consistency, int64_t* num_points);
 /* The single-level schedule of main.cpp:1249-1291 followed by fusion. */
 int hpm_host_run_pipeline(const char* dense_folder, int precision, uint64_t seed, int64_t* num_points);
 /* The pyramid schedule of main.cpp:1183-1352 followed by fusion
@@ -39,9 +45,10 @@ int hpm_host_run_pipeline(const char* dense_folder, int precision, uint64_t seed
  * is fixed at 1000).  Coarsest level: photometric, prior and geometric
  * passes; each finer level: JBU of the coarser depths_geom, a hierarchy pass,
  * prior passes from 2 and 1 levels coarser down to the level itself, and three
- * geometric passes. */
+ * geometric passes.  mask_fusion != 0 ends with hpm_host_fusion_mask's
+ * fusion (--mask, main.cpp:1353-1356) instead of RunFusion. */
 int hpm_host_run_pyramid_pipeline(const char* dense_folder, int initial_image_scale, int size_bound, int precision,
-                                  uint64_t seed, int64_t* num_points);
+                                  uint64_t seed, int mask_fusion, int64_t* num_points);
 
 /* Wall time of the pipeline stages, summed process-wide since the last reset
  * (no counterpart in the reference, which prints progress lines).  Stage order:

@@ -213,6 +213,7 @@ float orc_acosf(float x) { return orc::acosf_hpm(x); }
 //   position, reference normal) may be null.
 // mode 1: out_conf[i] (w*h floats, zero-filled by the caller) receives the
 //   consistency map of image i.
+// mode 2: as mode 0 with RunFusion_Sky_Strict's float threshold (main.cpp:659-821).
 int orc_consistency(int mode, int n, const Cam* cams, const float* const* depths, const float* const* normals,
                     const int* src_offsets, const int* src_ids, float* const* out_conf, int32_t* out_pairs,
                     float* out_points, int64_t capacity, int64_t* count) {
@@ -267,8 +268,12 @@ int orc_consistency(int mode, int n, const Cam* cams, const float* const* depths
             }
           }
         }
-        if (num_consistent >= 1 && ((double)dynamic_consistency > 0.3 * num_consistent)) {
-          if (mode == 0) {
+        // RunFusion compares with the double 0.3 * n; RunFusion_Sky_Strict
+        // (main.cpp:793-797) with `float factor = 0.3`, i.e. in float
+        const bool pass = mode == 2 ? dynamic_consistency > 0.3f * (float)num_consistent
+                                    : (double)dynamic_consistency > 0.3 * num_consistent;
+        if (num_consistent >= 1 && pass) {
+          if (mode != 1) {
             if (npts < capacity) {
               out_pairs[2 * npts] = i;
               out_pairs[2 * npts + 1] = (int32_t)center;

@@ -234,8 +234,9 @@ def jbu_prior(fine, coarse, coarse_normals):
 
 
 def consistency(mode, cams, depths, normals, sources):
-    """RunFusion (mode 0) -> (pairs[k, 2], points[k, 6]); ConfidenceEvaluation
-    (mode 1) -> list of confidence maps."""
+    """RunFusion (mode 0) and RunFusion_Sky_Strict's pass (mode 2) ->
+    (pairs[k, 2], points[k, 6]); ConfidenceEvaluation (mode 1) -> list of
+    confidence maps."""
     n = len(cams)
     ca = _cams_array(cams)
     d = [np.ascontiguousarray(x, np.float32) for x in depths]
@@ -254,7 +255,8 @@ def consistency(mode, cams, depths, normals, sources):
     cap = sum(x.size for x in d)
     pairs = np.zeros((cap, 2), np.int32)
     pts = np.zeros((cap, 6), np.float32)
-    lib().orc_consistency(0, n, ca, dptr, nptr, _p(off), _p(ids), None, _p(pairs), _p(pts), cap, ctypes.byref(count))
+    lib().orc_consistency(mode, n, ca, dptr, nptr, _p(off), _p(ids), None, _p(pairs), _p(pts), cap,
+                          ctypes.byref(count))
     return pairs[:count.value], pts[:count.value]
 
 

@@ -77,6 +77,20 @@ def test_fusion_bitexact():
     np.testing.assert_array_equal(gx, rx)
 
 
+def test_fusion_mask_mode_bitexact():
+    """RunFusion_Sky_Strict's consistency pass (float acceptance threshold)."""
+    import hpm_mi355x as H
+    print("case: consistency fusion-mask 5 x 96x72", flush=True)
+    sc, depths, normals, sources = _noisy_maps(17)
+    hc = [H.camera_from_scene(c) for c in sc.cameras]
+    oc = [O.camera_from_scene(c) for c in sc.cameras]
+    gp, gx = H.consistency_check(H.CHECK_FUSION_MASK, hc, depths, normals, sources)
+    rp, rx = O.consistency(2, oc, depths, normals, sources)
+    assert len(rp) > 1000
+    np.testing.assert_array_equal(gp, rp)
+    np.testing.assert_array_equal(gx, rx)
+
+
 def test_confidence_bitexact():
     import hpm_mi355x as H
     print("case: consistency confidence 5 x 96x72", flush=True)

@@ -86,6 +86,54 @@ def test_pipeline_fusion_equals_oracle(dense_folder):
     np.testing.assert_array_equal(gx, rx)
 
 
+def test_sky_mask_fusion_equals_oracle(dense_folder):
+    """--mask fusion (RunFusion_Sky_Strict) over the pipeline's final maps:
+    masks/%08d.ppm paint a block of view 0 and a block of view 3 in the sky
+    colour (R 55, G 235, B 234) and single channels of it elsewhere; the kept
+    points are the oracle's strict-threshold pass filtered by the mask test,
+    in order, and the PLY holds exactly them."""
+    from hpm_mi355x import pipeline as P
+    import hpm_mi355x as H
+    sc, folder = dense_folder
+    out = os.path.join(folder, "HPM_MVS_plusplus")
+    if not os.path.exists(os.path.join(out, "2333_00000004", "depths_geom.dmb")):  # run alone
+        P.run_pipeline(folder, precision="fp32", seed=3)
+    os.makedirs(os.path.join(folder, "masks"), exist_ok=True)
+    masks = []
+    for i in range(5):
+        m = np.full((120, 160, 3), 128, np.uint8)  # R G B
+        if i == 0:
+            m[20:60, 30:90] = (55, 235, 234)
+        if i == 3:
+            m[60:110, 100:150, 0] = 55     # R alone at its sky value also rejects
+            m[0:30, 0:40, 1] = 235         # and G alone
+        masks.append(m)
+        with open(os.path.join(folder, "masks", f"{i:08d}.ppm"), "wb") as fh:
+            fh.write(b"P6\n160 120\n255\n" + m.tobytes())
+    n = P.fusion_mask(folder, geom=True)
+    depths = [np.ascontiguousarray(P.read_dmb(os.path.join(out, f"2333_{i:08d}", "depths_geom.dmb"))) for i in range(5)]
+    normals = [np.ascontiguousarray(P.read_dmb(os.path.join(out, f"2333_{i:08d}", "normals.dmb"))) for i in range(5)]
+    sources = [[j for j, s in sc.pairs[i] if s > 0] for i in range(5)]
+    cams = []
+    for i, c in enumerate(sc.cameras):
+        tok = open(os.path.join(folder, "cams", f"{i:08d}_cam.txt")).read().split()
+        ext = np.array(tok[1:17], np.float32).reshape(4, 4)
+        K = np.array(tok[18:27], np.float32).reshape(3, 3)
+        cams.append(O.camera(K, ext[:3, :3], ext[:3, 3], c.width, c.height, np.float32(tok[27]), np.float32(tok[30])))
+    rp, _ = O.consistency(2, cams, depths, normals, sources)
+    keep = []
+    for v, p in rp:
+        r, g, b = masks[v].reshape(-1, 3)[p]
+        keep.append(b != 234 and g != 235 and r != 55)
+    keep = np.array(keep)
+    print(f"strict pass {len(rp)} points, kept {keep.sum()}")
+    assert 0 < keep.sum() < len(rp)
+    assert n == keep.sum()
+    with open(os.path.join(out, "HPM_MVS_plusplus_mask.ply"), "rb") as fh:
+        head = fh.read(400).split(b"end_header\n")[0].decode()
+    assert f"element vertex {n}" in head
+
+
 def test_pyramid_pipeline_three_levels(tmp_path):
     from hpm_mi355x import pipeline as P
     from hpm_mi355x import scenes
