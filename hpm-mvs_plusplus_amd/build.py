@@ -18,7 +18,7 @@ SOURCES = [os.path.join(HERE, "csrc", f) for f in ("hpm_capi.hip", "hpm_batch.hi
 HEADERS = ["hpm_math.h", "hpm_device.h", "hpm_kernels.h", "hpm_jbu.h", "hpm_fast.h", "hpm_fusion.h", "hpm_grad.h",
            "hpm_internal.h", "hpm_multi.h", "hpm_batch_impl.h"]
 HOST_LIB = os.path.join(HERE, "lib", "libhpm_host.so")
-HOST_SOURCES = [os.path.join(HERE, "host", "hpm_host.cpp"), os.path.join(HERE, "host", "hpm_host_capi.cpp")]
+HOST_SOURCES = [os.path.join(HERE, "host", f) for f in ("hpm_host.cpp", "hpm_jpeg.cpp", "hpm_host_capi.cpp")]
 GXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-ffp-contract=off", "-fno-fast-math", "-Wall",
              "-Wextra"]
 

@@ -129,9 +129,9 @@ MapF ToGrayFloat(const MapU8& img) {
   for (size_t i = 0, n = (size_t)img.width * img.height; i < n; ++i) {
     if (img.channels == 1) {
       g.data[i] = (float)img.data[i];
-    } else {  // P6 holds R G B
-      const double y = 0.299 * img.data[3 * i] + 0.587 * img.data[3 * i + 1] + 0.114 * img.data[3 * i + 2];
-      g.data[i] = (float)std::floor(y + 0.5);
+    } else {  // R G B; BT.601 weights in Q14 with rounding, as cv::cvtColor(COLOR_BGR2GRAY) on 8-bit images
+      const int y = (4899 * img.data[3 * i] + 9617 * img.data[3 * i + 1] + 1868 * img.data[3 * i + 2] + (1 << 13)) >> 14;
+      g.data[i] = (float)y;
     }
   }
   return g;
@@ -544,14 +544,18 @@ static void ImageSize(const std::string& dense_folder, int id, int& w, int& h) {
     std::fclose(fp);
     if (ok && (w = std::atoi(sw.c_str())) > 0 && (h = std::atoi(sh.c_str())) > 0) return;
   }
-  throw Error("cannot read the size of image " + ImagePath(dense_folder, id, ".pgm"));
+  if (ReadJpegSize(ImagePath(dense_folder, id, ".jpg"), w, h)) return;
+  throw Error("cannot read the size of image " + ImagePath(dense_folder, id, ".pgm|.ppm|.jpg"));
 }
 
 static MapU8 LoadImage(const std::string& dense_folder, int id) {
   MapU8 img;
   if (ReadNetpbm(ImagePath(dense_folder, id, ".pgm"), img) || ReadNetpbm(ImagePath(dense_folder, id, ".ppm"), img))
     return img;
-  throw Error("cannot read image " + ImagePath(dense_folder, id, ".pgm") + " (binary PGM/PPM)");
+  std::string err;
+  const std::string jpg = ImagePath(dense_folder, id, ".jpg");
+  if (ReadJpeg(jpg, img, &err)) return img;
+  throw Error("cannot read image " + ImagePath(dense_folder, id, ".pgm|.ppm|.jpg") + ": " + err);
 }
 
 void SceneImages::ensure(int id) {

@@ -48,8 +48,17 @@ bool WriteDmb(const std::string& path, const MapF& m);
 // Binary PGM (P5) or PPM (P6), 8 bit.  The reference decodes JPEG with
 // OpenCV; this build reads the uncompressed Netpbm formats.
 bool ReadNetpbm(const std::string& path, MapU8& out);
-// Grey float image of an 8-bit image: P5 as is; P6 with the ITU-R BT.601
-// weights 0.299 R + 0.587 G + 0.114 B rounded to the nearest integer.
+// JPEG (hpm_jpeg.cpp), Huffman-coded sequential or progressive: the pixels
+// libjpeg's default decoding gives (accurate integer IDCT, fancy upsampling,
+// BT.601 YCbCr -> RGB), as grey (1 channel) or R G B (3 channels).  False with
+// a message for arithmetic-coded, lossless or 12-bit files and for corrupt data.
+bool DecodeJpeg(const std::vector<uint8_t>& data, MapU8& out, std::string* err = nullptr);
+bool ReadJpeg(const std::string& path, MapU8& out, std::string* err = nullptr);
+bool ReadJpegSize(const std::string& path, int& w, int& h);
+// Grey float image of an 8-bit image: grey as is; R G B with the ITU-R BT.601
+// weights in Q14, (4899 R + 9617 G + 1868 B + 2^13) >> 14 -- the fixed point
+// cv::cvtColor(COLOR_BGR2GRAY) uses for 8-bit images (InuputInitialization,
+// HPM.cpp:660-661).
 MapF ToGrayFloat(const MapU8& img);
 
 // cv::resize(src, dst, size, 0, 0, INTER_LINEAR): each destination pixel

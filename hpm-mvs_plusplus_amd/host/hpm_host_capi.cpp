@@ -99,6 +99,22 @@ void hpm_host_set_verbose(int level) { hpmhost::SetVerbose(level); }
 
 void hpm_host_set_threads(int n) { hpmhost::SetPassThreads(n); }
 
+int hpm_host_decode_jpeg(const uint8_t* data, int64_t size, int* w, int* h, int* channels, uint8_t* out,
+                         int64_t capacity) {
+  if (!data || size <= 0 || !w || !h || !channels) return HPM_E_INVALID;
+  return guarded([&] {
+    hpmhost::MapU8 img;
+    std::string err;
+    if (!hpmhost::DecodeJpeg(std::vector<uint8_t>(data, data + size), img, &err)) throw hpmhost::Error(err);
+    *w = img.width;
+    *h = img.height;
+    *channels = img.channels;
+    if (!out) return;  // size query
+    if (capacity < (int64_t)img.data.size()) throw hpmhost::Error("output buffer smaller than w * h * channels");
+    std::memcpy(out, img.data.data(), img.data.size());
+  });
+}
+
 int hpm_host_support_points(const float* costs, const float* confidences, int w, int h, int32_t* out_xy,
                             int capacity, int* count) {
   if (!costs || !confidences || w <= 0 || h <= 0 || !count || (capacity > 0 && !out_xy)) return HPM_E_INVALID;

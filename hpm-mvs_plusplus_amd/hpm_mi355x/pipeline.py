@@ -41,6 +41,8 @@ def host_lib():
     L.hpm_host_set_verbose.restype = None
     L.hpm_host_set_threads.argtypes = [ctypes.c_int]
     L.hpm_host_set_threads.restype = None
+    L.hpm_host_decode_jpeg.argtypes = [u8p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(ctypes.c_int), u8p, ctypes.c_int64]
     L.hpm_host_support_points.argtypes = [fp, fp, ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_int)]
     L.hpm_host_delaunay.argtypes = [ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int, i32p, ctypes.c_int,
@@ -128,6 +130,21 @@ def set_verbose(level=1):
 def set_threads(n=0):
     """Host threads for the order-free passes (0: default, 1: view order)."""
     host_lib().hpm_host_set_threads(int(n))
+
+
+def decode_jpeg(data: bytes):
+    """The host library's JPEG decoder (Huffman-coded sequential or progressive,
+    8-bit): (h, w) grey or (h, w, 3) R G B uint8, as libjpeg's default decoding
+    gives them."""
+    buf = np.frombuffer(data, np.uint8)
+    w, h, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    L = host_lib()
+    _check(L.hpm_host_decode_jpeg(_ptr(buf, ctypes.c_uint8), buf.size, ctypes.byref(w), ctypes.byref(h),
+                                  ctypes.byref(c), None, 0), "decode_jpeg")
+    out = np.zeros((h.value, w.value, c.value), np.uint8)
+    _check(L.hpm_host_decode_jpeg(_ptr(buf, ctypes.c_uint8), buf.size, ctypes.byref(w), ctypes.byref(h),
+                                  ctypes.byref(c), _ptr(out, ctypes.c_uint8), out.size), "decode_jpeg")
+    return out[..., 0] if c.value == 1 else out
 
 
 def support_points(costs, confidences):

@@ -420,7 +420,8 @@ def random_planes_per_pixel(cam: Camera, seed, depth_lo=1.0, depth_hi=5.0, max_a
 def write_dense_folder(scene: Scene, folder: str, depth_lo=1.0, depth_hi=5.0, n_depth=192):
     """images/%08d.pgm, cams/%08d_cam.txt and pair.txt in the layout the
     reference reads (ReadCamera HPM.cpp:159-184, GenerateSampleList
-    main.cpp:171-200).  Images are binary PGM (no JPEG codec here)."""
+    main.cpp:171-200).  Images are binary PGM; the pipeline also reads
+    PPM and JPEG (images/%08d.ppm, images/%08d.jpg)."""
     os.makedirs(os.path.join(folder, "images"), exist_ok=True)
     os.makedirs(os.path.join(folder, "cams"), exist_ok=True)
     for i, (cam, im) in enumerate(zip(scene.cameras, scene.images)):

@@ -67,6 +67,18 @@ void hpm_host_set_verbose(int level);
  * reference runs in view order); the output files do not depend on it. */
 void hpm_host_set_threads(int n);
 
+/* JPEG decoding as the pipeline reads images/%08d.jpg, which the reference
+ * reads with cv::imread (main.cpp:125, main.cpp:213): Huffman-coded
+ * sequential or progressive, 8-bit, the pixels of libjpeg's default decoding
+ * (accurate integer IDCT, fancy upsampling, BT.601 YCbCr -> RGB).  *w, *h,
+ * *channels (1 grey, 3 R G B) are set; `out` (NULL to query the size)
+ * receives w*h*channels bytes and must hold that many (`capacity`).
+ * HPM_E_INVALID for data it cannot decode (arithmetic coding, lossless,
+ * 12-bit, corrupt) or a short buffer, with the reason in
+ * hpm_host_last_error(). */
+int hpm_host_decode_jpeg(const uint8_t* data, int64_t size, int* w, int* h, int* channels, uint8_t* out,
+                         int64_t capacity);
+
 /* GetSupportPoints_Classify_Check (HPM.cpp:1026-1073) at factor 1: costs and
  * confidences w*h; out_xy receives up to `capacity` (x, y) pairs. */
 int hpm_host_support_points(const float* costs, const float* confidences, int w, int h, int32_t* out_xy,

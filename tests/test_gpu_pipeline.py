@@ -186,3 +186,48 @@ def test_pipeline_files_do_not_depend_on_host_threads(tmp_path):
     for rel in names:
         assert filecmp.cmp(os.path.join(runs[1][0], rel), os.path.join(runs[4][0], rel), shallow=False), rel
 
+
+
+def test_pipeline_reads_jpeg_images(tmp_path):
+    """images/%08d.jpg (what the reference reads, main.cpp:125): a colour JPEG
+    folder and a folder holding the decoded pixels as PPM give byte-identical
+    output files, and the depth maps keep their ground-truth accuracy."""
+    import filecmp
+    import io
+    Image = pytest.importorskip("PIL.Image")
+    from hpm_mi355x import pipeline as P
+    from hpm_mi355x import scenes
+    sc = scenes.make_plane_scene(seed=6, n_cams=5, width=160, height=120, f=131.25, n_src=4)
+    runs = {}
+    for kind in ("jpg", "ppm"):
+        folder = str(tmp_path / kind)
+        scenes.write_dense_folder(sc, folder)
+        for i, im in enumerate(sc.images):
+            g = np.asarray(im, np.int32)
+            rgb = np.stack([np.clip(g + 12, 0, 255), g, np.clip(g - 9, 0, 255)], -1).astype(np.uint8)
+            buf = io.BytesIO()
+            Image.fromarray(rgb).save(buf, format="JPEG", quality=95)
+            os.remove(os.path.join(folder, "images", f"{i:08d}.pgm"))
+            if kind == "jpg":
+                with open(os.path.join(folder, "images", f"{i:08d}.jpg"), "wb") as fh:
+                    fh.write(buf.getvalue())
+            else:
+                dec = np.asarray(Image.open(io.BytesIO(buf.getvalue())).convert("RGB"))
+                with open(os.path.join(folder, "images", f"{i:08d}.ppm"), "wb") as fh:
+                    fh.write(f"P6\n{dec.shape[1]} {dec.shape[0]}\n255\n".encode() + dec.tobytes())
+        print(f"case: pyramid pipeline, 5 views 160x120 from {kind} images", flush=True)
+        runs[kind] = (folder, P.run_pyramid_pipeline(folder, initial_image_scale=1, size_bound=100,
+                                                     precision="fp16", seed=4))
+    assert runs["jpg"][1] == runs["ppm"][1] > 4 * 2000
+    names = []
+    for root, _, files in os.walk(os.path.join(runs["jpg"][0], "HPM_MVS_plusplus")):
+        for f in files:
+            names.append(os.path.relpath(os.path.join(root, f), runs["jpg"][0]))
+    assert len(names) > 5 * 6
+    for rel in names:
+        assert filecmp.cmp(os.path.join(runs["jpg"][0], rel), os.path.join(runs["ppm"][0], rel), shallow=False), rel
+    for i in range(5):
+        d = P.read_dmb(os.path.join(runs["jpg"][0], "HPM_MVS_plusplus", f"2333_{i:08d}", "depths_geom.dmb"))
+        rel = np.abs(d - sc.depths[i]) / sc.depths[i]
+        print(f"view {i}: depth within 1% of GT {np.mean(rel <= 0.01):.3f}")
+        assert np.mean(rel <= 0.01) > 0.75
