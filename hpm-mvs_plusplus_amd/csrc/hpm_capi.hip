@@ -15,6 +15,7 @@
 #include "hpm_internal.h"
 #include "hpm_grad.h"
 #include "hpm_jbu.h"
+#include "hpm_prior.h"
 #include "hpm_kernels.h"
 
 using namespace hpm;
@@ -55,6 +56,10 @@ void free_views(hpm_ctx* c) {
   c->d_planes = nullptr; c->d_costs = nullptr; c->d_pre_costs = nullptr; c->d_conf = nullptr;
   c->d_texture = nullptr; c->d_sel = nullptr; c->d_masks = nullptr; c->d_canny = nullptr;
   c->d_prior = nullptr; c->d_scaled = nullptr;
+  if (c->d_prior_ds) hipFree(c->d_prior_ds);
+  if (c->d_prior_tmp) hipFree(c->d_prior_tmp);
+  c->d_prior_ds = nullptr; c->d_prior_tmp = nullptr; c->prior_tmp_bytes = 0;
+  c->prior_dist = -1; c->prior_w = c->prior_h = 0;
   c->nimg = 0; c->W = c->H = 0;
 }
 
@@ -551,6 +556,186 @@ int hpm_set_prior(hpm_ctx* ctx, const float* prior_planes, const uint32_t* masks
   if ((rc = dalloc(ctx, &ctx->d_prior, npix(ctx))) || (rc = dalloc(ctx, &ctx->d_masks, npix(ctx)))) return rc;
   if ((rc = upload(ctx, ctx->d_prior, prior_planes, npix(ctx) * sizeof(float4)))) return rc;
   return upload(ctx, ctx->d_masks, masks, npix(ctx) * sizeof(uint32_t));
+}
+
+}  // extern "C"
+
+namespace {
+
+DevCam dev_cam(const hpm_camera& c) {
+  static_assert(sizeof(DevCam) == sizeof(hpm_camera), "camera layouts");
+  DevCam d;
+  std::memcpy(&d, &c, sizeof(d));
+  return d;
+}
+
+unsigned grid1(size_t n) { return (unsigned)((n + 255) / 256); }
+
+// Grow-only device scratch of the prior construction, carved into 256-byte
+// aligned pieces.
+int prior_scratch(hpm_ctx* ctx, const std::vector<size_t>& sizes, std::vector<char*>& out) {
+  std::vector<size_t> off;
+  size_t total = 0;
+  for (size_t b : sizes) {
+    off.push_back(total);
+    total += (b + 255) & ~(size_t)255;
+  }
+  if (total > ctx->prior_tmp_bytes) {
+    if (ctx->d_prior_tmp) HIP_OK(hipFree(ctx->d_prior_tmp));
+    ctx->d_prior_tmp = nullptr;
+    ctx->prior_tmp_bytes = 0;
+    if (hipMalloc(&ctx->d_prior_tmp, total) != hipSuccess) return fail(ctx, HPM_E_NOMEM, "hipMalloc (prior scratch)");
+    ctx->prior_tmp_bytes = total;
+  }
+  out.clear();
+  for (size_t o : off) out.push_back((char*)ctx->d_prior_tmp + o);
+  return HPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hpm_prior_support_points(hpm_ctx* ctx, int distance, int32_t* out_xy, int capacity, int* count) {
+  if (!ctx || !count || distance < 0 || distance > 2 || capacity < 0 || (capacity > 0 && !out_xy)) return HPM_E_INVALID;
+  *count = 0;
+  if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
+  HIP_OK(hipSetDevice(ctx->device));
+  const int W = ctx->W, H = ctx->H;
+  FloatMap depth{(const float*)ctx->d_planes + 3, W, H, 4}, cost{ctx->d_costs, W, H, 1}, conf{ctx->d_conf, W, H, 1};
+  int scan_w = W, scan_h = H, mw = W, mh = H;
+  if (distance > 0) {
+    // the maps resized by 1 / (2 distance) (main.cpp:448-454); the support
+    // points scan (int)(width * factor) columns (HPM.cpp:1030-1031)
+    const float factor = (float)(1.0 / (distance * 2));
+    const int hw = (int)std::round(W * factor), hh = (int)std::round(H * factor);
+    if (hw < 1 || hh < 1) return fail(ctx, HPM_E_INVALID, "reference image too small for a prior at this distance");
+    const size_t n = (size_t)hw * hh;
+    int rc = dalloc(ctx, &ctx->d_prior_ds, 3 * n);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_prior_downsample, dim3(grid1(n)), dim3(256), 0, ctx->stream, depth, cost, conf, hw, hh,
+                       (double)W / hw, (double)H / hh, ctx->d_prior_ds);
+    HIP_OK(hipGetLastError());
+    depth = FloatMap{ctx->d_prior_ds, hw, hh, 1};
+    cost = FloatMap{ctx->d_prior_ds + n, hw, hh, 1};
+    conf = FloatMap{ctx->d_prior_ds + 2 * n, hw, hh, 1};
+    scan_w = std::min((int)(W * factor), hw);
+    scan_h = std::min((int)(H * factor), hh);
+    mw = hw;
+    mh = hh;
+  }
+  const size_t nb = (size_t)((scan_w + 4) / 5) * ((scan_h + 4) / 5);
+  std::vector<char*> buf;
+  int rc = prior_scratch(ctx, {nb * sizeof(int4)}, buf);
+  if (rc) return rc;
+  int4* d_sp = (int4*)buf[0];
+  hipLaunchKernelGGL(k_support_points, dim3(grid1(nb)), dim3(256), 0, ctx->stream, cost, conf, scan_w, scan_h, d_sp);
+  HIP_OK(hipGetLastError());
+  std::vector<int4> h(nb);
+  HIP_OK(hipMemcpyAsync(h.data(), d_sp, nb * sizeof(int4), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_OK(hipStreamSynchronize(ctx->stream));
+  int n = 0;
+  auto push = [&](int x, int y) {
+    if (n < capacity) { out_xy[2 * n] = x; out_xy[2 * n + 1] = y; }
+    ++n;
+  };
+  for (const int4& o : h) {  // the textured candidate first (HPM.cpp:1067-1070)
+    if (o.x >= 0) push(o.x, o.y);
+    if (o.z >= 0) push(o.z, o.w);
+  }
+  *count = n;
+  ctx->prior_dist = distance;
+  ctx->prior_w = mw;
+  ctx->prior_h = mh;
+  return HPM_OK;
+}
+
+int hpm_prior_build(hpm_ctx* ctx, const int32_t* tris, int n) {
+  if (!ctx || n < 0 || (n > 0 && !tris)) return HPM_E_INVALID;
+  if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
+  if (ctx->prior_dist < 0) return fail(ctx, HPM_E_STATE, "hpm_prior_support_points first");
+  // the raster stores triangle index + 1 in a float map: exact below 2^24
+  if (n >= (1 << 24) - 1) return fail(ctx, HPM_E_INVALID, "too many triangles");
+  const int W = ctx->W, H = ctx->H, mw = ctx->prior_w, mh = ctx->prior_h, distance = ctx->prior_dist;
+  std::vector<PriorTri> ht((size_t)n);
+  for (int t = 0; t < n; ++t) {
+    const int32_t* v = tris + 6 * (size_t)t;
+    for (int k = 0; k < 3; ++k)
+      if (v[2 * k] < 0 || v[2 * k] >= mw || v[2 * k + 1] < 0 || v[2 * k + 1] >= mh)
+        return fail(ctx, HPM_E_INVALID, "triangle corner outside the prior grid");
+    // step = 1 / longest edge, as the raster loop computes it (main.cpp:393-398)
+    auto len = [](int ax, int ay, int bx, int by) {
+      const double dx = ax - bx, dy = ay - by;
+      return (float)std::sqrt(dx * dx + dy * dy);
+    };
+    const float max_edge = std::max(len(v[0], v[1], v[2], v[3]), std::max(len(v[0], v[1], v[4], v[5]), len(v[2], v[3], v[4], v[5])));
+    ht[t] = PriorTri{v[0], v[1], v[2], v[3], v[4], v[5], (float)(1.0 / max_edge), 0};
+  }
+  HIP_OK(hipSetDevice(ctx->device));
+  const size_t P = npix(ctx), Pc = (size_t)mw * mh;
+  int rc;
+  if ((!ctx->d_prior && (rc = dalloc(ctx, &ctx->d_prior, P))) || (!ctx->d_masks && (rc = dalloc(ctx, &ctx->d_masks, P))))
+    return rc;
+  std::vector<char*> buf;
+  const size_t nt = std::max(n, 1);
+  if (distance == 0) rc = prior_scratch(ctx, {nt * sizeof(PriorTri), nt * sizeof(float4)}, buf);
+  else
+    rc = prior_scratch(ctx, {nt * sizeof(PriorTri), nt * sizeof(float4), Pc * 4, Pc * 4, Pc * sizeof(float4), P * 4}, buf);
+  if (rc) return rc;
+  PriorTri* d_tris = (PriorTri*)buf[0];
+  float4* d_tplanes = (float4*)buf[1];
+  if (n > 0) HIP_OK(hipMemcpyAsync(d_tris, ht.data(), (size_t)n * sizeof(PriorTri), hipMemcpyHostToDevice, ctx->stream));
+  const DevCam cam = dev_cam(ctx->cams[0]);
+  const float dmin = ctx->cams[0].depth_min * 0.6f, dmax = ctx->cams[0].depth_max * 1.2f;  // HPM.cpp:685-686
+  const float factor = distance > 0 ? (float)(1.0 / (distance * 2)) : 1.0f;
+  const FloatMap depth = distance == 0 ? FloatMap{(const float*)ctx->d_planes + 3, W, H, 4}
+                                       : FloatMap{ctx->d_prior_ds, mw, mh, 1};
+  unsigned int* d_raster = distance == 0 ? ctx->d_masks : (unsigned int*)buf[2];
+  HIP_OK(hipMemsetAsync(d_raster, 0, Pc * sizeof(unsigned int), ctx->stream));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_tri_planes, dim3(grid1(n)), dim3(256), 0, ctx->stream, d_tris, n, depth, cam, distance > 0 ? 1 : 0,
+                       factor, d_tplanes);
+    hipLaunchKernelGGL(k_raster, dim3(grid1(n)), dim3(256), 0, ctx->stream, d_tris, n, mw, mh, d_raster);
+    HIP_OK(hipGetLastError());
+  }
+  if (distance == 0) {
+    hipLaunchKernelGGL(k_prior_level, dim3(grid1(P)), dim3(256), 0, ctx->stream, d_tplanes, W, H, cam, dmin, dmax,
+                       ctx->d_masks, ctx->d_prior);
+  } else {
+    float* d_cd = (float*)buf[3];
+    float4* d_cn = (float4*)buf[4];
+    float* d_up = (float*)buf[5];
+    hipLaunchKernelGGL(k_prior_coarse, dim3(grid1(Pc)), dim3(256), 0, ctx->stream, d_tplanes, d_raster, mw, mh, cam,
+                       factor, dmin, dmax, d_cd, d_cn);
+    // JointBilateralUpsampling_prior (HPM.cpp:1323-1392) to the reference
+    // size; with an integer scale of 1 nothing is upsampled and the upsampled
+    // maps stay zero (main.cpp:548-576 then keeps no prior)
+    JbuDims jd;
+    jd.height = H; jd.width = W; jd.s_height = mh; jd.s_width = mw;
+    jd.imagescale = std::max(H / mh, W / mw);
+    if (jd.imagescale != 1) {
+      hipLaunchKernelGGL(k_jbu_prior, dim3((W + 15) / 16, (H + 15) / 16), dim3(16, 16), 0, ctx->stream, jd,
+                         (const float*)ctx->d_img[0], d_cd, d_cn, d_up, ctx->d_prior);
+    } else {
+      HIP_OK(hipMemsetAsync(d_up, 0, P * 4, ctx->stream));
+      HIP_OK(hipMemsetAsync(ctx->d_prior, 0, P * sizeof(float4), ctx->stream));
+    }
+    hipLaunchKernelGGL(k_prior_reload, dim3(grid1(P)), dim3(256), 0, ctx->stream, d_up, ctx->d_prior, W, H, cam, dmin,
+                       dmax, ctx->d_masks, ctx->d_prior);
+  }
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(ctx->stream));
+  return HPM_OK;
+}
+
+int hpm_get_prior(hpm_ctx* ctx, float* prior_planes, uint32_t* masks) {
+  if (!ctx || !prior_planes || !masks) return HPM_E_INVALID;
+  if (!ctx->d_prior || !ctx->d_masks) return fail(ctx, HPM_E_STATE, "no prior on the context");
+  HIP_OK(hipSetDevice(ctx->device));
+  HIP_OK(hipMemcpyAsync(prior_planes, ctx->d_prior, npix(ctx) * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_OK(hipMemcpyAsync(masks, ctx->d_masks, npix(ctx) * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_OK(hipStreamSynchronize(ctx->stream));
+  return HPM_OK;
 }
 
 int hpm_set_coarse(hpm_ctx* ctx, const float* coarse, int cw, int ch, const float* fine_depth) {

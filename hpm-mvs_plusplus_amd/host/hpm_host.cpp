@@ -792,6 +792,31 @@ void HPM::CudaHypothesesReload(const MapF& depths, const MapF& costs, const MapF
   check(hpm_reload_hypotheses(ctx_, planes.data(), costs.data.data()), "hpm_reload_hypotheses");
 }
 
+void HPM::BuildPlanarPrior(int distance) {
+  const int width = GetReferenceImageWidth(), height = GetReferenceImageHeight();
+  int mw = width, mh = height;
+  if (distance > 0) {  // the grid of PriorFromCoarser
+    const float factor = (float)(1.0 / (distance * 2));
+    mw = (int)std::round(width * factor);
+    mh = (int)std::round(height * factor);
+  }
+  const int cap = 2 * ((width + 4) / 5) * ((height + 4) / 5);
+  std::vector<int32_t> xy((size_t)2 * cap);
+  int count = 0;
+  check(hpm_prior_support_points(ctx_, distance, xy.data(), cap, &count), "hpm_prior_support_points");
+  if (count > cap) throw Error("hpm_prior_support_points: more points than blocks");
+  std::vector<Point2i> support((size_t)count);
+  for (int i = 0; i < count; ++i) support[i] = {xy[2 * i], xy[2 * i + 1]};
+  const std::vector<Triangle> tris = DelaunayTriangulation(mw, mh, support);
+  std::vector<int32_t> tv(6 * tris.size());
+  for (size_t t = 0; t < tris.size(); ++t) {
+    const Triangle& T = tris[t];
+    const int32_t v[6] = {T.pt1.x, T.pt1.y, T.pt2.x, T.pt2.y, T.pt3.x, T.pt3.y};
+    std::memcpy(&tv[6 * t], v, sizeof(v));
+  }
+  check(hpm_prior_build(ctx_, tv.data(), (int)tris.size()), "hpm_prior_build");
+}
+
 void HPM::CudaPlanarPriorInitialization(const std::vector<float>& plane_params, const MapF& masks) {
   const size_t n = (size_t)masks.width * masks.height;
   std::vector<float> prior(4 * n, 0.0f);
@@ -1083,6 +1108,11 @@ bool order_free(const PassFlags& f) { return !(f.geom_consistency && f.multi_geo
 
 void SetPassThreads(int n) { g_pass_threads = n; }
 
+namespace {
+std::atomic<int> g_prior_on_device{1};
+}
+void SetPriorOnDevice(int enable) { g_prior_on_device = enable != 0; }
+
 void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>& problems, int idx,
                     const PassFlags& flags, int precision, uint64_t seed, SceneImages* images) {
   StageClock clk;
@@ -1132,14 +1162,20 @@ void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>&
     if (flags.hpm_scale_distance >= 0 && flags.hpm_scale_distance <= 2)
       (void)LoadDmb(rf + "/texture" + std::to_string(tex_level) + ".dmb");
     t_setup += clk.lap(kSetup);
-    if (flags.hpm_scale_distance == 0)
-      PriorAtLevel(hpm, depths, costs, confidences);
-    else if (flags.hpm_scale_distance == 1 || flags.hpm_scale_distance == 2)
-      PriorFromCoarser(hpm, depths, costs, confidences, flags.hpm_scale_distance);
-    // any other distance runs nothing: the loaded maps are written back
+    // any distance but 0, 1, 2 runs nothing: the loaded maps are written back
     // unchanged, as main.cpp:349-597 does when neither branch is taken
     run = flags.hpm_scale_distance >= 0 && flags.hpm_scale_distance <= 2;
-    if (run) hpm.CudaHypothesesReload(depths, costs, normals);
+    if (run) {
+      // the reload and the prior fill separate buffers; the device build
+      // reads the reloaded depths and costs
+      hpm.CudaHypothesesReload(depths, costs, normals);
+      if (g_prior_on_device)
+        hpm.BuildPlanarPrior(flags.hpm_scale_distance);
+      else if (flags.hpm_scale_distance == 0)
+        PriorAtLevel(hpm, depths, costs, confidences);
+      else
+        PriorFromCoarser(hpm, depths, costs, confidences, flags.hpm_scale_distance);
+    }
     t_prior += clk.lap(kPrior);
   } else {
     t_setup += clk.lap(kSetup);

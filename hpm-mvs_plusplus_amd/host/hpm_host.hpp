@@ -190,6 +190,12 @@ class HPM {
   void CudaConfidenceInitialization(const std::string& dense_folder, const std::vector<Problem>& problems, int idx);
   void TextureInformationInitialization() {}
   void CudaHypothesesReload(const MapF& depths, const MapF& costs, const MapF& normals);
+  // The planar prior of a prior pass at `distance` 0, 1 or 2 built on the
+  // device from the reloaded hypotheses and confidences (after
+  // CudaHypothesesReload and CudaConfidenceInitialization): support points
+  // and everything after the triangulation run there, the Delaunay
+  // triangulation on the host.
+  void BuildPlanarPrior(int distance);
   void CudaPlanarPriorInitialization(const std::vector<float>& plane_params, const MapF& masks);
   void ReloadPlanarPriorInitialization(const MapF& masks, const std::vector<float>& prior_planes);
   void RunPatchMatch(uint64_t seed);
@@ -297,5 +303,9 @@ void SetVerbose(int level);
 // order-free: its views read no file that another view of the pass writes,
 // so the files it produces do not depend on the thread count.
 void SetPassThreads(int n);
+// Where the prior passes build the planar prior: 1 (default) on the device
+// (hpm_prior_support_points / hpm_prior_build; only the Delaunay
+// triangulation runs on the host), 0 on the host.  The files are identical.
+void SetPriorOnDevice(int enable);
 
 }  // namespace hpmhost

@@ -60,7 +60,7 @@ EXPORTS = [
     "hpm_geom_cost", "hpm_jbu_depth", "hpm_jbu_prior", "hpm_consistency_check", "hpm_set_graph",
     "hpm_run_patchmatch_batch", "hpm_pool_create", "hpm_pool_destroy", "hpm_set_views_pooled",
     "hpm_cost_grad", "hpm_cost_grad_device", "hpm_refine_gn", "hpm_refine_random", "hpm_refine_hybrid",
-    "hpm_refine_gn_smoothed",
+    "hpm_refine_gn_smoothed", "hpm_prior_support_points", "hpm_prior_build", "hpm_get_prior",
     "hpm_batch_create", "hpm_batch_destroy", "hpm_batch_last_error", "hpm_batch_run", "hpm_batch_set_graph",
     "hpm_batch_set_shape", "hpm_batch_get_shape", "hpm_batch_last_ms", "hpm_batch_autotune",
 ]
@@ -93,6 +93,10 @@ def lib():
     L.hpm_set_confidence.argtypes = [P, fp]
     L.hpm_reload_hypotheses.argtypes = [P, fp, fp]
     L.hpm_set_prior.argtypes = [P, fp, up]
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    L.hpm_prior_support_points.argtypes = [P, ctypes.c_int, i32p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.hpm_prior_build.argtypes = [P, i32p, ctypes.c_int]
+    L.hpm_get_prior.argtypes = [P, fp, up]
     L.hpm_set_coarse.argtypes = [P, fp, ctypes.c_int, ctypes.c_int, fp]
     L.hpm_run_patchmatch.argtypes = [P, ctypes.c_uint64]
     L.hpm_run_patchmatch_partial.argtypes = [P, ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
@@ -254,6 +258,32 @@ class Context:
         p = np.ascontiguousarray(prior_planes, dtype=np.float32)
         m = np.ascontiguousarray(masks, dtype=np.uint32)
         self._check(self._L.hpm_set_prior(self._h, _fptr(p), _uptr(m)), "hpm_set_prior")
+
+    def prior_support_points(self, distance=0):
+        """Support points of the planar prior at `distance` 0, 1 or 2, built
+        on the device from the reloaded hypotheses, costs and confidences:
+        (n, 2) int32 (x, y) in the reference's order."""
+        cap = 2 * ((self.width + 4) // 5) * ((self.height + 4) // 5)
+        out = np.empty((cap, 2), np.int32)
+        n = ctypes.c_int()
+        self._check(self._L.hpm_prior_support_points(self._h, int(distance),
+                                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), cap,
+                                                     ctypes.byref(n)), "hpm_prior_support_points")
+        return out[:n.value].copy()
+
+    def prior_build(self, triangles):
+        """The rest of the planar prior from the Delaunay triangles of the
+        last support points ((n, 6) int32 corners)."""
+        t = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1, 6)
+        self._check(self._L.hpm_prior_build(self._h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(t)),
+                    "hpm_prior_build")
+
+    def prior(self):
+        """The context's prior planes (h, w, 4) and triangle masks (h, w)."""
+        p = np.empty((self.height, self.width, 4), np.float32)
+        m = np.empty((self.height, self.width), np.uint32)
+        self._check(self._L.hpm_get_prior(self._h, _fptr(p), _uptr(m)), "hpm_get_prior")
+        return p, m
 
     def set_coarse(self, coarse_planes, fine_depth):
         cp = np.ascontiguousarray(coarse_planes, dtype=np.float32)

@@ -41,6 +41,8 @@ def host_lib():
     L.hpm_host_set_verbose.restype = None
     L.hpm_host_set_threads.argtypes = [ctypes.c_int]
     L.hpm_host_set_threads.restype = None
+    L.hpm_host_set_prior_on_device.argtypes = [ctypes.c_int]
+    L.hpm_host_set_prior_on_device.restype = None
     L.hpm_host_decode_jpeg.argtypes = [u8p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                        ctypes.POINTER(ctypes.c_int), u8p, ctypes.c_int64]
     L.hpm_host_support_points.argtypes = [fp, fp, ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int,
@@ -130,6 +132,13 @@ def set_verbose(level=1):
 def set_threads(n=0):
     """Host threads for the order-free passes (0: default, 1: view order)."""
     host_lib().hpm_host_set_threads(int(n))
+
+
+def set_prior_on_device(enable=True):
+    """Build the prior passes' planar prior on the device (default; only the
+    Delaunay triangulation runs on the host) or entirely on the host; the
+    output files are identical."""
+    host_lib().hpm_host_set_prior_on_device(int(bool(enable)))
 
 
 def decode_jpeg(data: bytes):

@@ -126,6 +126,33 @@ int hpm_reload_hypotheses(hpm_ctx* ctx, const float* planes, const float* costs)
  * (0 = no prior, else triangle index + 1). */
 int hpm_set_prior(hpm_ctx* ctx, const float* prior_planes, const uint32_t* masks);
 
+/* The planar prior built on the device from the maps the context holds:
+ * reloaded hypotheses (hpm_reload_hypotheses: the depth in w) and costs, and
+ * the confidences (hpm_set_confidence), as ProcessProblem's prior branch
+ * builds it from depths/costs/confidence.dmb (main.cpp:324-597).
+ *
+ * hpm_prior_support_points: GetSupportPoints_Classify_Check (HPM.cpp:1026-1073)
+ * at `distance` 0 (the reference grid) or 1, 2 (the maps resized by
+ * 1 / (2 distance), main.cpp:443-454).  Writes up to `capacity` (x, y) pairs
+ * in the reference's order and sets *count to their number.
+ *
+ * hpm_prior_build: the rest, from the Delaunay triangles of those points
+ * (DelaunayTriangulation, HPM.cpp:1075-1097; 6 int32 per triangle, corners on
+ * the same grid): per-triangle planes (GetPriorPlaneParams[_factor],
+ * HPM.cpp:1099-1167), the barycentric raster (main.cpp:389-409, 493-514),
+ * the depth range test and, at distance 0, the prior planes and triangle
+ * masks of CudaPlanarPriorInitialization (HPM.cpp:964-984); at distance 1, 2
+ * the coarse depth and world normal upsampled by JBU_cu_prior and reloaded
+ * per pixel (main.cpp:516-576, ReloadPlanarPriorInitialization,
+ * HPM.cpp:1433-1453).  Replaces hpm_set_prior; bit-identical to building the
+ * prior on the host with the same steps.
+ *
+ * hpm_get_prior: the context's prior planes and masks (as hpm_set_prior takes
+ * them). */
+int hpm_prior_support_points(hpm_ctx* ctx, int distance, int32_t* out_xy, int capacity, int* count);
+int hpm_prior_build(hpm_ctx* ctx, const int32_t* triangles, int num_triangles);
+int hpm_get_prior(hpm_ctx* ctx, float* prior_planes, uint32_t* masks);
+
 /* Hierarchy reload (HPM.cpp:849-905): the coarse level's maps (cw x ch) and the
  * JBU-upsampled fine depth.  coarse = (world normal xyz, w) where w is the
  * coarse cost when the sizes differ (`upsample`) else the coarse depth.  The

@@ -67,6 +67,12 @@ void hpm_host_set_verbose(int level);
  * reference runs in view order); the output files do not depend on it. */
 void hpm_host_set_threads(int n);
 
+/* Where prior passes build the planar prior (main.cpp:324-597): 1 (default)
+ * on the device with hpm_prior_support_points / hpm_prior_build, only the
+ * Delaunay triangulation on the host; 0 entirely on the host.  The output
+ * files are identical. */
+void hpm_host_set_prior_on_device(int enable);
+
 /* JPEG decoding as the pipeline reads images/%08d.jpg, which the reference
  * reads with cv::imread (main.cpp:125, main.cpp:213): Huffman-coded
  * sequential or progressive, 8-bit, the pixels of libjpeg's default decoding

@@ -191,7 +191,8 @@ def test_pipeline_files_do_not_depend_on_host_threads(tmp_path):
 def test_pipeline_reads_jpeg_images(tmp_path):
     """images/%08d.jpg (what the reference reads, main.cpp:125): a colour JPEG
     folder and a folder holding the decoded pixels as PPM give byte-identical
-    output files, and the depth maps keep their ground-truth accuracy."""
+    output files, and the depth maps from the lossy images stay close to the
+    ground-truth accuracy of the lossless grey images."""
     import filecmp
     import io
     Image = pytest.importorskip("PIL.Image")
@@ -199,10 +200,10 @@ def test_pipeline_reads_jpeg_images(tmp_path):
     from hpm_mi355x import scenes
     sc = scenes.make_plane_scene(seed=6, n_cams=5, width=160, height=120, f=131.25, n_src=4)
     runs = {}
-    for kind in ("jpg", "ppm"):
+    for kind in ("pgm", "jpg", "ppm"):
         folder = str(tmp_path / kind)
         scenes.write_dense_folder(sc, folder)
-        for i, im in enumerate(sc.images):
+        for i, im in enumerate(sc.images if kind != "pgm" else []):
             g = np.asarray(im, np.int32)
             rgb = np.stack([np.clip(g + 12, 0, 255), g, np.clip(g - 9, 0, 255)], -1).astype(np.uint8)
             buf = io.BytesIO()
@@ -226,8 +227,11 @@ def test_pipeline_reads_jpeg_images(tmp_path):
     assert len(names) > 5 * 6
     for rel in names:
         assert filecmp.cmp(os.path.join(runs["jpg"][0], rel), os.path.join(runs["ppm"][0], rel), shallow=False), rel
-    for i in range(5):
-        d = P.read_dmb(os.path.join(runs["jpg"][0], "HPM_MVS_plusplus", f"2333_{i:08d}", "depths_geom.dmb"))
-        rel = np.abs(d - sc.depths[i]) / sc.depths[i]
-        print(f"view {i}: depth within 1% of GT {np.mean(rel <= 0.01):.3f}")
-        assert np.mean(rel <= 0.01) > 0.75
+    acc = {}
+    for kind in ("pgm", "jpg"):
+        acc[kind] = []
+        for i in range(5):
+            d = P.read_dmb(os.path.join(runs[kind][0], "HPM_MVS_plusplus", f"2333_{i:08d}", "depths_geom.dmb"))
+            acc[kind].append(float(np.mean(np.abs(d - sc.depths[i]) / sc.depths[i] <= 0.01)))
+        print(f"{kind}: depth within 1% of GT per view", " ".join(f"{a:.3f}" for a in acc[kind]))
+    assert np.mean(acc["jpg"]) > np.mean(acc["pgm"]) - 0.05
