@@ -16,6 +16,7 @@
 #include "hpm_grad.h"
 #include "hpm_jbu.h"
 #include "hpm_prior.h"
+#include "hpm_canny.h"
 #include "hpm_kernels.h"
 
 using namespace hpm;
@@ -726,6 +727,56 @@ int hpm_prior_build(hpm_ctx* ctx, const int32_t* tris, int n) {
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(ctx->stream));
   return HPM_OK;
+}
+
+int hpm_canny(int device, const uint8_t* image, int w, int h, int channels, float low, float high, uint8_t* out_edges) {
+  if (!image || !out_edges || w <= 0 || h <= 0 || (channels != 1 && channels != 3)) return HPM_E_INVALID;
+  if ((size_t)w * h >= ((size_t)1 << 31)) return HPM_E_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return HPM_E_DEVICE;
+  const size_t P = (size_t)w * h;
+  // image, gx, gy, mag, cls, edge, flag; 256-byte aligned pieces
+  const size_t sizes[7] = {P * channels, 2 * P, 2 * P, 2 * P, P, P, 4};
+  size_t off[7], total = 0;
+  for (int k = 0; k < 7; ++k) { off[k] = total; total += (sizes[k] + 255) & ~(size_t)255; }
+  hipStream_t st = nullptr;
+  char* d = nullptr;
+  int rc = HPM_OK;
+  auto ok = [&](hipError_t e) {
+    if (e != hipSuccess && rc == HPM_OK) rc = (e == hipErrorOutOfMemory ? HPM_E_NOMEM : HPM_E_DEVICE);
+    return rc == HPM_OK;
+  };
+  // a stream of its own and stream-ordered memory: concurrent passes on other
+  // streams are neither waited for nor made to wait
+  if (ok(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) && ok(hipMallocAsync((void**)&d, total, st)) &&
+      ok(hipMemcpyAsync(d + off[0], image, sizes[0], hipMemcpyHostToDevice, st))) {
+    const uint8_t* d_img = (const uint8_t*)(d + off[0]);
+    int16_t* gx = (int16_t*)(d + off[1]);
+    int16_t* gy = (int16_t*)(d + off[2]);
+    int16_t* mag = (int16_t*)(d + off[3]);
+    uint8_t* cls = (uint8_t*)(d + off[4]);
+    uint8_t* edge = (uint8_t*)(d + off[5]);
+    int* flag = (int*)(d + off[6]);
+    const dim3 g8((w + 31) / 32, (h + 7) / 8), g32((w + 31) / 32, (h + 31) / 32);
+    hipLaunchKernelGGL(k_canny_grad, g8, dim3(256), 0, st, d_img, w, h, channels, gx, gy, mag);
+    hipLaunchKernelGGL(k_canny_nms, g8, dim3(256), 0, st, gx, gy, mag, w, h, low, high, cls, edge);
+    ok(hipGetLastError());
+    // every pass that goes on adds at least one pixel: at most P + 1 passes
+    for (size_t it = 0; it <= P && rc == HPM_OK; ++it) {
+      int changed = 0;
+      if (!ok(hipMemsetAsync(flag, 0, sizeof(int), st))) break;
+      hipLaunchKernelGGL(k_canny_hyst, g32, dim3(256), 0, st, cls, w, h, edge, flag);
+      if (!ok(hipGetLastError()) || !ok(hipMemcpyAsync(&changed, flag, sizeof(int), hipMemcpyDeviceToHost, st)) ||
+          !ok(hipStreamSynchronize(st)) || !changed)
+        break;
+    }
+    if (rc == HPM_OK) ok(hipMemcpyAsync(out_edges, edge, P, hipMemcpyDeviceToHost, st));
+  }
+  if (d) hipFreeAsync(d, st);
+  if (st) {
+    ok(hipStreamSynchronize(st));
+    hipStreamDestroy(st);
+  }
+  return rc;
 }
 
 int hpm_get_prior(hpm_ctx* ctx, float* prior_planes, uint32_t* masks) {

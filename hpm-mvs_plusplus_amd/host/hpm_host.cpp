@@ -589,10 +589,19 @@ const MapF& SceneImages::CurrentGray(int id) {
   return gray_[id];
 }
 
+// cv::Canny(image, 50, 150) of ProcessProblem (main.cpp:280-289), computed on
+// device 0 (the reference's device, main.cpp:250); the same map as CannyEdges.
+static MapU8 DeviceEdges(const MapU8& img) {
+  MapU8 e(img.width, img.height);
+  if (hpm_canny(0, img.data.data(), img.width, img.height, img.channels, 50.0f, 150.0f, e.data.data()) != HPM_OK)
+    throw Error("hpm_canny failed");
+  return e;
+}
+
 const MapU8& SceneImages::CurrentEdges(int id) {
   ensure(id);
   if (!have_edges_[id]) {
-    edges_[id] = CannyEdges(Current(id), 50.0f, 150.0f);
+    edges_[id] = DeviceEdges(Current(id));
     have_edges_[id] = true;
   }
   return edges_[id];
@@ -1142,7 +1151,7 @@ void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>&
     const MapU8 colour = LoadImage(dense_folder, problem.ref_image_id);
     if (colour.width != width || colour.height != height)
       throw Error("reference image size does not match the camera size");
-    hpm.CudaCannyInitialization(CannyEdges(colour, 50.0f, 150.0f));
+    hpm.CudaCannyInitialization(DeviceEdges(colour));
   }
   t_edges += clk.lap(kEdges);
   if (flags.mand_consistency || flags.prior_consistency) hpm.CudaConfidenceInitialization(dense_folder, problems, idx);

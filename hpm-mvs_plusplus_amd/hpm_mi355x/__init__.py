@@ -60,7 +60,7 @@ EXPORTS = [
     "hpm_geom_cost", "hpm_jbu_depth", "hpm_jbu_prior", "hpm_consistency_check", "hpm_set_graph",
     "hpm_run_patchmatch_batch", "hpm_pool_create", "hpm_pool_destroy", "hpm_set_views_pooled",
     "hpm_cost_grad", "hpm_cost_grad_device", "hpm_refine_gn", "hpm_refine_random", "hpm_refine_hybrid",
-    "hpm_refine_gn_smoothed", "hpm_prior_support_points", "hpm_prior_build", "hpm_get_prior",
+    "hpm_refine_gn_smoothed", "hpm_prior_support_points", "hpm_prior_build", "hpm_get_prior", "hpm_canny",
     "hpm_batch_create", "hpm_batch_destroy", "hpm_batch_last_error", "hpm_batch_run", "hpm_batch_set_graph",
     "hpm_batch_set_shape", "hpm_batch_get_shape", "hpm_batch_last_ms", "hpm_batch_autotune",
 ]
@@ -97,6 +97,9 @@ def lib():
     L.hpm_prior_support_points.argtypes = [P, ctypes.c_int, i32p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.hpm_prior_build.argtypes = [P, i32p, ctypes.c_int]
     L.hpm_get_prior.argtypes = [P, fp, up]
+    u8p = ctypes.POINTER(ctypes.c_uint8)
+    L.hpm_canny.argtypes = [ctypes.c_int, u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                            ctypes.c_float, u8p]
     L.hpm_set_coarse.argtypes = [P, fp, ctypes.c_int, ctypes.c_int, fp]
     L.hpm_run_patchmatch.argtypes = [P, ctypes.c_uint64]
     L.hpm_run_patchmatch_partial.argtypes = [P, ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
@@ -436,6 +439,20 @@ def jbu_prior(fine_gray, coarse_depth, coarse_normals):
 CHECK_FUSION = 0
 CHECK_CONFIDENCE = 1
 CHECK_FUSION_MASK = 2
+
+
+def canny(image, low=50.0, high=150.0, device=0):
+    """Edge map (255 = edge) of an (h, w) grey or (h, w, 3) R G B uint8 image
+    computed on the device (hpm_canny)."""
+    g = np.ascontiguousarray(image, np.uint8)
+    channels = 1 if g.ndim == 2 else g.shape[2]
+    out = np.zeros(g.shape[:2], np.uint8)
+    u8p = ctypes.POINTER(ctypes.c_uint8)
+    rc = lib().hpm_canny(device, g.ctypes.data_as(u8p), g.shape[1], g.shape[0], channels, low, high,
+                         out.ctypes.data_as(u8p))
+    if rc != HPM_OK:
+        raise HpmError(f"hpm_canny failed ({rc})")
+    return out
 
 
 def _pack_views(cameras, depths, normals, sources):

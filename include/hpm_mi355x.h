@@ -153,6 +153,13 @@ int hpm_prior_support_points(hpm_ctx* ctx, int distance, int32_t* out_xy, int ca
 int hpm_prior_build(hpm_ctx* ctx, const int32_t* triangles, int num_triangles);
 int hpm_get_prior(hpm_ctx* ctx, float* prior_planes, uint32_t* masks);
 
+/* The edge map ProcessProblem uploads with CudaCannyInitialization:
+ * cv::Canny(image, low, high) with a 3 x 3 aperture and the L1 gradient
+ * (main.cpp:280-289) of a w x h image of 1 (grey) or 3 (R G B) channels;
+ * out_edges = 255 on edges, else 0.  Runs on `device` on a stream of its own
+ * and returns when the map is on the host. */
+int hpm_canny(int device, const uint8_t* image, int w, int h, int channels, float low, float high, uint8_t* out_edges);
+
 /* Hierarchy reload (HPM.cpp:849-905): the coarse level's maps (cw x ch) and the
  * JBU-upsampled fine depth.  coarse = (world normal xyz, w) where w is the
  * coarse cost when the sizes differ (`upsample`) else the coarse depth.  The
