@@ -7,12 +7,13 @@
 // expensive part does not depend on the masks: for each (reference pixel,
 // source view) it is two projections, a reprojection, a normal angle and an
 // exp.  k_consistency computes it for all pixels of one image at once and
-// stores, per (source slot j, pixel), the matched source pixel (or -1 when the
-// pair fails any test) and its score exp(-(e + 200 dd + 10 angle)).  The
-// greedy, ordered part -- masks, the score sum over unmasked matches, the
+// stores, per pixel, the sources that confirm it (slot j, matched source
+// pixel, score exp(-(e + 200 dd + 10 angle))) in slot order, and their count.
+// The greedy, ordered part -- masks, the score sum over unmasked matches, the
 // acceptance test and the mask updates -- is a short sequential pass on the
-// host over those arrays, with the reference's float operations in the
-// reference's order.
+// host over those lists, with the reference's float operations in the
+// reference's order; a pixel no source confirms changes nothing and is
+// skipped without reading its list.
 //
 // Arithmetic: plain IEEE float operations in the reference's order (no fused
 // multiply-adds; the library is built with -ffp-contract=off); pow(x, 2) of a
@@ -21,6 +22,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -69,19 +73,23 @@ struct FusionImage {
 
 constexpr int kFusionMaxImages = 1024;
 
-// One (matched source pixel, score) pair per (pixel, source slot); m = -1
-// when the source does not confirm the pixel.
+// A source that confirms a pixel: its slot j (top 5 bits) and the matched
+// source pixel m (low 27 bits), and the score.
+constexpr int kMatchPixelBits = 27;
+constexpr int kFusionMaxSources = 32;
 struct MatchTerm {
-  int m;
+  unsigned int jm;
   float term;
 };
 
-// One thread per pixel p0 + i (i < count) of reference image `ref`; writes
-// out[i * num_ngb + j] for its num_ngb source slots (pixel-major, so the
-// host's ordered pass reads each pixel's slots contiguously).
+// One thread per pixel p0 + i (i < count) of reference image `ref`.  Its row
+// out[i * num_ngb ...] holds the sources that confirm it, in slot order, and
+// n_match[i] their number (pixel-major, so the host's ordered pass reads a
+// pixel's matches contiguously and skips pixels without one).
 __global__ __launch_bounds__(256) void k_consistency(const DevCam* __restrict__ cams, const FusionImage* __restrict__ imgs,
                                                      const int* __restrict__ srcs, int num_ngb, int ref, int mode,
-                                                     int p0, int count, MatchTerm* __restrict__ out) {
+                                                     int p0, int count, MatchTerm* __restrict__ out,
+                                                     unsigned char* __restrict__ n_match) {
   const DevCam& rc = cams[ref];
   const int cols = rc.width;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -94,43 +102,41 @@ __global__ __launch_bounds__(256) void k_consistency(const DevCam* __restrict__ 
   if (mode == 1 && (ref_depth < rc.depth_min || ref_depth > rc.depth_max)) ref_ok = false;
   float X[3];
   point_on_world(c, r, ref_depth, rc, X);
-  for (int j = 0; j < num_ngb; ++j) {
-    int m = -1;
-    float score = 0.0f;
+  int n = 0;
+  for (int j = 0; j < num_ngb && ref_ok; ++j) {
     const int s = srcs[j];
     const DevCam& sc = cams[s];
-    if (ref_ok) {
-      float pu, pv, proj_depth;
-      project_on_camera(X, sc, pu, pv, proj_depth);
-      const int src_r = round_index(pv);
-      const int src_c = round_index(pu);
-      if (src_c >= 0 && src_c < sc.width && src_r >= 0 && src_r < sc.height) {
-        const int sp = src_r * sc.width + src_c;
-        const float src_depth = imgs[s].depth[sp];
-        const float* sn = imgs[s].normal + 3 * (size_t)sp;
-        bool ok = src_depth > 0.0f;
-        if (mode == 1 && (src_depth < rc.depth_min || src_depth > rc.depth_max)) ok = false;
-        if (ok) {
-          float tX[3];
-          point_on_world(src_c, src_r, src_depth, sc, tX);
-          float tu, tv;
-          project_on_camera(tX, rc, tu, tv, proj_depth);
-          const float dx = (float)c - tu, dy = (float)r - tv;
-          const float reproj_error = sqrtf(dx * dx + dy * dy);
-          const float relative_depth_diff = fabsf(proj_depth - ref_depth) / ref_depth;
-          const float dot = rn[0] * sn[0] + rn[1] * sn[1] + rn[2] * sn[2];
-          float angle = acosf_hpm(dot);  // GetAngle, HPM.cpp:281-290
-          if (angle != angle) angle = 0.0f;
-          if (reproj_error < 2.0f && relative_depth_diff < 0.01f && angle < 0.174533f) {
-            m = sp;
-            const float tmp_index = reproj_error + 200 * relative_depth_diff + angle * 10;
-            score = expf_hpm(-tmp_index);
-          }
+    float pu, pv, proj_depth;
+    project_on_camera(X, sc, pu, pv, proj_depth);
+    const int src_r = round_index(pv);
+    const int src_c = round_index(pu);
+    if (src_c >= 0 && src_c < sc.width && src_r >= 0 && src_r < sc.height) {
+      const int sp = src_r * sc.width + src_c;
+      const float src_depth = imgs[s].depth[sp];
+      const float* sn = imgs[s].normal + 3 * (size_t)sp;
+      bool ok = src_depth > 0.0f;
+      if (mode == 1 && (src_depth < rc.depth_min || src_depth > rc.depth_max)) ok = false;
+      if (ok) {
+        float tX[3];
+        point_on_world(src_c, src_r, src_depth, sc, tX);
+        float tu, tv;
+        project_on_camera(tX, rc, tu, tv, proj_depth);
+        const float dx = (float)c - tu, dy = (float)r - tv;
+        const float reproj_error = sqrtf(dx * dx + dy * dy);
+        const float relative_depth_diff = fabsf(proj_depth - ref_depth) / ref_depth;
+        const float dot = rn[0] * sn[0] + rn[1] * sn[1] + rn[2] * sn[2];
+        float angle = acosf_hpm(dot);  // GetAngle, HPM.cpp:281-290
+        if (angle != angle) angle = 0.0f;
+        if (reproj_error < 2.0f && relative_depth_diff < 0.01f && angle < 0.174533f) {
+          const float tmp_index = reproj_error + 200 * relative_depth_diff + angle * 10;
+          out[(size_t)i * num_ngb + n] = MatchTerm{((unsigned int)j << kMatchPixelBits) | (unsigned int)sp,
+                                                   expf_hpm(-tmp_index)};
+          ++n;
         }
       }
     }
-    out[(size_t)i * num_ngb + j] = MatchTerm{m, score};
   }
+  n_match[i] = (unsigned char)n;
 }
 
 }  // namespace hpm
@@ -158,9 +164,9 @@ int hpm_consistency_check(int mode, int num_images, const hpm_camera* cameras, c
   int max_ngb = 0;
   for (int i = 0; i < num_images; ++i) {
     if (cameras[i].width <= 0 || cameras[i].height <= 0 || !depths[i] || !normals[i]) return HPM_E_INVALID;
-    if ((size_t)cameras[i].width * cameras[i].height > ((size_t)1 << 28)) return HPM_E_INVALID;
+    if ((size_t)cameras[i].width * cameras[i].height > ((size_t)1 << kMatchPixelBits)) return HPM_E_INVALID;
     const int n = src_offsets[i + 1] - src_offsets[i];
-    if (n < 0) return HPM_E_INVALID;
+    if (n < 0 || n > kFusionMaxSources) return HPM_E_INVALID;
     for (int j = src_offsets[i]; j < src_offsets[i + 1]; ++j)
       if (src_ids[j] < 0 || src_ids[j] >= num_images) return HPM_E_INVALID;
     max_p = std::max(max_p, (size_t)cameras[i].width * cameras[i].height);
@@ -187,14 +193,19 @@ int hpm_consistency_check(int mode, int num_images, const hpm_camera* cameras, c
   const size_t chunk_elems = chunk_px * (size_t)std::max(max_ngb, 1);
   MatchTerm* d_out = nullptr;
   MatchTerm* h_out[2] = {nullptr, nullptr};
+  unsigned char* d_cnt = nullptr;
+  unsigned char* h_cnt[2] = {nullptr, nullptr};
   hipStream_t st = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
   if (ok(hipMalloc((void**)&d_cams, sizeof(DevCam) * num_images)) &&
       ok(hipMalloc((void**)&d_imgs, sizeof(FusionImage) * num_images)) &&
       ok(hipMalloc((void**)&d_srcs, sizeof(int) * std::max(total_src, 1))) &&
       ok(hipMalloc((void**)&d_out, sizeof(MatchTerm) * chunk_elems)) &&
+      ok(hipMalloc((void**)&d_cnt, chunk_px)) &&
       ok(hipHostMalloc((void**)&h_out[0], sizeof(MatchTerm) * chunk_elems, hipHostMallocDefault)) &&
       ok(hipHostMalloc((void**)&h_out[1], sizeof(MatchTerm) * chunk_elems, hipHostMallocDefault)) &&
+      ok(hipHostMalloc((void**)&h_cnt[0], chunk_px, hipHostMallocDefault)) &&
+      ok(hipHostMalloc((void**)&h_cnt[1], chunk_px, hipHostMallocDefault)) &&
       ok(hipStreamCreate(&st)) && ok(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming)) &&
       ok(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming))) {
     for (int i = 0; i < num_images && rc == HPM_OK; ++i) {
@@ -223,9 +234,10 @@ int hpm_consistency_check(int mode, int num_images, const hpm_camera* cameras, c
     const Chunk& c = chunks[k];
     const int num_ngb = src_offsets[c.img + 1] - src_offsets[c.img];
     hipLaunchKernelGGL(k_consistency, dim3((unsigned)((c.n + 255) / 256)), dim3(256), 0, st, d_cams, d_imgs,
-                       d_srcs + src_offsets[c.img], num_ngb, c.img, mode, (int)c.p0, (int)c.n, d_out);
+                       d_srcs + src_offsets[c.img], num_ngb, c.img, mode, (int)c.p0, (int)c.n, d_out, d_cnt);
     return ok(hipGetLastError()) &&
            ok(hipMemcpyAsync(h_out[k & 1], d_out, sizeof(MatchTerm) * c.n * num_ngb, hipMemcpyDeviceToHost, st)) &&
+           ok(hipMemcpyAsync(h_cnt[k & 1], d_cnt, c.n, hipMemcpyDeviceToHost, st)) &&
            ok(hipEventRecord(ev[k & 1], st));
   };
 
@@ -238,6 +250,12 @@ int hpm_consistency_check(int mode, int num_images, const hpm_camera* cameras, c
   for (size_t k = 0; k < chunks.size() && k < 2 && rc == HPM_OK; ++k) issue(k);
   size_t k = 0;
   std::vector<int> used;
+  // HPM_VERBOSE=1: time waiting for chunks vs resolving them, match counts
+  const char* verbose_env = std::getenv("HPM_VERBOSE");
+  const bool verbose = verbose_env && std::atoi(verbose_env) > 0;
+  double t_wait = 0.0;
+  int64_t n_pixels = 0, n_matched_pixels = 0, n_matches = 0;
+  const auto t_begin = std::chrono::steady_clock::now();
   for (int i = 0; i < num_images && rc == HPM_OK; ++i) {
     const int cols = cameras[i].width, rows = cameras[i].height;
     const size_t P = (size_t)cols * rows;
@@ -248,26 +266,38 @@ int hpm_consistency_check(int mode, int num_images, const hpm_camera* cameras, c
     used.assign(num_ngb, -1);
     size_t p_begin = 0, p_end = 0;
     const MatchTerm* mt = nullptr;
+    const unsigned char* nm = nullptr;
     // chunk k consumed: its pinned buffer takes chunk k + 2
     auto release = [&]() { return !(k + 2 < chunks.size() && !issue(k + 2)) && (++k, true); };
     for (size_t p = 0; p < P; ++p) {
       if (num_ngb > 0 && p == p_end) {  // next chunk of this image
         if (mt && !release()) break;
+        const auto tw = std::chrono::steady_clock::now();
         if (!ok(hipEventSynchronize(ev[k & 1]))) break;
+        t_wait += std::chrono::duration<double>(std::chrono::steady_clock::now() - tw).count();
         mt = h_out[k & 1];
+        nm = h_cnt[k & 1];
         p_begin = chunks[k].p0;
         p_end = p_begin + chunks[k].n;
       }
+      // a pixel no source confirms changes nothing (num_consistent stays 0)
+      const int n_row = num_ngb > 0 ? nm[p - p_begin] : 0;
+      if (verbose) {
+        ++n_pixels;
+        n_matched_pixels += n_row > 0;
+        n_matches += n_row;
+      }
+      if (n_row == 0) continue;
       if (masks[i][p] == 1) continue;
       int num_consistent = 0;
       float dynamic_consistency = 0;
-      const MatchTerm* row = mt + (p - p_begin) * num_ngb;  // num_ngb == 0: not read
-      for (int j = 0; j < num_ngb; ++j) {
-        const int m = row[j].m;
-        if (m < 0) continue;
+      const MatchTerm* row = mt + (p - p_begin) * num_ngb;
+      for (int e = 0; e < n_row; ++e) {  // slots in increasing order, as the reference sums them
+        const int j = (int)(row[e].jm >> kMatchPixelBits);
+        const int m = (int)(row[e].jm & ((1u << kMatchPixelBits) - 1u));
         if (masks[srcs[j]][m] == 1) continue;
         used[j] = m;
-        dynamic_consistency += row[j].term;
+        dynamic_consistency += row[e].term;
         num_consistent++;
       }
       // RunFusion compares with the double 0.3 * n; RunFusion_Sky_Strict
@@ -301,9 +331,21 @@ int hpm_consistency_check(int mode, int num_images, const hpm_camera* cameras, c
     }
     if (mt && rc == HPM_OK) release();
   }
+  if (verbose) {
+    const double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count();
+    std::fprintf(stderr,
+                 "[hpm] consistency check: %.1f ms (waiting for chunks %.1f ms), %lld pixels, %.3f with a match, "
+                 "%.2f matches per pixel\n",
+                 1e3 * total, 1e3 * t_wait, (long long)n_pixels,
+                 n_pixels ? (double)n_matched_pixels / n_pixels : 0.0, n_pixels ? (double)n_matches / n_pixels : 0.0);
+    std::fflush(stderr);
+  }
   if (st) hipStreamSynchronize(st);
   for (MatchTerm* h : h_out)
     if (h) hipHostFree(h);
+  for (unsigned char* h : h_cnt)
+    if (h) hipHostFree(h);
+  hipFree(d_cnt);
   for (hipEvent_t e : ev)
     if (e) hipEventDestroy(e);
   if (st) hipStreamDestroy(st);

@@ -328,6 +328,8 @@ int hpm_refine_gn_smoothed(hpm_ctx* ctx, float* params, const float* weights, in
  *               normal, as normals.dmb holds);
  *   src_ids[src_offsets[i] .. src_offsets[i+1]) are image i's source images
  *               (Problem::src_image_ids), src_offsets has num_images+1 entries.
+ * Limits: at most 1024 images, 32 sources per image (the reference's
+ * per-pixel arrays hold 32) and 2^27 pixels per image.
  * HPM_CHECK_FUSION: out_pairs receives (image, pixel) of every fused point in
  *   the reference's PointCloud order, out_points (may be NULL) 6 floats per
  *   point (world position, normal), both up to `capacity` points;

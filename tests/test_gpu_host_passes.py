@@ -127,6 +127,31 @@ def test_consistency_chunked_bitexact(mode):
             np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_consistency_self_and_repeated_sources_bitexact(mode):
+    """Source lists that name the image itself or one source twice (two
+    slots then write the same mask and confidence locations, and an image
+    masks its own pixels): fusion and confidence equal the oracle."""
+    import hpm_mi355x as H
+    print(f"case: consistency mode {mode} 5 x 96x72, self and repeated sources", flush=True)
+    sc, depths, normals, sources = _noisy_maps(21 + mode)
+    sources[0] = [0] + sources[0]
+    sources[2] = sources[2][:2] + sources[2][:2] + sources[2][2:]
+    hc = [H.camera_from_scene(c) for c in sc.cameras]
+    oc = [O.camera_from_scene(c) for c in sc.cameras]
+    kind = {0: H.CHECK_FUSION, 1: H.CHECK_CONFIDENCE, 2: H.CHECK_FUSION_MASK}[mode]
+    g = H.consistency_check(kind, hc, depths, normals, sources)
+    r = O.consistency(mode, oc, depths, normals, sources)
+    if mode == 1:
+        assert sum(int((x > 0).sum()) for x in r) > 1000
+        for a, b in zip(g, r):
+            np.testing.assert_array_equal(a, b)
+    else:
+        assert len(r[0]) > 1000
+        np.testing.assert_array_equal(g[0], r[0])
+        np.testing.assert_array_equal(g[1], r[1])
+
+
 def test_fusion_two_view_closed_form_on_gpu():
     """The integer-disparity pair of the oracle pins, through the C ABI."""
     import hpm_mi355x as H
