@@ -19,9 +19,10 @@ writes, per registered image in increasing image-id order (output index i):
   the score of a pair is the number of 3D points both observe, set to 0 when
   the 75th percentile of their triangulation angles is below 1 degree
   (colmap2mvsnet_acm.py:281-303, 428-435);
-* images/%08d.<ext>: the image itself.  Only binary PGM / PPM inputs are
-  copied (this build has no JPEG / PNG codec; the reference re-encodes every
-  image as JPEG with OpenCV).
+* images/%08d.<ext>: the image itself.  JPEG, PGM and PPM files are copied
+  unchanged (the reference copies JPEG files and re-encodes any other format
+  as JPEG with OpenCV); other formats are decoded with Pillow and written
+  losslessly as PGM / PPM.
 """
 from __future__ import annotations
 
@@ -306,12 +307,30 @@ def convert(dense_folder, save_folder, max_d=192, interval_scale=1.0, model_ext=
         for i, sel in enumerate(views):
             fh.write(f"{i}\n{len(sel)} " + "".join(f"{j} {int(s)} " for j, s in sel) + "\n")
     for i, im in enumerate(imgs):
-        src = os.path.join(dense_folder, "images", im.name)
-        ext = os.path.splitext(im.name)[1].lower()
-        if ext not in (".pgm", ".ppm"):
-            raise ValueError(f"{src}: only binary PGM / PPM images can be copied (no JPEG / PNG codec)")
-        shutil.copyfile(src, os.path.join(img_dir, f"{i:08d}{ext}"))
+        _copy_image(os.path.join(dense_folder, "images", im.name), img_dir, i)
     return n
+
+
+def _copy_image(src, img_dir, i):
+    """images/%08d.<ext> of output index i.  JPEG files are copied unchanged,
+    as colmap2mvsnet_acm.py:450-451 does, and so are binary PGM / PPM files;
+    the reference re-encodes any other format as JPEG (:447-449), here it is
+    decoded with Pillow, when that is importable, and written losslessly as
+    PGM (grey) or PPM (colour)."""
+    ext = os.path.splitext(src)[1].lower()
+    if ext in (".jpg", ".jpeg"):
+        shutil.copyfile(src, os.path.join(img_dir, f"{i:08d}.jpg"))
+        return
+    if ext in (".pgm", ".ppm"):
+        shutil.copyfile(src, os.path.join(img_dir, f"{i:08d}{ext}"))
+        return
+    try:
+        from PIL import Image
+    except ImportError as e:
+        raise ValueError(f"{src}: only JPEG, PGM and PPM images can be copied without Pillow") from e
+    with Image.open(src) as img:
+        grey = img.mode in ("L", "I;16", "I", "1")
+        img.convert("L" if grey else "RGB").save(os.path.join(img_dir, f"{i:08d}.{'pgm' if grey else 'ppm'}"))
 
 
 def main(argv=None):

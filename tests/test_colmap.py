@@ -246,3 +246,25 @@ def test_inverse_depth_steps(workspace, tmp_path):
     expect = (1 / dmin - 1 / dmax) / (1 / dmin - 1 / (dmin + s))
     assert abs(num - expect) <= 1e-5 * expect + 1e-6
     assert abs(interval - (dmax - dmin) / (num - 1)) <= 1e-5 * interval
+
+
+def test_image_formats_are_copied_or_converted(tmp_path):
+    """JPEG, PGM and PPM files are copied byte for byte (JPEG as %08d.jpg, as
+    colmap2mvsnet_acm.py:450-451 copies them); a PNG is decoded and written
+    losslessly as PPM."""
+    Image = pytest.importorskip("PIL.Image")
+    rng = np.random.default_rng(0)
+    rgb = rng.integers(0, 255, (12, 16, 3), dtype=np.uint8)
+    src = tmp_path / "src"
+    out = tmp_path / "out"
+    src.mkdir()
+    out.mkdir()
+    Image.fromarray(rgb).save(src / "a.JPG", format="JPEG", quality=90)
+    Image.fromarray(rgb).save(src / "b.png")
+    (src / "c.pgm").write_bytes(b"P5\n2 1\n255\n\x01\x02")
+    for i, name in enumerate(["a.JPG", "b.png", "c.pgm"]):
+        colmap._copy_image(str(src / name), str(out), i)
+    assert (out / "00000000.jpg").read_bytes() == (src / "a.JPG").read_bytes()
+    ppm = (out / "00000001.ppm").read_bytes()
+    assert ppm.startswith(b"P6") and ppm.endswith(rgb.tobytes())
+    assert (out / "00000002.pgm").read_bytes() == (src / "c.pgm").read_bytes()
