@@ -11,10 +11,13 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
 #include <exception>
 #include <thread>
 #include <cmath>
 #include <mutex>
+#include <unordered_map>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -676,11 +679,148 @@ const MapU8& SceneImages::Current(int id) {
   return have_cur_[id] ? cur_[id] : Original(id);
 }
 
-static MapF LoadDmb(const std::string& path) {
-  MapF m;
-  if (!ReadDmb(path, m)) throw Error("cannot read " + path);
+// ------------------------------------------------------------- map store
+//
+// During a pipeline run every .dmb the passes write is also kept in memory,
+// and later passes read it from there: a pass reads many maps the passes
+// before it wrote (a geometric view reads the depth maps of all its
+// sources).  The files are still written, by a background thread, in the
+// order the maps were produced; a map replaced before its file was written
+// is written once, in its last version.  The run waits for the writer before
+// it returns, so the files are those the passes would have written.  Outside
+// a run (single passes through the C ABI) maps are read from and written to
+// the files directly.
+namespace {
+class MapStore {
+ public:
+  MapStore() : writer_([this] { run(); }) {}
+  ~MapStore() {
+    {
+      std::lock_guard<std::mutex> l(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    writer_.join();
+  }
+  MapRef get(const std::string& path) {
+    std::lock_guard<std::mutex> l(mu_);
+    auto it = maps_.find(path);
+    return it == maps_.end() ? nullptr : it->second;
+  }
+  void put(const std::string& path, MapRef m) {
+    {
+      std::lock_guard<std::mutex> l(mu_);
+      maps_[path] = m;
+      auto it = pending_.find(path);
+      if (it != pending_.end()) {
+        it->second = std::move(m);  // still queued: write the last version only
+      } else {
+        pending_.emplace(path, std::move(m));
+        order_.push_back(path);
+      }
+    }
+    cv_.notify_all();
+  }
+  // A map read from its file: kept for later reads, not written back.
+  void put_read(const std::string& path, MapRef m) {
+    std::lock_guard<std::mutex> l(mu_);
+    maps_.emplace(path, std::move(m));
+  }
+  // Waits until every queued map is on disk; throws on a failed write.
+  void flush() {
+    std::unique_lock<std::mutex> l(mu_);
+    cv_.wait(l, [this] { return order_.empty() && !busy_; });
+    if (!error_.empty()) {
+      const std::string e = error_;
+      error_.clear();
+      throw Error(e);
+    }
+  }
+
+ private:
+  void run() {
+    std::unique_lock<std::mutex> l(mu_);
+    for (;;) {
+      cv_.wait(l, [this] { return stop_ || !order_.empty(); });
+      if (order_.empty()) return;  // stop requested and nothing left
+      const std::string path = order_.front();
+      order_.pop_front();
+      MapRef m = std::move(pending_[path]);
+      pending_.erase(path);
+      busy_ = true;
+      l.unlock();
+      const bool ok = WriteDmb(path, *m);
+      m.reset();
+      l.lock();
+      busy_ = false;
+      if (!ok && error_.empty()) error_ = "cannot write " + path;
+      cv_.notify_all();
+    }
+  }
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::unordered_map<std::string, MapRef> maps_, pending_;
+  std::deque<std::string> order_;
+  bool busy_ = false, stop_ = false;
+  std::string error_;
+  std::thread writer_;
+};
+
+std::mutex g_store_mu;
+std::shared_ptr<MapStore> g_store;
+
+std::shared_ptr<MapStore> active_store() {
+  std::lock_guard<std::mutex> l(g_store_mu);
+  return g_store;
+}
+}  // namespace
+
+MapStoreScope::MapStoreScope() {
+  std::lock_guard<std::mutex> l(g_store_mu);
+  if (!g_store) {
+    g_store = std::make_shared<MapStore>();
+    owner_ = true;
+  }
+}
+
+void MapStoreScope::Finish() {
+  if (!owner_) return;
+  std::shared_ptr<MapStore> s = active_store();
+  if (s) s->flush();
+}
+
+MapStoreScope::~MapStoreScope() {
+  if (!owner_) return;
+  std::shared_ptr<MapStore> s;
+  {
+    std::lock_guard<std::mutex> l(g_store_mu);
+    s = std::move(g_store);
+    g_store.reset();
+  }
+  // the destructor writes what is still queued before it returns
+}
+
+MapRef LoadMap(const std::string& path) {
+  std::shared_ptr<MapStore> s = active_store();
+  if (s) {
+    if (MapRef m = s->get(path)) return m;
+  }
+  auto m = std::make_shared<MapF>();
+  if (!ReadDmb(path, *m)) throw Error("cannot read " + path);
+  if (s) s->put_read(path, m);
   return m;
 }
+
+void SaveMap(const std::string& path, MapF&& m) {
+  std::shared_ptr<MapStore> s = active_store();
+  if (!s) {
+    if (!WriteDmb(path, m)) throw Error("cannot write " + path);
+    return;
+  }
+  s->put(path, std::make_shared<const MapF>(std::move(m)));
+}
+
+
 
 HPM::HPM() {
   hpm_default_params(&params_);
@@ -737,7 +877,7 @@ void HPM::InuputInitialization(const std::string& dense_folder, const std::vecto
   }
   if (params_.geom_consistency) {  // HPM.cpp:693-732
     const char* suffix = multi_geometry_ ? "/depths_geom.dmb" : "/depths.dmb";
-    for (int id : ids) depths_.push_back(LoadDmb(ResultFolder(dense_folder, id) + suffix));
+    for (int id : ids) depths_.push_back(LoadMap(ResultFolder(dense_folder, id) + suffix));
   }
 }
 
@@ -752,17 +892,18 @@ void HPM::CudaSpaceInitialization(const std::string& dense_folder, const Problem
   }
   if (params_.geom_consistency) {
     std::vector<const float*> dp;
-    for (const MapF& m : depths_) dp.push_back(m.data.data());
+    for (const MapRef& m : depths_) dp.push_back(m->data.data());
     check(hpm_set_src_depths(ctx_, dp.data()), "hpm_set_src_depths");
     // reload of the reference's own maps, HPM.cpp:813-846
     const std::string rf = ResultFolder(dense_folder, problem.ref_image_id);
-    CudaHypothesesReload(depths_[0], LoadDmb(rf + "/costs.dmb"), LoadDmb(rf + "/normals.dmb"));
+    CudaHypothesesReload(*depths_[0], *LoadMap(rf + "/costs.dmb"), *LoadMap(rf + "/normals.dmb"));
   }
   if (params_.hierarchy) {  // HPM.cpp:849-905
     const std::string rf = ResultFolder(dense_folder, problem.ref_image_id);
-    const MapF fine_depth = LoadDmb(rf + "/depths.dmb");  // the JBU-upsampled depth
-    const MapF cn = LoadDmb(rf + "/normals.dmb");         // the coarser level's maps
-    const MapF cc = LoadDmb(rf + "/costs.dmb");
+    const MapRef fine_ref = LoadMap(rf + "/depths.dmb");  // the JBU-upsampled depth
+    const MapRef cn_ref = LoadMap(rf + "/normals.dmb");   // the coarser level's maps
+    const MapRef cc_ref = LoadMap(rf + "/costs.dmb");
+    const MapF &fine_depth = *fine_ref, &cn = *cn_ref, &cc = *cc_ref;
     const int cw = cn.width, ch = cn.height;
     if (fine_depth.width != GetReferenceImageWidth() || fine_depth.height != GetReferenceImageHeight())
       throw Error("hierarchy pass: depths.dmb does not have the reference image size");
@@ -785,8 +926,8 @@ void HPM::CudaCannyInitialization(const MapU8& canny) {
 }
 
 void HPM::CudaConfidenceInitialization(const std::string& dense_folder, const std::vector<Problem>& problems, int idx) {
-  const MapF conf = LoadDmb(ResultFolder(dense_folder, problems.at(idx).ref_image_id) + "/confidence.dmb");
-  check(hpm_set_confidence(ctx_, conf.data.data()), "hpm_set_confidence");
+  const MapRef conf = LoadMap(ResultFolder(dense_folder, problems.at(idx).ref_image_id) + "/confidence.dmb");
+  check(hpm_set_confidence(ctx_, conf->data.data()), "hpm_set_confidence");
 }
 
 void HPM::CudaHypothesesReload(const MapF& depths, const MapF& costs, const MapF& normals) {
@@ -1156,20 +1297,21 @@ void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>&
   t_edges += clk.lap(kEdges);
   if (flags.mand_consistency || flags.prior_consistency) hpm.CudaConfidenceInitialization(dense_folder, problems, idx);
 
-  MapF depths(width, height), normals(width, height, 3), costs(width, height), texture(width, height);
   bool run = true;
+  MapRef in_depths, in_normals, in_costs;  // a prior pass's loaded maps
   if (flags.prior_consistency) {  // main.cpp:324-597
     const char* suffix = flags.multi_geometry ? "/depths_geom.dmb" : "/depths.dmb";
-    depths = LoadDmb(rf + suffix);
-    normals = LoadDmb(rf + "/normals.dmb");
-    const MapF confidences = LoadDmb(rf + "/confidence.dmb");
-    costs = LoadDmb(rf + "/costs.dmb");
+    in_depths = LoadMap(rf + suffix);
+    in_normals = LoadMap(rf + "/normals.dmb");
+    const MapRef conf_ref = LoadMap(rf + "/confidence.dmb");
+    in_costs = LoadMap(rf + "/costs.dmb");
+    const MapF &depths = *in_depths, &normals = *in_normals, &costs = *in_costs, &confidences = *conf_ref;
     if (depths.width != width || depths.height != height) throw Error("prior pass: maps of another size in " + rf);
     // the prior texture map must exist (main.cpp:353-355, 458-460); its
     // texture flag is never used by the support-point selection
     const int tex_level = flags.image_scale + flags.hpm_scale_distance;
     if (flags.hpm_scale_distance >= 0 && flags.hpm_scale_distance <= 2)
-      (void)LoadDmb(rf + "/texture" + std::to_string(tex_level) + ".dmb");
+      (void)LoadMap(rf + "/texture" + std::to_string(tex_level) + ".dmb");
     t_setup += clk.lap(kSetup);
     // any distance but 0, 1, 2 runs nothing: the loaded maps are written back
     // unchanged, as main.cpp:349-597 does when neither branch is taken
@@ -1189,8 +1331,11 @@ void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>&
   } else {
     t_setup += clk.lap(kSetup);
   }
+  MapF depths, normals, costs;
   if (run) {
     hpm.RunPatchMatch(seed);
+    depths = MapF(width, height);
+    normals = MapF(width, height, 3);
     std::vector<float> planes;
     hpm.GetPlanes(planes);
     for (size_t i = 0; i < (size_t)width * height; ++i) {
@@ -1201,14 +1346,19 @@ void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>&
     }
     hpm.GetCosts(costs);
     t_pm += clk.lap(kPatchMatch);
+  } else {  // the loaded maps go back unchanged
+    depths = *in_depths;
+    normals = *in_normals;
+    costs = *in_costs;
   }
   const std::string dpath = rf + (flags.geom_consistency ? "/depths_geom.dmb" : "/depths.dmb");
-  if (!WriteDmb(dpath, depths) || !WriteDmb(rf + "/normals.dmb", normals) || !WriteDmb(rf + "/costs.dmb", costs))
-    throw Error("cannot write results under " + rf);
+  SaveMap(dpath, std::move(depths));
+  SaveMap(rf + "/normals.dmb", std::move(normals));
+  SaveMap(rf + "/costs.dmb", std::move(costs));
   if (!flags.mand_consistency && !flags.prior_consistency) {
+    MapF texture;
     hpm.GetTexture(texture);
-    if (!WriteDmb(rf + "/texture" + std::to_string(flags.image_scale) + ".dmb", texture))
-      throw Error("cannot write texture under " + rf);
+    SaveMap(rf + "/texture" + std::to_string(flags.image_scale) + ".dmb", std::move(texture));
   }
   t_write += clk.lap(kWrite);
   if (g_verbose > 0) {
@@ -1233,7 +1383,8 @@ void JointBilateralUpsampling(const std::string& dense_folder, const Problem& pr
     ~Lap() { c.lap(kJbu); }
   } lap{clk};
   const std::string rf = ResultFolder(dense_folder, problem.ref_image_id);
-  const MapF coarse = LoadDmb(rf + "/depths_geom.dmb");
+  const MapRef coarse_ref = LoadMap(rf + "/depths_geom.dmb");
+  const MapF& coarse = *coarse_ref;
   const MapF& fine = images.CurrentGray(problem.ref_image_id);
   const int scale = std::max(fine.height / coarse.height, fine.width / coarse.width);  // HPM.cpp:1270
   if (scale == 1) return;  // HPM.cpp:1272-1275
@@ -1245,12 +1396,12 @@ void JointBilateralUpsampling(const std::string& dense_folder, const Problem& pr
     for (int j = 0; j < fine.height; ++j)
       if (out.at(i, j) != out.at(i, j))
         out.at(i, j) = coarse.at(std::min(i / 2, coarse.width - 1), std::min(j / 2, coarse.height - 1));
-  if (!WriteDmb(rf + "/depths.dmb", out)) throw Error("cannot write " + rf + "/depths.dmb");
+  SaveMap(rf + "/depths.dmb", std::move(out));
 }
 
 struct ViewMaps {
   std::vector<hpm_camera> cams;
-  std::vector<MapF> depths, normals;
+  std::vector<MapRef> depths, normals;
   std::vector<int32_t> offsets, ids;
 };
 
@@ -1260,14 +1411,14 @@ static ViewMaps LoadViewMaps(const std::string& dense_folder, const std::vector<
   ViewMaps v;
   for (const Problem& p : problems) {
     const std::string rf = ResultFolder(dense_folder, p.ref_image_id);
-    v.depths.push_back(LoadDmb(rf + (geom ? "/depths_geom.dmb" : "/depths.dmb")));
-    v.normals.push_back(LoadDmb(rf + "/normals.dmb"));
+    v.depths.push_back(LoadMap(rf + (geom ? "/depths_geom.dmb" : "/depths.dmb")));
+    v.normals.push_back(LoadMap(rf + "/normals.dmb"));
     std::ostringstream cp;
     cp << dense_folder << "/cams/" << std::setw(8) << std::setfill('0') << p.ref_image_id << "_cam.txt";
     hpm_camera c = ReadCamera(cp.str());
     int iw = 0, ih = 0;
     ImageSize(dense_folder, p.ref_image_id, iw, ih);
-    const MapF& d = v.depths.back();
+    const MapF& d = *v.depths.back();
     if (iw != d.width || ih != d.height) {
       const float sx = d.width / (float)iw, sy = d.height / (float)ih;
       c.K[0] *= sx; c.K[2] *= sx; c.K[4] *= sy; c.K[5] *= sy;
@@ -1311,9 +1462,9 @@ void ConfidenceEvaluation(const std::string& dense_folder, const std::vector<Pro
   std::vector<const float*> dp, np;
   std::vector<float*> cp;
   for (int i = 0; i < n; ++i) {
-    conf.emplace_back(v.depths[i].width, v.depths[i].height);
-    dp.push_back(v.depths[i].data.data());
-    np.push_back(v.normals[i].data.data());
+    conf.emplace_back(v.depths[i]->width, v.depths[i]->height);
+    dp.push_back(v.depths[i]->data.data());
+    np.push_back(v.normals[i]->data.data());
   }
   for (auto& c : conf) cp.push_back(c.data.data());
   int64_t count = 0;
@@ -1321,9 +1472,7 @@ void ConfidenceEvaluation(const std::string& dense_folder, const std::vector<Pro
                                        v.ids.data(), cp.data(), nullptr, nullptr, 0, &count);
   if (rc != HPM_OK) throw Error("hpm_consistency_check (confidence) failed");
   const double t2 = now_s();
-  for (int i = 0; i < n; ++i)
-    if (!WriteDmb(ResultFolder(dense_folder, problems[i].ref_image_id) + "/confidence.dmb", conf[i]))
-      throw Error("cannot write confidence.dmb");
+  for (int i = 0; i < n; ++i) SaveMap(ResultFolder(dense_folder, problems[i].ref_image_id) + "/confidence.dmb", std::move(conf[i]));
   if (g_verbose > 0) {
     std::fprintf(stderr, "[hpm] confidence (%d views): load %.1f ms, check %.1f ms, write %.1f ms\n", n,
                  1e3 * (t1 - t0), 1e3 * (t2 - t1), 1e3 * (now_s() - t2));
@@ -1346,9 +1495,9 @@ Fused FuseViews(const std::string& dense_folder, const std::vector<Problem>& pro
   std::vector<const float*> dp, np;
   size_t cap = 0;
   for (int i = 0; i < n; ++i) {
-    dp.push_back(v.depths[i].data.data());
-    np.push_back(v.normals[i].data.data());
-    cap += v.depths[i].data.size();
+    dp.push_back(v.depths[i]->data.data());
+    np.push_back(v.normals[i]->data.data());
+    cap += v.depths[i]->data.size();
   }
   // room for every pixel, left uninitialised: only the fused ones are written
   std::unique_ptr<int32_t[]> pairs(new int32_t[2 * std::max<size_t>(cap, 1)]);
@@ -1363,11 +1512,11 @@ Fused FuseViews(const std::string& dense_folder, const std::vector<Problem>& pro
   Fused F;
   for (int i = 0; i < n; ++i) {
     MapU8 img = LoadImage(dense_folder, problems[i].ref_image_id);
-    if (img.width != v.depths[i].width || img.height != v.depths[i].height)
-      img = ResizeLinear(img, v.depths[i].width, v.depths[i].height);
+    if (img.width != v.depths[i]->width || img.height != v.depths[i]->height)
+      img = ResizeLinear(img, v.depths[i]->width, v.depths[i]->height);
     imgs.push_back(std::move(img));
-    F.width.push_back(v.depths[i].width);
-    F.height.push_back(v.depths[i].height);
+    F.width.push_back(v.depths[i]->width);
+    F.height.push_back(v.depths[i]->height);
   }
   F.pts.resize((size_t)count);
   F.view.resize((size_t)count);
@@ -1444,6 +1593,7 @@ std::vector<FusedPoint> RunFusionSkyStrict(const std::string& dense_folder, cons
 }
 
 std::vector<FusedPoint> RunPipeline(const std::string& dense_folder, int precision, uint64_t seed) {
+  MapStoreScope store;  // the passes' maps stay in memory for the passes after them
   const std::vector<Problem> problems = ReadPairFile(dense_folder + "/pair.txt");
   const int n = (int)problems.size();
   PassFlags f;
@@ -1463,13 +1613,16 @@ std::vector<FusedPoint> RunPipeline(const std::string& dense_folder, int precisi
     if (order_free(f)) parallel_views(n, one);
     else for (int i = 0; i < n; ++i) one(i);
   }
-  return RunFusion(dense_folder, problems, true);
+  std::vector<FusedPoint> pts = RunFusion(dense_folder, problems, true);
+  store.Finish();
+  return pts;
 }
 
 std::vector<FusedPoint> RunPyramidPipeline(const std::string& dense_folder, int initial_image_scale, int size_bound,
                                            int precision, uint64_t seed, bool mask_fusion) {
   if (initial_image_scale < 1) throw Error("initial image scale must be >= 1");
   if (size_bound < 1) throw Error("size bound must be >= 1");
+  MapStoreScope store;  // the passes' maps stay in memory for the passes after them
   const std::vector<Problem> problems = ReadPairFile(dense_folder + "/pair.txt");
   const int n = (int)problems.size();
   MakeDir(dense_folder + "/HPM_MVS_plusplus");
@@ -1547,7 +1700,10 @@ std::vector<FusedPoint> RunPyramidPipeline(const std::string& dense_folder, int 
       each_view(f, nullptr);
     }
   }
-  return mask_fusion ? RunFusionSkyStrict(dense_folder, problems, true) : RunFusion(dense_folder, problems, true);
+  std::vector<FusedPoint> pts =
+      mask_fusion ? RunFusionSkyStrict(dense_folder, problems, true) : RunFusion(dense_folder, problems, true);
+  store.Finish();
+  return pts;
 }
 
 }  // namespace hpmhost

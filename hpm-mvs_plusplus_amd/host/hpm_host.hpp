@@ -6,6 +6,7 @@
 #pragma once
 #include <cstdint>
 #include <stdexcept>
+#include <memory>
 #include <string>
 #include <utility>
 #include <vector>
@@ -45,6 +46,25 @@ hpm_camera ReadCamera(const std::string& path);
 bool ReadDmb(const std::string& path, MapF& out);
 // writeDepthDmb / writeNormalDmb (HPM.cpp:323-407).
 bool WriteDmb(const std::string& path, const MapF& m);
+
+// Maps of a pipeline run.  While a MapStoreScope is alive, SaveMap keeps the
+// map in memory for LoadMap and queues its file write to a background
+// thread; Finish() waits for the files (and reports a failed write).  With
+// no scope alive, LoadMap reads the file and SaveMap writes it at once.
+using MapRef = std::shared_ptr<const MapF>;
+MapRef LoadMap(const std::string& path);
+void SaveMap(const std::string& path, MapF&& m);
+class MapStoreScope {
+ public:
+  MapStoreScope();
+  ~MapStoreScope();  // writes what is still queued
+  MapStoreScope(const MapStoreScope&) = delete;
+  MapStoreScope& operator=(const MapStoreScope&) = delete;
+  void Finish();
+
+ private:
+  bool owner_ = false;  // false inside another scope, which keeps the store
+};
 // Binary PGM (P5) or PPM (P6), 8 bit.  The reference decodes JPEG with
 // OpenCV; this build reads the uncompressed Netpbm formats.
 bool ReadNetpbm(const std::string& path, MapU8& out);
@@ -231,7 +251,7 @@ class HPM {
   std::vector<hpm_camera> cams_;
   std::vector<const MapF*> images_;  // into owned_images_ or a SceneImages cache
   std::vector<MapF> owned_images_;
-  std::vector<MapF> depths_;
+  std::vector<MapRef> depths_;
   hpm_pool* pool_ = nullptr;  // the level's device images, when every view is there
   std::vector<int32_t> pool_ids_;
 };
