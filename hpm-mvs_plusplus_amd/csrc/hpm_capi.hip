@@ -58,8 +58,8 @@ void free_views(hpm_ctx* c) {
   c->d_texture = nullptr; c->d_sel = nullptr; c->d_masks = nullptr; c->d_canny = nullptr;
   c->d_prior = nullptr; c->d_scaled = nullptr;
   if (c->d_prior_ds) hipFree(c->d_prior_ds);
-  if (c->d_prior_tmp) hipFree(c->d_prior_tmp);
-  c->d_prior_ds = nullptr; c->d_prior_tmp = nullptr; c->prior_tmp_bytes = 0;
+  if (c->d_scratch) hipFree(c->d_scratch);
+  c->d_prior_ds = nullptr; c->d_scratch = nullptr; c->scratch_bytes = 0;
   c->prior_dist = -1; c->prior_w = c->prior_h = 0;
   c->nimg = 0; c->W = c->H = 0;
 }
@@ -572,24 +572,24 @@ DevCam dev_cam(const hpm_camera& c) {
 
 unsigned grid1(size_t n) { return (unsigned)((n + 255) / 256); }
 
-// Grow-only device scratch of the prior construction, carved into 256-byte
-// aligned pieces.
-int prior_scratch(hpm_ctx* ctx, const std::vector<size_t>& sizes, std::vector<char*>& out) {
+// Grow-only device scratch of the context (prior construction, map
+// splitting), carved into 256-byte aligned pieces.
+int ctx_scratch(hpm_ctx* ctx, const std::vector<size_t>& sizes, std::vector<char*>& out) {
   std::vector<size_t> off;
   size_t total = 0;
   for (size_t b : sizes) {
     off.push_back(total);
     total += (b + 255) & ~(size_t)255;
   }
-  if (total > ctx->prior_tmp_bytes) {
-    if (ctx->d_prior_tmp) HIP_OK(hipFree(ctx->d_prior_tmp));
-    ctx->d_prior_tmp = nullptr;
-    ctx->prior_tmp_bytes = 0;
-    if (hipMalloc(&ctx->d_prior_tmp, total) != hipSuccess) return fail(ctx, HPM_E_NOMEM, "hipMalloc (prior scratch)");
-    ctx->prior_tmp_bytes = total;
+  if (total > ctx->scratch_bytes) {
+    if (ctx->d_scratch) HIP_OK(hipFree(ctx->d_scratch));
+    ctx->d_scratch = nullptr;
+    ctx->scratch_bytes = 0;
+    if (hipMalloc(&ctx->d_scratch, total) != hipSuccess) return fail(ctx, HPM_E_NOMEM, "hipMalloc (scratch)");
+    ctx->scratch_bytes = total;
   }
   out.clear();
-  for (size_t o : off) out.push_back((char*)ctx->d_prior_tmp + o);
+  for (size_t o : off) out.push_back((char*)ctx->d_scratch + o);
   return HPM_OK;
 }
 
@@ -627,7 +627,7 @@ int hpm_prior_support_points(hpm_ctx* ctx, int distance, int32_t* out_xy, int ca
   }
   const size_t nb = (size_t)((scan_w + 4) / 5) * ((scan_h + 4) / 5);
   std::vector<char*> buf;
-  int rc = prior_scratch(ctx, {nb * sizeof(int4)}, buf);
+  int rc = ctx_scratch(ctx, {nb * sizeof(int4)}, buf);
   if (rc) return rc;
   int4* d_sp = (int4*)buf[0];
   hipLaunchKernelGGL(k_support_points, dim3(grid1(nb)), dim3(256), 0, ctx->stream, cost, conf, scan_w, scan_h, d_sp);
@@ -679,9 +679,9 @@ int hpm_prior_build(hpm_ctx* ctx, const int32_t* tris, int n) {
     return rc;
   std::vector<char*> buf;
   const size_t nt = std::max(n, 1);
-  if (distance == 0) rc = prior_scratch(ctx, {nt * sizeof(PriorTri), nt * sizeof(float4)}, buf);
+  if (distance == 0) rc = ctx_scratch(ctx, {nt * sizeof(PriorTri), nt * sizeof(float4)}, buf);
   else
-    rc = prior_scratch(ctx, {nt * sizeof(PriorTri), nt * sizeof(float4), Pc * 4, Pc * 4, Pc * sizeof(float4), P * 4}, buf);
+    rc = ctx_scratch(ctx, {nt * sizeof(PriorTri), nt * sizeof(float4), Pc * 4, Pc * 4, Pc * sizeof(float4), P * 4}, buf);
   if (rc) return rc;
   PriorTri* d_tris = (PriorTri*)buf[0];
   float4* d_tplanes = (float4*)buf[1];
@@ -921,6 +921,41 @@ int hpm_get_costs(hpm_ctx* ctx, float* out) { return download(ctx, out, ctx ? ct
 int hpm_get_texture(hpm_ctx* ctx, float* out) { return download(ctx, out, ctx ? ctx->d_texture : nullptr, ctx ? npix(ctx) * 4 : 0); }
 int hpm_get_selected_views(hpm_ctx* ctx, uint32_t* out) { return download(ctx, out, ctx ? ctx->d_sel : nullptr, ctx ? npix(ctx) * 4 : 0); }
 int hpm_get_confidence(hpm_ctx* ctx, float* out) { return download(ctx, out, ctx ? ctx->d_conf : nullptr, ctx ? npix(ctx) * 4 : 0); }
+
+int hpm_reload_hypotheses_maps(hpm_ctx* ctx, const float* depth, const float* normals, const float* costs) {
+  if (!ctx || !depth || !normals || !costs) return HPM_E_INVALID;
+  if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
+  HIP_OK(hipSetDevice(ctx->device));
+  const size_t P = npix(ctx);
+  std::vector<char*> buf;
+  int rc = ctx_scratch(ctx, {P * sizeof(float), 3 * P * sizeof(float)}, buf);
+  if (rc) return rc;
+  HIP_OK(hipMemcpyAsync(buf[0], depth, P * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  HIP_OK(hipMemcpyAsync(buf[1], normals, 3 * P * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  HIP_OK(hipMemcpyAsync(ctx->d_costs, costs, P * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_join_planes, dim3(grid1(P)), dim3(256), 0, ctx->stream, (const float*)buf[0],
+                     (const float*)buf[1], P, ctx->d_planes);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(ctx->stream));
+  return HPM_OK;
+}
+
+int hpm_get_depth_normals(hpm_ctx* ctx, float* out_depth, float* out_normals) {
+  if (!ctx || !out_depth || !out_normals) return HPM_E_INVALID;
+  if (ctx->nimg == 0 || !ctx->d_planes) return fail(ctx, HPM_E_STATE, "nothing to read yet");
+  HIP_OK(hipSetDevice(ctx->device));
+  const size_t P = npix(ctx);
+  std::vector<char*> buf;
+  int rc = ctx_scratch(ctx, {P * sizeof(float), 3 * P * sizeof(float)}, buf);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_split_planes, dim3(grid1(P)), dim3(256), 0, ctx->stream, ctx->d_planes, P, (float*)buf[0],
+                     (float*)buf[1]);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(out_depth, buf[0], P * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_OK(hipMemcpyAsync(out_normals, buf[1], 3 * P * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_OK(hipStreamSynchronize(ctx->stream));
+  return HPM_OK;
+}
 
 int hpm_get_size(const hpm_ctx* ctx, int* w, int* h, int* n) {
   if (!ctx) return HPM_E_INVALID;

@@ -55,12 +55,12 @@ struct hpm_ctx {
   float4* d_scaled = nullptr;
   int scaled_w = 0, scaled_h = 0;
   // planar prior built on the device: the distance and grid of the support
-  // points last computed, the maps resized to that grid (distance > 0), and
-  // grow-only scratch for the triangles and intermediate maps
+  // points last computed and the maps resized to that grid (distance > 0)
   int prior_dist = -1, prior_w = 0, prior_h = 0;
   float* d_prior_ds = nullptr;
-  void* d_prior_tmp = nullptr;
-  size_t prior_tmp_bytes = 0;
+  // grow-only scratch: the prior's triangles and intermediate maps, split maps
+  void* d_scratch = nullptr;
+  size_t scratch_bytes = 0;
   bool upsample = false;
   hpm::RunConsts* d_consts = nullptr;
   unsigned long long* d_counters = nullptr;

@@ -21,6 +21,9 @@
 //                       (main.cpp:516-546), upsampled by k_jbu_prior
 //   k_prior_reload      the per-pixel reload of main.cpp:548-576 and
 //                       ReloadPlanarPriorInitialization (HPM.cpp:1433-1453)
+//   k_split_planes,     (not the prior) the output planes as a depth map and
+//   k_join_planes       a normal map and back, for hpm_get_depth_normals and
+//                       hpm_reload_hypotheses_maps
 //
 // The Delaunay triangulation between the support points and the triangles
 // stays on the host: cv::Subdiv2D inserts points one at a time and its result
@@ -267,6 +270,28 @@ __global__ __launch_bounds__(256) void k_prior_reload(const float* __restrict__ 
   }
   mask[p] = m;
   prior[p] = out;
+}
+
+// The planes after the output kernel (world normal xyz, depth w) as the
+// host's depth map and 3-channel normal map (main.cpp:311-322).
+__global__ __launch_bounds__(256) void k_split_planes(const float4* __restrict__ planes, size_t n,
+                                                      float* __restrict__ depth, float* __restrict__ normals) {
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const float4 v = planes[p];
+  depth[p] = v.w;
+  normals[3 * p] = v.x;
+  normals[3 * p + 1] = v.y;
+  normals[3 * p + 2] = v.z;
+}
+
+// The inverse, for hpm_reload_hypotheses_maps: (normal xyz, depth) planes
+// from a depth map and a 3-channel normal map (HPM.cpp:942-962).
+__global__ __launch_bounds__(256) void k_join_planes(const float* __restrict__ depth, const float* __restrict__ normals,
+                                                     size_t n, float4* __restrict__ planes) {
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  planes[p] = make_float4(normals[3 * p], normals[3 * p + 1], normals[3 * p + 2], depth[p]);
 }
 
 }  // namespace hpm

@@ -931,15 +931,12 @@ void HPM::CudaConfidenceInitialization(const std::string& dense_folder, const st
 }
 
 void HPM::CudaHypothesesReload(const MapF& depths, const MapF& costs, const MapF& normals) {
-  const size_t n = (size_t)depths.width * depths.height;
-  std::vector<float> planes(4 * n);
-  for (size_t i = 0; i < n; ++i) {
-    planes[4 * i + 0] = normals.data[3 * i + 0];
-    planes[4 * i + 1] = normals.data[3 * i + 1];
-    planes[4 * i + 2] = normals.data[3 * i + 2];
-    planes[4 * i + 3] = depths.data[i];
-  }
-  check(hpm_reload_hypotheses(ctx_, planes.data(), costs.data.data()), "hpm_reload_hypotheses");
+  const int w = GetReferenceImageWidth(), h = GetReferenceImageHeight();
+  for (const MapF* m : {&depths, &costs, &normals})
+    if (m->width != w || m->height != h) throw Error("reload: maps of another size than the reference image");
+  if (normals.channels != 3 || depths.channels != 1 || costs.channels != 1) throw Error("reload: bad map channels");
+  check(hpm_reload_hypotheses_maps(ctx_, depths.data.data(), normals.data.data(), costs.data.data()),
+        "hpm_reload_hypotheses_maps");
 }
 
 void HPM::BuildPlanarPrior(int distance) {
@@ -1002,6 +999,12 @@ void HPM::GetPlanes(std::vector<float>& planes) const {
   planes.resize((size_t)4 * GetReferenceImageWidth() * GetReferenceImageHeight());
   check(hpm_get_planes(ctx_, planes.data()), "hpm_get_planes");
 }
+void HPM::GetDepthNormals(MapF& depths, MapF& normals) const {
+  depths = MapF(GetReferenceImageWidth(), GetReferenceImageHeight());
+  normals = MapF(GetReferenceImageWidth(), GetReferenceImageHeight(), 3);
+  check(hpm_get_depth_normals(ctx_, depths.data.data(), normals.data.data()), "hpm_get_depth_normals");
+}
+
 void HPM::GetCosts(MapF& costs) const {
   costs = MapF(GetReferenceImageWidth(), GetReferenceImageHeight());
   check(hpm_get_costs(ctx_, costs.data.data()), "hpm_get_costs");
@@ -1334,16 +1337,7 @@ void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>&
   MapF depths, normals, costs;
   if (run) {
     hpm.RunPatchMatch(seed);
-    depths = MapF(width, height);
-    normals = MapF(width, height, 3);
-    std::vector<float> planes;
-    hpm.GetPlanes(planes);
-    for (size_t i = 0; i < (size_t)width * height; ++i) {
-      depths.data[i] = planes[4 * i + 3];
-      normals.data[3 * i + 0] = planes[4 * i + 0];
-      normals.data[3 * i + 1] = planes[4 * i + 1];
-      normals.data[3 * i + 2] = planes[4 * i + 2];
-    }
+    hpm.GetDepthNormals(depths, normals);  // main.cpp:311-322
     hpm.GetCosts(costs);
     t_pm += clk.lap(kPatchMatch);
   } else {  // the loaded maps go back unchanged

@@ -228,6 +228,8 @@ class HPM {
   float GetMaxDepth() const;
   // Bulk getters (GetPlaneHypothesis / GetCost / GetTexture for every pixel).
   void GetPlanes(std::vector<float>& planes) const;  // (world normal, depth)
+  // depth map and world normal map of the result (main.cpp:311-322)
+  void GetDepthNormals(MapF& depths, MapF& normals) const;
   void GetCosts(MapF& costs) const;
   void GetTexture(MapF& texture) const;
 

@@ -60,7 +60,8 @@ EXPORTS = [
     "hpm_geom_cost", "hpm_jbu_depth", "hpm_jbu_prior", "hpm_consistency_check", "hpm_set_graph",
     "hpm_run_patchmatch_batch", "hpm_pool_create", "hpm_pool_destroy", "hpm_set_views_pooled",
     "hpm_cost_grad", "hpm_cost_grad_device", "hpm_refine_gn", "hpm_refine_random", "hpm_refine_hybrid",
-    "hpm_refine_gn_smoothed", "hpm_prior_support_points", "hpm_prior_build", "hpm_get_prior", "hpm_canny",
+    "hpm_refine_gn_smoothed", "hpm_prior_support_points", "hpm_prior_build", "hpm_get_prior", "hpm_canny", "hpm_get_depth_normals",
+    "hpm_reload_hypotheses_maps",
     "hpm_batch_create", "hpm_batch_destroy", "hpm_batch_last_error", "hpm_batch_run", "hpm_batch_set_graph",
     "hpm_batch_set_shape", "hpm_batch_get_shape", "hpm_batch_last_ms", "hpm_batch_autotune",
 ]
@@ -97,6 +98,8 @@ def lib():
     L.hpm_prior_support_points.argtypes = [P, ctypes.c_int, i32p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.hpm_prior_build.argtypes = [P, i32p, ctypes.c_int]
     L.hpm_get_prior.argtypes = [P, fp, up]
+    L.hpm_get_depth_normals.argtypes = [P, fp, fp]
+    L.hpm_reload_hypotheses_maps.argtypes = [P, fp, fp, fp]
     u8p = ctypes.POINTER(ctypes.c_uint8)
     L.hpm_canny.argtypes = [ctypes.c_int, u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                             ctypes.c_float, u8p]
@@ -257,6 +260,14 @@ class Context:
         c = np.ascontiguousarray(costs, dtype=np.float32)
         self._check(self._L.hpm_reload_hypotheses(self._h, _fptr(p), _fptr(c)), "hpm_reload_hypotheses")
 
+    def reload_maps(self, depth, normals, costs):
+        """hpm_reload_hypotheses from a depth map, a (h, w, 3) normal map and costs."""
+        d = np.ascontiguousarray(depth, dtype=np.float32)
+        n = np.ascontiguousarray(normals, dtype=np.float32)
+        c = np.ascontiguousarray(costs, dtype=np.float32)
+        self._check(self._L.hpm_reload_hypotheses_maps(self._h, _fptr(d), _fptr(n), _fptr(c)),
+                    "hpm_reload_hypotheses_maps")
+
     def set_prior(self, prior_planes, masks):
         p = np.ascontiguousarray(prior_planes, dtype=np.float32)
         m = np.ascontiguousarray(masks, dtype=np.uint32)
@@ -372,6 +383,13 @@ class Context:
         out = np.empty((self.height, self.width), np.float32)
         self._check(self._L.hpm_get_costs(self._h, _fptr(out)), "hpm_get_costs")
         return out
+
+    def depth_normals(self):
+        """The result as a depth map (h, w) and a world normal map (h, w, 3)."""
+        d = np.empty((self.height, self.width), np.float32)
+        n = np.empty((self.height, self.width, 3), np.float32)
+        self._check(self._L.hpm_get_depth_normals(self._h, _fptr(d), _fptr(n)), "hpm_get_depth_normals")
+        return d, n
 
     def texture(self):
         out = np.empty((self.height, self.width), np.float32)

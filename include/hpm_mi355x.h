@@ -120,6 +120,10 @@ int hpm_set_confidence(hpm_ctx* ctx, const float* confidences);
 /* CudaHypothesesReload (HPM.cpp:942-962) and the geometric-pass reload
  * (HPM.cpp:813-846): planes = (world normal xyz, depth w) per pixel, costs. */
 int hpm_reload_hypotheses(hpm_ctx* ctx, const float* planes, const float* costs);
+/* The same from the maps as ProcessProblem keeps them (depths.dmb /
+ * normals.dmb / costs.dmb): a depth map, a 3-channel world normal map and
+ * the costs; the planes are assembled on the device. */
+int hpm_reload_hypotheses_maps(hpm_ctx* ctx, const float* depth, const float* normals, const float* costs);
 
 /* CudaPlanarPriorInitialization / ReloadPlanarPriorInitialization
  * (HPM.cpp:964-984, 1433-1453): prior plane per pixel and triangle mask
@@ -227,6 +231,9 @@ int hpm_synchronize(hpm_ctx* ctx);
  * (HPM.cpp:1001-1014) and the D2H copies of HPM.cu:3029-3033.  Each copies
  * width*height elements (planes: 4 floats per pixel). */
 int hpm_get_planes(hpm_ctx* ctx, float* out_planes);
+/* The planes as ProcessProblem stores them (main.cpp:311-322): the depth map
+ * (width*height floats) and the world normal map (3 floats per pixel). */
+int hpm_get_depth_normals(hpm_ctx* ctx, float* out_depth, float* out_normals);
 int hpm_get_costs(hpm_ctx* ctx, float* out_costs);
 int hpm_get_texture(hpm_ctx* ctx, float* out_texture);
 int hpm_get_selected_views(hpm_ctx* ctx, uint32_t* out_masks);

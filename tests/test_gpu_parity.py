@@ -107,6 +107,10 @@ def test_full_run_bitexact_and_fp64(wide_scene):
     assert _bits_equal(gp, r["planes"])
     assert _bits_equal(ctx.costs(), r["costs"])
     assert np.array_equal(ctx.selected_views(), r["selected"])
+    # the same planes as ProcessProblem's depth and normal maps
+    dm, nm = ctx.depth_normals()
+    assert _bits_equal(dm, r["planes"][..., 3])
+    assert _bits_equal(nm, r["planes"][..., :3])
     # FP64 oracle: depth within 1 % on >= 99 % of pixels, normals within 5 deg
     r64 = O.run_patchmatch(oc, imgs, prm, SEED, dtype=np.float64)
     d, d64 = gp[..., 3].astype(np.float64), r64["planes"][..., 3]
@@ -139,6 +143,14 @@ def test_geom_pass_bitexact(small_scene):
     assert _bits_equal(ctx.planes(), r["planes"])
     assert _bits_equal(ctx.costs(), r["costs"])
     assert np.array_equal(ctx.selected_views(), r["selected"])
+    # the reload from separate depth / normal maps (as ProcessProblem keeps
+    # them) assembles the same hypotheses
+    ctx2 = _gpu_ctx(cams, imgs, geom_consistency=1)
+    ctx2.set_src_depths(depths)
+    ctx2.reload_maps(base["planes"][..., 3], base["planes"][..., :3], base["costs"])
+    ctx2.run(SEED + 1)
+    assert _bits_equal(ctx2.planes(), r["planes"])
+    assert _bits_equal(ctx2.costs(), r["costs"])
 
 
 def test_geom_cost_bitexact(small_scene):
