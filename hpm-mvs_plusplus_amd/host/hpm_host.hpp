@@ -4,7 +4,11 @@
 // driver with its single-level schedule.  Everything that touches the GPU
 // goes through the C ABI of include/hpm_mi355x.h.
 #pragma once
+#include <sys/mman.h>
+
 #include <cstdint>
+#include <cstdlib>
+#include <new>
 #include <stdexcept>
 #include <memory>
 #include <string>
@@ -22,10 +26,40 @@ struct Problem {  // main.h:49-58 (the fields the pass driver uses)
   std::vector<int> src_image_ids;
 };
 
+// Storage of the per-pixel maps.  Blocks of 4 MiB and more (a 3072 x 2048
+// float map is 24 MiB, and every pass allocates several) are 2 MiB aligned
+// and advised as transparent huge pages, so that a fresh map faults in a few
+// huge pages instead of thousands of small ones.
+template <class T>
+struct MapAllocator {
+  using value_type = T;
+  static constexpr size_t kHuge = (size_t)2 << 20;
+  MapAllocator() = default;
+  template <class U>
+  MapAllocator(const MapAllocator<U>&) {}
+  T* allocate(size_t n) {
+    const size_t bytes = n * sizeof(T);
+    if (bytes < 2 * kHuge) return static_cast<T*>(::operator new(bytes));
+    const size_t rounded = (bytes + kHuge - 1) / kHuge * kHuge;
+    void* p = std::aligned_alloc(kHuge, rounded);
+    if (!p) throw std::bad_alloc();
+    madvise(p, rounded, MADV_HUGEPAGE);  // advice only: ignored where huge pages are off
+    return static_cast<T*>(p);
+  }
+  void deallocate(T* p, size_t n) {
+    if (n * sizeof(T) < 2 * kHuge) ::operator delete(p);
+    else std::free(p);
+  }
+  template <class U>
+  bool operator==(const MapAllocator<U>&) const { return true; }
+  template <class U>
+  bool operator!=(const MapAllocator<U>&) const { return false; }
+};
+
 template <class T>
 struct Map {  // row-major image / per-pixel map
   int width = 0, height = 0, channels = 1;
-  std::vector<T> data;
+  std::vector<T, MapAllocator<T>> data;
   Map() = default;
   Map(int w, int h, int c = 1, T v = T()) : width(w), height(h), channels(c), data((size_t)w * h * c, v) {}
   T& at(int x, int y, int c = 0) { return data[((size_t)y * width + x) * channels + c]; }

@@ -650,7 +650,7 @@ void Decoder::idct_and_upsample(MapU8& out) {
   }
   if (comp_.size() == 1) {
     out = MapU8(W_, H_, 1);
-    out.data = std::move(planes[0]);
+    out.data.assign(planes[0].begin(), planes[0].end());
     return;
   }
   // YCbCr -> RGB with 16-bit fixed-point BT.601 tables
