@@ -922,6 +922,21 @@ int hpm_get_texture(hpm_ctx* ctx, float* out) { return download(ctx, out, ctx ? 
 int hpm_get_selected_views(hpm_ctx* ctx, uint32_t* out) { return download(ctx, out, ctx ? ctx->d_sel : nullptr, ctx ? npix(ctx) * 4 : 0); }
 int hpm_get_confidence(hpm_ctx* ctx, float* out) { return download(ctx, out, ctx ? ctx->d_conf : nullptr, ctx ? npix(ctx) * 4 : 0); }
 
+int hpm_set_canny_u8(hpm_ctx* ctx, const uint8_t* canny) {
+  if (!ctx || !canny) return HPM_E_INVALID;
+  if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
+  HIP_OK(hipSetDevice(ctx->device));
+  const size_t P = npix(ctx);
+  std::vector<char*> buf;
+  int rc = ctx_scratch(ctx, {P}, buf);
+  if (rc) return rc;
+  HIP_OK(hipMemcpyAsync(buf[0], canny, P, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_widen_edges, dim3(grid1(P)), dim3(256), 0, ctx->stream, (const uint8_t*)buf[0], P, ctx->d_canny);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(ctx->stream));
+  return HPM_OK;
+}
+
 int hpm_reload_hypotheses_maps(hpm_ctx* ctx, const float* depth, const float* normals, const float* costs) {
   if (!ctx || !depth || !normals || !costs) return HPM_E_INVALID;
   if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");

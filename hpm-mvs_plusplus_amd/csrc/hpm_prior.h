@@ -24,6 +24,7 @@
 //   k_split_planes,     (not the prior) the output planes as a depth map and
 //   k_join_planes       a normal map and back, for hpm_get_depth_normals and
 //                       hpm_reload_hypotheses_maps
+//   k_widen_edges       (not the prior) 8-bit edge map to 32 bits
 //
 // The Delaunay triangulation between the support points and the triangles
 // stays on the host: cv::Subdiv2D inserts points one at a time and its result
@@ -292,6 +293,13 @@ __global__ __launch_bounds__(256) void k_join_planes(const float* __restrict__ d
   const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   planes[p] = make_float4(normals[3 * p], normals[3 * p + 1], normals[3 * p + 2], depth[p]);
+}
+
+// An 8-bit edge map widened to the 32-bit map the kernels read, for
+// hpm_set_canny_u8.
+__global__ __launch_bounds__(256) void k_widen_edges(const uint8_t* __restrict__ in, size_t n, uint32_t* __restrict__ out) {
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) out[p] = in[p];
 }
 
 }  // namespace hpm

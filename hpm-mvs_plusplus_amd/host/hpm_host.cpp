@@ -269,7 +269,7 @@ struct Delaunay {
   struct V { int64_t x, y; };
   struct T { int v[3]; int nb[3]; bool alive; };  // nb[i] is across the edge opposite v[i]
   std::vector<V> vs;
-  std::vector<T> ts;
+  std::vector<T, MapAllocator<T>> ts;  // about 5 per point; dead ones stay (their order is the output order)
   int last = 0;
   // per-insert scratch kept across inserts (reset by stamp or by the entries
   // touched), so that an insert costs the size of its cavity, not of the mesh
@@ -277,14 +277,28 @@ struct Delaunay {
   uint32_t stamp = 0;
   std::vector<int> vstart, vend;  // per vertex: new triangle starting / ending there
   std::vector<int> cavity, stack, created;
+  struct E { int a, b, out; };
+  std::vector<E> boundary;
 
   static int64_t orient(const V& a, const V& b, const V& c) {
     return (b.x - a.x) * (c.y - a.y) - (b.y - a.y) * (c.x - a.x);
   }
-  // > 0 when d lies strictly inside the circumcircle of the CCW triangle abc
+  // > 0 when d lies strictly inside the circumcircle of the CCW triangle abc.
+  // Exact: with every coordinate difference below 2^14 each term of the
+  // determinant is below 2^59 and 64-bit integers suffice; tests that reach
+  // the far bounding vertices take 128 bits.
   static int incircle(const V& a, const V& b, const V& c, const V& d) {
-    const __int128 adx = a.x - d.x, ady = a.y - d.y, bdx = b.x - d.x, bdy = b.y - d.y, cdx = c.x - d.x,
-                   cdy = c.y - d.y;
+    const int64_t adx = a.x - d.x, ady = a.y - d.y, bdx = b.x - d.x, bdy = b.y - d.y, cdx = c.x - d.x,
+                  cdy = c.y - d.y;
+    const int64_t lim = (int64_t)1 << 14;
+    if (std::max({std::abs(adx), std::abs(ady), std::abs(bdx), std::abs(bdy), std::abs(cdx), std::abs(cdy)}) < lim) {
+      const int64_t ad = adx * adx + ady * ady, bd = bdx * bdx + bdy * bdy, cd = cdx * cdx + cdy * cdy;
+      const int64_t det = adx * (bdy * cd - bd * cdy) - ady * (bdx * cd - bd * cdx) + ad * (bdx * cdy - bdy * cdx);
+      return det > 0 ? 1 : (det < 0 ? -1 : 0);
+    }
+    return incircle128(adx, ady, bdx, bdy, cdx, cdy);
+  }
+  static int incircle128(__int128 adx, __int128 ady, __int128 bdx, __int128 bdy, __int128 cdx, __int128 cdy) {
     const __int128 ad = adx * adx + ady * ady, bd = bdx * bdx + bdy * bdy, cd = cdx * cdx + cdy * cdy;
     const __int128 det = adx * (bdy * cd - bd * cdy) - ady * (bdx * cd - bd * cdx) + ad * (bdx * cdy - bdy * cdx);
     return det > 0 ? 1 : (det < 0 ? -1 : 0);
@@ -361,8 +375,7 @@ struct Delaunay {
     }
     // boundary edges (a, b) in CCW order of their cavity triangle, with the
     // outside neighbour; each becomes the triangle (a, b, p)
-    struct E { int a, b, out; };
-    std::vector<E> boundary;
+    boundary.clear();
     for (int t : cavity)
       for (int k = 0; k < 3; ++k) {
         const int n = ts[t].nb[k];
@@ -399,6 +412,13 @@ std::vector<Triangle> DelaunayTriangulation(int width, int height, const std::ve
   std::vector<Triangle> out;
   if (points.empty()) return out;
   Delaunay d(width, height);
+  // room for the triangles an incremental build allocates (about 5 per
+  // point), so that the arrays are not copied while they grow
+  d.ts.reserve(5 * points.size() + 64);
+  d.mark.resize(5 * points.size() + 64, 0);
+  d.vs.reserve(points.size() + 3);
+  d.vstart.reserve(points.size() + 3);
+  d.vend.reserve(points.size() + 3);
   for (const Point2i& p : points)
     if (p.x >= 0 && p.x < width && p.y >= 0 && p.y < height) d.insert(2 * (int64_t)p.x, 2 * (int64_t)p.y);
   for (const auto& t : d.ts) {
@@ -921,8 +941,9 @@ void HPM::CudaSpaceInitialization(const std::string& dense_folder, const Problem
 }
 
 void HPM::CudaCannyInitialization(const MapU8& canny) {
-  std::vector<uint32_t> c(canny.data.begin(), canny.data.end());
-  check(hpm_set_canny(ctx_, c.data()), "hpm_set_canny");
+  if (canny.width != GetReferenceImageWidth() || canny.height != GetReferenceImageHeight() || canny.channels != 1)
+    throw Error("edge map of another size than the reference image");
+  check(hpm_set_canny_u8(ctx_, canny.data.data()), "hpm_set_canny_u8");
 }
 
 void HPM::CudaConfidenceInitialization(const std::string& dense_folder, const std::vector<Problem>& problems, int idx) {
@@ -1335,8 +1356,11 @@ void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>&
     t_setup += clk.lap(kSetup);
   }
   MapF depths, normals, costs;
+  double t_run = 0;
   if (run) {
+    const double r0 = now_s();
     hpm.RunPatchMatch(seed);
+    t_run = now_s() - r0;
     hpm.GetDepthNormals(depths, normals);  // main.cpp:311-322
     hpm.GetCosts(costs);
     t_pm += clk.lap(kPatchMatch);
@@ -1362,10 +1386,12 @@ void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>&
                        : flags.geom_consistency ? "geometric"
                                                 : "photometric";
     std::fprintf(stderr,
-                 "[hpm] view %08d %dx%d %s%s: setup %.1f ms, edges %.1f, prior %.1f, patchmatch %.1f, write %.1f\n",
+                 "[hpm] view %08d %dx%d %s%s: setup %.1f ms, edges %.1f, prior %.1f, patchmatch %.1f (run %.1f, "
+                 "read back %.1f), write %.1f\n",
                  problem.ref_image_id, width, height, mode,
                  flags.prior_consistency ? (" d=" + std::to_string(flags.hpm_scale_distance)).c_str() : "",
-                 1e3 * t_setup, 1e3 * t_edges, 1e3 * t_prior, 1e3 * t_pm, 1e3 * t_write);
+                 1e3 * t_setup, 1e3 * t_edges, 1e3 * t_prior, 1e3 * t_pm, 1e3 * t_run, 1e3 * (t_pm - t_run),
+                 1e3 * t_write);
     std::fflush(stderr);
   }
 }

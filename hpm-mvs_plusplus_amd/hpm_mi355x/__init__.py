@@ -61,7 +61,7 @@ EXPORTS = [
     "hpm_run_patchmatch_batch", "hpm_pool_create", "hpm_pool_destroy", "hpm_set_views_pooled",
     "hpm_cost_grad", "hpm_cost_grad_device", "hpm_refine_gn", "hpm_refine_random", "hpm_refine_hybrid",
     "hpm_refine_gn_smoothed", "hpm_prior_support_points", "hpm_prior_build", "hpm_get_prior", "hpm_canny", "hpm_get_depth_normals",
-    "hpm_reload_hypotheses_maps",
+    "hpm_reload_hypotheses_maps", "hpm_set_canny_u8",
     "hpm_batch_create", "hpm_batch_destroy", "hpm_batch_last_error", "hpm_batch_run", "hpm_batch_set_graph",
     "hpm_batch_set_shape", "hpm_batch_get_shape", "hpm_batch_last_ms", "hpm_batch_autotune",
 ]
@@ -100,6 +100,7 @@ def lib():
     L.hpm_get_prior.argtypes = [P, fp, up]
     L.hpm_get_depth_normals.argtypes = [P, fp, fp]
     L.hpm_reload_hypotheses_maps.argtypes = [P, fp, fp, fp]
+    L.hpm_set_canny_u8.argtypes = [P, ctypes.POINTER(ctypes.c_uint8)]
     u8p = ctypes.POINTER(ctypes.c_uint8)
     L.hpm_canny.argtypes = [ctypes.c_int, u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                             ctypes.c_float, u8p]
@@ -250,6 +251,11 @@ class Context:
     def set_canny(self, canny):
         a = np.ascontiguousarray(canny, dtype=np.uint32)
         self._check(self._L.hpm_set_canny(self._h, _uptr(a)), "hpm_set_canny")
+
+    def set_canny_u8(self, canny):
+        a = np.ascontiguousarray(canny, dtype=np.uint8)
+        self._check(self._L.hpm_set_canny_u8(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))),
+                    "hpm_set_canny_u8")
 
     def set_confidence(self, conf):
         a = np.ascontiguousarray(conf, dtype=np.float32)

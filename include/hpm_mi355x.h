@@ -113,6 +113,8 @@ int hpm_set_src_depths(hpm_ctx* ctx, const float* const* depths);
 /* CudaCannyInitialization (HPM.cpp:908-920): edge map of the reference image,
  * nonzero = edge (cv::Canny output, main.cpp:287). */
 int hpm_set_canny(hpm_ctx* ctx, const uint32_t* canny);
+/* The same from an 8-bit edge map (cv::Canny's own output type). */
+int hpm_set_canny_u8(hpm_ctx* ctx, const uint8_t* canny);
 
 /* CudaConfidenceInitialization (HPM.cpp:922-940). */
 int hpm_set_confidence(hpm_ctx* ctx, const float* confidences);

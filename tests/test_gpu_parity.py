@@ -80,6 +80,12 @@ def test_init_bitexact(small_scene):
     assert _bits_equal(ctx.costs(), r["costs"])
     assert np.array_equal(ctx.selected_views(), r["selected"])
     assert _bits_equal(ctx.texture(), r["texture"])
+    # the same edge map handed over as 8 bits (cv::Canny's type)
+    ctx8 = _gpu_ctx(cams, imgs)
+    ctx8.set_canny_u8((canny * 255).astype(np.uint8))
+    ctx8.run(SEED, n_halfpasses=0, do_output=False)
+    assert _bits_equal(ctx8.texture(), r["texture"])
+    assert _bits_equal(ctx8.planes(), r["planes"])
 
 
 @pytest.mark.parametrize("n_half", [1, 2])
