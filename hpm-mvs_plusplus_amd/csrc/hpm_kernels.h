@@ -662,18 +662,21 @@ template <class E>
 __device__ __forceinline__ void propagate_pixel_mand(const RunConsts& C, const DevState& S, int px, int py, int iter, const typename E::Win& rw,
                                      Sample& s, unsigned& nncc, unsigned& n_ext) {
   const DevCam& cam = C.cams[0];
-  const int V = C.V;
   const int center = py * C.W + px;
   Philox rng(C.seed, (uint32_t)center, (uint32_t)(1 + iter));
   sample_neighbours<E>(C, S, px, py, iter, rw, s, nncc, n_ext);
   ViewWeights vw; uint32_t tsel; float wn; int nsel;
   view_selection(C, S, px, py, iter, s, rng, vw, tsel, wn, nsel);
+  // Both branches of the reference compute the 8 direction costs (the low-
+  // confidence branch then rescales them) and the current plane's cost:
+  // computed once, before the branch, so that a wave holding pixels of both
+  // kinds does not evaluate these NCCs twice.  The operations are the same.
+  float fc[8];
+  final_costs(C, S, px, py, s, vw, wn, fc);
+  float cost_now = current_cost<E>(C, px, py, S.planes[center], vw, wn, rw, nncc);
+  S.costs[center] = cost_now;
   if (S.conf[center] >= 0.3f) {
-    float fc[8];
-    final_costs(C, S, px, py, s, vw, wn, fc);
     const int min_idx = find_min_idx(fc);
-    float cost_now = current_cost<E>(C, px, py, S.planes[center], vw, wn, rw, nncc);
-    S.costs[center] = cost_now;
     float depth_now = depth_from_plane(cam, S.planes[center], px, py);
     float4 plane_now = S.planes[center];
     if (s.flag[min_idx]) {
@@ -691,21 +694,9 @@ __device__ __forceinline__ void propagate_pixel_mand(const RunConsts& C, const D
     S.planes[center] = plane_now;
     return;
   }
-  float fc[8], rfc[8];
+  float rfc[8];
   for (int i = 0; i < 8; i++) {
-    float f = 0.0f;
-    for (int j = 0; j < V; ++j) {
-      if (vw.on(j)) {
-        const float c = s.ca[s.cs * (i * V + j)];
-        if (C.geom) {
-          if (s.flag[i]) f += vw.w(j) * (c + 0.2f * geom_cost(C, j + 1, S.planes[s.pos[i]], px, py));
-          else f += vw.w(j) * (c + 0.1f * 3.0f);
-        } else {
-          f += vw.w(j) * c;
-        }
-      }
-    }
-    f /= wn;
+    float f = fc[i];
     rfc[i] = 0.0f;
     if (s.flag[i]) {
       const float gce = expf_hpm(-f * f / 0.18f);
@@ -719,8 +710,6 @@ __device__ __forceinline__ void propagate_pixel_mand(const RunConsts& C, const D
     fc[i] = f;
   }
   const int max_idx = find_max_idx(rfc);
-  float cost_now = current_cost<E>(C, px, py, S.planes[center], vw, wn, rw, nncc);
-  S.costs[center] = cost_now;
   const float gce = expf_hpm(-cost_now * cost_now / 0.18f);
   const float mmc = (float)nsel * 0.5f;
   const float cc = S.conf[center];
