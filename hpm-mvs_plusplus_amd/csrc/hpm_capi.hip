@@ -2,6 +2,9 @@
 // per-run constant block and kernel launches on one HIP stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <mutex>
+#include <cstdint>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -34,31 +37,49 @@ int fail(hpm_ctx* c, int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(ctx, HPM_E_DEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
   } while (0)
 
+// Context buffers are allocated and freed in the order of the context's
+// stream: unlike hipFree, hipFreeAsync does not wait for the whole device,
+// so contexts driven from other host threads keep their work running.
 template <class T> int dalloc(hpm_ctx* ctx, T** p, size_t n) {
-  if (*p) { hipFree(*p); *p = nullptr; }
+  if (*p) { hipFreeAsync(*p, ctx->stream); *p = nullptr; }
   if (n == 0) return HPM_OK;
-  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
-  if (e != hipSuccess) { *p = nullptr; return fail(ctx, HPM_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)); }
+  hipError_t e = hipMallocAsync((void**)p, n * sizeof(T), ctx->stream);
+  if (e != hipSuccess) { *p = nullptr; return fail(ctx, HPM_E_NOMEM, std::string("hipMallocAsync: ") + hipGetErrorString(e)); }
   return HPM_OK;
+}
+
+// The device's default memory pool keeps what contexts free for the next
+// ones instead of returning it to the driver at every synchronisation.
+void keep_pool_memory(int device) {
+  static std::mutex mu;
+  static std::vector<int> done;
+  std::lock_guard<std::mutex> lock(mu);
+  if (std::find(done.begin(), done.end(), device) != done.end()) return;
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+    uint64_t threshold = UINT64_MAX;
+    hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
+  }
+  done.push_back(device);
 }
 
 void free_views(hpm_ctx* c) {
   if (!c->pool) {
-    for (float* p : c->d_img) hipFree(p);
-    for (void* p : c->d_img_lp) hipFree(p);
+    for (float* p : c->d_img) if (p) hipFreeAsync(p, c->stream);
+    for (void* p : c->d_img_lp) if (p) hipFreeAsync(p, c->stream);
   }
   c->pool = nullptr;
   c->pool_ids.clear();
-  for (float* p : c->d_dimg) hipFree(p);
+  for (float* p : c->d_dimg) if (p) hipFreeAsync(p, c->stream);
   c->d_img.clear(); c->d_dimg.clear(); c->d_img_lp.clear(); c->lp_kind = -1;
   void* bufs[] = {c->d_planes, c->d_costs, c->d_pre_costs, c->d_conf, c->d_texture, c->d_sel,
                   c->d_masks,  c->d_canny, c->d_prior,     c->d_scaled};
-  for (void* p : bufs) if (p) hipFree(p);
+  for (void* p : bufs) if (p) hipFreeAsync(p, c->stream);
   c->d_planes = nullptr; c->d_costs = nullptr; c->d_pre_costs = nullptr; c->d_conf = nullptr;
   c->d_texture = nullptr; c->d_sel = nullptr; c->d_masks = nullptr; c->d_canny = nullptr;
   c->d_prior = nullptr; c->d_scaled = nullptr;
-  if (c->d_prior_ds) hipFree(c->d_prior_ds);
-  if (c->d_scratch) hipFree(c->d_scratch);
+  if (c->d_prior_ds) hipFreeAsync(c->d_prior_ds, c->stream);
+  if (c->d_scratch) hipFreeAsync(c->d_scratch, c->stream);
   c->d_prior_ds = nullptr; c->d_scratch = nullptr; c->scratch_bytes = 0;
   c->prior_dist = -1; c->prior_w = c->prior_h = 0;
   c->nimg = 0; c->W = c->H = 0;
@@ -148,12 +169,12 @@ template <class S> int build_quads(hpm_ctx* ctx) {
     if ((size_t)(ctx->cams[i].width + 1) * (ctx->cams[i].height + 1) >= ((size_t)1 << 24))
       return fail(ctx, HPM_E_INVALID, "the mixed-precision path takes views below 2^24 texels (e.g. 4096 x 4095); "
                                       "use HPM_PREC_FP32_EXACT for larger images");
-  for (void* p : ctx->d_img_lp) hipFree(p);
+  for (void* p : ctx->d_img_lp) if (p) hipFreeAsync(p, ctx->stream);
   ctx->d_img_lp.assign(ctx->nimg, nullptr);
   for (int i = 0; i < ctx->nimg; ++i) {
     const int w = ctx->cams[i].width, h = ctx->cams[i].height;
     const size_t n = (size_t)(w + 1) * (h + 1);
-    hipError_t e = hipMalloc(&ctx->d_img_lp[i], n * sizeof(typename S::Q));
+    hipError_t e = hipMallocAsync(&ctx->d_img_lp[i], n * sizeof(typename S::Q), ctx->stream);
     if (e != hipSuccess) { ctx->d_img_lp[i] = nullptr; return fail(ctx, HPM_E_NOMEM, "quad image allocation failed"); }
     const dim3 b(16, 16), g((w + 1 + 15) / 16, (h + 1 + 15) / 16);
     hipLaunchKernelGGL(k_build_quad<S>, g, b, 0, ctx->stream, ctx->d_img[i], w, h, ctx->d_img_lp[i]);
@@ -356,9 +377,12 @@ int hpm_create(hpm_ctx** out, int device) {
     delete ctx;
     return HPM_E_DEVICE;
   }
-  if (hipMalloc((void**)&ctx->d_consts, sizeof(RunConsts)) != hipSuccess ||
-      hipMalloc((void**)&ctx->d_counters, 8 * sizeof(unsigned long long)) != hipSuccess) {
-    hipFree(ctx->d_consts);
+  keep_pool_memory(device);
+  if (hipMallocAsync((void**)&ctx->d_consts, sizeof(RunConsts), ctx->stream) != hipSuccess ||
+      hipMallocAsync((void**)&ctx->d_counters, 8 * sizeof(unsigned long long), ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    if (ctx->d_consts) hipFreeAsync(ctx->d_consts, ctx->stream);
+    hipStreamSynchronize(ctx->stream);
     hipStreamDestroy(ctx->stream);
     delete ctx;
     return HPM_E_NOMEM;
@@ -374,8 +398,9 @@ void hpm_destroy(hpm_ctx* ctx) {
   hipStreamSynchronize(ctx->stream);
   if (ctx->gexec) hipGraphExecDestroy(ctx->gexec);
   free_views(ctx);
-  hipFree(ctx->d_consts);
-  hipFree(ctx->d_counters);
+  hipFreeAsync(ctx->d_consts, ctx->stream);
+  hipFreeAsync(ctx->d_counters, ctx->stream);
+  hipStreamSynchronize(ctx->stream);  // the frees above; no other stream is waited for
   for (auto& e : ctx->ev) hipEventDestroy(e);
   hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -510,7 +535,7 @@ int hpm_set_src_depths(hpm_ctx* ctx, const float* const* depths) {
   if (!ctx || !depths) return HPM_E_INVALID;
   if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
   HIP_OK(hipSetDevice(ctx->device));
-  for (float* p : ctx->d_dimg) hipFree(p);
+  for (float* p : ctx->d_dimg) if (p) hipFreeAsync(p, ctx->stream);
   ctx->d_dimg.assign(ctx->nimg, nullptr);
   for (int i = 0; i < ctx->nimg; ++i) {
     if (!depths[i]) { if (i == 0) continue; return fail(ctx, HPM_E_INVALID, "null source depth map"); }
@@ -582,10 +607,11 @@ int ctx_scratch(hpm_ctx* ctx, const std::vector<size_t>& sizes, std::vector<char
     total += (b + 255) & ~(size_t)255;
   }
   if (total > ctx->scratch_bytes) {
-    if (ctx->d_scratch) HIP_OK(hipFree(ctx->d_scratch));
+    if (ctx->d_scratch) HIP_OK(hipFreeAsync(ctx->d_scratch, ctx->stream));
     ctx->d_scratch = nullptr;
     ctx->scratch_bytes = 0;
-    if (hipMalloc(&ctx->d_scratch, total) != hipSuccess) return fail(ctx, HPM_E_NOMEM, "hipMalloc (scratch)");
+    if (hipMallocAsync(&ctx->d_scratch, total, ctx->stream) != hipSuccess)
+      return fail(ctx, HPM_E_NOMEM, "hipMallocAsync (scratch)");
     ctx->scratch_bytes = total;
   }
   out.clear();

@@ -9,6 +9,9 @@
 // loads; SpatialGauss / RangeGauss (HPM.cu:142-152) square in float and take
 // exp in double, which uses the project's own double exp.
 #pragma once
+#include <initializer_list>
+#include <vector>
+
 #include "hpm_device.h"
 
 namespace hpm {
@@ -108,45 +111,75 @@ static int jbu_common(const float* fine_gray, int w, int h, const float* coarse_
   return 0;
 }
 
+// Both run on a non-blocking stream of their own with stream-ordered
+// memory, so that they neither wait for nor hold up the contexts other host
+// threads drive; they return when the result is on the host.
+namespace {
+struct JbuRun {
+  hipStream_t st = nullptr;
+  char* d = nullptr;
+  int rc = 0;
+  bool ok(hipError_t e) {
+    if (e != hipSuccess && rc == 0) rc = e == hipErrorOutOfMemory ? -4 : -3;
+    return rc == 0;
+  }
+  // one device block holding pieces of the given sizes, 256-byte aligned
+  std::vector<char*> alloc(std::initializer_list<size_t> sizes) {
+    size_t total = 0;
+    std::vector<size_t> off;
+    for (size_t b : sizes) { off.push_back(total); total += (b + 255) & ~(size_t)255; }
+    std::vector<char*> out;
+    if (ok(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) && ok(hipMallocAsync((void**)&d, total, st)))
+      for (size_t o : off) out.push_back(d + o);
+    return out;
+  }
+  ~JbuRun() {
+    if (d) hipFreeAsync(d, st);
+    if (st) {
+      hipStreamSynchronize(st);
+      hipStreamDestroy(st);
+    }
+  }
+};
+}  // namespace
+
 int hpm_jbu_depth(const float* fine_gray, int w, int h, const float* coarse_depth, int cw, int ch, float* out_depth) {
   hpm::JbuDims d;
   if (!out_depth || jbu_common(fine_gray, w, h, coarse_depth, cw, ch, &d)) return -1;
-  float *df = nullptr, *dc = nullptr, *dout = nullptr;
   const size_t nf = (size_t)w * h, nc = (size_t)cw * ch;
-  int rc = 0;
-  if (hipMalloc((void**)&df, nf * 4) != hipSuccess || hipMalloc((void**)&dc, nc * 4) != hipSuccess ||
-      hipMalloc((void**)&dout, nf * 4) != hipSuccess) rc = -4;
-  if (!rc && (hipMemcpy(df, fine_gray, nf * 4, hipMemcpyHostToDevice) != hipSuccess ||
-              hipMemcpy(dc, coarse_depth, nc * 4, hipMemcpyHostToDevice) != hipSuccess)) rc = -3;
-  if (!rc) {
-    hipLaunchKernelGGL(hpm::k_jbu_depth, dim3((w + 15) / 16, (h + 15) / 16), dim3(16, 16), 0, 0, d, df, dc, dout);
-    if (hipGetLastError() != hipSuccess || hipMemcpy(out_depth, dout, nf * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = -3;
+  JbuRun r;
+  const std::vector<char*> b = r.alloc({nf * 4, nc * 4, nf * 4});
+  if (r.rc) return r.rc;
+  float *df = (float*)b[0], *dc = (float*)b[1], *dout = (float*)b[2];
+  if (r.ok(hipMemcpyAsync(df, fine_gray, nf * 4, hipMemcpyHostToDevice, r.st)) &&
+      r.ok(hipMemcpyAsync(dc, coarse_depth, nc * 4, hipMemcpyHostToDevice, r.st))) {
+    hipLaunchKernelGGL(hpm::k_jbu_depth, dim3((w + 15) / 16, (h + 15) / 16), dim3(16, 16), 0, r.st, d, df, dc, dout);
+    if (r.ok(hipGetLastError()) && r.ok(hipMemcpyAsync(out_depth, dout, nf * 4, hipMemcpyDeviceToHost, r.st)))
+      r.ok(hipStreamSynchronize(r.st));
   }
-  hipFree(df); hipFree(dc); hipFree(dout);
-  return rc;
+  return r.rc;
 }
 
 int hpm_jbu_prior(const float* fine_gray, int w, int h, const float* coarse_depth, const float* coarse_normals, int cw,
                   int ch, float* out_depth, float* out_normals) {
   hpm::JbuDims d;
   if (!out_depth || !out_normals || !coarse_normals || jbu_common(fine_gray, w, h, coarse_depth, cw, ch, &d)) return -1;
-  float *df = nullptr, *dc = nullptr, *dout = nullptr;
-  float4 *dn = nullptr, *don = nullptr;
   const size_t nf = (size_t)w * h, nc = (size_t)cw * ch;
-  int rc = 0;
-  if (hipMalloc((void**)&df, nf * 4) != hipSuccess || hipMalloc((void**)&dc, nc * 4) != hipSuccess ||
-      hipMalloc((void**)&dout, nf * 4) != hipSuccess || hipMalloc((void**)&dn, nc * 16) != hipSuccess ||
-      hipMalloc((void**)&don, nf * 16) != hipSuccess) rc = -4;
-  if (!rc && (hipMemcpy(df, fine_gray, nf * 4, hipMemcpyHostToDevice) != hipSuccess ||
-              hipMemcpy(dc, coarse_depth, nc * 4, hipMemcpyHostToDevice) != hipSuccess ||
-              hipMemcpy(dn, coarse_normals, nc * 16, hipMemcpyHostToDevice) != hipSuccess)) rc = -3;
-  if (!rc) {
-    hipLaunchKernelGGL(hpm::k_jbu_prior, dim3((w + 15) / 16, (h + 15) / 16), dim3(16, 16), 0, 0, d, df, dc, dn, dout, don);
-    if (hipGetLastError() != hipSuccess || hipMemcpy(out_depth, dout, nf * 4, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(out_normals, don, nf * 16, hipMemcpyDeviceToHost) != hipSuccess) rc = -3;
+  JbuRun r;
+  const std::vector<char*> b = r.alloc({nf * 4, nc * 4, nf * 4, nc * 16, nf * 16});
+  if (r.rc) return r.rc;
+  float *df = (float*)b[0], *dc = (float*)b[1], *dout = (float*)b[2];
+  float4 *dn = (float4*)b[3], *don = (float4*)b[4];
+  if (r.ok(hipMemcpyAsync(df, fine_gray, nf * 4, hipMemcpyHostToDevice, r.st)) &&
+      r.ok(hipMemcpyAsync(dc, coarse_depth, nc * 4, hipMemcpyHostToDevice, r.st)) &&
+      r.ok(hipMemcpyAsync(dn, coarse_normals, nc * 16, hipMemcpyHostToDevice, r.st))) {
+    hipLaunchKernelGGL(hpm::k_jbu_prior, dim3((w + 15) / 16, (h + 15) / 16), dim3(16, 16), 0, r.st, d, df, dc, dn, dout,
+                       don);
+    if (r.ok(hipGetLastError()) && r.ok(hipMemcpyAsync(out_depth, dout, nf * 4, hipMemcpyDeviceToHost, r.st)) &&
+        r.ok(hipMemcpyAsync(out_normals, don, nf * 16, hipMemcpyDeviceToHost, r.st)))
+      r.ok(hipStreamSynchronize(r.st));
   }
-  hipFree(df); hipFree(dc); hipFree(dout); hipFree(dn); hipFree(don);
-  return rc;
+  return r.rc;
 }
 
 }  // extern "C"
