@@ -182,6 +182,33 @@ MapF ResizeLinear(const MapF& src, int w, int h) {
   return d;
 }
 
+namespace {
+// Records serialised into blocks of about 8 MB, each written with one fwrite.
+class BlockWriter {
+ public:
+  explicit BlockWriter(FILE* fp) : fp_(fp) { buf_.reserve(kBlock); }
+  void put(const void* p, size_t n) {
+    if (buf_.size() + n > kBlock) flush();
+    const char* c = static_cast<const char*>(p);
+    buf_.insert(buf_.end(), c, c + n);
+  }
+  bool finish() {
+    flush();
+    return ok_;
+  }
+
+ private:
+  void flush() {
+    if (!buf_.empty()) ok_ = ok_ && std::fwrite(buf_.data(), 1, buf_.size(), fp_) == buf_.size();
+    buf_.clear();
+  }
+  static constexpr size_t kBlock = (size_t)8 << 20;
+  FILE* fp_;
+  std::vector<char> buf_;
+  bool ok_ = true;
+};
+}  // namespace
+
 bool ExportPointCloud(const std::string& path, const std::vector<FusedPoint>& pts) {
   FILE* fp = std::fopen(path.c_str(), "wb");
   if (!fp) return false;
@@ -189,14 +216,16 @@ bool ExportPointCloud(const std::string& path, const std::vector<FusedPoint>& pt
                "ply\nformat binary_little_endian 1.0\nelement vertex %zu\nproperty float x\nproperty float y\n"
                "property float z\nproperty uchar red\nproperty uchar green\nproperty uchar blue\nend_header\n",
                pts.size());
-  bool ok = true;
+  BlockWriter w(fp);
   for (const FusedPoint& p : pts) {
     float X[3] = {p.coord[0], p.coord[1], p.coord[2]};
     const bool finite = std::isfinite(X[0]) && std::isfinite(X[1]) && std::isfinite(X[2]);
     if (!finite) X[0] = X[1] = X[2] = 0.0f;  // HPM.cpp:477-481
     const char rgb[3] = {(char)(int)p.color[2], (char)(int)p.color[1], (char)(int)p.color[0]};
-    ok = ok && std::fwrite(X, sizeof(float), 3, fp) == 3 && std::fwrite(rgb, 1, 3, fp) == 3;
+    w.put(X, sizeof(X));
+    w.put(rgb, 3);
   }
+  const bool ok = w.finish();
   return std::fclose(fp) == 0 && ok;
 }
 
@@ -218,18 +247,20 @@ bool ExportFusedPointCloud(const std::string& path, const std::vector<FusedPoint
                "property float z\nproperty float nx\nproperty float ny\nproperty float nz\nproperty uchar red\n"
                "property uchar green\nproperty uchar blue\nend_header\n",
                count);
-  bool ok = true;
   auto channel = [](float v) -> unsigned char {
     return std::isfinite(v) ? (unsigned char)std::max(0.0f, std::min(255.0f, v)) : 0;
   };
+  BlockWriter w(fp);
   for (const FusedPoint& p : pts) {
     if (!valid(p)) continue;
     const double len = std::sqrt(norm2(p));
     const float v[6] = {p.coord[0], p.coord[1], p.coord[2], (float)(p.normal[0] / len), (float)(p.normal[1] / len),
                         (float)(p.normal[2] / len)};
     const unsigned char rgb[3] = {channel(p.color[2]), channel(p.color[1]), channel(p.color[0])};
-    ok = ok && std::fwrite(v, sizeof(float), 6, fp) == 6 && std::fwrite(rgb, 1, 3, fp) == 3;
+    w.put(v, sizeof(v));
+    w.put(rgb, 3);
   }
+  const bool ok = w.finish();
   return std::fclose(fp) == 0 && ok;
 }
 
@@ -1558,8 +1589,11 @@ Fused FuseViews(const std::string& dense_folder, const std::vector<Problem>& pro
 void ExportBoth(const std::string& dense_folder, const char* name, const std::vector<FusedPoint>& pts) {
   const std::string out = dense_folder + "/HPM_MVS_plusplus";
   MakeDir(out);
-  if (!ExportPointCloud(out + "/" + name, pts) || !ExportFusedPointCloud(out + "/fused.ply", pts))
-    throw Error("cannot write the point clouds under " + out);
+  bool fused_ok = false;
+  std::thread fused([&] { fused_ok = ExportFusedPointCloud(out + "/fused.ply", pts); });  // the two files at once
+  const bool ok = ExportPointCloud(out + "/" + name, pts);
+  fused.join();
+  if (!ok || !fused_ok) throw Error("cannot write the point clouds under " + out);
 }
 }  // namespace
 

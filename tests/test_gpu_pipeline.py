@@ -84,6 +84,26 @@ def test_pipeline_fusion_equals_oracle(dense_folder):
     gp, gx = H.consistency_check(H.CHECK_FUSION, hc, depths, normals, sources)
     np.testing.assert_array_equal(gp, rp)
     np.testing.assert_array_equal(gx, rx)
+    # the PLY files hold those points in that order (ExportPointCloud /
+    # ExportFusedPointCloud, HPM.cpp:410-498): position, normalised normal,
+    # and the colour of the fused pixel (a grey image repeats its value)
+    grey = [np.fromfile(os.path.join(folder, "images", f"{i:08d}.pgm"), np.uint8)[-160 * 120:] for i in range(5)]
+    col = np.array([grey[v][p] for v, p in rp], np.uint8)
+    with open(os.path.join(out, "HPM_MVS_plusplus.ply"), "rb") as fh:
+        body = fh.read().split(b"end_header\n", 1)[1]
+    rec = np.frombuffer(body, np.dtype([("x", "<f4", 3), ("c", "u1", 3)]))
+    assert len(rec) == n
+    np.testing.assert_array_equal(rec["x"], rx[:, :3])
+    np.testing.assert_array_equal(rec["c"], np.repeat(col[:, None], 3, axis=1))
+    with open(os.path.join(out, "fused.ply"), "rb") as fh:
+        body = fh.read().split(b"end_header\n", 1)[1]
+    rec = np.frombuffer(body, np.dtype([("x", "<f4", 3), ("n", "<f4", 3), ("c", "u1", 3)]))
+    nrm = rx[:, 3:].astype(np.float64)
+    keep = np.isfinite(rx[:, :3]).all(1) & (np.sum(nrm * nrm, 1) > 1e-20)
+    assert len(rec) == int(keep.sum())
+    np.testing.assert_array_equal(rec["x"], rx[keep, :3])
+    np.testing.assert_allclose(rec["n"], (nrm[keep] / np.linalg.norm(nrm[keep], axis=1, keepdims=True)), atol=1e-6)
+    np.testing.assert_array_equal(rec["c"], np.repeat(col[keep, None], 3, axis=1))
 
 
 def test_sky_mask_fusion_equals_oracle(dense_folder):
