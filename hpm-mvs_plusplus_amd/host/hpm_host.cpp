@@ -1715,8 +1715,17 @@ std::vector<FusedPoint> RunPyramidPipeline(const std::string& dense_folder, int 
       if (distances) fi.hpm_scale_distance = (*distances)[i];
       ProcessProblem(dense_folder, problems, i, fi, precision, seed + 10007 * pass + i, &images);
     };
-    if (order_free(f)) parallel_views(n, one);
+    const double t0 = now_s();
+    const bool concurrent = order_free(f);
+    if (concurrent) parallel_views(n, one);
     else for (int i = 0; i < n; ++i) one(i);
+    if (g_verbose > 0) {
+      std::fprintf(stderr, "[hpm] level %d pass %llu (%s, %s): %d views in %.1f ms\n", f.image_scale,
+                   (unsigned long long)pass,
+                   f.prior_consistency ? "prior" : f.hierarchy ? "hierarchy" : f.geom_consistency ? "geometric" : "photometric",
+                   concurrent ? "concurrent" : "in view order", n, 1e3 * (now_s() - t0));
+      std::fflush(stderr);
+    }
     ++pass;
   };
   const int max_hpm_scale = max_down;
