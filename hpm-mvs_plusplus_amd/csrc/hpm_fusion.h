@@ -147,6 +147,14 @@ int hpm_consistency_check(int mode, int num_images, const hpm_camera* cameras, c
                           const float* const* normals, const int32_t* src_offsets, const int32_t* src_ids,
                           float* const* out_confidence, int32_t* out_pairs, float* out_points, int64_t capacity,
                           int64_t* out_count) {
+  return hpm_consistency_check_cb(mode, num_images, cameras, depths, normals, src_offsets, src_ids, out_confidence,
+                                  out_pairs, out_points, capacity, out_count, nullptr, nullptr);
+}
+
+int hpm_consistency_check_cb(int mode, int num_images, const hpm_camera* cameras, const float* const* depths,
+                             const float* const* normals, const int32_t* src_offsets, const int32_t* src_ids,
+                             float* const* out_confidence, int32_t* out_pairs, float* out_points, int64_t capacity,
+                             int64_t* out_count, void (*image_done)(int image, void* user), void* user) {
   using namespace hpm;
   if ((mode != HPM_CHECK_FUSION && mode != HPM_CHECK_CONFIDENCE && mode != HPM_CHECK_FUSION_MASK) || num_images < 1 ||
       num_images > kFusionMaxImages ||
@@ -330,6 +338,7 @@ int hpm_consistency_check(int mode, int num_images, const hpm_camera* cameras, c
       }
     }
     if (mt && rc == HPM_OK) release();
+    if (rc == HPM_OK && image_done) image_done(i, user);
   }
   if (verbose) {
     const double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count();

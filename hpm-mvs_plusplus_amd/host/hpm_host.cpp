@@ -1318,6 +1318,65 @@ std::atomic<int> g_prior_on_device{1};
 }
 void SetPriorOnDevice(int enable) { g_prior_on_device = enable != 0; }
 
+namespace {
+std::atomic<int> g_confidence_overlap{1};
+
+// For each problem (pair.txt position), the problem after which the
+// confidence evaluation no longer writes its map: itself or the last problem
+// that lists it as a source.
+std::vector<int> ConfidenceFinalAfter(const std::vector<Problem>& problems) {
+  const int n = (int)problems.size();
+  std::vector<int> pos;
+  for (int i = 0; i < n; ++i) {
+    const int id = problems[i].ref_image_id;
+    if (id < 0) throw Error("negative image id in pair.txt");
+    if ((size_t)id >= pos.size()) pos.resize(id + 1, -1);
+    pos[id] = i;
+  }
+  std::vector<int> final_after(n);
+  for (int i = 0; i < n; ++i) final_after[i] = i;
+  for (int k = 0; k < n; ++k)
+    for (int s : problems[k].src_image_ids)
+      if (s >= 0 && (size_t)s < pos.size() && pos[s] >= 0) final_after[pos[s]] = std::max(final_after[pos[s]], k);
+  return final_after;
+}
+
+// ConfidenceEvaluation, then `pass(ready)`.  With the overlap on, the pass
+// runs alongside the evaluation (a host pass with little GPU work) and each
+// of its views starts once its own confidence map -- the only one it reads
+// -- is final; the files are those of running one after the other.
+template <class Pass>
+void ConfidenceThenPass(const std::string& dense_folder, const std::vector<Problem>& problems, bool geom, Pass pass) {
+  if (!g_confidence_overlap) {
+    ConfidenceEvaluation(dense_folder, problems, geom);
+    pass(nullptr);
+    return;
+  }
+  ReadyViews ready((int)problems.size());
+  ready.SetFinalAfter(ConfidenceFinalAfter(problems));
+  std::exception_ptr conf_err;
+  std::thread conf([&] {
+    try {
+      ConfidenceEvaluation(dense_folder, problems, geom, &ready);
+    } catch (...) {
+      conf_err = std::current_exception();
+      ready.Fail("confidence evaluation failed");
+    }
+  });
+  try {
+    pass(&ready);
+  } catch (...) {
+    conf.join();
+    if (conf_err) std::rethrow_exception(conf_err);
+    throw;
+  }
+  conf.join();
+  if (conf_err) std::rethrow_exception(conf_err);
+}
+}  // namespace
+
+void SetConfidenceOverlap(int enable) { g_confidence_overlap = enable != 0; }
+
 void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>& problems, int idx,
                     const PassFlags& flags, int precision, uint64_t seed, SceneImages* images) {
   StageClock clk;
@@ -1499,7 +1558,47 @@ static ViewMaps LoadViewMaps(const std::string& dense_folder, const std::vector<
   return v;
 }
 
-void ConfidenceEvaluation(const std::string& dense_folder, const std::vector<Problem>& problems, bool geom) {
+ReadyViews::ReadyViews(int n) : ready_((size_t)n, 0), final_after_((size_t)n) {
+  for (int i = 0; i < n; ++i) final_after_[i] = i;
+}
+
+void ReadyViews::SetFinalAfter(const std::vector<int>& final_after) {
+  std::lock_guard<std::mutex> l(mu_);
+  final_after_ = final_after;
+}
+
+std::vector<int> ReadyViews::Order() const {
+  std::lock_guard<std::mutex> l(mu_);
+  std::vector<int> order(final_after_.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return final_after_[a] < final_after_[b]; });
+  return order;
+}
+
+void ReadyViews::Mark(int i) {
+  {
+    std::lock_guard<std::mutex> l(mu_);
+    ready_.at(i) = 1;
+  }
+  cv_.notify_all();
+}
+
+void ReadyViews::Fail(const std::string& msg) {
+  {
+    std::lock_guard<std::mutex> l(mu_);
+    if (error_.empty()) error_ = msg;
+  }
+  cv_.notify_all();
+}
+
+void ReadyViews::Wait(int i) {
+  std::unique_lock<std::mutex> l(mu_);
+  cv_.wait(l, [&] { return ready_.at(i) || !error_.empty(); });
+  if (!ready_[i]) throw Error(error_);
+}
+
+void ConfidenceEvaluation(const std::string& dense_folder, const std::vector<Problem>& problems, bool geom,
+                          ReadyViews* ready) {
   StageClock clk;
   struct Lap {
     StageClock& c;
@@ -1518,15 +1617,38 @@ void ConfidenceEvaluation(const std::string& dense_folder, const std::vector<Pro
     np.push_back(v.normals[i]->data.data());
   }
   for (auto& c : conf) cp.push_back(c.data.data());
+  // Image i's map is written by its own pixels and by the images after it
+  // that list it as a source: it is final once the last of those is done.
+  std::vector<int> final_after(n);
+  for (int i = 0; i < n; ++i) final_after[i] = i;
+  for (int k = 0; k < n; ++k)
+    for (int j = v.offsets[k]; j < v.offsets[k + 1]; ++j) final_after[v.ids[j]] = std::max(final_after[v.ids[j]], k);
+  if (ready) ready->SetFinalAfter(final_after);
+  std::vector<std::vector<int>> due(n);  // views whose map is final after image k
+  for (int i = 0; i < n; ++i) due[final_after[i]].push_back(i);
+  struct Progress {
+    const std::string* folder;
+    const std::vector<Problem>* problems;
+    std::vector<MapF>* conf;
+    std::vector<std::vector<int>>* due;
+    ReadyViews* ready;
+  } prog{&dense_folder, &problems, &conf, &due, ready};
+  auto image_done = [](int k, void* user) {
+    auto* g = static_cast<Progress*>(user);
+    for (int i : (*g->due)[k]) {
+      SaveMap(ResultFolder(*g->folder, (*g->problems)[i].ref_image_id) + "/confidence.dmb", std::move((*g->conf)[i]));
+      if (g->ready) g->ready->Mark(i);
+    }
+  };
   int64_t count = 0;
-  const int rc = hpm_consistency_check(HPM_CHECK_CONFIDENCE, n, v.cams.data(), dp.data(), np.data(), v.offsets.data(),
-                                       v.ids.data(), cp.data(), nullptr, nullptr, 0, &count);
+  const int rc = hpm_consistency_check_cb(HPM_CHECK_CONFIDENCE, n, v.cams.data(), dp.data(), np.data(),
+                                          v.offsets.data(), v.ids.data(), cp.data(), nullptr, nullptr, 0, &count,
+                                          image_done, &prog);
   if (rc != HPM_OK) throw Error("hpm_consistency_check (confidence) failed");
   const double t2 = now_s();
-  for (int i = 0; i < n; ++i) SaveMap(ResultFolder(dense_folder, problems[i].ref_image_id) + "/confidence.dmb", std::move(conf[i]));
   if (g_verbose > 0) {
-    std::fprintf(stderr, "[hpm] confidence (%d views): load %.1f ms, check %.1f ms, write %.1f ms\n", n,
-                 1e3 * (t1 - t0), 1e3 * (t2 - t1), 1e3 * (now_s() - t2));
+    std::fprintf(stderr, "[hpm] confidence (%d views): load %.1f ms, check and hand-over %.1f ms\n", n,
+                 1e3 * (t1 - t0), 1e3 * (t2 - t1));
     std::fflush(stderr);
   }
 }
@@ -1652,20 +1774,35 @@ std::vector<FusedPoint> RunPipeline(const std::string& dense_folder, int precisi
   const int n = (int)problems.size();
   PassFlags f;
   parallel_views(n, [&](int i) { ProcessProblem(dense_folder, problems, i, f, precision, seed + i); });  // photometric
-  ConfidenceEvaluation(dense_folder, problems, false);
   f.prior_consistency = true;
-  parallel_views(n, [&](int i) { ProcessProblem(dense_folder, problems, i, f, precision, seed + 1000 + i); });
+  ConfidenceThenPass(dense_folder, problems, false, [&](ReadyViews* ready) {
+    const std::vector<int> order = ready ? ready->Order() : std::vector<int>();
+    parallel_views(n, [&](int k) {
+      const int i = ready ? order[k] : k;
+      if (ready) ready->Wait(i);
+      ProcessProblem(dense_folder, problems, i, f, precision, seed + 1000 + i);
+    });
+  });
   f = PassFlags();
   for (int it = 0; it < 3; ++it) {  // main.cpp:1269-1290
     f.multi_geometry = it > 0;
-    if (it > 0) {
-      f.mand_consistency = true;
-      ConfidenceEvaluation(dense_folder, problems, f.geom_consistency);
-    }
+    const bool conf_geom = f.geom_consistency;  // the previous pass's maps
+    if (it > 0) f.mand_consistency = true;
     f.geom_consistency = true;
-    auto one = [&](int i) { ProcessProblem(dense_folder, problems, i, f, precision, seed + 2000 * (it + 2) + i); };
-    if (order_free(f)) parallel_views(n, one);
-    else for (int i = 0; i < n; ++i) one(i);
+    auto run = [&](ReadyViews* ready) {
+      auto one = [&](int i) {
+        if (ready) ready->Wait(i);
+        ProcessProblem(dense_folder, problems, i, f, precision, seed + 2000 * (it + 2) + i);
+      };
+      if (order_free(f)) {
+        const std::vector<int> order = ready ? ready->Order() : std::vector<int>();
+        parallel_views(n, [&](int k) { one(ready ? order[k] : k); });
+      } else {
+        for (int i = 0; i < n; ++i) one(i);
+      }
+    };
+    if (it > 0) ConfidenceThenPass(dense_folder, problems, conf_geom, run);
+    else run(nullptr);
   }
   std::vector<FusedPoint> pts = RunFusion(dense_folder, problems, true);
   store.Finish();
@@ -1709,16 +1846,24 @@ std::vector<FusedPoint> RunPyramidPipeline(const std::string& dense_folder, int 
   };
 
   uint64_t pass = 0;
-  auto each_view = [&](const PassFlags& f, const std::vector<int>* distances) {
+  // one pass over the views; with `ready`, each view first waits for its
+  // confidence map, and an order-free pass takes the views in the order the
+  // maps become ready
+  auto each_view = [&](const PassFlags& f, const std::vector<int>* distances, ReadyViews* ready) {
     auto one = [&](int i) {
+      if (ready) ready->Wait(i);
       PassFlags fi = f;
       if (distances) fi.hpm_scale_distance = (*distances)[i];
       ProcessProblem(dense_folder, problems, i, fi, precision, seed + 10007 * pass + i, &images);
     };
     const double t0 = now_s();
     const bool concurrent = order_free(f);
-    if (concurrent) parallel_views(n, one);
-    else for (int i = 0; i < n; ++i) one(i);
+    if (concurrent) {
+      const std::vector<int> order = ready ? ready->Order() : std::vector<int>();
+      parallel_views(n, [&](int k) { one(ready ? order[k] : k); });
+    } else {
+      for (int i = 0; i < n; ++i) one(i);
+    }
     if (g_verbose > 0) {
       std::fprintf(stderr, "[hpm] level %d pass %llu (%s, %s): %d views in %.1f ms\n", f.image_scale,
                    (unsigned long long)pass,
@@ -1744,23 +1889,24 @@ std::vector<FusedPoint> RunPyramidPipeline(const std::string& dense_folder, int 
       parallel_views(n, [&](int i) { JointBilateralUpsampling(dense_folder, problems[i], images); });
       f.hierarchy = true;
     }
-    each_view(f, nullptr);  // photometric or hierarchy pass
+    each_view(f, nullptr, nullptr);  // photometric or hierarchy pass
     f.hierarchy = false;
     f.prior_consistency = true;
     for (int hs = max_hpm_scale; hs >= level; --hs) {  // prior passes, coarsest prior first
-      ConfidenceEvaluation(dense_folder, problems, false);
       std::vector<int> dist(n);
       for (int i = 0; i < n; ++i) dist[i] = hs - num_down[i] - 1;
-      each_view(f, &dist);
+      ConfidenceThenPass(dense_folder, problems, false, [&](ReadyViews* ready) { each_view(f, &dist, ready); });
     }
     f.prior_consistency = false;
     f.hpm_scale_distance = 0;
     for (int it = 0; it < 3; ++it) {  // geometric passes
       f.multi_geometry = it > 0;
       f.mand_consistency = it > 0;
-      if (it > 0) ConfidenceEvaluation(dense_folder, problems, true);  // over the previous depths_geom
       f.geom_consistency = true;
-      each_view(f, nullptr);
+      if (it > 0)  // over the previous depths_geom
+        ConfidenceThenPass(dense_folder, problems, true, [&](ReadyViews* ready) { each_view(f, nullptr, ready); });
+      else
+        each_view(f, nullptr, nullptr);
     }
   }
   std::vector<FusedPoint> pts =

@@ -6,8 +6,10 @@
 #pragma once
 #include <sys/mman.h>
 
+#include <condition_variable>
 #include <cstdint>
 #include <cstdlib>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <memory>
@@ -315,7 +317,26 @@ void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>&
 // guided by the reference image, written as depths.dmb.
 void JointBilateralUpsampling(const std::string& dense_folder, const Problem& problem, SceneImages& images);
 // ConfidenceEvaluation (main.cpp:959-1109) -> confidence.dmb per view.
-void ConfidenceEvaluation(const std::string& dense_folder, const std::vector<Problem>& problems, bool geom);
+// Views whose confidence map a running ConfidenceEvaluation has finished
+// (saved with SaveMap): a pass started alongside it waits per view.
+class ReadyViews {
+ public:
+  explicit ReadyViews(int n);
+  void SetFinalAfter(const std::vector<int>& final_after);  // image after which each view's map is final
+  std::vector<int> Order() const;                           // views in the order they become ready
+  void Mark(int i);
+  void Fail(const std::string& msg);  // wakes the waiters, which throw
+  void Wait(int i);
+
+ private:
+  mutable std::mutex mu_;
+  std::condition_variable cv_;
+  std::vector<char> ready_;
+  std::vector<int> final_after_;
+  std::string error_;
+};
+void ConfidenceEvaluation(const std::string& dense_folder, const std::vector<Problem>& problems, bool geom,
+                          ReadyViews* ready = nullptr);
 // RunFusion (main.cpp:823-957) -> HPM_MVS_plusplus.ply and fused.ply; returns
 // the fused points in the reference's order.
 std::vector<FusedPoint> RunFusion(const std::string& dense_folder, const std::vector<Problem>& problems, bool geom);
@@ -363,5 +384,9 @@ void SetPassThreads(int n);
 // (hpm_prior_support_points / hpm_prior_build; only the Delaunay
 // triangulation runs on the host), 0 on the host.  The files are identical.
 void SetPriorOnDevice(int enable);
+// Whether the pipeline runs a pass alongside the confidence evaluation
+// before it, each view starting when its confidence map is final (1,
+// default), or after it (0).  The files are identical.
+void SetConfidenceOverlap(int enable);
 
 }  // namespace hpmhost

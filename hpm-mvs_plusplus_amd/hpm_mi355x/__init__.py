@@ -61,7 +61,7 @@ EXPORTS = [
     "hpm_run_patchmatch_batch", "hpm_pool_create", "hpm_pool_destroy", "hpm_set_views_pooled",
     "hpm_cost_grad", "hpm_cost_grad_device", "hpm_refine_gn", "hpm_refine_random", "hpm_refine_hybrid",
     "hpm_refine_gn_smoothed", "hpm_prior_support_points", "hpm_prior_build", "hpm_get_prior", "hpm_canny", "hpm_get_depth_normals",
-    "hpm_reload_hypotheses_maps", "hpm_set_canny_u8",
+    "hpm_reload_hypotheses_maps", "hpm_set_canny_u8", "hpm_consistency_check_cb",
     "hpm_batch_create", "hpm_batch_destroy", "hpm_batch_last_error", "hpm_batch_run", "hpm_batch_set_graph",
     "hpm_batch_set_shape", "hpm_batch_get_shape", "hpm_batch_last_ms", "hpm_batch_autotune",
 ]

@@ -43,6 +43,8 @@ def host_lib():
     L.hpm_host_set_threads.restype = None
     L.hpm_host_set_prior_on_device.argtypes = [ctypes.c_int]
     L.hpm_host_set_prior_on_device.restype = None
+    L.hpm_host_set_confidence_overlap.argtypes = [ctypes.c_int]
+    L.hpm_host_set_confidence_overlap.restype = None
     L.hpm_host_decode_jpeg.argtypes = [u8p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                        ctypes.POINTER(ctypes.c_int), u8p, ctypes.c_int64]
     L.hpm_host_support_points.argtypes = [fp, fp, ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int,
@@ -139,6 +141,12 @@ def set_prior_on_device(enable=True):
     Delaunay triangulation runs on the host) or entirely on the host; the
     output files are identical."""
     host_lib().hpm_host_set_prior_on_device(int(bool(enable)))
+
+
+def set_confidence_overlap(enable=True):
+    """Run each pass that follows a confidence evaluation alongside it (each
+    view waits for its own map; default) or after it; same files."""
+    host_lib().hpm_host_set_confidence_overlap(int(bool(enable)))
 
 
 def decode_jpeg(data: bytes):

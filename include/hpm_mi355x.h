@@ -358,6 +358,16 @@ int hpm_consistency_check(int mode, int num_images, const hpm_camera* cameras, c
                           const float* const* normals, const int32_t* src_offsets, const int32_t* src_ids,
                           float* const* out_confidence, int32_t* out_pairs, float* out_points, int64_t capacity,
                           int64_t* out_count);
+/* The same, calling image_done(i, user) on the calling thread as soon as the
+ * ordered pass has finished image i (images in order).  From then on the
+ * pass writes out_confidence[i] only through images after i that list i as
+ * a source: a caller can hand out a confidence map once the last of those
+ * is done, while the check goes on (no counterpart in the reference; for
+ * overlapping the next pass with the check). */
+int hpm_consistency_check_cb(int mode, int num_images, const hpm_camera* cameras, const float* const* depths,
+                             const float* const* normals, const int32_t* src_offsets, const int32_t* src_ids,
+                             float* const* out_confidence, int32_t* out_pairs, float* out_points, int64_t capacity,
+                             int64_t* out_count, void (*image_done)(int image, void* user), void* user);
 
 #ifdef __cplusplus
 }

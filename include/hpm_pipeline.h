@@ -73,6 +73,11 @@ void hpm_host_set_threads(int n);
  * files are identical. */
 void hpm_host_set_prior_on_device(int enable);
 
+/* Whether the pipelines run each pass that follows a confidence evaluation
+ * alongside it, each view starting once its own confidence map is final
+ * (1, default), or after it (0).  The output files are identical. */
+void hpm_host_set_confidence_overlap(int enable);
+
 /* JPEG decoding as the pipeline reads images/%08d.jpg, which the reference
  * reads with cv::imread (main.cpp:125, main.cpp:213): Huffman-coded
  * sequential or progressive, 8-bit, the pixels of libjpeg's default decoding

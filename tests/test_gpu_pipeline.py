@@ -255,3 +255,36 @@ def test_pipeline_reads_jpeg_images(tmp_path):
             acc[kind].append(float(np.mean(np.abs(d - sc.depths[i]) / sc.depths[i] <= 0.01)))
         print(f"{kind}: depth within 1% of GT per view", " ".join(f"{a:.3f}" for a in acc[kind]))
     assert np.mean(acc["jpg"]) > np.mean(acc["pgm"]) - 0.05
+
+
+@pytest.mark.parametrize("schedule", ["pyramid", "single"])
+def test_confidence_overlap_gives_the_same_files(tmp_path, schedule):
+    """The passes after each confidence evaluation run alongside it, each
+    view waiting for its own confidence map; every file equals that of
+    running them one after the other."""
+    import filecmp
+    from hpm_mi355x import pipeline as P
+    from hpm_mi355x import scenes
+    sc = scenes.make_plane_scene(seed=8, n_cams=6, width=320, height=240, f=262.5, n_src=4)
+    runs = {}
+    try:
+        for overlap in (True, False):
+            folder = str(tmp_path / f"overlap{int(overlap)}")
+            scenes.write_dense_folder(sc, folder)
+            P.set_confidence_overlap(overlap)
+            print(f"case: {schedule} pipeline 6 views 320x240, confidence overlap {overlap}", flush=True)
+            if schedule == "pyramid":
+                n = P.run_pyramid_pipeline(folder, initial_image_scale=1, size_bound=100, precision="fp16", seed=2)
+            else:
+                n = P.run_pipeline(folder, precision="fp16", seed=2)
+            runs[overlap] = (folder, n)
+    finally:
+        P.set_confidence_overlap(True)
+    assert runs[True][1] == runs[False][1] > 1000
+    names = []
+    for root, _, files in os.walk(os.path.join(runs[True][0], "HPM_MVS_plusplus")):
+        for f in files:
+            names.append(os.path.relpath(os.path.join(root, f), runs[True][0]))
+    assert len(names) > 6 * 6
+    for rel in names:
+        assert filecmp.cmp(os.path.join(runs[True][0], rel), os.path.join(runs[False][0], rel), shallow=False), rel
