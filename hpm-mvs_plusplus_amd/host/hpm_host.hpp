@@ -87,6 +87,8 @@ bool WriteDmb(const std::string& path, const MapF& m);
 // map in memory for LoadMap and queues its file write to a background
 // thread; Finish() waits for the files (and reports a failed write).  With
 // no scope alive, LoadMap reads the file and SaveMap writes it at once.
+// Runs in one process share the store of the first scope; when that scope
+// ends (its files written), the others go on with the files directly.
 using MapRef = std::shared_ptr<const MapF>;
 MapRef LoadMap(const std::string& path);
 void SaveMap(const std::string& path, MapF&& m);
