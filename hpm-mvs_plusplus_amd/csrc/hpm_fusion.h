@@ -195,27 +195,30 @@ int hpm_consistency_check_cb(int mode, int num_images, const hpm_camera* cameras
   std::vector<FusionImage> himgs(num_images);
   const int total_src = src_offsets[num_images];
   // The per-pixel tests run on the GPU in chunks of rows, in the order the
-  // ordered pass consumes them; each chunk is copied into one of two pinned
-  // host buffers while the host resolves the previous one.
+  // ordered pass consumes them; each chunk is copied into one of a few
+  // pinned host buffers while the host resolves an earlier one.
   const size_t chunk_px = std::min<size_t>(max_p, (size_t)1 << 20);
   const size_t chunk_elems = chunk_px * (size_t)std::max(max_ngb, 1);
   MatchTerm* d_out = nullptr;
-  MatchTerm* h_out[2] = {nullptr, nullptr};
+  // kChunkBuffers chunks in flight: the GPU runs ahead of the ordered host
+  // pass, also while other work keeps it busy
+  constexpr int kChunkBuffers = 4;
+  MatchTerm* h_out[kChunkBuffers] = {};
   unsigned char* d_cnt = nullptr;
-  unsigned char* h_cnt[2] = {nullptr, nullptr};
+  unsigned char* h_cnt[kChunkBuffers] = {};
   hipStream_t st = nullptr;
-  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipEvent_t ev[kChunkBuffers] = {};
   if (ok(hipMalloc((void**)&d_cams, sizeof(DevCam) * num_images)) &&
       ok(hipMalloc((void**)&d_imgs, sizeof(FusionImage) * num_images)) &&
       ok(hipMalloc((void**)&d_srcs, sizeof(int) * std::max(total_src, 1))) &&
       ok(hipMalloc((void**)&d_out, sizeof(MatchTerm) * chunk_elems)) &&
       ok(hipMalloc((void**)&d_cnt, chunk_px)) &&
-      ok(hipHostMalloc((void**)&h_out[0], sizeof(MatchTerm) * chunk_elems, hipHostMallocDefault)) &&
-      ok(hipHostMalloc((void**)&h_out[1], sizeof(MatchTerm) * chunk_elems, hipHostMallocDefault)) &&
-      ok(hipHostMalloc((void**)&h_cnt[0], chunk_px, hipHostMallocDefault)) &&
-      ok(hipHostMalloc((void**)&h_cnt[1], chunk_px, hipHostMallocDefault)) &&
-      ok(hipStreamCreate(&st)) && ok(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming)) &&
-      ok(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming))) {
+      ok(hipStreamCreate(&st))) {
+    for (int b = 0; b < kChunkBuffers; ++b)
+      if (!(ok(hipHostMalloc((void**)&h_out[b], sizeof(MatchTerm) * chunk_elems, hipHostMallocDefault)) &&
+            ok(hipHostMalloc((void**)&h_cnt[b], chunk_px, hipHostMallocDefault)) &&
+            ok(hipEventCreateWithFlags(&ev[b], hipEventDisableTiming))))
+        break;
     for (int i = 0; i < num_images && rc == HPM_OK; ++i) {
       const size_t P = (size_t)cameras[i].width * cameras[i].height;
       if (ok(hipMalloc((void**)&d_buf[2 * i], P * 4)) && ok(hipMalloc((void**)&d_buf[2 * i + 1], P * 12)) &&
@@ -244,9 +247,10 @@ int hpm_consistency_check_cb(int mode, int num_images, const hpm_camera* cameras
     hipLaunchKernelGGL(k_consistency, dim3((unsigned)((c.n + 255) / 256)), dim3(256), 0, st, d_cams, d_imgs,
                        d_srcs + src_offsets[c.img], num_ngb, c.img, mode, (int)c.p0, (int)c.n, d_out, d_cnt);
     return ok(hipGetLastError()) &&
-           ok(hipMemcpyAsync(h_out[k & 1], d_out, sizeof(MatchTerm) * c.n * num_ngb, hipMemcpyDeviceToHost, st)) &&
-           ok(hipMemcpyAsync(h_cnt[k & 1], d_cnt, c.n, hipMemcpyDeviceToHost, st)) &&
-           ok(hipEventRecord(ev[k & 1], st));
+           ok(hipMemcpyAsync(h_out[k % kChunkBuffers], d_out, sizeof(MatchTerm) * c.n * num_ngb,
+                             hipMemcpyDeviceToHost, st)) &&
+           ok(hipMemcpyAsync(h_cnt[k % kChunkBuffers], d_cnt, c.n, hipMemcpyDeviceToHost, st)) &&
+           ok(hipEventRecord(ev[k % kChunkBuffers], st));
   };
 
   std::vector<std::vector<uint8_t>> masks(num_images);
@@ -255,7 +259,7 @@ int hpm_consistency_check_cb(int mode, int num_images, const hpm_camera* cameras
     masks[i].assign((size_t)cameras[i].width * cameras[i].height, 0);
     if (mode == HPM_CHECK_CONFIDENCE) std::memset(out_confidence[i], 0, masks[i].size() * sizeof(float));
   }
-  for (size_t k = 0; k < chunks.size() && k < 2 && rc == HPM_OK; ++k) issue(k);
+  for (size_t k = 0; k < chunks.size() && k < (size_t)kChunkBuffers && rc == HPM_OK; ++k) issue(k);
   size_t k = 0;
   std::vector<int> used;
   // HPM_VERBOSE=1: time waiting for chunks vs resolving them, match counts
@@ -275,16 +279,18 @@ int hpm_consistency_check_cb(int mode, int num_images, const hpm_camera* cameras
     size_t p_begin = 0, p_end = 0;
     const MatchTerm* mt = nullptr;
     const unsigned char* nm = nullptr;
-    // chunk k consumed: its pinned buffer takes chunk k + 2
-    auto release = [&]() { return !(k + 2 < chunks.size() && !issue(k + 2)) && (++k, true); };
+    // chunk k consumed: its pinned buffer takes chunk k + kChunkBuffers
+    auto release = [&]() {
+      return !(k + kChunkBuffers < chunks.size() && !issue(k + kChunkBuffers)) && (++k, true);
+    };
     for (size_t p = 0; p < P; ++p) {
       if (num_ngb > 0 && p == p_end) {  // next chunk of this image
         if (mt && !release()) break;
         const auto tw = std::chrono::steady_clock::now();
-        if (!ok(hipEventSynchronize(ev[k & 1]))) break;
+        if (!ok(hipEventSynchronize(ev[k % kChunkBuffers]))) break;
         t_wait += std::chrono::duration<double>(std::chrono::steady_clock::now() - tw).count();
-        mt = h_out[k & 1];
-        nm = h_cnt[k & 1];
+        mt = h_out[k % kChunkBuffers];
+        nm = h_cnt[k % kChunkBuffers];
         p_begin = chunks[k].p0;
         p_end = p_begin + chunks[k].n;
       }

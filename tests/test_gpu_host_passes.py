@@ -106,12 +106,13 @@ def test_confidence_bitexact():
 
 @pytest.mark.parametrize("mode", [0, 1])
 def test_consistency_chunked_bitexact(mode):
-    """Views of 1280 x 960 (1.23 Mpix) are resolved in two chunks of rows each,
-    and a view without sources sits between them: fusion and confidence still
+    """Views of 1280 x 960 (1.23 Mpix) are resolved in two chunks of rows each
+    (8 chunks, more than the pinned buffers that rotate among them), and a
+    view without sources sits between them: fusion and confidence still
     equal the oracle's bit for bit."""
     import hpm_mi355x as H
-    print(f"case: consistency mode {mode} 3 x 1280x960, chunked", flush=True)
-    sc, depths, normals, sources = _noisy_maps(13 + mode, n_cams=3, width=1280, height=960)
+    print(f"case: consistency mode {mode} 5 x 1280x960, chunked", flush=True)
+    sc, depths, normals, sources = _noisy_maps(13 + mode, n_cams=5, width=1280, height=960)
     sources[1] = []
     hc = [H.camera_from_scene(c) for c in sc.cameras]
     oc = [O.camera_from_scene(c) for c in sc.cameras]
