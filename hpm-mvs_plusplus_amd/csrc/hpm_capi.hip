@@ -63,6 +63,11 @@ void keep_pool_memory(int device) {
   done.push_back(device);
 }
 
+// Frees a buffer from dalloc in the order of the context's stream.
+template <class T> void dfree(hpm_ctx* ctx, T* p) {
+  if (p) hipFreeAsync(p, ctx->stream);
+}
+
 void free_views(hpm_ctx* c) {
   if (!c->pool) {
     for (float* p : c->d_img) if (p) hipFreeAsync(p, c->stream);
@@ -1032,12 +1037,12 @@ int hpm_cost_fixed(hpm_ctx* ctx, const float* planes, float* out, int precision)
   float4* dp = nullptr;
   float* dout = nullptr;
   int rc;
-  if ((rc = dalloc(ctx, &dp, P)) || (rc = dalloc(ctx, &dout, P * V))) { hipFree(dp); return rc; }
+  if ((rc = dalloc(ctx, &dp, P)) || (rc = dalloc(ctx, &dout, P * V))) { dfree(ctx, dp); return rc; }
   rc = upload(ctx, dp, planes, P * sizeof(float4));
   if (!rc) rc = hpm_cost_fixed_device(ctx, dp, dout, precision);
   if (!rc) rc = download(ctx, out, dout, P * V * sizeof(float));
-  hipFree(dp);
-  hipFree(dout);
+  dfree(ctx, dp);
+  dfree(ctx, dout);
   return rc;
 }
 
@@ -1051,7 +1056,7 @@ int hpm_init_cost(hpm_ctx* ctx, const float* planes, float* out_cost, uint32_t* 
   uint32_t* dm = nullptr;
   int rc;
   if ((rc = dalloc(ctx, &dp, P)) || (rc = dalloc(ctx, &dc, P)) || (rc = dalloc(ctx, &dm, P))) {
-    hipFree(dp); hipFree(dc); return rc;
+    dfree(ctx, dp); dfree(ctx, dc); return rc;
   }
   RunConsts C;
   build_consts(ctx, C);
@@ -1064,7 +1069,7 @@ int hpm_init_cost(hpm_ctx* ctx, const float* planes, float* out_cost, uint32_t* 
   }
   if (!rc) rc = download(ctx, out_cost, dc, P * sizeof(float));
   if (!rc) rc = download(ctx, out_masks, dm, P * sizeof(uint32_t));
-  hipFree(dp); hipFree(dc); hipFree(dm);
+  dfree(ctx, dp); dfree(ctx, dc); dfree(ctx, dm);
   return rc;
 }
 
@@ -1077,7 +1082,7 @@ int hpm_geom_cost(hpm_ctx* ctx, const float* planes, float* out) {
   float4* dp = nullptr;
   float* dout = nullptr;
   int rc;
-  if ((rc = dalloc(ctx, &dp, P)) || (rc = dalloc(ctx, &dout, P * V))) { hipFree(dp); return rc; }
+  if ((rc = dalloc(ctx, &dp, P)) || (rc = dalloc(ctx, &dout, P * V))) { dfree(ctx, dp); return rc; }
   RunConsts C;
   build_consts(ctx, C);
   rc = upload(ctx, ctx->d_consts, &C, sizeof(C));
@@ -1088,7 +1093,7 @@ int hpm_geom_cost(hpm_ctx* ctx, const float* planes, float* out) {
     if (hipGetLastError() != hipSuccess) rc = fail(ctx, HPM_E_DEVICE, "k_geom_cost launch failed");
   }
   if (!rc) rc = download(ctx, out, dout, P * V * sizeof(float));
-  hipFree(dp); hipFree(dout);
+  dfree(ctx, dp); dfree(ctx, dout);
   return rc;
 }
 
