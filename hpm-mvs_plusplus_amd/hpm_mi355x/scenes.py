@@ -279,16 +279,24 @@ def _render_job(args):
     return render(cam, planes, noise_sigma, np.random.default_rng(seed))
 
 
-def render_all(cams, planes, noise_sigma, seed, workers=None):
-    """Renders every camera (camera k's pixel noise seeded by (seed, k), so the
-    result does not depend on the number of worker processes)."""
-    jobs = [(c, planes, noise_sigma, [seed, k]) for k, c in enumerate(cams)]
-    workers = workers if workers is not None else min(16, os.cpu_count() or 1, len(jobs))
+def render_all(cams, planes, noise_sigma, seed, workers=None, views=None):
+    """Renders every camera, or those in `views` (the others give (None, None,
+    None)).  Camera k's pixel noise is seeded by (seed, k), so the result does
+    not depend on the number of worker processes or on which views are
+    rendered."""
+    keep = [k for k in range(len(cams)) if views is None or k in views]
+    jobs = [(cams[k], planes, noise_sigma, [seed, k]) for k in keep]
+    workers = workers if workers is not None else min(16, os.cpu_count() or 1, max(1, len(jobs)))
     if workers <= 1:
-        return [_render_job(j) for j in jobs]
-    import multiprocessing as mp
-    with mp.get_context("fork").Pool(workers) as pool:
-        return pool.map(_render_job, jobs, chunksize=1)
+        done = [_render_job(j) for j in jobs]
+    else:
+        import multiprocessing as mp
+        with mp.get_context("fork").Pool(workers) as pool:
+            done = pool.map(_render_job, jobs, chunksize=1)
+    out = [(None, None, None)] * len(cams)
+    for k, r in zip(keep, done):
+        out[k] = r
+    return out
 
 
 def random_patches(rng, n, lo=(-1.6, -1.2, -1.4), hi=(1.6, 1.2, 1.4), ext=(0.4, 1.2), max_tilt_deg=50.0):
@@ -312,7 +320,8 @@ def random_spheres(rng, n, lo=(-1.0, -0.8, -1.6), hi=(1.0, 0.8, -0.4), r_lo=0.2,
 
 
 def make_orbit_scene(seed=3, n_views=30, width=3072, height=2048, n_planes=24, n_spheres=6, radius=4.0,
-                     elevations_deg=(15.0, 35.0), span_deg=60.0, n_src=10, noise_sigma=2.0, workers=None) -> Scene:
+                     elevations_deg=(15.0, 35.0), span_deg=60.0, n_src=10, noise_sigma=2.0, workers=None,
+                     render_refs=None) -> Scene:
     """Full-resolution scene: n_planes textured square plane patches (half
     sides U[0.4, 1.2] m, normals within 50 deg of the scene axis) plus
     n_spheres textured spheres (radius U[0.2, 0.6] m) in front of a background
@@ -320,7 +329,9 @@ def make_orbit_scene(seed=3, n_views=30, width=3072, height=2048, n_planes=24, n
     cameras on two rings (radius 4 m, elevations 15 and 35 deg) over a 60 deg
     arc, focal length 525 px per 640 px of width, each with its n_src
     top-scored sources.  Rays that leave the scene within 8 m (a few per cent
-    near the top corners of the high ring) hit nothing: depth 0 there."""
+    near the top corners of the high ring) hit nothing: depth 0 there.
+    render_refs: render only these reference views and their sources (the
+    other views' images, depths and normals are None)."""
     rng = np.random.default_rng(seed)
     f = 525.0 * width / 640.0
     cams = ring_cameras(n_views, radius, elevations_deg, f, width / 2, height / 2, width, height, span_deg=span_deg)
@@ -330,9 +341,15 @@ def make_orbit_scene(seed=3, n_views=30, width=3072, height=2048, n_planes=24, n
                           tex_seed=int(rng.integers(1 << 30))))
     for c in cams:
         c.depth_min, c.depth_max = 1.0, 7.5
-    out = render_all(cams, surfaces, noise_sigma, seed + 2, workers)
+    pairs = pair_scores(cams, n_src)
+    views = None
+    if render_refs is not None:
+        views = set(render_refs)
+        for r in render_refs:
+            views.update(j for j, _ in pairs[r])
+    out = render_all(cams, surfaces, noise_sigma, seed + 2, workers, views)
     return Scene(planes=surfaces, cameras=cams, images=[o[0] for o in out], depths=[o[1] for o in out],
-                 normals=[o[2] for o in out], pairs=pair_scores(cams, n_src))
+                 normals=[o[2] for o in out], pairs=pairs)
 
 
 def scaled_camera(cam: Camera, width: int, height: int) -> Camera:
