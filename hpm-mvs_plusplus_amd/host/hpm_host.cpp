@@ -70,11 +70,28 @@ hpm_camera ReadCamera(const std::string& path) {
   return c;
 }
 
+// Bytes from the read position to the end of the file, or -1.
+static int64_t BytesLeft(FILE* fp) {
+  const off_t here = ftello(fp);
+  if (here < 0 || fseeko(fp, 0, SEEK_END) != 0) return -1;
+  const off_t end = ftello(fp);
+  if (end < 0 || fseeko(fp, here, SEEK_SET) != 0) return -1;
+  return (int64_t)(end - here);
+}
+
+// A header's w x h x c elements of `elem` bytes fit in what the file holds: a
+// corrupt header is refused before anything is allocated for it.
+static bool FitsFile(FILE* fp, int64_t w, int64_t h, int64_t c, int64_t elem) {
+  const int64_t left = BytesLeft(fp);
+  return left >= 0 && w > 0 && h > 0 && c > 0 && w <= left / elem && h <= left / elem / w &&
+         c <= left / elem / w / h;
+}
+
 bool ReadDmb(const std::string& path, MapF& out) {
   FILE* fp = std::fopen(path.c_str(), "rb");
   if (!fp) return false;
   int32_t hdr[4] = {-1, 0, 0, 0};
-  bool ok = std::fread(hdr, sizeof(int32_t), 4, fp) == 4 && hdr[0] == 1 && hdr[1] > 0 && hdr[2] > 0 && hdr[3] > 0;
+  bool ok = std::fread(hdr, sizeof(int32_t), 4, fp) == 4 && hdr[0] == 1 && FitsFile(fp, hdr[2], hdr[1], hdr[3], 4);
   if (ok) {
     out = MapF(hdr[2], hdr[1], hdr[3]);
     ok = std::fread(out.data.data(), sizeof(float), out.data.size(), fp) == out.data.size();
@@ -116,8 +133,10 @@ bool ReadNetpbm(const std::string& path, MapU8& out) {
   bool ok = next_token(fp, magic) && (magic == "P5" || magic == "P6") && next_token(fp, sw) && next_token(fp, sh) &&
             next_token(fp, sm) && std::atoi(sm.c_str()) == 255;
   if (ok) {
-    const int w = std::atoi(sw.c_str()), h = std::atoi(sh.c_str()), c = magic == "P6" ? 3 : 1;
-    ok = w > 0 && h > 0;
+    const long long lw = std::strtoll(sw.c_str(), nullptr, 10), lh = std::strtoll(sh.c_str(), nullptr, 10);
+    const int c = magic == "P6" ? 3 : 1;
+    ok = lw < (1 << 30) && lh < (1 << 30) && FitsFile(fp, lw, lh, c, 1);
+    const int w = (int)lw, h = (int)lh;
     if (ok) {
       out = MapU8(w, h, c);
       ok = std::fread(out.data.data(), 1, out.data.size(), fp) == out.data.size();
@@ -862,6 +881,18 @@ MapRef LoadMap(const std::string& path) {
   return m;
 }
 
+// A loaded map must have the shape its consumer reads: a .dmb of another size
+// or channel count would otherwise be read past its end.
+static const MapF& Shaped(const MapRef& m, int w, int h, int c, const std::string& path) {
+  if (m->width != w || m->height != h || m->channels != c) {
+    std::ostringstream os;
+    os << path << ": a " << m->width << "x" << m->height << "x" << m->channels << " map where " << w << "x" << h
+       << "x" << c << " is needed";
+    throw Error(os.str());
+  }
+  return *m;
+}
+
 void SaveMap(const std::string& path, MapF&& m) {
   std::shared_ptr<MapStore> s = active_store();
   if (!s) {
@@ -928,7 +959,11 @@ void HPM::InuputInitialization(const std::string& dense_folder, const std::vecto
   }
   if (params_.geom_consistency) {  // HPM.cpp:693-732
     const char* suffix = multi_geometry_ ? "/depths_geom.dmb" : "/depths.dmb";
-    for (int id : ids) depths_.push_back(LoadMap(ResultFolder(dense_folder, id) + suffix));
+    for (size_t k = 0; k < ids.size(); ++k) {
+      const std::string path = ResultFolder(dense_folder, ids[k]) + suffix;
+      depths_.push_back(LoadMap(path));
+      Shaped(depths_.back(), cams_[k].width, cams_[k].height, 1, path);
+    }
   }
 }
 
@@ -954,10 +989,11 @@ void HPM::CudaSpaceInitialization(const std::string& dense_folder, const Problem
     const MapRef fine_ref = LoadMap(rf + "/depths.dmb");  // the JBU-upsampled depth
     const MapRef cn_ref = LoadMap(rf + "/normals.dmb");   // the coarser level's maps
     const MapRef cc_ref = LoadMap(rf + "/costs.dmb");
-    const MapF &fine_depth = *fine_ref, &cn = *cn_ref, &cc = *cc_ref;
-    const int cw = cn.width, ch = cn.height;
-    if (fine_depth.width != GetReferenceImageWidth() || fine_depth.height != GetReferenceImageHeight())
-      throw Error("hierarchy pass: depths.dmb does not have the reference image size");
+    const MapF& fine_depth =
+        Shaped(fine_ref, GetReferenceImageWidth(), GetReferenceImageHeight(), 1, rf + "/depths.dmb");
+    const int cw = cn_ref->width, ch = cn_ref->height;
+    const MapF& cn = Shaped(cn_ref, cw, ch, 3, rf + "/normals.dmb");
+    const MapF& cc = Shaped(cc_ref, cw, ch, 1, rf + "/costs.dmb");
     // the reference compares the coarse width with the fine rows (HPM.cpp:869)
     const bool upsample = cw != GetReferenceImageHeight() || ch != GetReferenceImageWidth();
     std::vector<float> coarse((size_t)4 * cw * ch);
@@ -978,7 +1014,9 @@ void HPM::CudaCannyInitialization(const MapU8& canny) {
 }
 
 void HPM::CudaConfidenceInitialization(const std::string& dense_folder, const std::vector<Problem>& problems, int idx) {
-  const MapRef conf = LoadMap(ResultFolder(dense_folder, problems.at(idx).ref_image_id) + "/confidence.dmb");
+  const std::string path = ResultFolder(dense_folder, problems.at(idx).ref_image_id) + "/confidence.dmb";
+  const MapRef conf = LoadMap(path);
+  Shaped(conf, GetReferenceImageWidth(), GetReferenceImageHeight(), 1, path);
   check(hpm_set_confidence(ctx_, conf->data.data()), "hpm_set_confidence");
 }
 
@@ -1419,8 +1457,10 @@ void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>&
     in_normals = LoadMap(rf + "/normals.dmb");
     const MapRef conf_ref = LoadMap(rf + "/confidence.dmb");
     in_costs = LoadMap(rf + "/costs.dmb");
-    const MapF &depths = *in_depths, &normals = *in_normals, &costs = *in_costs, &confidences = *conf_ref;
-    if (depths.width != width || depths.height != height) throw Error("prior pass: maps of another size in " + rf);
+    const MapF& depths = Shaped(in_depths, width, height, 1, rf + suffix);
+    const MapF& normals = Shaped(in_normals, width, height, 3, rf + "/normals.dmb");
+    const MapF& costs = Shaped(in_costs, width, height, 1, rf + "/costs.dmb");
+    const MapF& confidences = Shaped(conf_ref, width, height, 1, rf + "/confidence.dmb");
     // the prior texture map must exist (main.cpp:353-355, 458-460); its
     // texture flag is never used by the support-point selection
     const int tex_level = flags.image_scale + flags.hpm_scale_distance;
@@ -1494,7 +1534,7 @@ void JointBilateralUpsampling(const std::string& dense_folder, const Problem& pr
   } lap{clk};
   const std::string rf = ResultFolder(dense_folder, problem.ref_image_id);
   const MapRef coarse_ref = LoadMap(rf + "/depths_geom.dmb");
-  const MapF& coarse = *coarse_ref;
+  const MapF& coarse = Shaped(coarse_ref, coarse_ref->width, coarse_ref->height, 1, rf + "/depths_geom.dmb");
   const MapF& fine = images.CurrentGray(problem.ref_image_id);
   const int scale = std::max(fine.height / coarse.height, fine.width / coarse.width);  // HPM.cpp:1270
   if (scale == 1) return;  // HPM.cpp:1272-1275
@@ -1521,8 +1561,11 @@ static ViewMaps LoadViewMaps(const std::string& dense_folder, const std::vector<
   ViewMaps v;
   for (const Problem& p : problems) {
     const std::string rf = ResultFolder(dense_folder, p.ref_image_id);
-    v.depths.push_back(LoadMap(rf + (geom ? "/depths_geom.dmb" : "/depths.dmb")));
+    const std::string dpath = rf + (geom ? "/depths_geom.dmb" : "/depths.dmb");
+    v.depths.push_back(LoadMap(dpath));
+    const MapF& dm = Shaped(v.depths.back(), v.depths.back()->width, v.depths.back()->height, 1, dpath);
     v.normals.push_back(LoadMap(rf + "/normals.dmb"));
+    Shaped(v.normals.back(), dm.width, dm.height, 3, rf + "/normals.dmb");
     std::ostringstream cp;
     cp << dense_folder << "/cams/" << std::setw(8) << std::setfill('0') << p.ref_image_id << "_cam.txt";
     hpm_camera c = ReadCamera(cp.str());

@@ -22,6 +22,13 @@ int guarded(F&& f) {
     return HPM_E_DEVICE;
   }
 }
+
+// An argument check that fails before any work: HPM_E_INVALID, with the
+// function's name as the message.
+int bad_args(const char* fn) {
+  g_error = std::string(fn) + ": invalid argument";
+  return HPM_E_INVALID;
+}
 }  // namespace
 
 extern "C" {
@@ -30,7 +37,7 @@ const char* hpm_host_last_error(void) { return g_error.c_str(); }
 
 int hpm_host_process_problem(const char* dense_folder, int idx, int geom, int prior, int mand, int multi_geometry,
                              int precision, uint64_t seed) {
-  if (!dense_folder) return HPM_E_INVALID;
+  if (!dense_folder) return bad_args(__func__);
   return guarded([&] {
     const auto problems = hpmhost::ReadPairFile(std::string(dense_folder) + "/pair.txt");
     if (idx < 0 || idx >= (int)problems.size()) throw hpmhost::Error("problem index out of range");
@@ -44,7 +51,7 @@ int hpm_host_process_problem(const char* dense_folder, int idx, int geom, int pr
 }
 
 int hpm_host_confidence(const char* dense_folder, int geom) {
-  if (!dense_folder) return HPM_E_INVALID;
+  if (!dense_folder) return bad_args(__func__);
   return guarded([&] {
     hpmhost::ConfidenceEvaluation(dense_folder, hpmhost::ReadPairFile(std::string(dense_folder) + "/pair.txt"),
                                   geom != 0);
@@ -52,7 +59,7 @@ int hpm_host_confidence(const char* dense_folder, int geom) {
 }
 
 int hpm_host_fusion(const char* dense_folder, int geom, int64_t* num_points) {
-  if (!dense_folder || !num_points) return HPM_E_INVALID;
+  if (!dense_folder || !num_points) return bad_args(__func__);
   return guarded([&] {
     *num_points = (int64_t)hpmhost::RunFusion(dense_folder,
                                                hpmhost::ReadPairFile(std::string(dense_folder) + "/pair.txt"),
@@ -61,13 +68,13 @@ int hpm_host_fusion(const char* dense_folder, int geom, int64_t* num_points) {
 }
 
 int hpm_host_run_pipeline(const char* dense_folder, int precision, uint64_t seed, int64_t* num_points) {
-  if (!dense_folder || !num_points) return HPM_E_INVALID;
+  if (!dense_folder || !num_points) return bad_args(__func__);
   return guarded([&] { *num_points = (int64_t)hpmhost::RunPipeline(dense_folder, precision, seed).size(); });
 }
 
 int hpm_host_run_pyramid_pipeline(const char* dense_folder, int initial_image_scale, int size_bound, int precision,
                                   uint64_t seed, int mask_fusion, int64_t* num_points) {
-  if (!dense_folder || !num_points || initial_image_scale < 1 || size_bound < 1) return HPM_E_INVALID;
+  if (!dense_folder || !num_points || initial_image_scale < 1 || size_bound < 1) return bad_args(__func__);
   return guarded([&] {
     *num_points = (int64_t)hpmhost::RunPyramidPipeline(dense_folder, initial_image_scale, size_bound, precision, seed,
                                                        mask_fusion != 0)
@@ -76,7 +83,7 @@ int hpm_host_run_pyramid_pipeline(const char* dense_folder, int initial_image_sc
 }
 
 int hpm_host_fusion_mask(const char* dense_folder, int geom, int64_t* num_points) {
-  if (!dense_folder || !num_points) return HPM_E_INVALID;
+  if (!dense_folder || !num_points) return bad_args(__func__);
   return guarded([&] {
     *num_points = (int64_t)hpmhost::RunFusionSkyStrict(
                       dense_folder, hpmhost::ReadPairFile(std::string(dense_folder) + "/pair.txt"), geom != 0)
@@ -105,7 +112,7 @@ void hpm_host_set_confidence_overlap(int enable) { hpmhost::SetConfidenceOverlap
 
 int hpm_host_decode_jpeg(const uint8_t* data, int64_t size, int* w, int* h, int* channels, uint8_t* out,
                          int64_t capacity) {
-  if (!data || size <= 0 || !w || !h || !channels) return HPM_E_INVALID;
+  if (!data || size <= 0 || !w || !h || !channels) return bad_args(__func__);
   return guarded([&] {
     hpmhost::MapU8 img;
     std::string err;
@@ -121,7 +128,7 @@ int hpm_host_decode_jpeg(const uint8_t* data, int64_t size, int* w, int* h, int*
 
 int hpm_host_support_points(const float* costs, const float* confidences, int w, int h, int32_t* out_xy,
                             int capacity, int* count) {
-  if (!costs || !confidences || w <= 0 || h <= 0 || !count || (capacity > 0 && !out_xy)) return HPM_E_INVALID;
+  if (!costs || !confidences || w <= 0 || h <= 0 || !count || (capacity > 0 && !out_xy)) return bad_args(__func__);
   return guarded([&] {
     hpmhost::MapF c(w, h), f(w, h);
     std::memcpy(c.data.data(), costs, sizeof(float) * w * h);
@@ -136,8 +143,8 @@ int hpm_host_support_points(const float* costs, const float* confidences, int w,
 }
 
 int hpm_host_delaunay(int w, int h, const int32_t* xy, int n, int32_t* out_tris, int capacity, int* count) {
-  if (w <= 0 || h <= 0 || n < 0 || (n > 0 && !xy) || !count || (capacity > 0 && !out_tris)) return HPM_E_INVALID;
-  if (w > (1 << 20) || h > (1 << 20)) return HPM_E_INVALID;
+  if (w <= 0 || h <= 0 || n < 0 || (n > 0 && !xy) || !count || (capacity > 0 && !out_tris)) return bad_args(__func__);
+  if (w > (1 << 20) || h > (1 << 20)) return bad_args(__func__);
   return guarded([&] {
     std::vector<hpmhost::Point2i> pts(n);
     for (int i = 0; i < n; ++i) pts[i] = {xy[2 * i], xy[2 * i + 1]};
@@ -152,7 +159,7 @@ int hpm_host_delaunay(int w, int h, const int32_t* xy, int n, int32_t* out_tris,
 }
 
 int hpm_host_raster(const int32_t* tris, int n, int w, int h, float* mask) {
-  if (w <= 0 || h <= 0 || n < 0 || (n > 0 && !tris) || !mask) return HPM_E_INVALID;
+  if (w <= 0 || h <= 0 || n < 0 || (n > 0 && !tris) || !mask) return bad_args(__func__);
   return guarded([&] {
     std::vector<hpmhost::Triangle> t(n);
     for (int i = 0; i < n; ++i)
@@ -164,12 +171,12 @@ int hpm_host_raster(const int32_t* tris, int n, int w, int h, float* mask) {
 }
 
 int hpm_host_plane_through_points(const float* X1, const float* X2, const float* X3, float* plane) {
-  if (!X1 || !X2 || !X3 || !plane) return HPM_E_INVALID;
+  if (!X1 || !X2 || !X3 || !plane) return bad_args(__func__);
   return guarded([&] { hpmhost::PlaneThroughPoints(X1, X2, X3, plane); });
 }
 
 int hpm_host_canny(const uint8_t* image, int w, int h, int channels, float low, float high, uint8_t* out) {
-  if (!image || !out || w <= 0 || h <= 0 || (channels != 1 && channels != 3)) return HPM_E_INVALID;
+  if (!image || !out || w <= 0 || h <= 0 || (channels != 1 && channels != 3)) return bad_args(__func__);
   return guarded([&] {
     hpmhost::MapU8 g(w, h, channels);
     std::memcpy(g.data.data(), image, (size_t)w * h * channels);
