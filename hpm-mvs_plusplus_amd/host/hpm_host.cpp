@@ -1415,11 +1415,29 @@ void ConfidenceThenPass(const std::string& dense_folder, const std::vector<Probl
 
 void SetConfidenceOverlap(int enable) { g_confidence_overlap = enable != 0; }
 
+// A PatchMatch pass takes the reference and 1 to HPM_MAX_IMAGES - 1 source
+// views, a repeated source counting each time.  main.cpp:1209-1220 refuses
+// a problem over MAX_IMAGES before any pass; a view whose sources all have
+// score <= 0 in pair.txt is refused here as well, since its costs would be
+// averages over no view.
+static void CheckSources(const Problem& p) {
+  if (p.src_image_ids.empty())
+    throw Error("view " + std::to_string(p.ref_image_id) + " has no source view with a positive score in pair.txt");
+  if (1 + p.src_image_ids.size() > (size_t)HPM_MAX_IMAGES)
+    throw Error("view " + std::to_string(p.ref_image_id) + " needs " + std::to_string(1 + p.src_image_ids.size()) +
+                " images, more than HPM_MAX_IMAGES = " + std::to_string(HPM_MAX_IMAGES));
+}
+
+static void CheckSources(const std::vector<Problem>& problems) {
+  for (const Problem& p : problems) CheckSources(p);
+}
+
 void ProcessProblem(const std::string& dense_folder, const std::vector<Problem>& problems, int idx,
                     const PassFlags& flags, int precision, uint64_t seed, SceneImages* images) {
   StageClock clk;
   double t_setup = 0, t_edges = 0, t_prior = 0, t_pm = 0, t_write = 0;
   const Problem& problem = problems.at(idx);
+  CheckSources(problem);
   MakeDir(dense_folder + "/HPM_MVS_plusplus");
   const std::string rf = ResultFolder(dense_folder, problem.ref_image_id);
   MakeDir(rf);
@@ -1814,6 +1832,7 @@ std::vector<FusedPoint> RunFusionSkyStrict(const std::string& dense_folder, cons
 std::vector<FusedPoint> RunPipeline(const std::string& dense_folder, int precision, uint64_t seed) {
   MapStoreScope store;  // the passes' maps stay in memory for the passes after them
   const std::vector<Problem> problems = ReadPairFile(dense_folder + "/pair.txt");
+  CheckSources(problems);
   const int n = (int)problems.size();
   PassFlags f;
   parallel_views(n, [&](int i) { ProcessProblem(dense_folder, problems, i, f, precision, seed + i); });  // photometric
@@ -1858,6 +1877,7 @@ std::vector<FusedPoint> RunPyramidPipeline(const std::string& dense_folder, int 
   if (size_bound < 1) throw Error("size bound must be >= 1");
   MapStoreScope store;  // the passes' maps stay in memory for the passes after them
   const std::vector<Problem> problems = ReadPairFile(dense_folder + "/pair.txt");
+  CheckSources(problems);
   const int n = (int)problems.size();
   MakeDir(dense_folder + "/HPM_MVS_plusplus");
   SceneImages images(dense_folder);

@@ -131,6 +131,32 @@ def test_source_without_pair_entry(folder):
     _invalid(_fusion(folder), "source image without a pair.txt entry")
 
 
+def test_source_count_checked_before_any_pass(folder):
+    """main.cpp:1209-1220 refuses a problem over MAX_IMAGES before any pass;
+    a view whose sources all score <= 0 (GenerateSampleList drops them,
+    main.cpp:193-195) has no source at all.  Both are refused up front by
+    the pipelines and by ProcessProblem, naming the view."""
+    L = _lib()
+    n = ctypes.c_int64(0)
+    srcs = " ".join(f"{j % 2 + 1} 1.0" for j in range(33))
+    for text, needle in [
+        ("3\n0\n2 1 10.0 2 5.0\n1\n2 0 0.0 2 -1.0\n2\n2 0 10.0 1 5.0\n",
+         "view 1 has no source view with a positive score in pair.txt"),
+        (f"3\n0\n33 {srcs}\n1\n2 0 10.0 2 5.0\n2\n2 0 10.0 1 5.0\n",
+         "view 0 needs 34 images, more than HPM_MAX_IMAGES = 33"),
+    ]:
+        with open(os.path.join(folder, "pair.txt"), "w") as fh:
+            fh.write(text)
+        assert L.hpm_host_run_pyramid_pipeline(folder.encode(), 1, 1000, 0, 1, 0, ctypes.byref(n)) == HPM_E_INVALID
+        assert L.hpm_host_last_error().decode() == needle
+        assert L.hpm_host_run_pipeline(folder.encode(), 0, 1, ctypes.byref(n)) == HPM_E_INVALID
+        assert L.hpm_host_last_error().decode() == needle
+        idx = int(needle.split()[1])
+        assert L.hpm_host_process_problem(folder.encode(), idx, 0, 0, 0, 0, 0, 1) == HPM_E_INVALID
+        assert L.hpm_host_last_error().decode() == needle
+    assert not os.path.exists(os.path.join(folder, "HPM_MVS_plusplus", "HPM_MVS_plusplus.ply"))
+
+
 def test_problem_index_out_of_range(folder):
     L = _lib()
     for idx in (3, -1, 1 << 30):
