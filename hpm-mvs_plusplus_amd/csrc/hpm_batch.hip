@@ -72,6 +72,7 @@ int ensure_layout(hpm_batch* b) {
     if (c->nimg - 1 > b->max_v) b->max_v = c->nimg - 1;
   }
   free_layout(b);
+  b->layout_key.clear();  // set again only once every table is in place
   BHIP_OK(hipMalloc((void**)&b->d_consts, sizeof(RunConsts) * n));
   BHIP_OK(hipMalloc((void**)&b->d_states, sizeof(DevState) * n));
   BHIP_OK(hipMalloc((void**)&b->d_views, sizeof(MultiView) * n));

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <mutex>
 #include <cstdint>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -37,13 +38,26 @@ int fail(hpm_ctx* c, int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(ctx, HPM_E_DEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
   } while (0)
 
+// True for the allocation that HPM_FAULT_ALLOC_AFTER names (counted per
+// context over every device allocation it makes); that one fails instead of
+// reaching the device.
+bool injected_fault(hpm_ctx* ctx) {
+  if (ctx->fault_alloc < 0) return false;
+  return ctx->fault_alloc-- == 0;
+}
+
+hipError_t ctx_malloc_async(hpm_ctx* ctx, void** p, size_t bytes) {
+  if (injected_fault(ctx)) return hipErrorOutOfMemory;
+  return hipMallocAsync(p, bytes, ctx->stream);
+}
+
 // Context buffers are allocated and freed in the order of the context's
 // stream: unlike hipFree, hipFreeAsync does not wait for the whole device,
 // so contexts driven from other host threads keep their work running.
 template <class T> int dalloc(hpm_ctx* ctx, T** p, size_t n) {
   if (*p) { hipFreeAsync(*p, ctx->stream); *p = nullptr; }
   if (n == 0) return HPM_OK;
-  hipError_t e = hipMallocAsync((void**)p, n * sizeof(T), ctx->stream);
+  hipError_t e = ctx_malloc_async(ctx, (void**)p, n * sizeof(T));
   if (e != hipSuccess) { *p = nullptr; return fail(ctx, HPM_E_NOMEM, std::string("hipMallocAsync: ") + hipGetErrorString(e)); }
   return HPM_OK;
 }
@@ -176,10 +190,11 @@ template <class S> int build_quads(hpm_ctx* ctx) {
                                       "use HPM_PREC_FP32_EXACT for larger images");
   for (void* p : ctx->d_img_lp) if (p) hipFreeAsync(p, ctx->stream);
   ctx->d_img_lp.assign(ctx->nimg, nullptr);
+  ctx->lp_kind = -1;  // until every view's quads are built
   for (int i = 0; i < ctx->nimg; ++i) {
     const int w = ctx->cams[i].width, h = ctx->cams[i].height;
     const size_t n = (size_t)(w + 1) * (h + 1);
-    hipError_t e = hipMallocAsync(&ctx->d_img_lp[i], n * sizeof(typename S::Q), ctx->stream);
+    hipError_t e = ctx_malloc_async(ctx, &ctx->d_img_lp[i], n * sizeof(typename S::Q));
     if (e != hipSuccess) { ctx->d_img_lp[i] = nullptr; return fail(ctx, HPM_E_NOMEM, "quad image allocation failed"); }
     const dim3 b(16, 16), g((w + 1 + 15) / 16, (h + 1 + 15) / 16);
     hipLaunchKernelGGL(k_build_quad<S>, g, b, 0, ctx->stream, ctx->d_img[i], w, h, ctx->d_img_lp[i]);
@@ -200,8 +215,11 @@ template <class S> int build_pool_quads(hpm_ctx* ctx) {
     q.assign(pool->d_img.size(), nullptr);
     for (size_t i = 0; i < pool->d_img.size(); ++i) {
       const int w = pool->cams[i].width, h = pool->cams[i].height;
-      if (hipMalloc(&q[i], (size_t)(w + 1) * (h + 1) * sizeof(typename S::Q)) != hipSuccess) {
-        q[i] = nullptr;
+      if (injected_fault(ctx) || hipMalloc(&q[i], (size_t)(w + 1) * (h + 1) * sizeof(typename S::Q)) != hipSuccess) {
+        // no partial set stays behind: the next run builds them all again
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        for (void* p : q) if (p) hipFree(p);
+        q.clear();
         return fail(ctx, HPM_E_NOMEM, "quad image allocation failed");
       }
       const dim3 b(16, 16), g((w + 1 + 15) / 16, (h + 1 + 15) / 16);
@@ -383,6 +401,7 @@ int hpm_create(hpm_ctx** out, int device) {
     return HPM_E_DEVICE;
   }
   keep_pool_memory(device);
+  if (const char* f = std::getenv("HPM_FAULT_ALLOC_AFTER")) ctx->fault_alloc = std::atoll(f);
   if (hipMallocAsync((void**)&ctx->d_consts, sizeof(RunConsts), ctx->stream) != hipSuccess ||
       hipMallocAsync((void**)&ctx->d_counters, 8 * sizeof(unsigned long long), ctx->stream) != hipSuccess ||
       hipStreamSynchronize(ctx->stream) != hipSuccess) {
@@ -423,8 +442,31 @@ int hpm_set_params(hpm_ctx* ctx, const hpm_params* params) {
 
 static int alloc_pixel_state(hpm_ctx* ctx);
 
+static int set_views(hpm_ctx* ctx, int num_images, const hpm_camera* cameras, const float* const* images,
+                     bool& dropped);
+static int set_views_pooled(hpm_ctx* ctx, hpm_pool* pool, int num_images, const int32_t* ids, bool& dropped);
+
+// A call refused for its arguments keeps the views the context had.  One
+// that fails after the old views were dropped leaves the context without
+// views (runs then return HPM_E_STATE), never with a part of them.
 int hpm_set_views(hpm_ctx* ctx, int num_images, const hpm_camera* cameras, const float* const* images) {
   if (!ctx) return HPM_E_INVALID;
+  bool dropped = false;
+  const int rc = set_views(ctx, num_images, cameras, images, dropped);
+  if (rc != HPM_OK && dropped) free_views(ctx);
+  return rc;
+}
+
+int hpm_set_views_pooled(hpm_ctx* ctx, hpm_pool* pool, int num_images, const int32_t* ids) {
+  if (!ctx) return HPM_E_INVALID;
+  bool dropped = false;
+  const int rc = set_views_pooled(ctx, pool, num_images, ids, dropped);
+  if (rc != HPM_OK && dropped) free_views(ctx);
+  return rc;
+}
+
+static int set_views(hpm_ctx* ctx, int num_images, const hpm_camera* cameras, const float* const* images,
+                     bool& dropped) {
   if (!cameras || !images) return fail(ctx, HPM_E_INVALID, "null cameras or images");
   if (num_images < 2 || num_images > HPM_MAX_IMAGES)
     return fail(ctx, HPM_E_INVALID, "num_images must be in [2, 33] (view masks are 32-bit)");
@@ -438,6 +480,7 @@ int hpm_set_views(hpm_ctx* ctx, int num_images, const hpm_camera* cameras, const
   HIP_OK(hipSetDevice(ctx->device));
   HIP_OK(hipStreamSynchronize(ctx->stream));
   free_views(ctx);
+  dropped = true;
   ctx->nimg = num_images;
   ctx->cams.assign(cameras, cameras + num_images);
   ctx->W = cameras[0].width;
@@ -490,8 +533,7 @@ void hpm_pool_destroy(hpm_pool* pool) {
   delete pool;
 }
 
-int hpm_set_views_pooled(hpm_ctx* ctx, hpm_pool* pool, int num_images, const int32_t* ids) {
-  if (!ctx) return HPM_E_INVALID;
+static int set_views_pooled(hpm_ctx* ctx, hpm_pool* pool, int num_images, const int32_t* ids, bool& dropped) {
   if (!pool || !ids) return fail(ctx, HPM_E_INVALID, "null pool or ids");
   if (pool->device != ctx->device) return fail(ctx, HPM_E_INVALID, "pool and context are on different devices");
   if (num_images < 2 || num_images > HPM_MAX_IMAGES)
@@ -503,6 +545,7 @@ int hpm_set_views_pooled(hpm_ctx* ctx, hpm_pool* pool, int num_images, const int
   HIP_OK(hipSetDevice(ctx->device));
   HIP_OK(hipStreamSynchronize(ctx->stream));
   free_views(ctx);
+  dropped = true;
   ctx->pool = pool;
   ctx->pool_ids.assign(ids, ids + num_images);
   ctx->nimg = num_images;
@@ -536,9 +579,22 @@ static int alloc_pixel_state(hpm_ctx* ctx) {
   return HPM_OK;
 }
 
+static int set_src_depths(hpm_ctx* ctx, const float* const* depths);
+
+// A failed hpm_set_src_depths leaves no source depths on the context
+// (geometric runs then return HPM_E_STATE).
 int hpm_set_src_depths(hpm_ctx* ctx, const float* const* depths) {
   if (!ctx || !depths) return HPM_E_INVALID;
   if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
+  const int rc = set_src_depths(ctx, depths);
+  if (rc != HPM_OK) {
+    for (float* p : ctx->d_dimg) if (p) hipFreeAsync(p, ctx->stream);
+    ctx->d_dimg.clear();
+  }
+  return rc;
+}
+
+static int set_src_depths(hpm_ctx* ctx, const float* const* depths) {
   HIP_OK(hipSetDevice(ctx->device));
   for (float* p : ctx->d_dimg) if (p) hipFreeAsync(p, ctx->stream);
   ctx->d_dimg.assign(ctx->nimg, nullptr);
@@ -615,8 +671,10 @@ int ctx_scratch(hpm_ctx* ctx, const std::vector<size_t>& sizes, std::vector<char
     if (ctx->d_scratch) HIP_OK(hipFreeAsync(ctx->d_scratch, ctx->stream));
     ctx->d_scratch = nullptr;
     ctx->scratch_bytes = 0;
-    if (hipMallocAsync(&ctx->d_scratch, total, ctx->stream) != hipSuccess)
+    if (ctx_malloc_async(ctx, (void**)&ctx->d_scratch, total) != hipSuccess) {
+      ctx->d_scratch = nullptr;
       return fail(ctx, HPM_E_NOMEM, "hipMallocAsync (scratch)");
+    }
     ctx->scratch_bytes = total;
   }
   out.clear();
@@ -632,6 +690,7 @@ int hpm_prior_support_points(hpm_ctx* ctx, int distance, int32_t* out_xy, int ca
   if (!ctx || !count || distance < 0 || distance > 2 || capacity < 0 || (capacity > 0 && !out_xy)) return HPM_E_INVALID;
   *count = 0;
   if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
+  ctx->prior_dist = -1;  // set again once the support points are found: a failed call leaves nothing to build on
   HIP_OK(hipSetDevice(ctx->device));
   const int W = ctx->W, H = ctx->H;
   FloatMap depth{(const float*)ctx->d_planes + 3, W, H, 4}, cost{ctx->d_costs, W, H, 1}, conf{ctx->d_conf, W, H, 1};
@@ -1156,7 +1215,7 @@ static int build_smoothed(hpm_ctx* ctx, float sigma, SmoothedViews& S) {
   for (int i = 0; i < ctx->nimg && rc == HPM_OK; ++i) {
     const int w = ctx->cams[i].width, h = ctx->cams[i].height;
     if ((rc = dalloc(ctx, &S.img[i], (size_t)w * h))) break;
-    if (hipMalloc(&S.quad[i], (size_t)(w + 1) * (h + 1) * sizeof(QuadF32::Q)) != hipSuccess) {
+    if (injected_fault(ctx) || hipMalloc(&S.quad[i], (size_t)(w + 1) * (h + 1) * sizeof(QuadF32::Q)) != hipSuccess) {
       S.quad[i] = nullptr;
       rc = fail(ctx, HPM_E_NOMEM, "smoothed quad allocation failed");
       break;

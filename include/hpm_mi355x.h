@@ -15,6 +15,16 @@
  *     per function, following the reference.
  *   - One context owns one HIP stream on one device; calls on a context are not
  *     thread-safe (the reference is single-threaded too, main.cpp:250).
+ *   - A call that fails leaves no half-built state on the context: a failed
+ *     hpm_set_views / hpm_set_views_pooled that got past its argument checks
+ *     leaves the context without views, a failed hpm_set_src_depths without
+ *     source depths, a failed hpm_prior_support_points without support
+ *     points (later calls that need them return HPM_E_STATE), and a failed
+ *     run or refinement rebuilds what it needs when called again.
+ *   - Fault injection for tests: with HPM_FAULT_ALLOC_AFTER=n in the
+ *     environment when hpm_create runs, that context's device allocation
+ *     number n (counted from 0 over the buffers it allocates after creation)
+ *     fails once with HPM_E_NOMEM, as if the device were out of memory.
  */
 #ifndef HPM_MI355X_H_
 #define HPM_MI355X_H_
