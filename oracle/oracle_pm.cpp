@@ -1073,13 +1073,23 @@ template <class T> void SetupThresholds(Ctx<T>& c) {
   }
 }
 
+// Sweep order inside a colour: 0 = rows top-down, columns left to right;
+// 1 = rows bottom-up, columns right to left.  Every pixel of a colour reads
+// only pixels of the other colour, so the order cannot change any result;
+// the reversed sweep exists to test exactly that.
+static int g_reverse_sweep = 0;
+
 // Red/black pass: colour 0 = black = (x + y) even (HPM.cu:2793-2809), 1 = red.
 template <class T> void ColourPass(Ctx<T>& c, int colour, int iter) {
   const int H2 = (c.H / 2) * 2;  // an odd last row is never visited (Q9, HPM.cu:3006)
   const int r0 = std::max(0, c.row_begin), r1 = std::min(H2, c.row_end);
+  const bool rev = g_reverse_sweep != 0;
 #pragma omp parallel for schedule(dynamic, 1)
-  for (int y = r0; y < r1; ++y) {
-    for (int x = ((y + colour) & 1); x < c.W; x += 2) {
+  for (int k = r0; k < r1; ++k) {
+    const int y = rev ? r1 - 1 - (k - r0) : k;
+    const int x0 = (y + colour) & 1, n = (c.W - x0 + 1) / 2;
+    for (int i = 0; i < n; ++i) {
+      const int x = x0 + 2 * (rev ? n - 1 - i : i);
       if (c.prm.mand_consistency) CheckerboardPropagationMand(c, x, y, iter);
       else CheckerboardPropagation(c, x, y, iter);
     }
@@ -1308,6 +1318,9 @@ template <class T> int GeomCostImpl(const orc_camera* cams, int num_images, cons
 }  // namespace
 
 extern "C" {
+
+void orc_set_reverse_sweep(int reverse) { g_reverse_sweep = reverse; }
+
 
 int orc_run_patchmatch_f32(const orc_camera* cams, int num_images, const float* const* images,
                            const float* const* depth_images, const orc_params* params, uint64_t seed, float* planes,

@@ -47,6 +47,7 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         f32 = ctypes.c_float
         L.orc_expf.argtypes = [f32]; L.orc_expf.restype = f32
+        L.orc_set_reverse_sweep.argtypes = [ctypes.c_int]; L.orc_set_reverse_sweep.restype = None
         L.orc_acosf.argtypes = [f32]; L.orc_acosf.restype = f32
         L.orc_cost_grad_f64.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
