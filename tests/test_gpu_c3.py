@@ -1,6 +1,6 @@
 """The C3 scene at full size -- 3072 x 2048, 30 cameras on two rings, 24
-plane patches and 6 spheres, V = 10 -- on two designated reference views
-(7 on the low ring, 22 on the high one), against the FP64 oracle and
+plane patches and 6 spheres, V = 10 -- on three designated reference views
+(7 on the low ring, 15 and 22 on the high one), against the FP64 oracle and
 against ground truth.  The oracle is far too slow for whole 3072 x 2048
 views, so it runs on a band of 256 rows:
 
@@ -13,8 +13,12 @@ views, so it runs on a band of 256 rows:
   band on the band's central 128 rows.  A band run lacks what propagates in
   from the rows outside it, and PatchMatch turns one different adoption into
   a different patch, so that agreement is bounded loosely (depth within 1 %
-  of FP64 on more than 95 % of those pixels; 98 % measured on MI355X) and
-  the ground-truth accuracy of the two is compared on the same rows.
+  of FP64 on more than 93 % of those pixels; 95-98 % measured on MI355X).
+  The parity bound is the ground-truth accuracy of the two on the same rows:
+  the GPU's within 0.02 of FP64's or better.  View 15, at the end of the high
+  ring with every source to one side, is the hardest of the three: about
+  0.76 of its pixels lie within 1 % of ground truth after one photometric
+  pass (0.90 for view 7), for the FP64 oracle as for the GPU.
 """
 import time
 
@@ -26,7 +30,7 @@ from conftest import gpu_available
 
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="no HIP device")]
 
-REFS = (7, 22)
+REFS = (7, 15, 22)
 SEED = 5
 
 
@@ -63,7 +67,7 @@ def test_c3_cost_vs_fp64_band(c3_scene, ref):
     assert np.quantile(err, 0.999) <= 1e-5 and err.mean() <= 2e-6
 
 
-@pytest.mark.parametrize("ref", REFS[:1])
+@pytest.mark.parametrize("ref", REFS)
 def test_c3_patchmatch_vs_gt_and_fp64_band(c3_scene, ref):
     import hpm_mi355x as H
     from hpm_mi355x import scenes
@@ -101,6 +105,6 @@ def test_c3_patchmatch_vs_gt_and_fp64_band(c3_scene, ref):
           f"FP64 {gt64:.3f}; GPU vs FP64: depth within 1 % {np.mean(rel <= 0.01):.3f}, normal within 5 deg "
           f"{np.mean(ang <= 5.0):.3f}")
     gpu_band = float(np.mean(np.abs(d[band][hb] - gt_band[hb]) / gt_band[hb] <= 0.01))
-    assert gt_frac > 0.85
+    assert gt_frac > 0.70
     assert gpu_band >= gt64 - 0.02  # as close to the ground truth as FP64 on the same rows
-    assert np.mean(rel <= 0.01) > 0.95
+    assert np.mean(rel <= 0.01) > 0.93
