@@ -47,6 +47,14 @@ runs the full pyramid pipeline with fusion over a synthetic dense folder
 reports seconds per depth map, the host stage breakdown and the accuracy
 against ground truth;
 
+    python bench.py --workload c3view
+
+times RunPatchMatch on view 7 of the 3072x2048 scene with 10 sources (the
+same k_propagate as the headline at the size where the 11 FP16 quad images,
+528 MiB, exceed the 256 MB Infinity Cache): k_propagate ms per launch, NCC
+evaluations per launch, ns per evaluation and algorithmic GB/s, for the
+counter runs of that size;
+
     python bench.py --workload gradient
 
 reports the plane-cost gradient at 1600x1200 with 8 sources (time, checks
@@ -372,6 +380,41 @@ def gradient_bench(reps=5):
             "data": "synthetic (seeded ray-cast: 12 textured planes + background, 9 cameras on a 30 deg arc)"}
 
 
+def c3view_bench(precision="fp16", steps=3, warmup=1, ref=7):
+    """RunPatchMatch of one designated reference view of the 3072x2048 orbit
+    scene (30 cameras on two rings, V = 10), timed as run_views times the
+    headline view."""
+    import hpm_mi355x as Hm
+    from hpm_mi355x import scenes
+    t0 = time.perf_counter()
+    sc = scenes.make_orbit_scene(seed=3, render_refs=(ref,))
+    ids, cams, imgs = scenes.reference_view_problem(sc, ref, NSRC)
+    print(f"[bench] c3 view {ref} rendered in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    ctx = Hm.Context(0)
+    ctx.set_params(Hm.default_params(precision=precision))
+    ctx.set_views([Hm.camera_from_scene(c) for c in cams], imgs)
+    r = run_views(ctx, steps, warmup, SEED)
+    planes = ctx.planes()
+    ctx.close()
+    gt = sc.depths[ref]
+    hit = gt > 0
+    bpt = {"fp32": 4, "fp32fast": 4, "fp16": 2, "bf16": 2}[precision]
+    launches = max(r["launches"], 1)
+    evals = r["ncc_prop"] / launches
+    ms = r["prop_ms"] / launches
+    alg = evals * 181 * bpt + (evals / NSRC) * (16 + 4 * NSRC)
+    quad_bytes = {"fp32": 16, "fp32fast": 16, "fp16": 8, "bf16": 8}[precision]
+    w, h = cams[0].width, cams[0].height
+    return {"workload": f"RunPatchMatch photometric, view {ref} of the orbit scene, {w}x{h}, {NSRC} sources",
+            "precision": precision, "views_s": round(r["views_s"], 3), "k_propagate_avg_ms": round(ms, 3),
+            "ncc_evals_per_launch": int(evals), "ns_per_ncc": round(1e6 * ms / evals, 4),
+            "alg_bytes_per_launch": int(alg), "alg_gbs": round(alg / (ms / 1e3) / 1e9, 1),
+            "image_footprint_mib": None if precision == "fp32" else
+            round(len(cams) * (w + 1) * (h + 1) * quad_bytes / 2 ** 20, 1),
+            "depth_within_1pct_of_gt": round(float(np.mean(np.abs(planes[..., 3][hit] - gt[hit]) / gt[hit] <= 0.01)), 4),
+            "steps": steps, "warmup": warmup}
+
+
 def cost_microbench(reps=20):
     """Cost at fixed hypotheses (hpm_cost_fixed_device): 1600x1200 reference +
     10 sources, one random plane per pixel (depth U[1, 5], normal within 60 deg
@@ -426,11 +469,13 @@ def cost_microbench(reps=20):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="patchmatch", choices=["patchmatch", "cost", "uneven", "pipeline", "gradient"],
+    ap.add_argument("--workload", default="patchmatch",
+                    choices=["patchmatch", "cost", "uneven", "pipeline", "gradient", "c3view"],
                     help="patchmatch: the RunPatchMatch throughput line; cost: the cost microbenchmark report; "
                          "uneven: 64 reference views of uneven size, one-at-a-time vs batched; "
                          "pipeline: the full pyramid pipeline over a dense folder (--config c1|c3); "
-                         "gradient: plane-cost gradient and Gauss-Newton vs random refinement")
+                         "gradient: plane-cost gradient and Gauss-Newton vs random refinement; "
+                         "c3view: RunPatchMatch of one 3072x2048 view with 10 sources")
     ap.add_argument("--config", default="c1", choices=["c1", "c3"], help="pipeline workload: scene")
     ap.add_argument("--scale", type=int, default=None, help="pipeline workload: initial image scale")
     ap.add_argument("--batch-mode", default="fused", choices=["fused", "streams"],
@@ -463,6 +508,9 @@ def main():
         return
     if args.workload == "gradient":
         print(json.dumps(gradient_bench()), flush=True)
+        return
+    if args.workload == "c3view":
+        print(json.dumps(c3view_bench(args.precision, args.steps, args.warmup)), flush=True)
         return
     if args.workload == "pipeline":
         print(json.dumps(pipeline_bench(args.config, args.precision, args.scale)), flush=True)
