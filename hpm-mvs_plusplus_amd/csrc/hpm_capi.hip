@@ -39,8 +39,8 @@ int fail(hpm_ctx* c, int code, const std::string& msg) {
   } while (0)
 
 // True for the allocation that HPM_FAULT_ALLOC_AFTER names (counted per
-// context over every device allocation it makes); that one fails instead of
-// reaching the device.
+// context over the device allocations it makes after hpm_create); that one
+// fails instead of reaching the device.
 bool injected_fault(hpm_ctx* ctx) {
   if (ctx->fault_alloc < 0) return false;
   return ctx->fault_alloc-- == 0;
@@ -401,7 +401,11 @@ int hpm_create(hpm_ctx** out, int device) {
     return HPM_E_DEVICE;
   }
   keep_pool_memory(device);
-  if (const char* f = std::getenv("HPM_FAULT_ALLOC_AFTER")) ctx->fault_alloc = std::atoll(f);
+  if (const char* f = std::getenv("HPM_FAULT_ALLOC_AFTER")) {  // a non-negative integer, else ignored
+    char* end = nullptr;
+    const long long n = std::strtoll(f, &end, 10);
+    if (end != f && *end == '\0' && n >= 0) ctx->fault_alloc = n;
+  }
   if (hipMallocAsync((void**)&ctx->d_consts, sizeof(RunConsts), ctx->stream) != hipSuccess ||
       hipMallocAsync((void**)&ctx->d_counters, 8 * sizeof(unsigned long long), ctx->stream) != hipSuccess ||
       hipStreamSynchronize(ctx->stream) != hipSuccess) {

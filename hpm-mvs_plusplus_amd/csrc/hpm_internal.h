@@ -44,8 +44,8 @@ struct hpm_ctx {
   std::vector<void*> d_img_lp;     // FP16 / BF16 copies (fast paths)
   int lp_kind = -1;                // precision of d_img_lp
   // Allocation fault injection: with HPM_FAULT_ALLOC_AFTER=n in the
-  // environment at hpm_create, the context's allocation number n (from 0)
-  // fails as if the device were out of memory, once; -1 = off.
+  // environment at hpm_create, the context's allocation number n (from 0,
+  // after creation) fails as if the device were out of memory, once; -1 = off.
   long long fault_alloc = -1;
   float4* d_planes = nullptr;
   float* d_costs = nullptr;
