@@ -386,6 +386,9 @@ void SetPassThreads(int n);
 // (hpm_prior_support_points / hpm_prior_build; only the Delaunay
 // triangulation runs on the host), 0 on the host.  The files are identical.
 void SetPriorOnDevice(int enable);
+// Reference views of at least n pixels run as a one-view fused launch with
+// 16-column tiles (mixed-precision engine only); default 4,000,000.
+void SetFusedMinPixels(int64_t n);
 // Whether the pipeline runs a pass alongside the confidence evaluation
 // before it, each view starting when its confidence map is final (1,
 // default), or after it (0).  The files are identical.

@@ -108,6 +108,8 @@ void hpm_host_set_threads(int n) { hpmhost::SetPassThreads(n); }
 
 void hpm_host_set_prior_on_device(int enable) { hpmhost::SetPriorOnDevice(enable); }
 
+void hpm_host_set_fused_min_pixels(int64_t n) { hpmhost::SetFusedMinPixels(n); }
+
 void hpm_host_set_confidence_overlap(int enable) { hpmhost::SetConfidenceOverlap(enable); }
 
 int hpm_host_decode_jpeg(const uint8_t* data, int64_t size, int* w, int* h, int* channels, uint8_t* out,

@@ -43,6 +43,8 @@ def host_lib():
     L.hpm_host_set_threads.restype = None
     L.hpm_host_set_prior_on_device.argtypes = [ctypes.c_int]
     L.hpm_host_set_prior_on_device.restype = None
+    L.hpm_host_set_fused_min_pixels.argtypes = [ctypes.c_int64]
+    L.hpm_host_set_fused_min_pixels.restype = None
     L.hpm_host_set_confidence_overlap.argtypes = [ctypes.c_int]
     L.hpm_host_set_confidence_overlap.restype = None
     L.hpm_host_decode_jpeg.argtypes = [u8p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
@@ -134,6 +136,12 @@ def set_verbose(level=1):
 def set_threads(n=0):
     """Host threads for the order-free passes (0: default, 1: view order)."""
     host_lib().hpm_host_set_threads(int(n))
+
+
+def set_fused_min_pixels(n=4000000):
+    """Reference views of at least n pixels run as a one-view fused launch with
+    16-column tiles (mixed precision only); the results are identical."""
+    host_lib().hpm_host_set_fused_min_pixels(int(n))
 
 
 def set_prior_on_device(enable=True):

@@ -73,6 +73,15 @@ void hpm_host_set_threads(int n);
  * files are identical. */
 void hpm_host_set_prior_on_device(int enable);
 
+/* The size from which a pass runs a reference view as a one-view fused launch
+ * (hpm_batch) with 16-column colour tiles and 4-tile XCD strips instead of the
+ * context's own launch: reference views of at least `n` pixels, default
+ * 4,000,000, mixed-precision engine only (the exact engine has one tile
+ * width).  At 3072 x 2048 with 10 sources this takes 7-20 % less time per
+ * RunPatchMatch.  The results are identical either way.  No reference
+ * counterpart: the reference has one launch shape (HPM.cu:2987-3035). */
+void hpm_host_set_fused_min_pixels(int64_t n);
+
 /* Whether the pipelines run each pass that follows a confidence evaluation
  * alongside it, each view starting once its own confidence map is final
  * (1, default), or after it (0).  The output files are identical. */

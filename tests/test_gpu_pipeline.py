@@ -208,6 +208,38 @@ def test_pipeline_files_do_not_depend_on_host_threads(tmp_path):
 
 
 
+def test_pipeline_fused_launch_for_large_views_identical(tmp_path):
+    """Reference views from hpm_host_set_fused_min_pixels on run as a one-view
+    fused launch with 16-column tiles (by default from 4 Mpx).  With the bound
+    at 1 pixel every pass of a 3-level pyramid (photometric, prior, geometric,
+    hierarchy, mandatory) takes that path, and every file is byte-identical
+    to a run through the contexts' own launches."""
+    import filecmp
+    from hpm_mi355x import pipeline as P
+    from hpm_mi355x import scenes
+    sc = scenes.make_plane_scene(seed=6, n_cams=5, width=320, height=240, f=262.5, n_src=4)
+    runs = {}
+    try:
+        for bound in (1, 1 << 40):
+            folder = str(tmp_path / f"b{bound}")
+            scenes.write_dense_folder(sc, folder)
+            P.set_fused_min_pixels(bound)
+            print(f"case: pyramid pipeline, 5 views 320x240, bound 100, fused launch from {bound} px", flush=True)
+            runs[bound] = (folder, P.run_pyramid_pipeline(folder, initial_image_scale=1, size_bound=100,
+                                                          precision="fp16", seed=4))
+    finally:
+        P.set_fused_min_pixels()
+    a, b = runs[1], runs[1 << 40]
+    assert a[1] == b[1] > 4 * 5000
+    names = []
+    for root, _, files in os.walk(os.path.join(a[0], "HPM_MVS_plusplus")):
+        for f in files:
+            names.append(os.path.relpath(os.path.join(root, f), a[0]))
+    assert len(names) > 5 * 6
+    for rel in names:
+        assert filecmp.cmp(os.path.join(a[0], rel), os.path.join(b[0], rel), shallow=False), rel
+
+
 def test_pipeline_reads_jpeg_images(tmp_path):
     """images/%08d.jpg (what the reference reads, main.cpp:125): a colour JPEG
     folder and a folder holding the decoded pixels as PPM give byte-identical
