@@ -144,6 +144,9 @@ bool Decoder::sof(size_t len, int marker) {
   const int nc = byte();
   if (P != 8) return fail("only 8-bit samples are supported");
   if (H_ <= 0 || W_ <= 0) return fail("bad image size (zero height: DNL is not supported)");
+  // the engine takes views of at most 2^28 pixels; a larger header (up to
+  // 65535 x 65535) is refused before its planes are allocated
+  if ((int64_t)W_ * H_ > ((int64_t)1 << 28)) return fail("image larger than 2^28 pixels");
   if (nc != 1 && nc != 3) return fail("only 1 or 3 components are supported");
   if (len != (size_t)(8 + 3 * nc)) return fail("bad SOF length");
   comp_.resize(nc);

@@ -115,6 +115,11 @@ def test_unsupported_and_corrupt_files_fail_with_a_reason():
     for cut in (2, 100):
         with pytest.raises(P.HpmError):
             P.decode_jpeg(data[:cut])
+    # a header claiming 65535 x 65535 pixels is refused before anything is
+    # allocated for it (the SOF0 segment: length, precision, height, width)
+    huge = data[:i + 5] + b"\xff\xff\xff\xff" + data[i + 9:]
+    with pytest.raises(P.HpmError, match="2\\^28"):
+        P.decode_jpeg(huge)
     # a file that ends before its EOI marker decodes to the same pixels
     # (libjpeg warns and goes on; the marker carries no data)
     np.testing.assert_array_equal(P.decode_jpeg(data[:-2]), P.decode_jpeg(data))
