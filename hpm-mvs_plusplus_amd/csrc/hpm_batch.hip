@@ -22,10 +22,14 @@ int hpm_batch_detail::bfail(hpm_batch* b, int code, const std::string& msg) {
 
 namespace {
 
+// The layout's device tables are allocated, uploaded and freed in the order
+// of the batch's own stream (stream-ordered allocator): a batch made and
+// dropped per RunPatchMatch, as the host pipeline does for large views, then
+// never waits for the kernels other host threads run on the device.
 void free_layout(hpm_batch* b) {
   void* bufs[] = {b->d_consts, b->d_states, b->d_views, b->d_wg[0], b->d_wg[1], b->d_wg[2]};
   for (void* p : bufs)
-    if (p) hipFree(p);
+    if (p) hipFreeAsync(p, b->stream);
   b->d_consts = nullptr;
   b->d_states = nullptr;
   b->d_views = nullptr;
@@ -52,9 +56,13 @@ int ensure_layout(hpm_batch* b) {
   }
   if (key == b->layout_key && b->d_views) return HPM_OK;
   const int n = (int)b->ctxs.size();
-  std::vector<MultiView> views(n);
-  std::vector<DevState> states(n);
-  std::vector<uint16_t> wg[3];
+  BHIP_OK(hipStreamSynchronize(b->stream));  // the previous layout's uploads have read their host sources
+  std::vector<MultiView>& views = b->h_views;
+  std::vector<DevState>& states = b->h_states;
+  std::vector<uint16_t>* wg = b->h_wg;
+  views.assign(n, MultiView());
+  states.assign(n, DevState());
+  for (int s = 0; s < 3; ++s) wg[s].clear();
   b->max_v = 0;
   for (int i = 0; i < n; ++i) {
     hpm_ctx* c = b->ctxs[i];
@@ -73,16 +81,17 @@ int ensure_layout(hpm_batch* b) {
   }
   free_layout(b);
   b->layout_key.clear();  // set again only once every table is in place
-  BHIP_OK(hipMalloc((void**)&b->d_consts, sizeof(RunConsts) * n));
-  BHIP_OK(hipMalloc((void**)&b->d_states, sizeof(DevState) * n));
-  BHIP_OK(hipMalloc((void**)&b->d_views, sizeof(MultiView) * n));
-  BHIP_OK(hipMemcpy(b->d_states, states.data(), sizeof(DevState) * n, hipMemcpyHostToDevice));
-  BHIP_OK(hipMemcpy(b->d_views, views.data(), sizeof(MultiView) * n, hipMemcpyHostToDevice));
+  hipStream_t st = b->stream;
+  BHIP_OK(hipMallocAsync((void**)&b->d_consts, sizeof(RunConsts) * n, st));
+  BHIP_OK(hipMallocAsync((void**)&b->d_states, sizeof(DevState) * n, st));
+  BHIP_OK(hipMallocAsync((void**)&b->d_views, sizeof(MultiView) * n, st));
+  BHIP_OK(hipMemcpyAsync(b->d_states, states.data(), sizeof(DevState) * n, hipMemcpyHostToDevice, st));
+  BHIP_OK(hipMemcpyAsync(b->d_views, views.data(), sizeof(MultiView) * n, hipMemcpyHostToDevice, st));
   for (int s = 0; s < 3; ++s) {
     if (wg[s].size() > (size_t)0x7fffffff) return bfail(b, HPM_E_INVALID, "batch grid too large");
     b->grid[s] = (int)wg[s].size();
-    BHIP_OK(hipMalloc((void**)&b->d_wg[s], sizeof(uint16_t) * wg[s].size()));
-    BHIP_OK(hipMemcpy(b->d_wg[s], wg[s].data(), sizeof(uint16_t) * wg[s].size(), hipMemcpyHostToDevice));
+    BHIP_OK(hipMallocAsync((void**)&b->d_wg[s], sizeof(uint16_t) * wg[s].size(), st));
+    BHIP_OK(hipMemcpyAsync(b->d_wg[s], wg[s].data(), sizeof(uint16_t) * wg[s].size(), hipMemcpyHostToDevice, st));
   }
   b->layout_key = key;
   return HPM_OK;
@@ -158,7 +167,10 @@ void hpm_batch_destroy(hpm_batch* b) {
   hipSetDevice(b->device);
   if (b->stream) hipStreamSynchronize(b->stream);
   if (b->gexec) hipGraphExecDestroy(b->gexec);
-  free_layout(b);
+  if (b->stream) {
+    free_layout(b);
+    hipStreamSynchronize(b->stream);
+  }
   for (auto e : b->ev) if (e) hipEventDestroy(e);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;

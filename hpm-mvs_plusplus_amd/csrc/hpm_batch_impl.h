@@ -23,6 +23,11 @@ struct hpm_batch {
   hpm::DevState* d_states = nullptr;
   hpm::MultiView* d_views = nullptr;
   uint16_t* d_wg[3] = {nullptr, nullptr, nullptr};
+  // host sources of the layout uploads, kept until the next layout (the
+  // uploads are asynchronous on `stream`)
+  std::vector<hpm::MultiView> h_views;
+  std::vector<hpm::DevState> h_states;
+  std::vector<uint16_t> h_wg[3];
   int grid[3] = {0, 0, 0};
   int max_v = 0;
   // launch shape

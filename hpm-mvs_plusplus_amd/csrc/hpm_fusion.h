@@ -16,8 +16,12 @@
 // skipped without reading its list.
 //
 // Arithmetic: plain IEEE float operations in the reference's order (no fused
-// multiply-adds; the library is built with -ffp-contract=off); pow(x, 2) of a
-// float is x * x, exp of a float is expf_hpm, acosf is acosf_hpm.
+// multiply-adds; the library is built with -ffp-contract=off); exp of a float
+// is expf_hpm, acosf is acosf_hpm.  The reprojection error
+// sqrt(pow(c - u, 2) + pow(r - v, 2)) (main.cpp:922,1064) is a double
+// expression: main.cpp is host C++11, where pow(float, int) returns double, so
+// the squares, their sum and the square root are double, rounded to float on
+// assignment.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -122,7 +126,7 @@ __global__ __launch_bounds__(256) void k_consistency(const DevCam* __restrict__ 
         float tu, tv;
         project_on_camera(tX, rc, tu, tv, proj_depth);
         const float dx = (float)c - tu, dy = (float)r - tv;
-        const float reproj_error = sqrtf(dx * dx + dy * dy);
+        const float reproj_error = (float)sqrt((double)dx * (double)dx + (double)dy * (double)dy);
         const float relative_depth_diff = fabsf(proj_depth - ref_depth) / ref_depth;
         const float dot = rn[0] * sn[0] + rn[1] * sn[1] + rn[2] * sn[2];
         float angle = acosf_hpm(dot);  // GetAngle, HPM.cpp:281-290
@@ -254,10 +258,15 @@ int hpm_consistency_check_cb(int mode, int num_images, const hpm_camera* cameras
   };
 
   std::vector<std::vector<uint8_t>> masks(num_images);
+  // ConfidenceEvaluation's in-memory consistency maps (cv::Mat::zeros,
+  // main.cpp:1005).  out_confidence[i] receives image i's confidence.dmb:
+  // the map as it stands when image i is done (main.cpp:1095-1100).  Later
+  // images still write into the in-memory map (main.cpp:1086), not the file.
+  std::vector<std::vector<float>> consistency(mode == HPM_CHECK_CONFIDENCE ? num_images : 0);
   int64_t npts = 0;
   for (int i = 0; i < num_images && rc == HPM_OK; ++i) {
     masks[i].assign((size_t)cameras[i].width * cameras[i].height, 0);
-    if (mode == HPM_CHECK_CONFIDENCE) std::memset(out_confidence[i], 0, masks[i].size() * sizeof(float));
+    if (mode == HPM_CHECK_CONFIDENCE) consistency[i].assign(masks[i].size(), 0.0f);
   }
   for (size_t k = 0; k < chunks.size() && k < (size_t)kChunkBuffers && rc == HPM_OK; ++k) issue(k);
   size_t k = 0;
@@ -334,16 +343,18 @@ int hpm_consistency_check_cb(int mode, int num_images, const hpm_camera* cameras
           }
           ++npts;
         } else {
-          out_confidence[i][p] = dynamic_consistency;
+          consistency[i][p] = dynamic_consistency;
         }
         for (int j = 0; j < num_ngb; ++j) {
           if (used[j] < 0) continue;
           masks[srcs[j]][used[j]] = 1;
-          if (mode == HPM_CHECK_CONFIDENCE) out_confidence[srcs[j]][used[j]] = dynamic_consistency;
+          if (mode == HPM_CHECK_CONFIDENCE) consistency[srcs[j]][used[j]] = dynamic_consistency;
         }
       }
     }
     if (mt && rc == HPM_OK) release();
+    if (rc == HPM_OK && mode == HPM_CHECK_CONFIDENCE)
+      std::memcpy(out_confidence[i], consistency[i].data(), P * sizeof(float));
     if (rc == HPM_OK && image_done) image_done(i, user);
   }
   if (verbose) {

@@ -1378,30 +1378,11 @@ void SetPriorOnDevice(int enable) { g_prior_on_device = enable != 0; }
 namespace {
 std::atomic<int> g_confidence_overlap{1};
 
-// For each problem (pair.txt position), the problem after which the
-// confidence evaluation no longer writes its map: itself or the last problem
-// that lists it as a source.
-std::vector<int> ConfidenceFinalAfter(const std::vector<Problem>& problems) {
-  const int n = (int)problems.size();
-  std::vector<int> pos;
-  for (int i = 0; i < n; ++i) {
-    const int id = problems[i].ref_image_id;
-    if (id < 0) throw Error("negative image id in pair.txt");
-    if ((size_t)id >= pos.size()) pos.resize(id + 1, -1);
-    pos[id] = i;
-  }
-  std::vector<int> final_after(n);
-  for (int i = 0; i < n; ++i) final_after[i] = i;
-  for (int k = 0; k < n; ++k)
-    for (int s : problems[k].src_image_ids)
-      if (s >= 0 && (size_t)s < pos.size() && pos[s] >= 0) final_after[pos[s]] = std::max(final_after[pos[s]], k);
-  return final_after;
-}
-
 // ConfidenceEvaluation, then `pass(ready)`.  With the overlap on, the pass
 // runs alongside the evaluation (a host pass with little GPU work) and each
 // of its views starts once its own confidence map -- the only one it reads
-// -- is final; the files are those of running one after the other.
+// -- is written, i.e. once the evaluation has finished that image; the files
+// are those of running one after the other.
 template <class Pass>
 void ConfidenceThenPass(const std::string& dense_folder, const std::vector<Problem>& problems, bool geom, Pass pass) {
   if (!g_confidence_overlap) {
@@ -1410,7 +1391,6 @@ void ConfidenceThenPass(const std::string& dense_folder, const std::vector<Probl
     return;
   }
   ReadyViews ready((int)problems.size());
-  ready.SetFinalAfter(ConfidenceFinalAfter(problems));
   std::exception_ptr conf_err;
   std::thread conf([&] {
     try {
@@ -1638,22 +1618,7 @@ static ViewMaps LoadViewMaps(const std::string& dense_folder, const std::vector<
   return v;
 }
 
-ReadyViews::ReadyViews(int n) : ready_((size_t)n, 0), final_after_((size_t)n) {
-  for (int i = 0; i < n; ++i) final_after_[i] = i;
-}
-
-void ReadyViews::SetFinalAfter(const std::vector<int>& final_after) {
-  std::lock_guard<std::mutex> l(mu_);
-  final_after_ = final_after;
-}
-
-std::vector<int> ReadyViews::Order() const {
-  std::lock_guard<std::mutex> l(mu_);
-  std::vector<int> order(final_after_.size());
-  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return final_after_[a] < final_after_[b]; });
-  return order;
-}
+ReadyViews::ReadyViews(int n) : ready_((size_t)n, 0) {}
 
 void ReadyViews::Mark(int i) {
   {
@@ -1697,28 +1662,21 @@ void ConfidenceEvaluation(const std::string& dense_folder, const std::vector<Pro
     np.push_back(v.normals[i]->data.data());
   }
   for (auto& c : conf) cp.push_back(c.data.data());
-  // Image i's map is written by its own pixels and by the images after it
-  // that list it as a source: it is final once the last of those is done.
-  std::vector<int> final_after(n);
-  for (int i = 0; i < n; ++i) final_after[i] = i;
-  for (int k = 0; k < n; ++k)
-    for (int j = v.offsets[k]; j < v.offsets[k + 1]; ++j) final_after[v.ids[j]] = std::max(final_after[v.ids[j]], k);
-  if (ready) ready->SetFinalAfter(final_after);
-  std::vector<std::vector<int>> due(n);  // views whose map is final after image k
-  for (int i = 0; i < n; ++i) due[final_after[i]].push_back(i);
+  // The reference writes image i's confidence.dmb as soon as image i is done
+  // (main.cpp:1095-1100): the file holds what images 0..i wrote into the map.
+  // hpm_consistency_check_cb fills conf[i] with exactly that when image i is
+  // done, reports it, and never writes conf[i] again (later images' writes
+  // go to its own in-memory maps), so the map is saved as it is.
   struct Progress {
     const std::string* folder;
     const std::vector<Problem>* problems;
     std::vector<MapF>* conf;
-    std::vector<std::vector<int>>* due;
     ReadyViews* ready;
-  } prog{&dense_folder, &problems, &conf, &due, ready};
+  } prog{&dense_folder, &problems, &conf, ready};
   auto image_done = [](int k, void* user) {
     auto* g = static_cast<Progress*>(user);
-    for (int i : (*g->due)[k]) {
-      SaveMap(ResultFolder(*g->folder, (*g->problems)[i].ref_image_id) + "/confidence.dmb", std::move((*g->conf)[i]));
-      if (g->ready) g->ready->Mark(i);
-    }
+    SaveMap(ResultFolder(*g->folder, (*g->problems)[k].ref_image_id) + "/confidence.dmb", std::move((*g->conf)[k]));
+    if (g->ready) g->ready->Mark(k);
   };
   int64_t count = 0;
   const int rc = hpm_consistency_check_cb(HPM_CHECK_CONFIDENCE, n, v.cams.data(), dp.data(), np.data(),
@@ -1857,9 +1815,7 @@ std::vector<FusedPoint> RunPipeline(const std::string& dense_folder, int precisi
   parallel_views(n, [&](int i) { ProcessProblem(dense_folder, problems, i, f, precision, seed + i); });  // photometric
   f.prior_consistency = true;
   ConfidenceThenPass(dense_folder, problems, false, [&](ReadyViews* ready) {
-    const std::vector<int> order = ready ? ready->Order() : std::vector<int>();
-    parallel_views(n, [&](int k) {
-      const int i = ready ? order[k] : k;
+    parallel_views(n, [&](int i) {
       if (ready) ready->Wait(i);
       ProcessProblem(dense_folder, problems, i, f, precision, seed + 1000 + i);
     });
@@ -1876,8 +1832,7 @@ std::vector<FusedPoint> RunPipeline(const std::string& dense_folder, int precisi
         ProcessProblem(dense_folder, problems, i, f, precision, seed + 2000 * (it + 2) + i);
       };
       if (order_free(f)) {
-        const std::vector<int> order = ready ? ready->Order() : std::vector<int>();
-        parallel_views(n, [&](int k) { one(ready ? order[k] : k); });
+        parallel_views(n, one);
       } else {
         for (int i = 0; i < n; ++i) one(i);
       }
@@ -1941,8 +1896,7 @@ std::vector<FusedPoint> RunPyramidPipeline(const std::string& dense_folder, int 
     const double t0 = now_s();
     const bool concurrent = order_free(f);
     if (concurrent) {
-      const std::vector<int> order = ready ? ready->Order() : std::vector<int>();
-      parallel_views(n, [&](int k) { one(ready ? order[k] : k); });
+      parallel_views(n, one);
     } else {
       for (int i = 0; i < n; ++i) one(i);
     }

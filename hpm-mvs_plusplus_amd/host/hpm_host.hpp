@@ -324,8 +324,6 @@ void JointBilateralUpsampling(const std::string& dense_folder, const Problem& pr
 class ReadyViews {
  public:
   explicit ReadyViews(int n);
-  void SetFinalAfter(const std::vector<int>& final_after);  // image after which each view's map is final
-  std::vector<int> Order() const;                           // views in the order they become ready
   void Mark(int i);
   void Fail(const std::string& msg);  // wakes the waiters, which throw
   void Wait(int i);
@@ -334,7 +332,6 @@ class ReadyViews {
   mutable std::mutex mu_;
   std::condition_variable cv_;
   std::vector<char> ready_;
-  std::vector<int> final_after_;
   std::string error_;
 };
 void ConfidenceEvaluation(const std::string& dense_folder, const std::vector<Problem>& problems, bool geom,

@@ -354,7 +354,10 @@ int hpm_refine_gn_smoothed(hpm_ctx* ctx, float* params, const float* weights, in
  *   point (world position, normal), both up to `capacity` points;
  *   *out_count is the total number of fused points.
  * HPM_CHECK_CONFIDENCE: out_confidence[i] (width*height floats each) receives
- *   the confidence map of image i (zeros where not consistent); the depth-range
+ *   image i's confidence.dmb: the consistency map as it stands when image i is
+ *   done, which is when the reference writes the file (main.cpp:1095-1100);
+ *   writes by later images that list i as a source (main.cpp:1086) change
+ *   their masks but not this map.  Zeros where not consistent; the depth-range
  *   filter uses cameras[i].depth_min/max for reference and source depths, as the
  *   reference does (main.cpp:1032,1057).
  * Runs on the current HIP device; blocking. */
@@ -369,11 +372,10 @@ int hpm_consistency_check(int mode, int num_images, const hpm_camera* cameras, c
                           float* const* out_confidence, int32_t* out_pairs, float* out_points, int64_t capacity,
                           int64_t* out_count);
 /* The same, calling image_done(i, user) on the calling thread as soon as the
- * ordered pass has finished image i (images in order).  From then on the
- * pass writes out_confidence[i] only through images after i that list i as
- * a source: a caller can hand out a confidence map once the last of those
- * is done, while the check goes on (no counterpart in the reference; for
- * overlapping the next pass with the check). */
+ * ordered pass has finished image i (images in order).  out_confidence[i] is
+ * then complete and never written again: a caller can hand it out while the
+ * check goes on (no counterpart in the reference; for overlapping the next
+ * pass with the check). */
 int hpm_consistency_check_cb(int mode, int num_images, const hpm_camera* cameras, const float* const* depths,
                              const float* const* normals, const int32_t* src_offsets, const int32_t* src_ids,
                              float* const* out_confidence, int32_t* out_pairs, float* out_points, int64_t capacity,

@@ -8,10 +8,12 @@
 //
 // Arithmetic: plain IEEE float operations in the order the reference writes
 // them (built with -ffp-contract=off, no fused multiply-adds).  Calls the
-// reference makes with float arguments use float overloads (pow(x, 2) is x*x,
-// exp(float) is the oracle's expf, acosf the oracle's acosf); expressions the
-// reference promotes with double literals are evaluated in double, with the
-// oracle's own double exp.
+// reference makes with float arguments use float overloads (exp(float) is the
+// oracle's expf, acosf the oracle's acosf); expressions the reference promotes
+// to double are evaluated in double: pow(float, int) returns double in C++11
+// (main.cpp is host C++11), so the reprojection error
+// sqrt(pow(c - u, 2) + pow(r - v, 2)) is double until its assignment to
+// float, and double literals are evaluated with the oracle's own double exp.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -171,6 +173,14 @@ inline int round_index(float v) {
   return (x > -2147483648.0f && x < 2147483648.0f) ? (int)x : -1;
 }
 
+// The reprojection error sqrt(pow(c - u, 2) + pow(r - v, 2)) of
+// main.cpp:778,922,1064: c - u is a float, pow(float, int) is double in C++11,
+// so the squares, the sum and sqrt are double; the result is stored as float.
+inline float ReprojError(int c, int r, float u, float v) {
+  const float dx = (float)c - u, dy = (float)r - v;
+  return (float)std::sqrt((double)dx * (double)dx + (double)dy * (double)dy);
+}
+
 // GetAngle, HPM.cpp:281-290.
 inline float GetAngle(const float* a, const float* b) {
   const float dot = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
@@ -200,6 +210,10 @@ void orc_jbu_prior(const float* fine, int w, int h, const float* coarse, const f
 
 float orc_acosf(float x) { return orc::acosf_hpm(x); }
 
+void orc_reproj_error(const int* c, const int* r, const float* u, const float* v, int n, float* out) {
+  for (int i = 0; i < n; ++i) out[i] = ReprojError(c[i], r[i], u[i], v[i]);
+}
+
 // RunFusion (mode 0, main.cpp:823-957) and ConfidenceEvaluation (mode 1,
 // main.cpp:959-1109), in the reference's order: images in order, pixels
 // row-major, source views in pair order; masks greedy across images.  The
@@ -211,14 +225,19 @@ float orc_acosf(float x) { return orc::acosf_hpm(x); }
 // mode 0: out_pairs receives (image, pixel) of every fused point, in order, up
 //   to capacity; *count is the total.  out_points (6 floats per point: world
 //   position, reference normal) may be null.
-// mode 1: out_conf[i] (w*h floats, zero-filled by the caller) receives the
-//   consistency map of image i.
+// mode 1: out_conf[i] (w*h floats) receives image i's confidence.dmb: the
+//   in-memory consistency map as it stands when image i is done
+//   (writeDepthDmb inside the image loop, main.cpp:1095-1100).  Images after
+//   i that list it as a source keep writing into the in-memory map
+//   (main.cpp:1086); those writes never reach the file.
 // mode 2: as mode 0 with RunFusion_Sky_Strict's float threshold (main.cpp:659-821).
 int orc_consistency(int mode, int n, const Cam* cams, const float* const* depths, const float* const* normals,
                     const int* src_offsets, const int* src_ids, float* const* out_conf, int32_t* out_pairs,
                     float* out_points, int64_t capacity, int64_t* count) {
   std::vector<std::vector<uint8_t>> masks(n);
   for (int i = 0; i < n; ++i) masks[i].assign((size_t)cams[i].width * cams[i].height, 0);
+  std::vector<std::vector<float>> consistency(mode == 1 ? n : 0);  // cv::Mat::zeros, main.cpp:1005
+  for (size_t i = 0; i < consistency.size(); ++i) consistency[i].assign((size_t)cams[i].width * cams[i].height, 0.0f);
   int64_t npts = 0;
   for (int i = 0; i < n; ++i) {
     const int cols = cams[i].width, rows = cams[i].height;
@@ -255,8 +274,7 @@ int orc_consistency(int mode, int n, const Cam* cams, const float* const* depths
             PointOnWorld(src_c, src_r, src_depth, cams[src_id], tmp_X);
             float tu, tv;
             ProjectOnCamera(tmp_X, cams[i], tu, tv, proj_depth);
-            const float dx = (float)c - tu, dy = (float)r - tv;
-            const float reproj_error = std::sqrt(dx * dx + dy * dy);
+            const float reproj_error = ReprojError(c, r, tu, tv);
             const float relative_depth_diff = std::fabs(proj_depth - ref_depth) / ref_depth;
             const float angle = GetAngle(ref_normal, src_normal);
             if (reproj_error < 2.0f && relative_depth_diff < 0.01f && angle < 0.174533f) {
@@ -285,18 +303,19 @@ int orc_consistency(int mode, int n, const Cam* cams, const float* const* depths
             }
             ++npts;
           } else {
-            out_conf[i][center] = dynamic_consistency;
+            consistency[i][center] = dynamic_consistency;
           }
           for (int j = 0; j < num_ngb; ++j) {
             if (used_x[j] == -1) continue;
             const int s = srcs[j];
             const size_t sc = (size_t)used_y[j] * cams[s].width + used_x[j];
             masks[s][sc] = 1;
-            if (mode == 1) out_conf[s][sc] = dynamic_consistency;
+            if (mode == 1) consistency[s][sc] = dynamic_consistency;
           }
         }
       }
     }
+    if (mode == 1) std::memcpy(out_conf[i], consistency[i].data(), consistency[i].size() * sizeof(float));
   }
   *count = npts;
   return 0;

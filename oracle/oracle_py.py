@@ -49,6 +49,8 @@ def lib():
         L.orc_expf.argtypes = [f32]; L.orc_expf.restype = f32
         L.orc_set_reverse_sweep.argtypes = [ctypes.c_int]; L.orc_set_reverse_sweep.restype = None
         L.orc_acosf.argtypes = [f32]; L.orc_acosf.restype = f32
+        L.orc_reproj_error.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_void_p]
+        L.orc_reproj_error.restype = None
         L.orc_cost_grad_f64.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_cost_params_f64.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
@@ -213,6 +215,15 @@ def philox4x32_10(ctr, key):
 
 def acosf(x):
     return lib().orc_acosf(float(x))
+
+
+def reproj_error(c, r, u, v):
+    """The fusion / confidence reprojection error (main.cpp:922,1064) per element."""
+    c = np.ascontiguousarray(c, np.int32); r = np.ascontiguousarray(r, np.int32)
+    u = np.ascontiguousarray(u, np.float32); v = np.ascontiguousarray(v, np.float32)
+    out = np.empty(c.shape, np.float32)
+    lib().orc_reproj_error(_p(c), _p(r), _p(u), _p(v), c.size, _p(out))
+    return out
 
 
 def jbu_depth(fine, coarse):

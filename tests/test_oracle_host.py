@@ -169,12 +169,62 @@ def test_confidence_two_view_mirrors_along_disparity():
     assert np.all(c1[:, W - dsp:] == 0)
     # depth range filter (main.cpp:1029,1057 use the reference camera's range
     # for both depths): view 0 with range [6, 10] fuses nothing as reference,
-    # view 1 then fuses and writes its scores into the view-0 pixels it used
+    # view 1 then fuses.  It writes its scores into the view-0 pixels it used,
+    # but only in memory (main.cpp:1086): view 0's confidence.dmb was written
+    # when view 0 was done (main.cpp:1095-1100), so that file stays zero
     K = np.array(cams[0].K).reshape(3, 3)
     far0 = O.camera(K, np.eye(3), [0, 0, 0], W, H, 6.0, 10.0)
     c0, c1 = O.consistency(1, [far0, cams[1]], depths, normals, [[1], [0]])
     assert np.all(c1[:, W - dsp:] == 0) and np.all(c1[:, :W - dsp] > 0.99)
-    np.testing.assert_array_equal(c0[:, dsp:], c1[:, :W - dsp])
+    assert np.all(c0 == 0)
+    # with the two images swapped in pair order, the scores reach the
+    # [6, 10] view's map before that view is done, so they are in its file
+    c1s, c0s = O.consistency(1, [cams[1], far0], [depths[1], depths[0]], [normals[1], normals[0]], [[1], [0]])
+    np.testing.assert_array_equal(c1s, c1)
+    np.testing.assert_array_equal(c0s[:, dsp:], c1s[:, :W - dsp])
+    assert np.all(c0s[:, :dsp] == 0)
     far1 = O.camera(K, np.eye(3), [-dsp * 5.0 / 100.0, 0, 0], W, H, 6.0, 10.0)
     c0, c1 = O.consistency(1, [far0, far1], depths, normals, [[1], [0]])
     assert np.all(c0 == 0) and np.all(c1 == 0)
+
+
+_REPROJ_SRC = r"""
+#include <cmath>
+// main.cpp:922 / 1064, compiled as main.cpp is (host C++11): c and r are int
+// pixel coordinates, tmp_pt.x / tmp_pt.y float.
+extern "C" void ref_reproj(const int* c, const int* r, const float* x, const float* y, int n, float* out) {
+  for (int i = 0; i < n; ++i) {
+    float reproj_error = sqrt(pow(c[i] - x[i], 2) + pow(r[i] - y[i], 2));
+    out[i] = reproj_error;
+  }
+}
+"""
+
+
+def test_reproj_error_is_the_cxx11_double_expression(tmp_path):
+    """The reprojection error of the consistency check equals the reference's
+    expression compiled with g++ -std=c++11: pow(float, int) returns double
+    there, so the squares, sum and square root are double.  A float
+    evaluation (sqrtf(dx*dx + dy*dy)) differs on about 17 % of these inputs."""
+    import ctypes
+    import subprocess
+    src = tmp_path / "reproj.cpp"
+    src.write_text(_REPROJ_SRC)
+    so = tmp_path / "libreproj.so"
+    subprocess.run(["g++", "-std=c++11", "-O2", "-ffp-contract=off", "-fPIC", "-shared", str(src), "-o", str(so)],
+                   check=True)
+    L = ctypes.CDLL(str(so))
+    rng = np.random.default_rng(7)
+    n = 200000
+    c = rng.integers(0, 1600, n).astype(np.int32)
+    r = rng.integers(0, 1200, n).astype(np.int32)
+    u = (c + rng.uniform(-2.5, 2.5, n)).astype(np.float32)
+    v = (r + rng.uniform(-2.5, 2.5, n)).astype(np.float32)
+    ref = np.empty(n, np.float32)
+    L.ref_reproj(*(ctypes.c_void_p(a.ctypes.data) for a in (c, r, u, v)), ctypes.c_int(n),
+                 ctypes.c_void_p(ref.ctypes.data))
+    got = O.reproj_error(c, r, u, v)
+    np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+    dx, dy = (c.astype(np.float32) - u), (r.astype(np.float32) - v)
+    single = np.sqrt(dx * dx + dy * dy, dtype=np.float32)
+    assert np.mean(single != ref) > 0.05  # the case distinguishes the two evaluations
