@@ -64,6 +64,29 @@ struct orc_params {
   int mand_consistency;
   int num_iterations;    // 3 (HPM.cu:3013)
 };
+
+// Test probes (orc_probe_pixel): a cost callback that replaces the bilateral
+// NCC or the geometric-consistency cost of one (view, plane), and what one
+// pixel's update computed along the way.
+typedef float (*orc_cost_fn)(void* user, int view, int px, int py, const float* plane);
+struct orc_probe_out {
+  int draws_used;          // uniforms the update consumed
+  int positions[8];        // neighbour search result per direction (pixel index)
+  int flags[8];            // direction valid
+  float rows[8][32];       // per-direction cost rows as the update reads them
+  int n_costvec;           // cost-vector evaluations of the neighbour search and extension
+  int costvec_pos[16];     // ... and the pixel whose plane each one evaluated
+  float pdf[32];           // view-selection probabilities times the neighbours' prior, before the CDF
+  float weights[32];       // view weights (draw counts)
+  uint32_t sel;            // view mask of the draws
+  float weight_norm;
+  float fc[8];             // weighted direction costs
+  float rfc[8];            // restricted costs (prior / mandatory-consistency updates)
+  int n_cand;              // refinement candidates evaluated
+  float cand[5][4];        // their planes (normal, d)
+  float cand_depth[5];     // the depth each candidate was built from
+  float cand_cost[5];      // their aggregated costs
+};
 }
 
 namespace {
@@ -209,6 +232,30 @@ template <class T> T ComputeBilateralWeight(T xd, T yd, T pix, T cpix, T ss, T s
   return EXP(-spatial / ((T)2 * ss * ss) - color / ((T)2 * sc * sc));
 }
 
+// ---------------------------------------------------------------- probes
+// Injected into one pixel's update by orc_probe_pixel; a normal run has none.
+struct Probe {
+  const float* draws = nullptr;  // the uniforms the update draws, in order
+  int n_draws = 0;
+  orc_cost_fn ncc = nullptr, geom = nullptr;
+  void* user = nullptr;
+  orc_probe_out* out = nullptr;
+};
+
+// The pixel's uniform stream: Philox (orc::PixelRng), or a probe's fixed list.
+struct Draws {
+  orc::PixelRng prng;
+  Probe* probe;
+  Draws(uint64_t seed, uint32_t pixel, uint32_t stage, Probe* p) : prng(seed, pixel, stage), probe(p) {}
+  float uniform() {
+    if (probe && probe->draws) {
+      const int k = probe->out->draws_used++;
+      return k < probe->n_draws ? probe->draws[k] : 0.5f;  // overrun: the caller sees draws_used > n_draws
+    }
+    return prng.uniform();
+  }
+};
+
 // ---------------------------------------------------------------- context
 template <class T> struct Ctx {
   int W = 0, H = 0, nv = 0;           // nv = num_images (ref + sources)
@@ -231,7 +278,13 @@ template <class T> struct Ctx {
   T vs_thr[3] = {};                   // view-selection cost threshold per iter (HPM.cu:1706)
   T vs_fallback[3] = {};              // expf(thr^2 / -0.32) per iter (HPM.cu:1724)
   int row_begin = 0, row_end = 0;     // pixel rows updated (CPU-baseline sampling)
+  Probe* probe = nullptr;             // test probe of a single pixel (orc_probe_pixel)
 };
+
+template <class T> float* ProbePlane(const V4<T>& pl, float out[4]) {
+  out[0] = (float)pl.x; out[1] = (float)pl.y; out[2] = (float)pl.z; out[3] = (float)pl.w;
+  return out;
+}
 
 template <class T> T ComputeTexture(const Ctx<T>& c, int px, int py) {  // HPM.cu:349-373
   int total = 0, edges = 0;
@@ -250,6 +303,10 @@ template <class T> T ComputeTexture(const Ctx<T>& c, int px, int py) {  // HPM.c
 // HPM.cu:375-448.  Outer loop i = x offset, inner j = y offset, per-column
 // partial sums added into the totals (HPM.cu:397-427).
 template <class T> T ComputeBilateralNCC(const Ctx<T>& c, int v, int px, int py, const V4<T>& pl) {
+  if (c.probe && c.probe->ncc) {
+    float q[4];
+    return (T)c.probe->ncc(c.probe->user, v, px, py, ProbePlane(pl, q));
+  }
   const T cost_max = 2;
   const int radius = c.prm.patch_size / 2;
   const orc_camera& rc = c.cams[0];
@@ -331,7 +388,7 @@ template <class T> void ComputeMultiViewCostVector(const Ctx<T>& c, int px, int 
 }
 
 // ---------------------------------------------------------------- random planes
-template <class T> V4<T> GenerateRandomNormal(const orc_camera& cam, int px, int py, orc::PixelRng& rng, T depth) {  // HPM.cu:159-185
+template <class T> V4<T> GenerateRandomNormal(const orc_camera& cam, int px, int py, Draws& rng, T depth) {  // HPM.cu:159-185
   T q1 = 1, q2 = 1, s = 2;
   while (s >= (T)1) {
     q1 = (T)2 * (T)rng.uniform() - (T)1;
@@ -347,7 +404,7 @@ template <class T> V4<T> GenerateRandomNormal(const orc_camera& cam, int px, int
   return n;
 }
 
-template <class T> V4<T> GeneratePerturbedNormal(const orc_camera& cam, int px, int py, const V4<T>& normal, orc::PixelRng& rng, T pert) {  // HPM.cu:187-222
+template <class T> V4<T> GeneratePerturbedNormal(const orc_camera& cam, int px, int py, const V4<T>& normal, Draws& rng, T pert) {  // HPM.cu:187-222
   const V4<T> vd = GetViewDirection(cam, px, py, (T)1);
   const T a1 = ((T)rng.uniform() - (T)0.5f) * pert;
   const T a2 = ((T)rng.uniform() - (T)0.5f) * pert;
@@ -371,7 +428,7 @@ template <class T> V4<T> GeneratePerturbedNormal(const orc_camera& cam, int px, 
   return np;
 }
 
-template <class T> V4<T> GenerateRandomPlaneHypothesis(const orc_camera& cam, int px, int py, orc::PixelRng& rng, T dmin, T dmax) {  // HPM.cu:224-230
+template <class T> V4<T> GenerateRandomPlaneHypothesis(const orc_camera& cam, int px, int py, Draws& rng, T dmin, T dmax) {  // HPM.cu:224-230
   const T depth = (T)rng.uniform() * (dmax - dmin) + dmin;
   V4<T> pl = GenerateRandomNormal(cam, px, py, rng, depth);
   pl.w = GetDistance2Origin(cam, px, py, depth, pl);
@@ -407,6 +464,10 @@ inline int TruncToTexel(double v, int n) {  // (int)v of HPM.cu:549 with a defin
   return (int)v;
 }
 template <class T> T ComputeGeomConsistencyCost(const Ctx<T>& c, int v, const V4<T>& pl, int px, int py) {  // HPM.cu:539-564
+  if (c.probe && c.probe->geom) {
+    float q[4];
+    return (T)c.probe->geom(c.probe->user, v, px, py, ProbePlane(pl, q));
+  }
   const T max_cost = 3;
   const orc_camera& rc = c.cams[0];
   const orc_camera& sc = c.cams[v];
@@ -488,7 +549,7 @@ template <class T> void ReloadPlane(Ctx<T>& c, int px, int py) {  // HPM.cu:749-
 
 template <class T> void RandomInitialization(Ctx<T>& c, int px, int py) {  // HPM.cu:567-763
   const size_t center = (size_t)py * c.W + px;
-  orc::PixelRng rng(c.seed, (uint32_t)center, 0u);
+  Draws rng(c.seed, (uint32_t)center, 0u, c.probe);
   const orc_params& P = c.prm;
   const orc_camera& cam = c.cams[0];
   if (!P.prior_consistency && !P.mand_consistency) c.texture[center] = ComputeTexture(c, px, py);
@@ -579,7 +640,7 @@ template <class T> Consts<T> MakeConsts(const orc_params& P) {  // HPM.cu:767-77
 
 // HPM.cu:765-841.  `views` limits the cost evaluation to the views the weights
 // select; unselected views' costs are never read by the reference either.
-template <class T> void PlaneHypothesisRefinement(const Ctx<T>& c, V4<T>* plane, T* depth, T* cost, orc::PixelRng& rng,
+template <class T> void PlaneHypothesisRefinement(const Ctx<T>& c, V4<T>* plane, T* depth, T* cost, Draws& rng,
                                                   const T* vw, T weight_norm, T* restricted_cost, int px, int py, T texture) {
   const orc_params& P = c.prm;
   const orc_camera& cam = c.cams[0];
@@ -618,6 +679,13 @@ template <class T> void PlaneHypothesisRefinement(const Ctx<T>& c, V4<T>* plane,
       }
     }
     tc /= weight_norm;
+    if (c.probe) {
+      orc_probe_out* o = c.probe->out;
+      ProbePlane(tp, o->cand[i]);
+      o->cand_depth[i] = (float)depths[i];
+      o->cand_cost[i] = (float)tc;
+      o->n_cand = i + 1;
+    }
     const T depth_before = ComputeDepthfromPlaneHypothesis(cam, tp, px, py);
     if (use_prior) {
       const T dd = depths[i] - depth_prior;
@@ -656,6 +724,15 @@ template <class T> void SampleNeighbours(const Ctx<T>& c, int px, int py, int it
   int right_down = center + 5 + 6 * W, left_down = center - 6 + 5 * W;
   int left_far = center - 5, right_far = center + 5, up_far = center - W * 5, down_far = center + W * 5;
   const T* costs = c.costs;
+  // ComputeMultiViewCostVector of the plane at pixel q, seen from this pixel
+  auto costvec = [&](int q, T* out) {
+    if (c.probe) {
+      orc_probe_out* o = c.probe->out;
+      if (o->n_costvec < 16) o->costvec_pos[o->n_costvec] = q;
+      ++o->n_costvec;
+    }
+    ComputeMultiViewCostVector(c, px, py, c.planes[q], out);
+  };
   if (py > 4) {
     s->flag[1] = true;
     T m = costs[up_far]; int mp = up_far;
@@ -663,7 +740,7 @@ template <class T> void SampleNeighbours(const Ctx<T>& c, int px, int py, int it
       if (py > 4 + 2 * i) { const int pt = up_far - 2 * i * W; if (costs[pt] < m) { m = costs[pt]; mp = pt; } } else break;
     }
     up_far = mp;
-    ComputeMultiViewCostVector(c, px, py, c.planes[up_far], s->cost_array[1]);
+    costvec(up_far, s->cost_array[1]);
   }
   if (py < Hh - 5) {
     s->flag[3] = true;
@@ -672,7 +749,7 @@ template <class T> void SampleNeighbours(const Ctx<T>& c, int px, int py, int it
       if (py < Hh - 5 - 2 * i) { const int pt = down_far + 2 * i * W; if (costs[pt] < m) { m = costs[pt]; mp = pt; } } else break;
     }
     down_far = mp;
-    ComputeMultiViewCostVector(c, px, py, c.planes[down_far], s->cost_array[3]);
+    costvec(down_far, s->cost_array[3]);
   }
   if (px > 4) {
     s->flag[5] = true;
@@ -681,7 +758,7 @@ template <class T> void SampleNeighbours(const Ctx<T>& c, int px, int py, int it
       if (px > 4 + 2 * i) { const int pt = left_far - 2 * i; if (costs[pt] < m) { m = costs[pt]; mp = pt; } } else break;
     }
     left_far = mp;
-    ComputeMultiViewCostVector(c, px, py, c.planes[left_far], s->cost_array[5]);
+    costvec(left_far, s->cost_array[5]);
   }
   if (px < W - 5) {
     s->flag[7] = true;
@@ -690,7 +767,7 @@ template <class T> void SampleNeighbours(const Ctx<T>& c, int px, int py, int it
       if (px < W - 5 - 2 * i) { const int pt = right_far + 2 * i; if (m < costs[pt]) { m = costs[pt]; mp = pt; } } else break;
     }
     right_far = mp;
-    ComputeMultiViewCostVector(c, px, py, c.planes[right_far], s->cost_array[7]);
+    costvec(right_far, s->cost_array[7]);
   }
   // Diagonal zig-zag searches: start (sx, sy), even steps move along `ex`, odd along `oy`.
   auto diag = [&](int dir, bool ok, int start, int sx, int sy, bool even_moves_x, int dx, int dy, int* out) {
@@ -707,7 +784,7 @@ template <class T> void SampleNeighbours(const Ctx<T>& c, int px, int py, int it
       } else break;
     }
     *out = mp;
-    ComputeMultiViewCostVector(c, px, py, c.planes[mp], s->cost_array[dir]);
+    costvec(mp, s->cost_array[dir]);
   };
   diag(0, px > 4 && py > 5, left_up, px - 5, py - 6, true, -2, -2, &left_up);
   diag(2, px < W - 6 && py > 4, right_up, px + 6, py - 5, false, +2, -2, &right_up);
@@ -747,7 +824,7 @@ template <class T> void SampleNeighbours(const Ctx<T>& c, int px, int py, int it
       }
       if (m > costs[pos[0]]) { m = costs[pos[0]]; mp = pos[0]; symbol = false; }
       newpos = mp;
-      if (symbol) ComputeMultiViewCostVector(c, px, py, c.planes[newpos], tmp);
+      if (symbol) costvec(newpos, tmp);
       else for (int v = 0; v < nvs; ++v) tmp[v] = s->cost_array[0][v];
     } else {
       newpos = pos[0];
@@ -757,10 +834,18 @@ template <class T> void SampleNeighbours(const Ctx<T>& c, int px, int py, int it
     for (int v = 0; v < nvs; ++v) s->cost_array[0][v] = tmp[v];
   }
   for (int d = 0; d < 8; ++d) s->positions[d] = pos[d];
+  if (c.probe) {
+    orc_probe_out* o = c.probe->out;
+    for (int d = 0; d < 8; ++d) {
+      o->positions[d] = pos[d];
+      o->flags[d] = s->flag[d];
+      for (int v = 0; v < 32; ++v) o->rows[d][v] = (float)s->cost_array[d][v];
+    }
+  }
 }
 
 // Multi-hypothesis joint view selection, HPM.cu:1686-1751.
-template <class T> void ViewSelection(const Ctx<T>& c, int px, int py, int iter, const Sampled<T>& s, orc::PixelRng& rng,
+template <class T> void ViewSelection(const Ctx<T>& c, int px, int py, int iter, const Sampled<T>& s, Draws& rng,
                                       T* vw, uint32_t* sel_out, T* weight_norm_out, int* nsel_out) {
   const int W = c.W;
   const int center = py * W + px;
@@ -788,6 +873,8 @@ template <class T> void ViewSelection(const Ctx<T>& c, int px, int py, int iter,
     else if (count_false < 3) probs[i] = c.vs_fallback[iter];
     probs[i] = probs[i] * prior[i];
   }
+  if (c.probe)
+    for (int i = 0; i < 32; ++i) c.probe->out->pdf[i] = (float)probs[i];
   // TransformPDFToCDF, HPM.cu:98-112
   T sum = 0;
   for (int i = 0; i < nvs; ++i) sum += probs[i];
@@ -803,6 +890,12 @@ template <class T> void ViewSelection(const Ctx<T>& c, int px, int py, int iter,
   uint32_t sel = 0; T wn = 0; int ns = 0;
   for (int i = 0; i < nvs; ++i) if (vw[i] > (T)0) { sel |= (1u << i); wn += vw[i]; ns++; }
   *sel_out = sel; *weight_norm_out = wn; *nsel_out = ns;
+  if (c.probe) {
+    orc_probe_out* o = c.probe->out;
+    for (int i = 0; i < 32; ++i) o->weights[i] = (float)vw[i];
+    o->sel = sel;
+    o->weight_norm = (float)wn;
+  }
 }
 
 // Weighted direction costs, HPM.cu:1753-1771.
@@ -824,6 +917,8 @@ template <class T> void FinalCosts(const Ctx<T>& c, int px, int py, const Sample
     }
     fc[i] /= wn;
   }
+  if (c.probe)
+    for (int i = 0; i < 8; ++i) c.probe->out->fc[i] = (float)fc[i];
 }
 
 // Cost of the current plane with the selected view weights, HPM.cu:1775-1786.
@@ -861,7 +956,7 @@ template <class T> void CheckerboardPropagation(Ctx<T>& c, int px, int py, int i
     if (c.conf[center] > (T)0.3) return;
     if (c.masks[center] == 0) return;
   }
-  orc::PixelRng rng(c.seed, (uint32_t)center, (uint32_t)(1 + iter));
+  Draws rng(c.seed, (uint32_t)center, (uint32_t)(1 + iter), c.probe);
   Sampled<T> s;
   SampleNeighbours(c, px, py, iter, &s);
   T vw[32]; uint32_t tsel; T wn; int nsel;
@@ -888,6 +983,8 @@ template <class T> void CheckerboardPropagation(Ctx<T>& c, int px, int py, int i
           rfc[i] = RestrictedCost(fc[i], k.beta, texture, prior);
         }
       }
+      if (c.probe)
+        for (int i = 0; i < 8; ++i) c.probe->out->rfc[i] = (float)rfc[i];
       const int max_idx = FindMaxCostIndex(rfc, 8);
       const T dn = ComputeDepthfromPlaneHypothesis(cam, c.planes[center], px, py);
       const T prior = PriorTerm(dn - depth_prior, NormDiff(c.prior[center], c.planes[center]), texture, k.two_dss, k.two_ass);
@@ -945,7 +1042,7 @@ template <class T> void CheckerboardPropagationMand(Ctx<T>& c, int px, int py, i
   const orc_camera& cam = c.cams[0];
   const int W = c.W;
   const int center = py * W + px;
-  orc::PixelRng rng(c.seed, (uint32_t)center, (uint32_t)(1 + iter));
+  Draws rng(c.seed, (uint32_t)center, (uint32_t)(1 + iter), c.probe);
   Sampled<T> s;
   SampleNeighbours(c, px, py, iter, &s);
   T vw[32]; uint32_t tsel; T wn; int nsel;
@@ -1011,6 +1108,8 @@ template <class T> void CheckerboardPropagationMand(Ctx<T>& c, int px, int py, i
       if (cp > (T)0) fc[i] = (T)((double)fc[i] * std::fmin((double)EXP(-cp * (T)2), 0.35));
     }
   }
+  if (c.probe)
+    for (int i = 0; i < 8; ++i) { c.probe->out->fc[i] = (float)fc[i]; c.probe->out->rfc[i] = (float)rfc[i]; }
   const int max_idx = FindMaxCostIndex(rfc, 8);
   T cost_now = 0;
   for (int i = 0; i < nvs; ++i) {
@@ -1455,6 +1554,45 @@ int orc_cost_params_f64(const orc_camera* cams, int num_images, const float* con
   return 0;
 }
 
+// One pixel's RandomInitialization (op 0) or red/black update at iteration
+// `iter` (op 1: CheckerboardPropagation, or the mandatory-consistency update
+// when prm->mand_consistency), FP32, on the caller's maps (the size of
+// cams[0]), with the pixel's uniforms taken from `draws` when it is not null,
+// and the bilateral NCC and geometric cost of (view, plane) given by `ncc` and
+// `geom` when they are not null.  `out` receives what the update computed.
+// ref_image (cams[0] size) is read by the texture-free paths that sample the
+// reference image (hierarchy upsampling); it may be null otherwise.
+int orc_probe_pixel(int op, const orc_camera* cams, int num_images, const float* ref_image, const orc_params* prm,
+                    float* planes, float* costs, float* pre_costs, uint32_t* selected, const float* prior_planes,
+                    const uint32_t* masks, float* confidences, const uint32_t* canny, float* texture,
+                    const float* scaled_planes, int px, int py, int iter, const float* draws, int n_draws,
+                    orc_cost_fn ncc, orc_cost_fn geom, void* user, orc_probe_out* out) {
+  if (num_images < 2 || num_images > 33 || !out || iter < 0 || iter > 2) return -1;
+  Ctx<float> c;
+  c.W = cams[0].width; c.H = cams[0].height; c.nv = num_images; c.cams = cams; c.prm = *prm; c.seed = 0;
+  c.prm.num_images = num_images;
+  if (px < 0 || py < 0 || px >= c.W || py >= c.H) return -1;
+  c.imgs.push_back(Image{ref_image, cams[0].width, cams[0].height});
+  for (int i = 1; i < num_images; ++i) c.imgs.push_back(Image{nullptr, cams[i].width, cams[i].height});
+  if (prm->geom_consistency)
+    for (int i = 0; i < num_images; ++i) c.dimgs.push_back(Image{nullptr, cams[i].width, cams[i].height});
+  c.planes = reinterpret_cast<V4<float>*>(planes);
+  c.costs = costs; c.pre_costs = pre_costs; c.sel = selected;
+  c.prior = reinterpret_cast<const V4<float>*>(prior_planes);
+  c.masks = masks; c.conf = confidences; c.canny = canny; c.texture = texture;
+  c.scaled = reinterpret_cast<const V4<float>*>(scaled_planes);
+  c.row_begin = 0; c.row_end = c.H;
+  SetupThresholds(c);
+  std::memset(out, 0, sizeof(*out));
+  Probe pr;
+  pr.draws = draws; pr.n_draws = n_draws; pr.ncc = ncc; pr.geom = geom; pr.user = user; pr.out = out;
+  c.probe = &pr;
+  if (op == 0) RandomInitialization(c, px, py);
+  else if (c.prm.mand_consistency) CheckerboardPropagationMand(c, px, py, iter);
+  else CheckerboardPropagation(c, px, py, iter);
+  return 0;
+}
+
 float orc_expf(float x) { return orc::expf_hpm(x); }
 double orc_exp(double x) { return orc::exp_hpm(x); }
 float orc_sinf(float x) { return orc::sinf_hpm(x); }
@@ -1474,7 +1612,7 @@ void orc_homography_f64(const orc_camera* rc, const orc_camera* sc, const double
 }
 void orc_random_plane_f32(const orc_camera* cam, int px, int py, uint64_t seed, uint32_t pixel, float dmin, float dmax,
                           float* out) {
-  orc::PixelRng rng(seed, pixel, 0u);
+  Draws rng(seed, pixel, 0u, nullptr);
   V4<float> p = GenerateRandomPlaneHypothesis<float>(*cam, px, py, rng, dmin, dmax);
   out[0] = p.x; out[1] = p.y; out[2] = p.z; out[3] = p.w;
 }

@@ -32,6 +32,19 @@ class orc_params(ctypes.Structure):
                 ("num_iterations", ctypes.c_int)]
 
 
+class orc_probe_out(ctypes.Structure):
+    _fields_ = [("draws_used", ctypes.c_int), ("positions", ctypes.c_int * 8), ("flags", ctypes.c_int * 8),
+                ("rows", (ctypes.c_float * 32) * 8), ("n_costvec", ctypes.c_int), ("costvec_pos", ctypes.c_int * 16),
+                ("pdf", ctypes.c_float * 32), ("weights", ctypes.c_float * 32), ("sel", ctypes.c_uint32),
+                ("weight_norm", ctypes.c_float), ("fc", ctypes.c_float * 8), ("rfc", ctypes.c_float * 8),
+                ("n_cand", ctypes.c_int), ("cand", (ctypes.c_float * 4) * 5), ("cand_depth", ctypes.c_float * 5),
+                ("cand_cost", ctypes.c_float * 5)]
+
+
+COST_FN = ctypes.CFUNCTYPE(ctypes.c_float, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                           ctypes.POINTER(ctypes.c_float))
+
+
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
@@ -78,6 +91,9 @@ def lib():
         L.orc_distance_to_origin_f64.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_void_p]
         L.orc_random_plane_f32.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32,
                                            f32, f32, ctypes.c_void_p]
+        L.orc_probe_pixel.argtypes = ([ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+                                      + [ctypes.c_void_p] * 10 + [ctypes.c_int] * 3 + [ctypes.c_void_p, ctypes.c_int]
+                                      + [COST_FN, COST_FN, ctypes.c_void_p, ctypes.POINTER(orc_probe_out)])
         for n in ("orc_run_patchmatch_f32", "orc_run_patchmatch_f64"):
             getattr(L, n).argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_uint64] + [ctypes.c_void_p] * 10 + [ctypes.c_int] * 4
@@ -162,6 +178,48 @@ def run_patchmatch(cams, images, prm: orc_params, seed, dtype=np.float32, planes
     if rc != 0:
         raise RuntimeError(f"oracle run failed ({rc})")
     return dict(planes=planes, costs=costs, pre_costs=pre_costs, selected=selected, conf=conf, texture=texture)
+
+
+def probe_pixel(op, cams, prm, maps, px, py, iter=0, draws=None, ncc=None, geom=None, ref_image=None):
+    """One pixel's RandomInitialization (op 0) or red/black update (op 1) in
+    the FP32 oracle, on the maps in `maps` (planes (H, W, 4), costs, pre_costs,
+    selected, conf, texture, canny, and optionally prior, masks, scaled), which
+    are updated in place.  draws: the uniforms the update consumes, in order;
+    ncc / geom: python callables (view, px, py, plane[4]) -> cost replacing the
+    bilateral NCC / the geometric cost.  Returns the probe record as a dict."""
+    W, H = cams[0].width, cams[0].height
+    for k, dt, shape in (("planes", np.float32, (H, W, 4)), ("costs", np.float32, (H, W)),
+                         ("pre_costs", np.float32, (H, W)), ("selected", np.uint32, (H, W)),
+                         ("conf", np.float32, (H, W)), ("texture", np.float32, (H, W)),
+                         ("canny", np.uint32, (H, W))):
+        if k not in maps:
+            maps[k] = np.zeros(shape, dt)
+        assert maps[k].dtype == dt and maps[k].shape == shape and maps[k].flags.c_contiguous, k
+    for k, dt in (("prior", np.float32), ("masks", np.uint32), ("scaled", np.float32)):
+        if k in maps:
+            assert maps[k].dtype == dt and maps[k].flags.c_contiguous, k
+
+    def wrap(fn):
+        if fn is None:
+            return COST_FN()
+        return COST_FN(lambda user, v, x, y, pl: float(fn(v, x, y, np.array(pl[:4], np.float32))))
+    cn, cg = wrap(ncc), wrap(geom)
+    d = None if draws is None else np.ascontiguousarray(draws, np.float32)
+    ref = None if ref_image is None else np.ascontiguousarray(ref_image, np.float32)
+    out = orc_probe_out()
+    rc = lib().orc_probe_pixel(op, _cams_array(cams), len(cams), _p(ref), ctypes.byref(prm), _p(maps["planes"]),
+                               _p(maps["costs"]), _p(maps["pre_costs"]), _p(maps["selected"]), _p(maps.get("prior")),
+                               _p(maps.get("masks")), _p(maps["conf"]), _p(maps["canny"]), _p(maps["texture"]),
+                               _p(maps.get("scaled")), px, py, iter, _p(d), 0 if d is None else d.size, cn, cg, None,
+                               ctypes.byref(out))
+    if rc != 0:
+        raise RuntimeError(f"orc_probe_pixel failed ({rc})")
+    r = {f: getattr(out, f) for f, _ in orc_probe_out._fields_}
+    for f in ("positions", "flags", "costvec_pos", "pdf", "weights", "fc", "rfc", "cand_depth", "cand_cost"):
+        r[f] = np.array(r[f][:])
+    r["rows"] = np.array([row[:] for row in out.rows], np.float32)
+    r["cand"] = np.array([c[:] for c in out.cand], np.float32)
+    return r
 
 
 def cost_fixed(cams, images, prm, planes, dtype=np.float32, rows=(-1, -1)):
