@@ -12,18 +12,29 @@ stage over all views' tiles, replayed from a hipGraph; --batch-mode streams
 issues one stream per view instead).  The headline `value`
 is views/s of that batched mixed-precision path (FP16 texel storage, FP32
 accumulation by default).  Beside it: the same path one view at a time
-(`single_view`), and the plain FP32 path (bit-exact with the FP32 oracle, one
-view at a time, no graph) as `plain_fp32`, with the fraction of pixels whose
-depth agrees with it within 1 %.  Prints one JSON line.
+(`single_view`), the plain FP32 path (bit-exact with the FP32 oracle, one
+view at a time, no graph) as `plain_fp32`, and the FP32 mixed-precision
+engine (FP32 texels, the FP16 path's arithmetic) as `fp32_fast`.
+`accuracy_vs_fp64` runs the FP64 oracle's RunPatchMatch of the same view
+with the same seed and gives, per precision, the share of pixels whose depth
+lies within 1 % of the FP64 result and whose normal lies within 5 degrees.
+`cpu_baseline` times the FP32 oracle on bands of the view, with the host's
+threads and with one.  Prints one JSON line.
 
-The roofline object prices the dominant kernel (the red/black propagation
-kernel) by its algorithmic bytes: 181 texels per NCC evaluation (37 reference
-texels + 36 bilinear samples x 4 source texels) x bytes per stored texel, plus
-16 B plane in and 4 B per view cost out per hypothesis, divided by its average
-launch time measured with HIP events on the context stream.  `traffic` is the
-HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by
-the same counters on a copy kernel of known size, read from
-profiles/hbm_traffic.json when that file holds the precision (else null).
+The roofline object is that of the dominant kernel, the red/black propagation
+kernel k_propagate.  It is bound by its vector ALU (and, about as much, by the
+texture data path), not by HBM: `achieved` is its VALU instructions per launch
+(rocprofv3 SQ_INSTS_VALU of the same launches, from
+profiles/k_propagate_counters.json) divided by the average launch time
+measured here with HIP events on the context stream, against 1024 SIMDs x
+2.4 GHz / 4 clocks per wave64 instruction.  `traffic` is the HBM bytes per
+launch measured with FETCH_SIZE / WRITE_SIZE (scaled by the same counters on
+a copy kernel of known size); `hbm_measured_frac` prices it at 8 TB/s.
+`algorithmic_gbs` is the byte count a gather-free kernel would move (181
+texels per NCC evaluation: 37 reference texels + 36 bilinear samples x 4
+source texels, x bytes per stored texel, plus 16 B plane in and 4 B per view
+cost out per hypothesis) per launch time, a labelled figure only: the
+gathers hit the caches.
 
     python bench.py --calibration-copy-mib 4096
 
@@ -102,24 +113,62 @@ def canny_like(img):
     return (g > 60).astype(np.uint32)
 
 
-def cpu_baseline(cams, imgs, canny, rows=1200):
-    """The FP32 oracle on a band of `rows` rows of the same view (all host
-    threads up to 16), scaled to views/s."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(cams, imgs, canny, rows_mt=120, rows_1t=16, reps=3):
+    """The FP32 oracle's RunPatchMatch of the headline view, timed on bands of
+    rows at the view's centre (each band also initialises the rows its
+    propagation reads, 32 above and 16 below), scaled to views/s: with the
+    host's thread count (OMP_NUM_THREADS, else all cores) and with 1 thread,
+    median of `reps` runs each; wall clock around the oracle call only."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as O
-    threads = min(16, os.cpu_count() or 1)
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     oc = [O.camera_from_scene(c) for c in cams]
     prm = O.params(len(oc), oc[0])
     H = cams[0].height
-    y0 = H // 2 - rows // 2
-    t0 = time.perf_counter()
-    O.run_patchmatch(oc, imgs, prm, SEED, canny=canny, rows=(y0, y0 + rows))
-    dt = time.perf_counter() - t0
-    return {"value": (rows / H) / dt, "unit": "views/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
-            "kind": "port",
-            "sample": f"FP32 oracle RunPatchMatch on rows {y0}-{y0 + rows} of the 1600x1200 view "
-                      f"(+ init halo), {dt:.2f} s, scaled by {H}/{rows}"}
+
+    def timed(n_threads, rows):
+        O.lib().orc_set_threads(n_threads)
+        y0 = H // 2 - rows // 2
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            O.run_patchmatch(oc, imgs, prm, SEED, canny=canny, rows=(y0, y0 + rows))
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), ts, y0
+
+    t_mt, all_mt, y_mt = timed(threads, rows_mt)
+    t_1t, all_1t, y_1t = timed(1, rows_1t)
+    O.lib().orc_set_threads(threads)
+    return {"value": (rows_mt / H) / t_mt, "unit": "views/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "sample": f"FP32 oracle RunPatchMatch, 1600x1200 view with 10 sources: rows {y_mt}-{y_mt + rows_mt} "
+                      f"(+ init halo) with {threads} threads, median of {reps} = {t_mt:.2f} s, scaled by {H}/{rows_mt}",
+            "single_thread": {"value": (rows_1t / H) / t_1t, "unit": "views/s", "cores": 1,
+                              "sample": f"rows {y_1t}-{y_1t + rows_1t} (+ init halo), median of {reps} = "
+                                        f"{t_1t:.2f} s, scaled by {H}/{rows_1t}"},
+            "runs_s": {"threads": [round(t, 3) for t in all_mt], "single_thread": [round(t, 3) for t in all_1t]}}
+
+
+def agreement(planes, ref):
+    """Share of pixels whose depth lies within 1 % of `ref`'s and whose normal
+    lies within 5 degrees of it (planes after the output step: world normal,
+    depth)."""
+    d, d0 = planes[..., 3].astype(np.float64), ref[..., 3].astype(np.float64)
+    n, n0 = planes[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
+    cosang = np.sum(n * n0, -1) / np.maximum(np.linalg.norm(n, axis=-1) * np.linalg.norm(n0, axis=-1), 1e-30)
+    ang = np.degrees(np.arccos(np.clip(cosang, -1, 1)))
+    return (round(float(np.mean(np.abs(d - d0) <= 0.01 * np.abs(d0))), 4), round(float(np.mean(ang <= 5.0)), 4))
 
 
 def calibration_copy(mib):
@@ -138,15 +187,20 @@ def calibration_copy(mib):
     return {"calibration_copy_bytes_read": n * 4, "calibration_copy_bytes_written": n * 4, "ok": ok}
 
 
-def hbm_traffic(precision):
-    """Measured HBM bytes per k_propagate launch for this precision, from the
-    committed rocprofv3 counter summary (None when absent)."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+VALU_PEAK_GINST = 1024 * 2.4 / 4  # wave64 VALU instructions/s (G): 1024 SIMDs, 2.4 GHz, 4 clocks each
+
+
+def kernel_counters(precision):
+    """rocprofv3 counters of the k_propagate launches of this precision's
+    RunPatchMatch on the headline view, from the committed summary
+    (profiles/k_propagate_counters.json): "hbm" (measured bytes per launch)
+    and "valu" (VALU instructions per launch); {} when absent."""
+    path = os.path.join(ROOT, "profiles", "k_propagate_counters.json")
     try:
         with open(path) as f:
-            return json.load(f).get(precision)
+            return json.load(f).get(precision) or {}
     except (OSError, ValueError):
-        return None
+        return {}
 
 
 def run_views(ctx, steps, warmup, seed0):
@@ -489,7 +543,7 @@ def main():
     ap.add_argument("--precision", default="fp16", choices=["fp16", "bf16", "fp32fast", "fp32"])
     ap.add_argument("--no-plain-fp32", action="store_true", help="skip timing the plain FP32 path beside it")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=1200)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 oracle run behind accuracy_vs_fp64")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="run only an elementwise copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -575,31 +629,51 @@ def main():
         c.close()
     pool.close()
 
+    # the other precisions, one view at a time, and the accuracy of each against
+    # the FP64 oracle on the same seed
+    last_seed = SEED + 100 + args.steps - 1
+    ctx.run(last_seed)
+    runs = {args.precision: (r, ctx.planes())}
     plain = None
-    if not args.no_plain_fp32 and args.precision != "fp32":
-        ctx.set_params(Hm.default_params(precision="fp32"))
+    others = [] if args.no_plain_fp32 else [p for p in ("fp32", "fp32fast") if p != args.precision]
+    for prec in others:
+        ctx.set_params(Hm.default_params(precision=prec))
         rp = run_views(ctx, max(2, args.steps // 2), 1, SEED)
-        pp = ctx.planes()
-        # same seed as the last timed step of the single-view run
-        ctx.set_params(Hm.default_params(precision=args.precision))
-        ctx.run(SEED + 100 + args.steps - 1)
-        pf = ctx.planes()
-        ctx.set_params(Hm.default_params(precision="fp32"))
-        ctx.run(SEED + 100 + args.steps - 1)
-        pe = ctx.planes()
-        agree = float(np.mean(np.abs(pf[..., 3] - pe[..., 3]) / np.abs(pe[..., 3]) <= 0.01))
-        # pixels the plain FP32 run gets right (within 1 % of ground truth): share the FP16 run also gets right
+        ctx.run(last_seed)
+        runs[prec] = (rp, ctx.planes())
+        print(f"[bench] {prec}: {rp['views_s']:.2f} views/s", file=sys.stderr, flush=True)
+    ctx.set_params(Hm.default_params(precision=args.precision))
+    accuracy = None
+    if not args.no_fp64:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        print("[bench] FP64 oracle RunPatchMatch of the view ...", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        oc = [O.camera_from_scene(c) for c in cams]
+        r64 = O.run_patchmatch(oc, imgs, O.params(len(oc), oc[0]), last_seed, dtype=np.float64, canny=canny)
+        accuracy = {"seed": last_seed, "oracle_s": round(time.perf_counter() - t0, 1)}
+        for prec, (_, pl) in runs.items():
+            dep, nrm = agreement(pl, r64["planes"])
+            accuracy[prec] = {"depth_within_1pct_of_fp64": dep, "normals_within_5deg_of_fp64": nrm}
+        accuracy["fp64_depth_within_1pct_of_gt"] = round(
+            float(np.mean(np.abs(r64["planes"][..., 3] - gt) / gt <= 0.01)), 4)
+    if "fp32" in runs and args.precision != "fp32":
+        rp, pe = runs["fp32"]
+        pf = runs[args.precision][1]
         ok_e = np.abs(pe[..., 3] - gt) / gt <= 0.01
         ok_f = np.abs(pf[..., 3] - gt) / gt <= 0.01
-        agree_ok = float(np.mean(ok_f[ok_e])) if ok_e.any() else None
         plain = {"views_s": round(rp["views_s"], 4), "ms_per_step": round(1e3 * rp["elapsed"] / max(2, args.steps // 2), 3),
                  "k_propagate_avg_ms": round(rp["prop_ms"] / max(rp["launches"], 1), 3),
-                 "depth_within_1pct_of_gt": round(float(np.mean(np.abs(pp[..., 3] - gt) / gt <= 0.01)), 4),
+                 "depth_within_1pct_of_gt": round(float(np.mean(ok_e)), 4),
                  "speedup_batched": round(views_s_b / rp["views_s"], 3),
                  "speedup_single_view": round(r["views_s"] / rp["views_s"], 3),
-                 "depth_within_1pct_of_plain_fp32": round(agree, 4),
-                 "fp16_right_where_fp32_right": None if agree_ok is None else round(agree_ok, 4)}
-        print(f"[bench] plain fp32: {rp['views_s']:.2f} views/s", file=sys.stderr, flush=True)
+                 "depth_within_1pct_of_plain_fp32": agreement(pf, pe)[0],
+                 "right_where_plain_fp32_right": round(float(np.mean(ok_f[ok_e])), 4) if ok_e.any() else None}
+    fast = None
+    if "fp32fast" in runs and args.precision != "fp32fast":
+        rp = runs["fp32fast"][0]
+        fast = {"views_s": round(rp["views_s"], 4), "k_propagate_avg_ms": round(rp["prop_ms"] / max(rp["launches"], 1), 3),
+                "speedup_of_headline_single_view": round(r["views_s"] / rp["views_s"], 3)}
 
     bytes_per_texel = {"fp32": 4, "fp32fast": 4, "fp16": 2, "bf16": 2}[args.precision]
     V = NSRC
@@ -607,8 +681,28 @@ def main():
     per_launch_evals = r["ncc_prop"] / launches
     alg_bytes = per_launch_evals * (181 * bytes_per_texel) + (per_launch_evals / V) * (16 + 4 * V)
     avg_launch_s = (r["prop_ms"] / launches) / 1e3
-    achieved = alg_bytes / avg_launch_s / 1e9
-    traffic = hbm_traffic(args.precision)
+    alg_gbs = alg_bytes / avg_launch_s / 1e9
+    counters = kernel_counters(args.precision)
+    hbm = counters.get("hbm") or {}
+    traffic = hbm.get("bytes_per_launch")
+    valu = (counters.get("valu") or {}).get("insts_per_launch")
+    labelled = {"kernel": "k_propagate", "avg_launch_ms": round(avg_launch_s * 1e3, 3),
+                "ncc_evals_per_launch": int(per_launch_evals),
+                "hbm_measured_gbs": None if traffic is None else round(traffic / avg_launch_s / 1e9, 1),
+                "hbm_measured_frac": None if traffic is None else round(traffic / avg_launch_s / 1e9 / HBM_PEAK_GBS, 4),
+                "algorithmic_bytes_per_launch": int(alg_bytes), "algorithmic_gbs": round(alg_gbs, 1),
+                "algorithmic_note": "181 texels per NCC evaluation priced at HBM rate; the gathers are served by "
+                                    "L1/L2/Infinity Cache, so this is not a bandwidth bound"}
+    if valu is not None:
+        achieved = valu / avg_launch_s / 1e9
+        roofline = {"bound": "valu", "achieved": round(achieved, 1), "peak": VALU_PEAK_GINST, "unit": "Ginst/s",
+                    "frac": round(achieved / VALU_PEAK_GINST, 4), "traffic": traffic,
+                    "valu_insts_per_launch": valu,
+                    "peak_note": "wave64 VALU instructions: 1024 SIMDs x 2.4 GHz / 4 clocks (no packed FP32)"}
+    else:
+        roofline = {"bound": "hbm", "achieved": round(alg_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(alg_gbs / HBM_PEAK_GBS, 4), "traffic": traffic}
+    roofline.update(labelled)
 
     out = {
         "metric": "views/s @1600x1200, 10 src: plain FP32 vs FP16+batched, at <=1% depth err",
@@ -627,14 +721,12 @@ def main():
                    "precision": args.precision, "width": W_FULL, "height": H_FULL, "sources": NSRC,
                    "views_per_step": len(ctxs), "batched": args.batch_mode, "hipgraph": True,
                    "launch_shape": tuned},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None if traffic is None else traffic["bytes_per_launch"],
-                     "kernel": "k_propagate", "avg_launch_ms": round(avg_launch_s * 1e3, 3),
-                     "alg_bytes_per_launch": int(alg_bytes), "ncc_evals_per_launch": int(per_launch_evals)},
+        "roofline": roofline,
         "single_view": {"views_s": round(r["views_s"], 4), "ms_per_step": round(1e3 * r["elapsed"] / args.steps, 3)},
         "batch_modes_views_s": {k: round(v[0], 4) for k, v in batch_rates.items()},
         "plain_fp32": plain,
+        "fp32_fast": fast,
+        "accuracy_vs_fp64": accuracy,
         "mpix_hyp_view_per_s": round(r["ncc_prop"] / (r["prop_ms"] / 1e3) / 1e6, 1) if r["prop_ms"] > 0 else None,
         "mpix_hyp_view_per_s_reference_count": round(r["ncc_ref"] / r["elapsed"] / 1e6, 1),
         "depth_within_1pct_of_gt": round(within1, 4),
@@ -642,8 +734,8 @@ def main():
     }
     ctx.close()
     if not args.no_cpu_baseline:
-        print("[bench] CPU baseline (oracle band) ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(cams, imgs, canny, rows=args.cpu_rows)
+        print("[bench] CPU baseline (oracle bands) ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(cams, imgs, canny)
     print(json.dumps(out), flush=True)
 
 

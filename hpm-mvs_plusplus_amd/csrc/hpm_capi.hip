@@ -649,6 +649,30 @@ int hpm_set_prior(hpm_ctx* ctx, const float* prior_planes, const uint32_t* masks
   return upload(ctx, ctx->d_masks, masks, npix(ctx) * sizeof(uint32_t));
 }
 
+int hpm_release_prior(hpm_ctx* ctx) {
+  if (!ctx) return HPM_E_INVALID;
+  HIP_OK(hipSetDevice(ctx->device));
+  void* bufs[] = {ctx->d_prior, ctx->d_masks, ctx->d_canny, ctx->d_prior_ds};
+  for (void* p : bufs)
+    if (p) hipFreeAsync(p, ctx->stream);
+  ctx->d_prior = nullptr; ctx->d_masks = nullptr; ctx->d_canny = nullptr; ctx->d_prior_ds = nullptr;
+  ctx->prior_dist = -1; ctx->prior_w = ctx->prior_h = 0;
+  return HPM_OK;
+}
+
+int hpm_release(hpm_ctx* ctx) {
+  if (!ctx) return HPM_E_INVALID;
+  HIP_OK(hipSetDevice(ctx->device));
+  if (ctx->gexec) {
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    hipGraphExecDestroy(ctx->gexec);
+    ctx->gexec = nullptr;
+    ctx->gkey.clear();
+  }
+  free_views(ctx);
+  return HPM_OK;
+}
+
 }  // extern "C"
 
 namespace {

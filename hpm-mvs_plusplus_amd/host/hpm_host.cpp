@@ -1044,7 +1044,7 @@ void HPM::BuildPlanarPrior(int distance) {
   if (count > cap) throw Error("hpm_prior_support_points: more points than blocks");
   std::vector<Point2i> support((size_t)count);
   for (int i = 0; i < count; ++i) support[i] = {xy[2 * i], xy[2 * i + 1]};
-  const std::vector<Triangle> tris = DelaunayTriangulation(mw, mh, support);
+  const std::vector<Triangle> tris = hpmhost::DelaunayTriangulation(mw, mh, support);
   std::vector<int32_t> tv(6 * tris.size());
   for (size_t t = 0; t < tris.size(); ++t) {
     const Triangle& T = tris[t];
@@ -1085,6 +1085,7 @@ static std::atomic<int64_t> g_fused_min_pixels{4000000};
 void SetFusedMinPixels(int64_t n) { g_fused_min_pixels = n; }
 
 void HPM::RunPatchMatch(uint64_t seed) {
+  have_results_ = false;
   check(hpm_set_params(ctx_, &params_), "hpm_set_params");
   // At 3072 x 2048 with 10 sources the narrower tiles keep more of the source
   // gathers in the caches: 7-20 % less time per RunPatchMatch.
@@ -1184,6 +1185,128 @@ float HPM::GetDistance2Origin(int x, int y, float depth, const float n[4]) const
   float X[3];
   PointOnRefCam(x, y, depth, cams_[0], X);
   return -(n[0] * X[0] + n[1] * X[1] + n[2] * X[2]);
+}
+
+void HPM::FetchResults() {
+  if (have_results_) return;
+  const size_t n = (size_t)GetReferenceImageWidth() * GetReferenceImageHeight();
+  planes_host_.resize(4 * n);
+  costs_host_.resize(n);
+  texture_host_.resize(n);
+  check(hpm_get_planes(ctx_, planes_host_.data()), "hpm_get_planes");
+  check(hpm_get_costs(ctx_, costs_host_.data()), "hpm_get_costs");
+  check(hpm_get_texture(ctx_, texture_host_.data()), "hpm_get_texture");
+  have_results_ = true;
+}
+
+Float4 HPM::GetPlaneHypothesis(int index) {  // HPM.cpp:1001-1004
+  FetchResults();
+  const float* p = &planes_host_.at(4 * (size_t)index);
+  return Float4{p[0], p[1], p[2], p[3]};
+}
+
+float HPM::GetCost(int index) {  // HPM.cpp:1006-1009
+  FetchResults();
+  return costs_host_.at((size_t)index);
+}
+
+float HPM::GetTexture(int index) {  // HPM.cpp:1011-1014
+  FetchResults();
+  return texture_host_.at((size_t)index);
+}
+
+void HPM::GetSupportPoints_Classify_Check(std::vector<Point2i>& support2DPoints, const MapF& costs,
+                                          const MapF& confidences, const MapF& texture, float hpm_factor) {
+  (void)texture;  // its flag is computed but never used (HPM.cpp:1041-1048)
+  const int width = (int)((float)GetReferenceImageWidth() * hpm_factor);
+  const int height = (int)((float)GetReferenceImageHeight() * hpm_factor);
+  support2DPoints = GetSupportPoints(costs, confidences, width, height);
+}
+
+std::vector<Triangle> HPM::DelaunayTriangulation(const Rect& boundRC, const std::vector<Point2i>& points) {
+  if (boundRC.x != 0 || boundRC.y != 0) throw Error("DelaunayTriangulation: the bounding rectangle starts at (0, 0)");
+  return hpmhost::DelaunayTriangulation(boundRC.width, boundRC.height, points);
+}
+
+Float4 HPM::GetPriorPlaneParams(const Triangle& triangle, const MapF& depths) const {
+  Float4 r;
+  GetPriorPlaneParams(triangle, depths, &r.x);
+  return r;
+}
+
+Float4 HPM::GetPriorPlaneParams_factor(const Triangle& triangle, const MapF& depths, float factor) const {
+  Float4 r;
+  GetPriorPlaneParamsFactor(triangle, depths, factor, &r.x);
+  return r;
+}
+
+float HPM::GetDepthFromPlaneParam(const Float4& plane, int x, int y) const {
+  return GetDepthFromPlaneParam(&plane.x, x, y);
+}
+
+float HPM::GetDepthFromPlaneParam_factor(const Float4& plane, int x, int y, float factor) const {
+  return GetDepthFromPlaneParamFactor(&plane.x, x, y, factor);
+}
+
+Float4 HPM::TransformNormal(const Float4& plane) const {
+  Float4 r;
+  TransformNormal(&plane.x, &r.x);
+  return r;
+}
+
+Float4 HPM::TransformNormal2RefCam(const Float4& plane) const {
+  Float4 r;
+  TransformNormal2RefCam(&plane.x, &r.x);
+  return r;
+}
+
+float HPM::GetDistance2Origin(const Int2& p, float depth, const Float4& normal) const {
+  return GetDistance2Origin(p.x, p.y, depth, &normal.x);
+}
+
+void HPM::CudaPlanarPriorInitialization(const std::vector<Float4>& plane_params, const MapF& masks) {
+  std::vector<float> flat(4 * plane_params.size());
+  if (!plane_params.empty()) std::memcpy(flat.data(), plane_params.data(), flat.size() * sizeof(float));
+  CudaPlanarPriorInitialization(flat, masks);
+}
+
+void HPM::ReloadPlanarPriorInitialization(const MapF& masks, const Float4* prior_plane_parameters) {
+  const size_t n = (size_t)masks.width * masks.height;
+  std::vector<float> flat(4 * n);
+  std::memcpy(flat.data(), prior_plane_parameters, flat.size() * sizeof(float));
+  ReloadPlanarPriorInitialization(masks, flat);
+}
+
+void HPM::JointBilateralUpsampling_prior(const MapF& scaled_image_float, const MapF& src_depthmap,
+                                         MapF& upsample_depthmap, const MapF& src_normal, MapF& upsample_normal) {
+  const int w = scaled_image_float.width, h = scaled_image_float.height;
+  const int cw = src_depthmap.width, ch = src_depthmap.height;
+  if (src_normal.width != cw || src_normal.height != ch || src_normal.channels != 3)
+    throw Error("JointBilateralUpsampling_prior: a 3-channel normal map of the depth map's size is needed");
+  if (cw <= 0 || ch <= 0 || std::max(h / ch, w / cw) == 1) return;  // HPM.cpp:1327-1331: the outputs stay as they are
+  std::vector<float> n4((size_t)4 * cw * ch, 0.0f);
+  for (size_t i = 0; i < (size_t)cw * ch; ++i) std::memcpy(&n4[4 * i], &src_normal.data[3 * i], 3 * sizeof(float));
+  upsample_depthmap = MapF(w, h);
+  std::vector<float> up4((size_t)4 * w * h);
+  if (hpm_jbu_prior(scaled_image_float.data.data(), w, h, src_depthmap.data.data(), n4.data(), cw, ch,
+                    upsample_depthmap.data.data(), up4.data()) != HPM_OK)
+    throw Error("hpm_jbu_prior failed");
+  upsample_normal = MapF(w, h, 3);
+  for (size_t i = 0; i < (size_t)w * h; ++i) std::memcpy(&upsample_normal.data[3 * i], &up4[4 * i], 3 * sizeof(float));
+}
+
+void HPM::CudaPlanarPriorRelease() { check(hpm_release_prior(ctx_), "hpm_release_prior"); }
+
+void HPM::CudaSpaceRelease(bool geom_consistency) {
+  (void)geom_consistency;  // the source depth maps go with the other buffers
+  FetchResults();
+  check(hpm_release(ctx_), "hpm_release");
+}
+
+void HPM::ReleaseProblemHostMemory() {
+  images_.clear();
+  owned_images_.clear();
+  depths_.clear();
 }
 
 // -------------------------------------------------------------- pass driver

@@ -190,6 +190,18 @@ struct Point2i {
 struct Triangle {  // main.h:87-90
   Point2i pt1, pt2, pt3;
 };
+// The value types of the reference interface: CUDA's float4 / int2 and
+// cv::Rect (x, y, width, height; contains() is half-open).
+struct Float4 {
+  float x = 0, y = 0, z = 0, w = 0;
+};
+struct Int2 {
+  int x = 0, y = 0;
+};
+struct Rect {
+  int x = 0, y = 0, width = 0, height = 0;
+  bool contains(const Point2i& p) const { return p.x >= x && p.x < x + width && p.y >= y && p.y < y + height; }
+};
 
 // HPM::GetSupportPoints_Classify_Check, HPM.cpp:1026-1073.
 std::vector<Point2i> GetSupportPoints(const MapF& costs, const MapF& confidences, int width, int height);
@@ -257,6 +269,11 @@ class HPM {
   void CudaPlanarPriorInitialization(const std::vector<float>& plane_params, const MapF& masks);
   void ReloadPlanarPriorInitialization(const MapF& masks, const std::vector<float>& prior_planes);
   void RunPatchMatch(uint64_t seed);
+  // RunPatchMatch() (HPM.cu:2987-3035) with the seed of SetSeed (0 by
+  // default); the reference seeds curand with clock64(), this build's
+  // counter-based generator with the seed.
+  void RunPatchMatch() { RunPatchMatch(seed_); }
+  void SetSeed(uint64_t seed) { seed_ = seed; }
 
   int GetReferenceImageWidth() const { return cams_.empty() ? 0 : cams_[0].width; }
   int GetReferenceImageHeight() const { return cams_.empty() ? 0 : cams_[0].height; }
@@ -283,6 +300,40 @@ class HPM {
   void TransformNormal2RefCam(const float in[4], float out[4]) const;
   float GetDistance2Origin(int x, int y, float depth, const float normal[4]) const;
 
+  // The reference's signatures of the same (HPM.h:74-101), for ProcessProblem
+  // (main.cpp:257-631) unchanged.  The per-pixel getters read host copies of
+  // the planes, costs and texture taken on the first call after a run
+  // (RunPatchMatch's D2H copies, HPM.cu:3029-3033).
+  Float4 GetPlaneHypothesis(int index);  // (world normal, depth)
+  float GetCost(int index);
+  float GetTexture(int index);
+  // support points of a (downsampled) map whose size is the reference size x
+  // hpm_factor, truncated (HPM.cpp:1026-1073); texture is not used there
+  void GetSupportPoints_Classify_Check(std::vector<Point2i>& support2DPoints, const MapF& costs,
+                                       const MapF& confidences, const MapF& texture, float hpm_factor);
+  std::vector<Triangle> DelaunayTriangulation(const Rect& boundRC, const std::vector<Point2i>& points);
+  Float4 GetPriorPlaneParams(const Triangle& triangle, const MapF& depths) const;
+  Float4 GetPriorPlaneParams_factor(const Triangle& triangle, const MapF& depths, float factor) const;
+  float GetDepthFromPlaneParam(const Float4& plane, int x, int y) const;
+  float GetDepthFromPlaneParam_factor(const Float4& plane, int x, int y, float factor) const;
+  Float4 TransformNormal(const Float4& plane) const;
+  Float4 TransformNormal2RefCam(const Float4& plane) const;
+  float GetDistance2Origin(const Int2& p, float depth, const Float4& normal) const;
+  void CudaPlanarPriorInitialization(const std::vector<Float4>& plane_params, const MapF& masks);
+  void ReloadPlanarPriorInitialization(const MapF& masks, const Float4* prior_plane_parameters);
+  // JointBilateralUpsampling_prior (HPM.cpp:1323-1392): the coarse prior
+  // depth and 3-channel normal maps upsampled to the reference image's size,
+  // guided by it (JBU_cu_prior, HPM.cu:3130-3197).
+  void JointBilateralUpsampling_prior(const MapF& scaled_image_float, const MapF& src_depthmap,
+                                      MapF& upsample_depthmap, const MapF& src_normal, MapF& upsample_normal);
+  // CudaPlanarPriorRelease (HPM.cpp:589-597), CudaSpaceRelease
+  // (HPM.cpp:599-633): device buffers freed; the getters keep working from
+  // their host copies.  ReleaseProblemHostMemory (HPM.cpp:635-642): images,
+  // cameras and source depths dropped.
+  void CudaPlanarPriorRelease();
+  void CudaSpaceRelease(bool geom_consistency);
+  void ReleaseProblemHostMemory();
+
  private:
   void check(int rc, const char* what) const;
   hpm_ctx* ctx_ = nullptr;
@@ -294,6 +345,12 @@ class HPM {
   std::vector<MapRef> depths_;
   hpm_pool* pool_ = nullptr;  // the level's device images, when every view is there
   std::vector<int32_t> pool_ids_;
+  uint64_t seed_ = 0;
+  // host copies for the per-pixel getters, valid from the first getter call
+  // after a run until the next run
+  void FetchResults();
+  bool have_results_ = false;
+  std::vector<float> planes_host_, costs_host_, texture_host_;
 };
 
 // ---------------------------------------------------------- pass driver --

@@ -141,6 +141,14 @@ int hpm_reload_hypotheses_maps(hpm_ctx* ctx, const float* depth, const float* no
  * (HPM.cpp:964-984, 1433-1453): prior plane per pixel and triangle mask
  * (0 = no prior, else triangle index + 1). */
 int hpm_set_prior(hpm_ctx* ctx, const float* prior_planes, const uint32_t* masks);
+/* CudaPlanarPriorRelease (HPM.cpp:589-597): frees the prior planes, the
+ * triangle masks and the edge map (freed in the order of the context's
+ * stream). */
+int hpm_release_prior(hpm_ctx* ctx);
+/* CudaSpaceRelease (HPM.cpp:599-633): frees every per-view device buffer
+ * (images, hypotheses, costs, masks, source depths, prior); the context keeps
+ * its parameters and takes new views with hpm_set_views. */
+int hpm_release(hpm_ctx* ctx);
 
 /* The planar prior built on the device from the maps the context holds:
  * reloaded hypotheses (hpm_reload_hypotheses: the depth in w) and costs, and

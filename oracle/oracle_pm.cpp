@@ -31,6 +31,8 @@
 #include <cstring>
 #include <vector>
 
+#include <omp.h>
+
 #include "oracle_math.h"
 
 extern "C" {
@@ -1419,6 +1421,10 @@ template <class T> int GeomCostImpl(const orc_camera* cams, int num_images, cons
 extern "C" {
 
 void orc_set_reverse_sweep(int reverse) { g_reverse_sweep = reverse; }
+
+// Threads of the oracle's OpenMP loops from now on (the CPU-baseline timing
+// runs them at the host's thread count and at 1).
+void orc_set_threads(int n) { omp_set_num_threads(n < 1 ? 1 : n); }
 
 
 int orc_run_patchmatch_f32(const orc_camera* cams, int num_images, const float* const* images,

@@ -61,6 +61,7 @@ def lib():
         f32 = ctypes.c_float
         L.orc_expf.argtypes = [f32]; L.orc_expf.restype = f32
         L.orc_set_reverse_sweep.argtypes = [ctypes.c_int]; L.orc_set_reverse_sweep.restype = None
+        L.orc_set_threads.argtypes = [ctypes.c_int]; L.orc_set_threads.restype = None
         L.orc_acosf.argtypes = [f32]; L.orc_acosf.restype = f32
         L.orc_reproj_error.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_void_p]
         L.orc_reproj_error.restype = None
