@@ -897,6 +897,79 @@ int hpm_canny(int device, const uint8_t* image, int w, int h, int channels, floa
   return rc;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// hpm_canny_noalloc's scratch: the gradient, magnitude and class planes of up
+// to HPM_CANNY_NOALLOC_MAX_PIXELS pixels (7 bytes each) and the hysteresis
+// flag, in a file-scope device array.  Every GPU the library runs on holds its
+// own copy until the process ends.  In host code `scratch` only names the
+// symbol: its device address on each device comes from hipGetSymbolAddress,
+// taken once per device (with that device current) on the first call there.
+namespace {
+constexpr size_t kCannyScratchPlanes = (size_t)HPM_CANNY_NOALLOC_MAX_PIXELS * 7;
+constexpr size_t kCannyScratchBytes = ((kCannyScratchPlanes + 255) & ~(size_t)255) + 256;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+static __device__ __align__(256) char scratch[kCannyScratchBytes];
+
+static int canny_scratch(int device, char** out) {
+  if (device < 0 || device >= kMaxDevices) return HPM_E_DEVICE;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    char* ptr = nullptr;
+    if (hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return HPM_E_DEVICE;
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return HPM_OK;
+}
+
+extern "C" {
+
+int hpm_canny_noalloc(const uint8_t* image_dev, int w, int h, int channels, float low, float high,
+                      uint8_t* out_edges_dev) {
+  if (!image_dev || !out_edges_dev || w <= 0 || h <= 0 || (channels != 1 && channels != 3)) return HPM_E_INVALID;
+  if ((size_t)w * h > (size_t)HPM_CANNY_NOALLOC_MAX_PIXELS) return HPM_E_TOO_LARGE;
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess) return HPM_E_DEVICE;
+  char* base = nullptr;
+  int rc = canny_scratch(device, &base);
+  if (rc) return rc;
+  const size_t P = (size_t)w * h;
+  // gx, gy, mag (int16), cls (uint8), flag (int): 256-byte aligned pieces
+  const size_t sizes[5] = {2 * P, 2 * P, 2 * P, P, 4};
+  size_t off[5], total = 0;
+  for (int k = 0; k < 5; ++k) { off[k] = total; total += (sizes[k] + 255) & ~(size_t)255; }
+  if (total > kCannyScratchBytes) return HPM_E_TOO_LARGE;
+  int16_t* gx = (int16_t*)(base + off[0]);
+  int16_t* gy = (int16_t*)(base + off[1]);
+  int16_t* mag = (int16_t*)(base + off[2]);
+  uint8_t* cls = (uint8_t*)(base + off[3]);
+  int* flag = (int*)(base + off[4]);
+  const hipStream_t st = hipStreamPerThread;
+  const dim3 g8((w + 31) / 32, (h + 7) / 8), g32((w + 31) / 32, (h + 31) / 32);
+  // every plane is written in full by the kernel that produces it; only the
+  // flag must start at zero, and it is cleared before every hysteresis pass
+  hipLaunchKernelGGL(k_canny_grad, g8, dim3(256), 0, st, image_dev, w, h, channels, gx, gy, mag);
+  hipLaunchKernelGGL(k_canny_nms, g8, dim3(256), 0, st, gx, gy, mag, w, h, low, high, cls, out_edges_dev);
+  if (hipGetLastError() != hipSuccess) return HPM_E_DEVICE;
+  // every pass that goes on adds at least one pixel: at most P + 1 passes
+  for (size_t it = 0; it <= P; ++it) {
+    int changed = 0;
+    if (hipMemsetAsync(flag, 0, sizeof(int), st) != hipSuccess) return HPM_E_DEVICE;
+    hipLaunchKernelGGL(k_canny_hyst, g32, dim3(256), 0, st, cls, w, h, out_edges_dev, flag);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&changed, flag, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return HPM_E_DEVICE;
+    if (!changed) break;
+  }
+  return HPM_OK;
+}
+
 int hpm_get_prior(hpm_ctx* ctx, float* prior_planes, uint32_t* masks) {
   if (!ctx || !prior_planes || !masks) return HPM_E_INVALID;
   if (!ctx->d_prior || !ctx->d_masks) return fail(ctx, HPM_E_STATE, "no prior on the context");

@@ -41,6 +41,7 @@ extern "C" {
 #define HPM_E_STATE (-2)        /* call out of order (e.g. run before set_views) */
 #define HPM_E_DEVICE (-3)       /* HIP runtime error */
 #define HPM_E_NOMEM (-4)        /* device allocation failed */
+#define HPM_E_TOO_LARGE (-5)    /* input larger than an entry point's fixed scratch */
 
 #define HPM_MAX_IMAGES 33       /* reference + 32 sources: view masks are uint32 (HPM.cu:67-75) */
 
@@ -183,6 +184,21 @@ int hpm_get_prior(hpm_ctx* ctx, float* prior_planes, uint32_t* masks);
  * out_edges = 255 on edges, else 0.  Runs on `device` on a stream of its own
  * and returns when the map is on the host. */
 int hpm_canny(int device, const uint8_t* image, int w, int h, int channels, float low, float high, uint8_t* out_edges);
+/* hpm_canny for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc); no
+ * counterpart in the reference.  It allocates nothing: image_dev
+ * (w*h*channels bytes) and out_edges_dev (w*h bytes) are device buffers on the
+ * current device, the gradient, magnitude and class planes live in a static
+ * device array of 7 bytes per pixel for up to HPM_CANNY_NOALLOC_MAX_PIXELS
+ * pixels (44 MiB, held by every GPU the library runs on until the process
+ * ends).  w*h above the limit returns HPM_E_TOO_LARGE before any launch.  Runs
+ * on the calling thread's default stream (hipStreamPerThread) and returns when
+ * the edges are written; the same edges as hpm_canny.  Calls share their
+ * device's array, which keeps the last call's contents: calls that overlap on
+ * one device race. */
+#define HPM_CANNY_NOALLOC_MAX_PIXELS (3072 * 2048)
+int hpm_canny_noalloc(const uint8_t* image_dev, int w, int h, int channels, float low, float high,
+                      uint8_t* out_edges_dev);
 
 /* Hierarchy reload (HPM.cpp:849-905): the coarse level's maps (cw x ch) and the
  * JBU-upsampled fine depth.  coarse = (world normal xyz, w) where w is the
