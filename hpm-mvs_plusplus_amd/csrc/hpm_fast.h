@@ -31,14 +31,6 @@
 #include "hpm_device.h"
 
 
-// Build-time knobs (measured variants): noinline keeps one copy of the 36-tap
-// cost body in the propagation kernel instead of one per call site.
-#ifdef HPM_NCC_NOINLINE
-#define HPM_NCC_INLINE __device__ __attribute__((noinline))
-#else
-#define HPM_NCC_INLINE __device__ __forceinline__
-#endif
-
 namespace hpm {
 
 __device__ __forceinline__ float half_bits_to_float(uint32_t b) {
@@ -56,13 +48,8 @@ __device__ __forceinline__ float lerp_cell(float t00, float t10, float t01, floa
   return ffma(ay, bot - top, top);
 }
 
-#ifndef HPM_MIX_LERP
-#define HPM_MIX_LERP 1
-#endif
 // columns of taps whose gathers are issued ahead of the column being summed
-#ifndef HPM_PIPE_COLS
-#define HPM_PIPE_COLS 1
-#endif
+constexpr int kPipeCols = 1;
 
 struct QuadF16 {
   using Q = uint2;
@@ -184,7 +171,7 @@ struct FastEngine {
     return dfma(r, dfma(-x, r, 1.0), r);
   }
 
-  HPM_NCC_INLINE static float ncc(const RunConsts& C, int v, int px, int py, float4 pl, const Win& fw) {
+  __device__ __forceinline__ static float ncc(const RunConsts& C, int v, int px, int py, float4 pl, const Win& fw) {
     // m = K_r^-T n / d (per plane), H = A - b m^T (per view pair), in double
     const double inv_d = recip((double)pl.w);
     const double m0 = (double)pl.x * C.kinv[0] * inv_d;
@@ -240,8 +227,8 @@ struct FastEngine {
       off = ((unsigned)rel + base) << S::kShift;
     };
     // Software pipeline over the six columns: the loads of columns
-    // ii + 1 .. ii + HPM_PIPE_COLS are in flight while column ii is consumed.
-    constexpr int L = HPM_PIPE_COLS;
+    // ii + 1 .. ii + kPipeCols are in flight while column ii is consumed.
+    constexpr int L = kPipeCols;
     constexpr int R = L + 1;  // ring of column buffers
     typename S::Q q[R][kTapsPerAxis];
     float axr[R][kTapsPerAxis], ayr[R][kTapsPerAxis];
@@ -300,159 +287,4 @@ __global__ __launch_bounds__(256) void k_build_quad(const float* __restrict__ im
 
 }  // namespace hpm
 
-namespace hpm {
-
-// Sum over the four lanes of a quad (lanes 4k..4k+3) with two DPP quad_perm
-// adds.  a + b == b + a in IEEE arithmetic, so every lane of the quad ends
-// with the same bits.
-__device__ __forceinline__ float quad_sum(float x) {
-  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));  // [1,0,3,2]
-  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x4E, 0xF, 0xF, false));  // [2,3,0,1]
-  return x;
-}
-
-// Four lanes per pixel: lane q of the quad owns the 3 x 3 block (q & 1, q >> 1)
-// of the 6 x 6 tap grid, i.e. x offsets -5 + 6 (q & 1) + {0, 2, 4} and y offsets
-// -5 + 6 (q >> 1) + {0, 2, 4}.  The window sums are reduced across the quad, so
-// a lane holds 9 weights instead of 36.  All four lanes run the pixel's
-// control flow with identical values.
-template <class S>
-struct QuadLaneEngine {
-  static constexpr int kLanes = 4;
-  struct Win {
-    float w[9];      // bilateral weights of the lane's taps
-    float wr[9];     // w * (r - r_centre)
-    float c_r;       // centre texel (shift of both windows)
-    float inv_w, m_r, var_r;
-    float bi, bj;    // offset of the lane's first tap
-  };
-
-  __device__ __forceinline__ static int quad_lane() { return (int)(__lane_id() & 3u); }
-
-  __device__ __forceinline__ static void build(const RunConsts& C, int px, int py, Win& fw) {
-    const float* ref = C.img[0];
-    const int W = C.cams[0].width, H = C.cams[0].height;
-    const float c = tex_at(ref, W, H, px, py);
-    const float den_s = 2.0f * C.sigma_spatial * C.sigma_spatial;
-    const float den_c = 2.0f * C.sigma_color * C.sigma_color;
-    const int q = quad_lane();
-    const int a = q & 1, b = q >> 1;
-    fw.c_r = c;
-    fw.bi = (float)(-5 + 6 * a);
-    fw.bj = (float)(-5 + 6 * b);
-    // window statistics over all 36 taps
-    double sw = 0.0, sr = 0.0, srr = 0.0;
-#pragma unroll
-    for (int ii = 0; ii < kTapsPerAxis; ++ii) {
-#pragma unroll
-      for (int jj = 0; jj < kTapsPerAxis; ++jj) {
-        const int i = -5 + 2 * ii, j = -5 + 2 * jj;
-        const float r = tex_at(ref, W, H, px + i, py + j);
-        const float spatial = sqrtf((float)(i * i + j * j));
-        const float w = expf_hpm(-spatial / den_s - fabsf(r - c) / den_c);
-        const float rs = r - c;
-        sw += (double)w;
-        sr += (double)w * (double)rs;
-        srr += (double)w * (double)rs * (double)rs;
-      }
-    }
-    // the lane's own 3 x 3 block (same weight formula, same values)
-    const int bi = -5 + 6 * a, bj = -5 + 6 * b;
-#pragma unroll
-    for (int di = 0; di < 3; ++di) {
-#pragma unroll
-      for (int dj = 0; dj < 3; ++dj) {
-        const int i = bi + 2 * di, j = bj + 2 * dj;
-        const float r = tex_at(ref, W, H, px + i, py + j);
-        const float spatial = sqrtf((float)(i * i + j * j));
-        const float w = expf_hpm(-spatial / den_s - fabsf(r - c) / den_c);
-        fw.w[di * 3 + dj] = w;
-        fw.wr[di * 3 + dj] = w * (r - c);
-      }
-    }
-    const double m = sr / sw;
-    fw.inv_w = (float)(1.0 / sw);
-    fw.m_r = (float)m;
-    fw.var_r = (float)(srr / sw - m * m);
-  }
-
-  __device__ __forceinline__ static float ncc(const RunConsts& C, int v, int px, int py, float4 pl, const Win& fw) {
-    const double inv_d = 1.0 / (double)pl.w;
-    const double m0 = (double)pl.x * C.kinv[0] * inv_d;
-    const double m1 = (double)pl.y * C.kinv[1] * inv_d;
-    const double m2 = ((double)pl.z - (double)pl.x * C.kinv[2] - (double)pl.y * C.kinv[3]) * inv_d;
-    const double* A = C.A[v - 1];
-    const double* b = C.bvec[v - 1];
-    const double h0 = A[0] - b[0] * m0, h1 = A[1] - b[0] * m1, h2 = A[2] - b[0] * m2;
-    const double h3 = A[3] - b[1] * m0, h4 = A[4] - b[1] * m1, h5 = A[5] - b[1] * m2;
-    const double h6 = A[6] - b[2] * m0, h7 = A[7] - b[2] * m1, h8 = A[8] - b[2] * m2;
-    const double X = h0 * px + h1 * py + h2;
-    const double Y = h3 * px + h4 * py + h5;
-    const double Z = h6 * px + h7 * py + h8;
-    const double rz = 1.0 / Z;
-    const double uc = X * rz, vc = Y * rz;
-    const DevCam& sc = C.cams[v];
-    const int sw = sc.width, sh = sc.height;
-    if (uc >= (double)sw || uc < 0.0 || vc >= (double)sh || vc < 0.0) return 2.0f;
-    if (fw.var_r < 1e-5f) return 2.0f;
-    const double g6 = h6 * rz, g7 = h7 * rz;
-    const float fg6 = (float)g6, fg7 = (float)g7;
-    const float au = (float)(h0 * rz - uc * g6), bu = (float)(h1 * rz - uc * g7);
-    const float av = (float)(h3 * rz - vc * g6), bv = (float)(h4 * rz - vc * g7);
-    const double flu = floor(uc), flv = floor(vc);
-    const int iu = (int)flu + 1, iv = (int)flv + 1;
-    const float fu = (float)(uc - flu), fv = (float)(vc - flv);
-    const float lox = (float)(-iu), hix = (float)(sw - iu);
-    const float loy = (float)(-iv), hiy = (float)(sh - iv);
-    const int qp = sw + 1;
-    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<void*>(C.quad[v]), (short)0, qp * (sh + 1) * (int)sizeof(typename S::Q), 0x00020000);
-    // lane-specific bases of the tap terms: value at (bi, bj)
-    const float den0 = ffma(fw.bi, fg6, ffma(fw.bj, fg7, 1.0f));
-    const float nu0 = ffma(fw.bi, au, fw.bj * bu);
-    const float nv0 = ffma(fw.bi, av, fw.bj * bv);
-    // all nine gathers of the lane are issued before any is consumed
-    typename S::Q q[9];
-    float axs[9], ays[9];
-#pragma unroll
-    for (int di = 0; di < 3; ++di) {
-#pragma unroll
-      for (int dj = 0; dj < 3; ++dj) {
-        const float oi = (float)(2 * di), oj = (float)(2 * dj);
-        const float den = ffma(oi, fg6, ffma(oj, fg7, den0));
-        const float r = __builtin_amdgcn_rcpf(den);
-        const float du = ffma(oi, au, ffma(oj, bu, nu0)) * r;
-        const float dv = ffma(oi, av, ffma(oj, bv, nv0)) * r;
-        const float x = fu + du, y = fv + dv;
-        const float fx = floorf(x), fy = floorf(y);
-        axs[di * 3 + dj] = x - fx;
-        ays[di * 3 + dj] = y - fy;
-        const int tx = iu + (int)__builtin_amdgcn_fmed3f(fx, lox, hix);
-        const int ty = iv + (int)__builtin_amdgcn_fmed3f(fy, loy, hiy);
-        const unsigned off = (__umul24((unsigned)ty, (unsigned)qp) + (unsigned)tx) * (unsigned)sizeof(typename S::Q);
-        q[di * 3 + dj] = S::load(rsrc, off);
-      }
-    }
-    float S1 = 0.0f, S2 = 0.0f, S3 = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      const float sp = S::bilinear(q[k], axs[k], ays[k]) - fw.c_r;
-      const float ws = fw.w[k] * sp;
-      S1 += ws;
-      S2 = ffma(ws, sp, S2);
-      S3 = ffma(fw.wr[k], sp, S3);
-    }
-    S1 = quad_sum(S1);
-    S2 = quad_sum(S2);
-    S3 = quad_sum(S3);
-    const float m_s = S1 * fw.inv_w;
-    const float var_s = ffma(-m_s, m_s, S2 * fw.inv_w);
-    if (var_s < 1e-5f) return 2.0f;
-    const float cov = ffma(-fw.m_r, m_s, S3 * fw.inv_w);
-    const float c = 1.0f - cov * __builtin_amdgcn_rsqf(fw.var_r * var_s);
-    return fmaxf(0.0f, fminf(2.0f, c));
-  }
-};
-
-}  // namespace hpm
 

@@ -12,12 +12,8 @@
 #include "../../include/hpm_mi355x.h"
 #include "hpm_fast.h"
 
-// Mixed-precision engine: four lanes per pixel (QuadLaneEngine) or one
-// (FastEngine); both give the same costs to within FP32 rounding.
-#ifndef HPM_FAST_LANES
-#define HPM_FAST_LANES 1
-#endif
-template <class S> using FastFor = typename std::conditional<HPM_FAST_LANES == 4, hpm::QuadLaneEngine<S>, hpm::FastEngine<S>>::type;
+// The mixed-precision engine of storage type S (one lane per pixel).
+template <class S> using FastFor = hpm::FastEngine<S>;
 
 // Images of a whole scene on one device, shared by the contexts of its
 // reference views: FP32 images plus, per storage precision, the quad images of

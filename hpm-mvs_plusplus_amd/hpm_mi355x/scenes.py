@@ -283,16 +283,18 @@ def render_all(cams, planes, noise_sigma, seed, workers=None, views=None):
     """Renders every camera, or those in `views` (the others give (None, None,
     None)).  Camera k's pixel noise is seeded by (seed, k), so the result does
     not depend on the number of worker processes or on which views are
-    rendered."""
+    rendered.  The views are rendered on threads (numpy releases the GIL in
+    the ray casting), not in forked processes: a caller may already have a
+    HIP runtime initialised, which a forked child must not inherit."""
     keep = [k for k in range(len(cams)) if views is None or k in views]
     jobs = [(cams[k], planes, noise_sigma, [seed, k]) for k in keep]
     workers = workers if workers is not None else min(16, os.cpu_count() or 1, max(1, len(jobs)))
     if workers <= 1:
         done = [_render_job(j) for j in jobs]
     else:
-        import multiprocessing as mp
-        with mp.get_context("fork").Pool(workers) as pool:
-            done = pool.map(_render_job, jobs, chunksize=1)
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(workers) as pool:
+            done = list(pool.map(_render_job, jobs))
     out = [(None, None, None)] * len(cams)
     for k, r in zip(keep, done):
         out[k] = r

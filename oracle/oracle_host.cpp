@@ -189,9 +189,134 @@ inline float GetAngle(const float* a, const float* b) {
   return angle;
 }
 
+// ------------------------------------------------------------ planar prior
+// GetSupportPoints_Classify_Check (HPM.cpp:1026-1073): per 5 x 5 block,
+// column-major over the blocks, the pixel minimising cost - confidence and
+// the one minimising cost + 0.2 - confidence (a double expression stored as
+// float), among pixels with cost < 2; each kept when below 0.1 (double).
+// The texture flag the reference computes is never read.
+int SupportPoints(const float* costs, const float* conf, int w, int h, int32_t* out_xy, int cap) {
+  int n = 0;
+  for (int col = 0; col < w; col += 5)
+    for (int row = 0; row < h; row += 5) {
+      float best_plain = 2.0f, best_shift = 2.0f;
+      int plain_x = 0, plain_y = 0, shift_x = 0, shift_y = 0;
+      for (int c = col; c < std::min(w, col + 5); ++c)
+        for (int r = row; r < std::min(h, row + 5); ++r) {
+          const float cost = costs[(size_t)r * w + c], cf = conf[(size_t)r * w + c];
+          if (!(cost < 2.0f)) continue;
+          const float plain = cost - cf;
+          const float shifted = (float)(cost + 0.2 - cf);
+          if (best_plain > plain) { best_plain = plain; plain_x = c; plain_y = r; }
+          if (best_shift > shifted) { best_shift = shifted; shift_x = c; shift_y = r; }
+        }
+      // the shifted winner is pushed first (HPM.cpp:1065-1070)
+      if (best_shift < 0.1) { if (n < cap) { out_xy[2 * n] = shift_x; out_xy[2 * n + 1] = shift_y; } ++n; }
+      if (best_plain < 0.1) { if (n < cap) { out_xy[2 * n] = plain_x; out_xy[2 * n + 1] = plain_y; } ++n; }
+    }
+  return n;
+}
+
+// Get3DPointonRefCam (HPM.cpp:209-219): the pixel back-projected at `depth`.
+inline void RefCamPoint(const Cam& c, int x, int y, float depth, float X[3]) {
+  X[0] = depth * ((float)x - c.K[2]) / c.K[0];
+  X[1] = depth * ((float)y - c.K[5]) / c.K[4];
+  X[2] = depth;
+}
+
+// GetPriorPlaneParams (HPM.cpp:1134-1167): the unit null vector (n, w) of the
+// rows [X_k 1] -- here n = (X2 - X1) x (X3 - X1), w = -n . X1 in double,
+// scaled to unit length and stored as float as cv::SVD::solveZ returns it --
+// then divided by |n| (a double sqrt of float squares), with the sign that
+// makes w >= 0.
+void PriorPlane(const float X1[3], const float X2[3], const float X3[3], float plane[4]) {
+  const double a[3] = {(double)X2[0] - X1[0], (double)X2[1] - X1[1], (double)X2[2] - X1[2]};
+  const double b[3] = {(double)X3[0] - X1[0], (double)X3[1] - X1[1], (double)X3[2] - X1[2]};
+  double v[4] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0], 0.0};
+  v[3] = -(v[0] * X1[0] + v[1] * X1[1] + v[2] * X1[2]);
+  const double len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + v[3] * v[3]);
+  float n4[4];
+  for (int k = 0; k < 4; ++k) n4[k] = (float)(v[k] / len);
+  float norm2 = (float)std::sqrt((double)n4[0] * n4[0] + (double)n4[1] * n4[1] + (double)n4[2] * n4[2]);
+  if (n4[3] < 0) norm2 *= -1;
+  for (int k = 0; k < 4; ++k) plane[k] = n4[k] / norm2;
+}
+
+// GetDepthFromPlaneParam (HPM.cpp:1169-1172).
+inline float PlaneDepth(const Cam& c, const float pl[4], int x, int y) {
+  return -pl[3] * c.K[0] / (((float)x - c.K[2]) * pl[0] + (c.K[0] / c.K[4]) * ((float)y - c.K[5]) * pl[1] + c.K[0] * pl[2]);
+}
+
+// The distance-0 prior of ProcessProblem (main.cpp:385-431) and
+// CudaPlanarPriorInitialization (HPM.cpp:964-984): triangles with all three
+// corners inside the image, in order, get index k = 1, 2, ...; each is
+// rasterised by the barycentric walk of main.cpp:396-404 (later triangles
+// overwrite earlier ones) and gets the plane through its back-projected
+// corners; a pixel keeps its triangle only when that plane's depth there is
+// within the pass's depth range.  Prior planes of pixels without a triangle
+// stay 0.
+void PriorAtLevel(const Cam& cam, const float* depths, const int32_t* tris, int ntri, float* prior, uint32_t* masks) {
+  const int w = cam.width, h = cam.height;
+  std::vector<float> mask((size_t)w * h, 0.0f);
+  std::vector<float> planes;
+  uint32_t idx = 0;
+  for (int t = 0; t < ntri; ++t) {
+    const int32_t* T = tris + 6 * t;
+    bool inside = true;
+    for (int k = 0; k < 3; ++k) inside = inside && T[2 * k] >= 0 && T[2 * k] < w && T[2 * k + 1] >= 0 && T[2 * k + 1] < h;
+    if (!inside) continue;
+    const int x1 = T[0], y1 = T[1], x2 = T[2], y2 = T[3], x3 = T[4], y3 = T[5];
+    // float L = sqrt(pow(int, 2) + pow(int, 2)): the squares and sqrt in double
+    const float L01 = (float)std::sqrt(std::pow((double)(x1 - x2), 2) + std::pow((double)(y1 - y2), 2));
+    const float L02 = (float)std::sqrt(std::pow((double)(x1 - x3), 2) + std::pow((double)(y1 - y3), 2));
+    const float L12 = (float)std::sqrt(std::pow((double)(x2 - x3), 2) + std::pow((double)(y2 - y3), 2));
+    const float longest = std::max(L01, std::max(L02, L12));
+    const float step = (float)(1.0 / longest);
+    for (float p = 0; p < 1.0; p += step)
+      for (float q = 0; q < 1.0 - p; q += step) {
+        // p * x1 + q * x2 are float products and a float sum; (1.0 - p - q) * x3 is double
+        const float fx = p * (float)x1 + q * (float)x2, fy = p * (float)y1 + q * (float)y2;
+        const int x = (int)((double)fx + (1.0 - p - q) * x3);
+        const int y = (int)((double)fy + (1.0 - p - q) * y3);
+        if (x >= 0 && x < w && y >= 0 && y < h) mask[(size_t)y * w + x] = (float)(idx + 1.0);
+      }
+    float X1[3], X2[3], X3[3], pl[4];
+    RefCamPoint(cam, x1, y1, depths[(size_t)y1 * w + x1], X1);
+    RefCamPoint(cam, x2, y2, depths[(size_t)y2 * w + x2], X2);
+    RefCamPoint(cam, x3, y3, depths[(size_t)y3 * w + x3], X3);
+    PriorPlane(X1, X2, X3, pl);
+    planes.insert(planes.end(), pl, pl + 4);
+    ++idx;
+  }
+  const float dmin = cam.depth_min * 0.6f, dmax = cam.depth_max * 1.2f;  // GetMinDepth / GetMaxDepth
+  for (int x = 0; x < w; ++x)
+    for (int y = 0; y < h; ++y) {
+      const size_t i = (size_t)y * w + x;
+      uint32_t m = (uint32_t)mask[i];
+      if (m > 0) {
+        const float d = PlaneDepth(cam, &planes[4 * (m - 1)], x, y);
+        if (!(d <= dmax && d >= dmin)) m = 0;
+      }
+      masks[i] = m;
+      for (int k = 0; k < 4; ++k) prior[4 * i + k] = m > 0 ? planes[4 * (m - 1) + k] : 0.0f;
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+int orc_support_points(const float* costs, const float* conf, int w, int h, int32_t* out_xy, int cap, int* count) {
+  *count = SupportPoints(costs, conf, w, h, out_xy, cap);
+  return 0;
+}
+
+void orc_prior_plane(const float* X1, const float* X2, const float* X3, float* plane) { PriorPlane(X1, X2, X3, plane); }
+
+int orc_prior_level(const Cam* cam, const float* depths, const int32_t* tris, int ntri, float* prior, uint32_t* masks) {
+  PriorAtLevel(*cam, depths, tris, ntri, prior, masks);
+  return 0;
+}
 
 void orc_jbu_depth(const float* fine, int w, int h, const float* coarse, int cw, int ch, float* out) {
   const Jbu jp = MakeJbu(w, h, cw, ch);

@@ -65,6 +65,12 @@ def lib():
         L.orc_acosf.argtypes = [f32]; L.orc_acosf.restype = f32
         L.orc_reproj_error.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_void_p]
         L.orc_reproj_error.restype = None
+        L.orc_support_points.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.orc_prior_level.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                      ctypes.c_void_p]
+        L.orc_prior_plane.argtypes = [ctypes.c_void_p] * 4
+        L.orc_prior_plane.restype = None
         L.orc_cost_grad_f64.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_cost_params_f64.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,

@@ -783,3 +783,85 @@ def test_mandatory_consistency_adoption_and_confidence_hand_over():
     assert maps["conf"][py, px] == f32(0.1)
     assert maps["costs"][py, px] == pytest.approx(cost_now * math.exp(0.2 + 2 * 0.1), rel=2e-6)
     del cur
+
+
+# ------------------------------------------------------------------ planar prior (main.cpp:324-441)
+
+def test_prior_plane_through_three_points():
+    """GetPriorPlaneParams (HPM.cpp:1134-1167): the plane n . X + w = 0 through
+    three points, |n| = 1 and w >= 0 whatever their order."""
+    rng = np.random.default_rng(8)
+    for _ in range(200):
+        n = rng.normal(size=3); n /= np.linalg.norm(n)
+        w = rng.uniform(0.5, 5.0)
+        # three points on the plane: a point on it plus two in-plane directions
+        p0 = -w * n
+        a = np.cross(n, [1.0, 0.0, 0.0] if abs(n[0]) < 0.9 else [0.0, 1.0, 0.0]); a /= np.linalg.norm(a)
+        b = np.cross(n, a)
+        pts = [p0 + rng.uniform(-1, 1) * a + rng.uniform(-1, 1) * b for _ in range(3)]
+        for order in ((0, 1, 2), (2, 1, 0), (1, 2, 0)):
+            X = [np.ascontiguousarray(pts[k], np.float32) for k in order]
+            out = np.zeros(4, np.float32)
+            O.lib().orc_prior_plane(X[0].ctypes.data, X[1].ctypes.data, X[2].ctypes.data, out.ctypes.data)
+            assert out[3] >= 0
+            np.testing.assert_allclose(out, np.r_[n, w], atol=2e-5 * max(1.0, w))
+
+
+def test_support_points_by_blocks():
+    """GetSupportPoints_Classify_Check (HPM.cpp:1026-1073), restated per block
+    with numpy: over 5 x 5 blocks in column-major order, among pixels with
+    cost < 2 the first (column by column) minimum of cost + 0.2 - conf and of
+    cost - conf, each kept below 0.1, the first one first."""
+    rng = np.random.default_rng(2)
+    h, w = 23, 31
+    costs = rng.uniform(-0.3, 2.5, (h, w)).astype(np.float32)
+    costs[rng.random((h, w)) < 0.2] = 2.0
+    conf = np.where(rng.random((h, w)) < 0.5, rng.uniform(0, 1, (h, w)), 0).astype(np.float32)
+    costs[5:10, 10:15] = 0.05  # one block of ties: its first pixel wins
+    conf[5:10, 10:15] = 0.0
+    xy = np.zeros((2 * 7 * 5, 2), np.int32)
+    cnt = ctypes.c_int(0)
+    O.lib().orc_support_points(costs.ctypes.data, conf.ctypes.data, w, h, xy.ctypes.data, len(xy), ctypes.byref(cnt))
+    expect = []
+    for c0 in range(0, w, 5):
+        for r0 in range(0, h, 5):
+            cand = [(c, r) for c in range(c0, min(w, c0 + 5)) for r in range(r0, min(h, r0 + 5))
+                    if costs[r, c] < 2.0]
+            if not cand:
+                continue
+            shifted = [np.float32(np.float64(costs[r, c]) + 0.2 - np.float64(conf[r, c])) for c, r in cand]
+            plain = [np.float32(costs[r, c] - conf[r, c]) for c, r in cand]
+            k1, k2 = int(np.argmin(shifted)), int(np.argmin(plain))
+            if shifted[k1] < 0.1:
+                expect.append(cand[k1])
+            if plain[k2] < 0.1:
+                expect.append(cand[k2])
+    assert [tuple(p) for p in xy[:cnt.value]] == expect
+    assert (10, 5) in expect  # the tie block: its first pixel
+
+
+def test_prior_level_raster_planes_and_depth_test():
+    """main.cpp:385-431 on a fronto-parallel depth map at Z: every pixel the
+    barycentric walk reaches gets the triangle's index and the plane
+    (0, 0, -1, Z); a triangle with a corner outside the image is skipped and
+    takes no index; the depth test against [0.6 dmin, 1.2 dmax] drops a
+    triangle whose plane lies outside it."""
+    W, H, Z = 40, 30, 3.0
+    cam = pinhole(W, H, 50.0)
+    depth = np.full((H, W), Z, np.float32)
+    tris = np.array([[2, 2, 30, 4, 10, 25], [0, 0, 45, 0, 0, 10], [30, 4, 38, 28, 10, 25]], np.int32)
+    prior = np.zeros((H, W, 4), np.float32)
+    masks = np.zeros((H, W), np.uint32)
+    O.lib().orc_prior_level(ctypes.byref(cam), depth.ctypes.data, tris.ctypes.data, len(tris), prior.ctypes.data,
+                            masks.ctypes.data)
+    assert set(np.unique(masks)) == {0, 1, 2}  # the second triangle leaves the image: index 2 is the third
+    assert masks[10, 12] == 1 and masks[20, 33] == 2 and masks[28, 2] == 0
+    for (y, x) in zip(*np.nonzero(masks)):
+        np.testing.assert_allclose(prior[y, x], [0, 0, -1, Z], atol=1e-6)
+    assert np.all(prior[masks == 0] == 0)
+    # the corners of a triangle are rasterised (p = 1, q = 1 - p, 1 - p - q = 1)
+    assert masks[2, 2] == 1 and masks[25, 10] >= 1
+    far = pinhole(W, H, 50.0, dmin=6.0, dmax=9.0)  # range [3.6, 10.8]: Z = 3 fails the test
+    O.lib().orc_prior_level(ctypes.byref(far), depth.ctypes.data, tris.ctypes.data, len(tris), prior.ctypes.data,
+                            masks.ctypes.data)
+    assert not masks.any() and not prior.any()
