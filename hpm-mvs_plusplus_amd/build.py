@@ -19,6 +19,8 @@ HEADERS = ["hpm_math.h", "hpm_device.h", "hpm_kernels.h", "hpm_jbu.h", "hpm_fast
            "hpm_internal.h", "hpm_multi.h", "hpm_batch_impl.h"]
 HOST_LIB = os.path.join(HERE, "lib", "libhpm_host.so")
 HOST_SOURCES = [os.path.join(HERE, "host", f) for f in ("hpm_host.cpp", "hpm_jpeg.cpp", "hpm_host_capi.cpp")]
+FACADE_CHECK_SRC = os.path.join(os.path.dirname(HERE), "tests", "facade_process_problem.cpp")
+FACADE_CHECK = os.path.join(os.path.dirname(HERE), "tests", "_build", "facade_process_problem")
 GXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-ffp-contract=off", "-fno-fast-math", "-Wall",
              "-Wextra"]
 
@@ -65,6 +67,16 @@ def build(verbose=False, extra=()):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True, cwd=HERE)
     os.replace(tmp, HOST_LIB)
+    # the HPM facade driven by main.cpp's ProcessProblem call sequence (a test program)
+    os.makedirs(os.path.dirname(FACADE_CHECK), exist_ok=True)
+    tmp = FACADE_CHECK + ".tmp"
+    cmd = (["g++"] + [f for f in GXX_FLAGS if f not in ("-shared", "-fPIC")] + ["-I" + os.path.join(HERE, "host"),
+           FACADE_CHECK_SRC, "-o", tmp, "-L" + os.path.dirname(LIB), "-lhpm_host", "-lhpm_mi355x",
+           "-Wl,-rpath,$ORIGIN/" + os.path.relpath(os.path.dirname(LIB), os.path.dirname(FACADE_CHECK))])
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=HERE)
+    os.replace(tmp, FACADE_CHECK)
     return LIB
 
 
