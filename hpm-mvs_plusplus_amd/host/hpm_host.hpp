@@ -444,7 +444,7 @@ void SetPriorOnDevice(int enable);
 // 16-column tiles (mixed-precision engine only); default 4,000,000.
 void SetFusedMinPixels(int64_t n);
 // Whether the pipeline runs a pass alongside the confidence evaluation
-// before it, each view starting when its confidence map is final (1,
+// before it, each view starting when its confidence map is written (1,
 // default), or after it (0).  The files are identical.
 void SetConfidenceOverlap(int enable);
 

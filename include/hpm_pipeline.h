@@ -83,7 +83,7 @@ void hpm_host_set_prior_on_device(int enable);
 void hpm_host_set_fused_min_pixels(int64_t n);
 
 /* Whether the pipelines run each pass that follows a confidence evaluation
- * alongside it, each view starting once its own confidence map is final
+ * alongside it, each view starting once its own confidence map is written
  * (1, default), or after it (0).  The output files are identical. */
 void hpm_host_set_confidence_overlap(int enable);
 
