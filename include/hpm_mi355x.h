@@ -4,7 +4,8 @@
  * helpers (HPM.h:158-202, RunJBU HPM.h:20).  The reference is a C++ class used
  * in-process by ProcessProblem (main.cpp:237-634); here the same operations are
  * plain C functions on an opaque context so that the C++ facade
- * (hpm-mvs_plusplus_amd/host/HPM.h), Python (ctypes) or any FFI can call them.
+ * (hpm-mvs_plusplus_amd/host/hpm_host.hpp, class hpmhost::HPM), Python
+ * (ctypes) or any FFI can call them.
  *
  * Conventions
  *   - Every function returns HPM_OK (0) or a negative HPM_E* status; it never
