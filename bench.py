@@ -61,8 +61,8 @@ against ground truth;
     python bench.py --workload c3view
 
 times RunPatchMatch on view 7 of the 3072x2048 scene with 10 sources (the
-same k_propagate as the headline at the size where the 11 FP16 quad images,
-528 MiB, exceed the 256 MB Infinity Cache): k_propagate ms per launch, NCC
+same k_propagate as the headline at the size where the 11 FP16 column-pair
+images take 264 MiB): k_propagate ms per launch, NCC
 evaluations per launch, ns per evaluation and algorithmic GB/s, for the
 counter runs of that size;
 
@@ -257,7 +257,7 @@ def uneven_batch_bench(precision="fp16", n_views=64, reps=3):
         return [hashlib.sha1(c.planes().tobytes()).hexdigest() for c in ctxs]
 
     def timed(fn):
-        fn()  # warm-up (quad images, graph capture)
+        fn()  # warm-up (pair images, graph capture)
         t = time.perf_counter()
         for _ in range(reps):
             fn()
@@ -457,14 +457,14 @@ def c3view_bench(precision="fp16", steps=3, warmup=1, ref=7):
     evals = r["ncc_prop"] / launches
     ms = r["prop_ms"] / launches
     alg = evals * 181 * bpt + (evals / NSRC) * (16 + 4 * NSRC)
-    quad_bytes = {"fp32": 16, "fp32fast": 16, "fp16": 8, "bf16": 8}[precision]
+    pair_bytes = {"fp32": 8, "fp32fast": 8, "fp16": 4, "bf16": 4}[precision]
     w, h = cams[0].width, cams[0].height
     return {"workload": f"RunPatchMatch photometric, view {ref} of the orbit scene, {w}x{h}, {NSRC} sources",
             "precision": precision, "views_s": round(r["views_s"], 3), "k_propagate_avg_ms": round(ms, 3),
             "ncc_evals_per_launch": int(evals), "ns_per_ncc": round(1e6 * ms / evals, 4),
             "alg_bytes_per_launch": int(alg), "alg_gbs": round(alg / (ms / 1e3) / 1e9, 1),
             "image_footprint_mib": None if precision == "fp32" else
-            round(len(cams) * (w + 1) * (h + 1) * quad_bytes / 2 ** 20, 1),
+            round(len(cams) * (w + 2) * (h + 1) * pair_bytes / 2 ** 20, 1),
             "depth_within_1pct_of_gt": round(float(np.mean(np.abs(planes[..., 3][hit] - gt[hit]) / gt[hit] <= 0.01)), 4),
             "steps": steps, "warmup": warmup}
 
@@ -495,7 +495,7 @@ def cost_microbench(reps=20):
     res = {}
     exact = None
     for name, prec, bpt in (("fp32_exact", 0, 4), ("fp32", 3, 4), ("fp16", 1, 2), ("bf16", 2, 2)):
-        ctx.cost_fixed_device(dp.value, dout.value, prec)  # builds the quad images once
+        ctx.cost_fixed_device(dp.value, dout.value, prec)  # builds the pair images once
         ctx.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):

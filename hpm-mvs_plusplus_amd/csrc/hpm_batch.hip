@@ -115,7 +115,7 @@ int run_once(hpm_batch* b, const uint64_t* seeds) {
     if (rc) return bfail(b, rc, "view " + std::to_string(i) + ": " + c->err);
   }
   BHIP_OK(hipSetDevice(b->device));
-  for (hpm_ctx* c : b->ctxs) BHIP_OK(hipStreamSynchronize(c->stream));  // quad builds, uploads
+  for (hpm_ctx* c : b->ctxs) BHIP_OK(hipStreamSynchronize(c->stream));  // pair-image builds, uploads
   if (!widths_for(precision, b->tile_w)) return bfail(b, HPM_E_INVALID, "colour tile width not available for this precision");
   int rc = ensure_layout(b);
   if (rc) return rc;
@@ -124,9 +124,9 @@ int run_once(hpm_batch* b, const uint64_t* seeds) {
     BHIP_OK(hipMemsetAsync(b->ctxs[i]->d_counters, 0, 8 * sizeof(unsigned long long), b->stream));
   BHIP_OK(hipEventRecord(b->ev[0], b->stream));
   switch (precision) {
-    case HPM_PREC_FP16: rc = run_engine<FastFor<QuadF16>, true>(b, precision, iters); break;
-    case HPM_PREC_BF16: rc = run_engine<FastFor<QuadBF16>, true>(b, precision, iters); break;
-    case HPM_PREC_FP32_FAST: rc = run_engine<FastFor<QuadF32>, true>(b, precision, iters); break;
+    case HPM_PREC_FP16: rc = run_engine<FastFor<PairF16>, true>(b, precision, iters); break;
+    case HPM_PREC_BF16: rc = run_engine<FastFor<PairBF16>, true>(b, precision, iters); break;
+    case HPM_PREC_FP32_FAST: rc = run_engine<FastFor<PairF32>, true>(b, precision, iters); break;
     default: rc = run_engine<ExactEngine, false>(b, precision, iters); break;
   }
   if (rc) return rc;

@@ -130,5 +130,5 @@ template <class E, bool kAllWidths> int run_engine(hpm_batch* b, int precision, 
 }  // namespace hpm_batch_detail
 
 // run_engine of the FP16 and BF16 engines is instantiated in hpm_batch_lp.hip
-extern template int hpm_batch_detail::run_engine<FastFor<hpm::QuadF16>, true>(hpm_batch*, int, int);
-extern template int hpm_batch_detail::run_engine<FastFor<hpm::QuadBF16>, true>(hpm_batch*, int, int);
+extern template int hpm_batch_detail::run_engine<FastFor<hpm::PairF16>, true>(hpm_batch*, int, int);
+extern template int hpm_batch_detail::run_engine<FastFor<hpm::PairBF16>, true>(hpm_batch*, int, int);

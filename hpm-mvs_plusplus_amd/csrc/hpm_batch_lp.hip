@@ -2,5 +2,5 @@
 // instantiations (compiled beside hpm_batch.hip).
 #include "hpm_batch_impl.h"
 
-template int hpm_batch_detail::run_engine<FastFor<hpm::QuadF16>, true>(hpm_batch*, int, int);
-template int hpm_batch_detail::run_engine<FastFor<hpm::QuadBF16>, true>(hpm_batch*, int, int);
+template int hpm_batch_detail::run_engine<FastFor<hpm::PairF16>, true>(hpm_batch*, int, int);
+template int hpm_batch_detail::run_engine<FastFor<hpm::PairBF16>, true>(hpm_batch*, int, int);

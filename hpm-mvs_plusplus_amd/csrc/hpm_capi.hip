@@ -177,54 +177,53 @@ int build_consts(hpm_ctx* ctx, RunConsts& C) {
     for (int a = 0; a < 3; ++a) C.bvec[v - 1][a] = Ks[3 * a] * trel[0] + Ks[3 * a + 1] * trel[1] + Ks[3 * a + 2] * trel[2];
   }
   C.storage = ctx->lp_kind;
-  for (int i = 0; i < ctx->nimg && i < (int)ctx->d_img_lp.size(); ++i) C.quad[i] = ctx->d_img_lp[i];
+  for (int i = 0; i < ctx->nimg && i < (int)ctx->d_img_lp.size(); ++i) C.pairs[i] = ctx->d_img_lp[i];
   return HPM_OK;
 }
 
-// Quad images of every view for the mixed-precision engine (rebuilt when the
-// storage precision changes).
-template <class S> int build_quads(hpm_ctx* ctx) {
+// Column-pair images of every view for the mixed-precision engine (rebuilt
+// when the storage precision changes).
+template <class S> int build_pair_images(hpm_ctx* ctx) {
   for (int i = 0; i < ctx->nimg; ++i)  // tap addresses are formed exactly in float
-    if ((size_t)(ctx->cams[i].width + 1) * (ctx->cams[i].height + 1) >= ((size_t)1 << 24))
-      return fail(ctx, HPM_E_INVALID, "the mixed-precision path takes views below 2^24 texels (e.g. 4096 x 4095); "
+    if ((size_t)(ctx->cams[i].width + 2) * (ctx->cams[i].height + 1) >= ((size_t)1 << 24))
+      return fail(ctx, HPM_E_INVALID, "the mixed-precision path takes views below 2^24 texels (e.g. 4094 x 4095); "
                                       "use HPM_PREC_FP32_EXACT for larger images");
   for (void* p : ctx->d_img_lp) if (p) hipFreeAsync(p, ctx->stream);
   ctx->d_img_lp.assign(ctx->nimg, nullptr);
-  ctx->lp_kind = -1;  // until every view's quads are built
+  ctx->lp_kind = -1;  // until every view's images are built
   for (int i = 0; i < ctx->nimg; ++i) {
     const int w = ctx->cams[i].width, h = ctx->cams[i].height;
-    const size_t n = (size_t)(w + 1) * (h + 1);
-    hipError_t e = ctx_malloc_async(ctx, &ctx->d_img_lp[i], n * sizeof(typename S::Q));
-    if (e != hipSuccess) { ctx->d_img_lp[i] = nullptr; return fail(ctx, HPM_E_NOMEM, "quad image allocation failed"); }
-    const dim3 b(16, 16), g((w + 1 + 15) / 16, (h + 1 + 15) / 16);
-    hipLaunchKernelGGL(k_build_quad<S>, g, b, 0, ctx->stream, ctx->d_img[i], w, h, ctx->d_img_lp[i]);
-    if (hipGetLastError() != hipSuccess) return fail(ctx, HPM_E_DEVICE, "k_build_quad launch failed");
+    hipError_t e = ctx_malloc_async(ctx, &ctx->d_img_lp[i], pair_image_bytes<S>(w, h));
+    if (e != hipSuccess) { ctx->d_img_lp[i] = nullptr; return fail(ctx, HPM_E_NOMEM, "pair image allocation failed"); }
+    const dim3 b(16, 16), g((w + 2 + 15) / 16, (h + 1 + 15) / 16);
+    hipLaunchKernelGGL(k_build_pairs<S>, g, b, 0, ctx->stream, ctx->d_img[i], w, h, ctx->d_img_lp[i]);
+    if (hipGetLastError() != hipSuccess) return fail(ctx, HPM_E_DEVICE, "k_build_pairs launch failed");
   }
   ctx->lp_kind = S::kind;
   return HPM_OK;
 }
 
-template <class S> int build_pool_quads(hpm_ctx* ctx) {
+template <class S> int build_pool_pair_images(hpm_ctx* ctx) {
   hpm_pool* pool = ctx->pool;
-  std::lock_guard<std::mutex> lock(pool->quads_mu);
-  std::vector<void*>& q = pool->quads[S::kind];
+  std::lock_guard<std::mutex> lock(pool->pairs_mu);
+  std::vector<void*>& q = pool->pairs[S::kind];
   if (q.size() != pool->d_img.size()) {
     for (size_t i = 0; i < pool->cams.size(); ++i)
-      if ((size_t)(pool->cams[i].width + 1) * (pool->cams[i].height + 1) >= ((size_t)1 << 24))
+      if ((size_t)(pool->cams[i].width + 2) * (pool->cams[i].height + 1) >= ((size_t)1 << 24))
         return fail(ctx, HPM_E_INVALID, "the mixed-precision path takes views below 2^24 texels");
     q.assign(pool->d_img.size(), nullptr);
     for (size_t i = 0; i < pool->d_img.size(); ++i) {
       const int w = pool->cams[i].width, h = pool->cams[i].height;
-      if (injected_fault(ctx) || hipMalloc(&q[i], (size_t)(w + 1) * (h + 1) * sizeof(typename S::Q)) != hipSuccess) {
+      if (injected_fault(ctx) || hipMalloc(&q[i], pair_image_bytes<S>(w, h)) != hipSuccess) {
         // no partial set stays behind: the next run builds them all again
         HIP_OK(hipStreamSynchronize(ctx->stream));
         for (void* p : q) if (p) hipFree(p);
         q.clear();
-        return fail(ctx, HPM_E_NOMEM, "quad image allocation failed");
+        return fail(ctx, HPM_E_NOMEM, "pair image allocation failed");
       }
-      const dim3 b(16, 16), g((w + 1 + 15) / 16, (h + 1 + 15) / 16);
-      hipLaunchKernelGGL(k_build_quad<S>, g, b, 0, ctx->stream, pool->d_img[i], w, h, q[i]);
-      if (hipGetLastError() != hipSuccess) return fail(ctx, HPM_E_DEVICE, "k_build_quad launch failed");
+      const dim3 b(16, 16), g((w + 2 + 15) / 16, (h + 1 + 15) / 16);
+      hipLaunchKernelGGL(k_build_pairs<S>, g, b, 0, ctx->stream, pool->d_img[i], w, h, q[i]);
+      if (hipGetLastError() != hipSuccess) return fail(ctx, HPM_E_DEVICE, "k_build_pairs launch failed");
     }
     // other contexts of the pool may run on other streams
     HIP_OK(hipStreamSynchronize(ctx->stream));
@@ -235,20 +234,20 @@ template <class S> int build_pool_quads(hpm_ctx* ctx) {
   return HPM_OK;
 }
 
-int ensure_quads(hpm_ctx* ctx, int precision) {
+int ensure_pair_images(hpm_ctx* ctx, int precision) {
   if (ctx->lp_kind == precision && (int)ctx->d_img_lp.size() == ctx->nimg) return HPM_OK;
   if (ctx->pool) {
     switch (precision) {
-      case HPM_PREC_FP16: return build_pool_quads<QuadF16>(ctx);
-      case HPM_PREC_BF16: return build_pool_quads<QuadBF16>(ctx);
-      case HPM_PREC_FP32_FAST: return build_pool_quads<QuadF32>(ctx);
+      case HPM_PREC_FP16: return build_pool_pair_images<PairF16>(ctx);
+      case HPM_PREC_BF16: return build_pool_pair_images<PairBF16>(ctx);
+      case HPM_PREC_FP32_FAST: return build_pool_pair_images<PairF32>(ctx);
       default: return fail(ctx, HPM_E_INVALID, "unknown precision");
     }
   }
   switch (precision) {
-    case HPM_PREC_FP16: return build_quads<QuadF16>(ctx);
-    case HPM_PREC_BF16: return build_quads<QuadBF16>(ctx);
-    case HPM_PREC_FP32_FAST: return build_quads<QuadF32>(ctx);
+    case HPM_PREC_FP16: return build_pair_images<PairF16>(ctx);
+    case HPM_PREC_BF16: return build_pair_images<PairBF16>(ctx);
+    case HPM_PREC_FP32_FAST: return build_pair_images<PairF32>(ctx);
     default: return fail(ctx, HPM_E_INVALID, "unknown precision");
   }
 }
@@ -532,7 +531,7 @@ void hpm_pool_destroy(hpm_pool* pool) {
   hipSetDevice(pool->device);
   hipDeviceSynchronize();
   for (float* p : pool->d_img) hipFree(p);
-  for (auto& q : pool->quads)
+  for (auto& q : pool->pairs)
     for (void* p : q) hipFree(p);
   delete pool;
 }
@@ -1017,7 +1016,7 @@ int prepare_run(hpm_ctx* ctx, uint64_t seed, RunConsts& C) {
   if (P.hierarchy && !ctx->d_scaled) return fail(ctx, HPM_E_STATE, "hierarchy needs hpm_set_coarse");
   HIP_OK(hipSetDevice(ctx->device));
   int rc;
-  if (P.precision != HPM_PREC_FP32_EXACT && (rc = ensure_quads(ctx, P.precision))) return rc;
+  if (P.precision != HPM_PREC_FP32_EXACT && (rc = ensure_pair_images(ctx, P.precision))) return rc;
   build_consts(ctx, C);
   C.seed = seed;
   return HPM_OK;
@@ -1040,9 +1039,9 @@ int hpm_run_patchmatch_partial(hpm_ctx* ctx, uint64_t seed, int n_halfpasses, in
   HIP_OK(hipMemcpyAsync(ctx->d_consts, &C, sizeof(C), hipMemcpyHostToDevice, ctx->stream));
   HIP_OK(hipMemsetAsync(ctx->d_counters, 0, 8 * sizeof(unsigned long long), ctx->stream));
   switch (P.precision) {
-    case HPM_PREC_FP16: return run_schedule<FastFor<QuadF16>>(ctx, n_halfpasses, do_output);
-    case HPM_PREC_BF16: return run_schedule<FastFor<QuadBF16>>(ctx, n_halfpasses, do_output);
-    case HPM_PREC_FP32_FAST: return run_schedule<FastFor<QuadF32>>(ctx, n_halfpasses, do_output);
+    case HPM_PREC_FP16: return run_schedule<FastFor<PairF16>>(ctx, n_halfpasses, do_output);
+    case HPM_PREC_BF16: return run_schedule<FastFor<PairBF16>>(ctx, n_halfpasses, do_output);
+    case HPM_PREC_FP32_FAST: return run_schedule<FastFor<PairF32>>(ctx, n_halfpasses, do_output);
     default: return run_schedule<ExactEngine>(ctx, n_halfpasses, do_output);
   }
 }
@@ -1177,14 +1176,14 @@ int hpm_cost_fixed_device(hpm_ctx* ctx, const void* planes_dev, void* out_dev, i
   if (precision < 0 || precision > 3) return fail(ctx, HPM_E_INVALID, "unknown precision");
   HIP_OK(hipSetDevice(ctx->device));
   int rc;
-  if (precision != HPM_PREC_FP32_EXACT && (rc = ensure_quads(ctx, precision))) return rc;
+  if (precision != HPM_PREC_FP32_EXACT && (rc = ensure_pair_images(ctx, precision))) return rc;
   RunConsts C;
   build_consts(ctx, C);
   HIP_OK(hipMemcpyAsync(ctx->d_consts, &C, sizeof(C), hipMemcpyHostToDevice, ctx->stream));
   switch (precision) {
-    case HPM_PREC_FP16: return launch_cost_fixed<FastFor<QuadF16>>(ctx, planes_dev, out_dev);
-    case HPM_PREC_BF16: return launch_cost_fixed<FastFor<QuadBF16>>(ctx, planes_dev, out_dev);
-    case HPM_PREC_FP32_FAST: return launch_cost_fixed<FastFor<QuadF32>>(ctx, planes_dev, out_dev);
+    case HPM_PREC_FP16: return launch_cost_fixed<FastFor<PairF16>>(ctx, planes_dev, out_dev);
+    case HPM_PREC_BF16: return launch_cost_fixed<FastFor<PairBF16>>(ctx, planes_dev, out_dev);
+    case HPM_PREC_FP32_FAST: return launch_cost_fixed<FastFor<PairF32>>(ctx, planes_dev, out_dev);
     default: return launch_cost_fixed<ExactEngine>(ctx, planes_dev, out_dev);
   }
 }
@@ -1263,7 +1262,7 @@ int hpm_geom_cost(hpm_ctx* ctx, const float* planes, float* out) {
 static int grad_setup(hpm_ctx* ctx) {
   if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
   HIP_OK(hipSetDevice(ctx->device));
-  int rc = ensure_quads(ctx, HPM_PREC_FP32_FAST);
+  int rc = ensure_pair_images(ctx, HPM_PREC_FP32_FAST);
   if (rc) return rc;
   RunConsts C;
   build_consts(ctx, C);
@@ -1286,16 +1285,17 @@ static int grad_launch(hpm_ctx* ctx, const void* params, const void* weights, vo
 }
 
 // The views smoothed by a Gaussian of `sigma` px (radius ceil(3 sigma), at
-// most 16), as FP32 images and FP32 quads, and a constant block reading them:
+// most 16), as FP32 images and FP32 column-pair images, and a constant block
+// reading them:
 // the cost of the smoothed images has a basin of convergence as wide as the
 // blur, where the sharp images' is about a texel of their finest texture.
 struct SmoothedViews {
   std::vector<float*> img;
-  std::vector<void*> quad;
+  std::vector<void*> pairs;
   RunConsts* d_consts = nullptr;
   ~SmoothedViews() {
     for (float* p : img) hipFree(p);
-    for (void* p : quad) hipFree(p);
+    for (void* p : pairs) hipFree(p);
     hipFree(d_consts);
   }
 };
@@ -1312,20 +1312,20 @@ static int build_smoothed(hpm_ctx* ctx, float sigma, SmoothedViews& S) {
   int rc = dalloc(ctx, &tmp, max_px);
   if (rc) return rc;
   S.img.assign(ctx->nimg, nullptr);
-  S.quad.assign(ctx->nimg, nullptr);
+  S.pairs.assign(ctx->nimg, nullptr);
   for (int i = 0; i < ctx->nimg && rc == HPM_OK; ++i) {
     const int w = ctx->cams[i].width, h = ctx->cams[i].height;
     if ((rc = dalloc(ctx, &S.img[i], (size_t)w * h))) break;
-    if (injected_fault(ctx) || hipMalloc(&S.quad[i], (size_t)(w + 1) * (h + 1) * sizeof(QuadF32::Q)) != hipSuccess) {
-      S.quad[i] = nullptr;
-      rc = fail(ctx, HPM_E_NOMEM, "smoothed quad allocation failed");
+    if (injected_fault(ctx) || hipMalloc(&S.pairs[i], pair_image_bytes<PairF32>(w, h)) != hipSuccess) {
+      S.pairs[i] = nullptr;
+      rc = fail(ctx, HPM_E_NOMEM, "smoothed pair image allocation failed");
       break;
     }
     const dim3 b(16, 16), g((w + 15) / 16, (h + 15) / 16);
     hipLaunchKernelGGL(k_blur<true>, g, b, 0, ctx->stream, ctx->d_img[i], tmp, w, h, t);
     hipLaunchKernelGGL(k_blur<false>, g, b, 0, ctx->stream, tmp, S.img[i], w, h, t);
-    const dim3 gq((w + 1 + 15) / 16, (h + 1 + 15) / 16);
-    hipLaunchKernelGGL(k_build_quad<QuadF32>, gq, b, 0, ctx->stream, S.img[i], w, h, S.quad[i]);
+    const dim3 gq((w + 2 + 15) / 16, (h + 1 + 15) / 16);
+    hipLaunchKernelGGL(k_build_pairs<PairF32>, gq, b, 0, ctx->stream, S.img[i], w, h, S.pairs[i]);
     if (hipGetLastError() != hipSuccess) rc = fail(ctx, HPM_E_DEVICE, "smoothing launch failed");
   }
   if (rc == HPM_OK) {
@@ -1333,7 +1333,7 @@ static int build_smoothed(hpm_ctx* ctx, float sigma, SmoothedViews& S) {
     build_consts(ctx, C);
     for (int i = 0; i < ctx->nimg; ++i) {
       C.img[i] = S.img[i];
-      C.quad[i] = S.quad[i];
+      C.pairs[i] = S.pairs[i];
     }
     C.storage = HPM_PREC_FP32_FAST;
     if ((rc = dalloc(ctx, &S.d_consts, 1)) == HPM_OK &&

@@ -61,8 +61,8 @@ struct RunConsts {
   PairBasis pair[kMaxSrc];
   const float* img[kMaxImages];
   const float* dimg[kMaxImages];
-  // mixed-precision engine: quad texel images and the FP64 homography basis
-  const void* quad[kMaxImages];  // (w+1) x (h+1) quads, origin at texel (-1, -1)
+  // mixed-precision engine: column-pair texel images and the FP64 homography basis
+  const void* pairs[kMaxImages];  // (w+2) x (h+1) elements, origin at texel (-1, -1)
   double A[kMaxSrc][9];          // K_s R_rel K_r^-1
   double bvec[kMaxSrc][3];       // K_s t_rel
   double kinv[4];                // 1/fx, 1/fy, cx/fx, cy/fy of the reference camera

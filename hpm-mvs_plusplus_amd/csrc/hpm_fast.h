@@ -10,16 +10,20 @@
 //  * Tap positions are offsets from the centre, u = u_c + (i a_u + j b_u) /
 //    (1 + i g6 + j g7), in FP32: small magnitudes keep ~1e-6 px accuracy where
 //    absolute FP32 pixel coordinates carry ~6e-5 px of rounding at 1600 px.
-//  * Texels.  A "quad" image holds, for every texel (x, y) of the (w+1) x (h+1)
-//    grid starting at (-1, -1), its bilinear cell with the reference's border
-//    clamp applied; a bilinear tap is one 8-byte load (FP16 / BF16) or one
-//    16-byte load (FP32).  FP16 and FP32 store the cell in difference form
-//    (t00, t10 - t00, t01 - t00, t11 - t10 - t01 + t00), so that the sample
-//    t00 + ax dx + ay (dy + ax dxy) is three fused multiply-adds; for 8-bit
-//    images every component is an integer of magnitude <= 510, exact in FP16.
-//    BF16 (8-bit mantissa) stores the four texels, exact for 8-bit images.
+//  * Texels.  A column-pair image holds, for every element (x, y) of the
+//    (w+2) x (h+1) grid starting at texel (-1, -1), the texel column segment
+//    (t(x, y), t(x, y+1) - t(x, y)) in grid coordinates, with the reference's
+//    border clamp applied; the bilinear cell at (x, y) is elements x and
+//    x + 1, one 8-byte
+//    load (FP16 / BF16) or one 16-byte load (FP32) at element alignment.  The
+//    sample is t0 + ay dv0 + ax ((t1 + ay dv1) - (t0 + ay dv0)): two fused
+//    multiply-adds reading the FP16 halves directly (v_fma_mix), a subtraction
+//    and one more fma.  For 8-bit images every component is an integer of
+//    magnitude <= 255, exact in FP16 and BF16.  At 4 B per texel for FP16
+//    (half the 8 B of a 2x2 cell per texel), the 32 lanes of a wave row that
+//    gather at source columns two apart touch 256 B of a row instead of 512 B.
 //  * Tap addresses: the clamped cell offset (dy, dx) is combined as the float
-//    dy * (w + 1) + dx, exact below 2^24 texels per view (checked on the host).
+//    dy * (w + 2) + dx, exact below 2^24 elements per view (checked on the host).
 //  * Sums use shifted data: the reference window is centred on its centre
 //    texel, the source window on its first sample, so E[x^2] - E[x]^2 never
 //    cancels large magnitudes.  The reference-side statistics are accumulated
@@ -33,91 +37,82 @@
 
 namespace hpm {
 
-__device__ __forceinline__ float half_bits_to_float(uint32_t b) {
-  _Float16 h;
-  const uint16_t s = (uint16_t)b;
-  __builtin_memcpy(&h, &s, 2);
-  return (float)h;
-}
-
-// Bilinear interpolation of one cell (t00, t10 / t01, t11) at fractions
-// (ax, ay): top = t00 + ax (t10 - t00), bottom likewise, then along y.
-__device__ __forceinline__ float lerp_cell(float t00, float t10, float t01, float t11, float ax, float ay) {
-  const float top = ffma(ax, t10 - t00, t00);
-  const float bot = ffma(ax, t11 - t01, t01);
-  return ffma(ay, bot - top, top);
-}
-
 // columns of taps whose gathers are issued ahead of the column being summed
 constexpr int kPipeCols = 1;
 
-struct QuadF16 {
-  using Q = uint2;
-  static constexpr int kind = 1;  // HPM_PREC_FP16
-  static constexpr int kShift = 3;  // log2 sizeof(Q)
-  __device__ __forceinline__ static uint32_t h(float f) {
-    const _Float16 x = (_Float16)f;  // round to nearest even
-    uint16_t b;
-    __builtin_memcpy(&b, &x, 2);
-    return b;
-  }
-  // difference form: (t00, t10 - t00) in x, (t01 - t00, t11 - t10 - t01 + t00) in y
-  __device__ __forceinline__ static Q pack(float a, float b, float c, float d) {
-    return make_uint2(h(a) | (h(b - a) << 16), h(c - a) | (h(d - b - c + a) << 16));
+// Column-pair images.  Element (x, y) of the (w + 2) x (h + 1) grid holds
+// texel column x - 1 at rows y - 1 and y, border-clamped, as (t, t_below - t).
+// The bilinear cell at grid position (x, y), whose top-left texel is
+// (x - 1, y - 1), is the two adjacent elements x and x + 1: one 8-byte (FP16,
+// BF16) or 16-byte (FP32) load at element alignment.  For 8-bit images both
+// components are integers of magnitude <= 255, exact in FP16 and BF16.  The
+// sample is  l = t0 + ay dv0,  r = t1 + ay dv1,  l + ax (r - l).
+__device__ __forceinline__ uint32_t f16_bits(float f) {
+  const _Float16 x = (_Float16)f;  // round to nearest even
+  uint16_t b;
+  __builtin_memcpy(&b, &x, 2);
+  return b;
+}
+__device__ __forceinline__ uint32_t bf16_bits(float f) {  // round to nearest even (finite inputs)
+  const uint32_t u = __float_as_uint(f);
+  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+
+struct PairF16 {
+  using E = uint32_t;  // one element
+  using Q = uint2;     // a cell: elements x and x + 1
+  static constexpr int kind = 1;    // HPM_PREC_FP16
+  static constexpr int kShift = 2;  // log2 sizeof(E)
+  __device__ __forceinline__ static E pack(float top, float bot) {
+    const _Float16 t = (_Float16)top;
+    return f16_bits(top) | (f16_bits(bot - (float)t) << 16);
   }
   __device__ __forceinline__ static Q load(__amdgpu_buffer_rsrc_t r, unsigned off) {
     const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
     return make_uint2(v[0], v[1]);
   }
-  // t00 + ax dx + ay (dy + ax dxy): three v_fma_mix_f32 reading the halves directly
+  // two v_fma_mix_f32 reading the halves directly, then one lerp along x
   __device__ __forceinline__ static float bilinear(Q q, float ax, float ay) {
-    const _Float16* h0 = reinterpret_cast<const _Float16*>(&q.x);
-    const _Float16* h1 = reinterpret_cast<const _Float16*>(&q.y);
-    const float b = ffma(ax, (float)h0[1], (float)h0[0]);
-    const float a = ffma(ax, (float)h1[1], (float)h1[0]);
-    return ffma(ay, a, b);
+    const _Float16* c0 = reinterpret_cast<const _Float16*>(&q.x);
+    const _Float16* c1 = reinterpret_cast<const _Float16*>(&q.y);
+    const float l = ffma(ay, (float)c0[1], (float)c0[0]);
+    const float r = ffma(ay, (float)c1[1], (float)c1[0]);
+    return ffma(ax, r - l, l);
   }
 };
-struct QuadBF16 {
+struct PairBF16 {
+  using E = uint32_t;
   using Q = uint2;
   static constexpr int kind = 2;  // HPM_PREC_BF16
-  static constexpr int kShift = 3;
-  __device__ __forceinline__ static uint32_t h(float f) {  // round to nearest even (finite inputs)
-    const uint32_t u = __float_as_uint(f);
-    return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
-  }
-  __device__ __forceinline__ static Q pack(float a, float b, float c, float d) {
-    return make_uint2(h(a) | (h(b) << 16), h(c) | (h(d) << 16));
+  static constexpr int kShift = 2;
+  __device__ __forceinline__ static E pack(float top, float bot) {
+    const uint32_t t = bf16_bits(top);
+    return t | (bf16_bits(bot - __uint_as_float(t << 16)) << 16);
   }
   __device__ __forceinline__ static Q load(__amdgpu_buffer_rsrc_t r, unsigned off) {
     const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
     return make_uint2(v[0], v[1]);
   }
-  __device__ __forceinline__ static void unpack(Q q, float& a, float& b, float& c, float& d) {
-    a = __uint_as_float(q.x << 16);
-    b = __uint_as_float(q.x & 0xffff0000u);
-    c = __uint_as_float(q.y << 16);
-    d = __uint_as_float(q.y & 0xffff0000u);
-  }
   __device__ __forceinline__ static float bilinear(Q q, float ax, float ay) {
-    float a, b, c, d;
-    unpack(q, a, b, c, d);
-    return lerp_cell(a, b, c, d, ax, ay);
+    const float l = ffma(ay, __uint_as_float(q.x & 0xffff0000u), __uint_as_float(q.x << 16));
+    const float r = ffma(ay, __uint_as_float(q.y & 0xffff0000u), __uint_as_float(q.y << 16));
+    return ffma(ax, r - l, l);
   }
 };
-struct QuadF32 {
+struct PairF32 {
+  using E = float2;
   using Q = float4;
   static constexpr int kind = 3;  // HPM_PREC_FP32_FAST
-  static constexpr int kShift = 4;
-  __device__ __forceinline__ static Q pack(float a, float b, float c, float d) {
-    return make_float4(a, b - a, c - a, d - b - c + a);  // difference form, as QuadF16
-  }
+  static constexpr int kShift = 3;
+  __device__ __forceinline__ static E pack(float top, float bot) { return make_float2(top, bot - top); }
   __device__ __forceinline__ static Q load(__amdgpu_buffer_rsrc_t r, unsigned off) {
     const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
     return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
   }
   __device__ __forceinline__ static float bilinear(Q q, float ax, float ay) {
-    return ffma(ay, ffma(ax, q.w, q.z), ffma(ax, q.y, q.x));
+    const float l = ffma(ay, q.y, q.x);
+    const float r = ffma(ay, q.w, q.z);
+    return ffma(ax, r - l, l);
   }
 };
 
@@ -196,17 +191,17 @@ struct FastEngine {
     const float au = (float)dfma(-uc, g6, h0 * rz), bu = (float)dfma(-uc, g7, h1 * rz);
     const float av = (float)dfma(-vc, g6, h3 * rz), bv = (float)dfma(-vc, g7, h4 * rz);
     const double flu = floor(uc), flv = floor(vc);
-    const int iu = (int)flu + 1, iv = (int)flv + 1;  // +1: the quad grid starts at (-1, -1)
+    const int iu = (int)flu + 1, iv = (int)flv + 1;  // +1: the pair grid starts at (-1, -1)
     const float fu = (float)(uc - flu), fv = (float)(vc - flv);
     // clamping the cell offset in float keeps the cell inside [0, w] x [0, h]
     // (the reference's border clamp); a NaN offset converts to 0, also inside.
     const float lox = (float)(-iu), hix = (float)(sw - iu);
     const float loy = (float)(-iv), hiy = (float)(sh - iv);
-    const int qp = sw + 1;
+    const int qp = sw + 2;  // elements per row of the pair image
     const float qpf = (float)qp;
     const unsigned base = (unsigned)(iv * qp + iu);
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<void*>(C.quad[v]), (short)0, qp * (sh + 1) * (int)sizeof(typename S::Q), 0x00020000);
+        const_cast<void*>(C.pairs[v]), (short)0, (qp * (sh + 1)) << S::kShift, 0x00020000);
     // Tap (i, j) in the source cell frame: x = (fu + i cu + j bx) / D,
     // y = (fv + i cv + j by) / D, D = 1 + i g6 + j g7 (the projective map of
     // the window offset around the centre, with the centre's fraction folded in).
@@ -272,17 +267,21 @@ struct FastEngine {
   }
 };
 
-// Builds the quad image of one view from its FP32 texels.  Thread (x, y) of
-// the (w+1) x (h+1) grid writes the cell whose top-left texel is (x-1, y-1).
+// Builds the column-pair image of one view from its FP32 texels.  Thread
+// (x, y) of the (w+2) x (h+1) grid writes the element of texel column x-1,
+// rows y-1 and y.
 template <class S>
-__global__ __launch_bounds__(256) void k_build_quad(const float* __restrict__ img, int w, int h, void* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_build_pairs(const float* __restrict__ img, int w, int h, void* __restrict__ out) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
-  if (x > w || y > h) return;
-  const float a = tex_at(img, w, h, x - 1, y - 1), b = tex_at(img, w, h, x, y - 1);
-  const float c = tex_at(img, w, h, x - 1, y), d = tex_at(img, w, h, x, y);
-  typename S::Q* q = (typename S::Q*)out;
-  q[y * (w + 1) + x] = S::pack(a, b, c, d);
+  if (x > w + 1 || y > h) return;
+  typename S::E* e = (typename S::E*)out;
+  e[y * (w + 2) + x] = S::pack(tex_at(img, w, h, x - 1, y - 1), tex_at(img, w, h, x - 1, y));
+}
+
+// Bytes of the column-pair image of a w x h view.
+template <class S> __host__ __device__ constexpr size_t pair_image_bytes(int w, int h) {
+  return ((size_t)(w + 2) * (size_t)(h + 1)) << S::kShift;
 }
 
 }  // namespace hpm

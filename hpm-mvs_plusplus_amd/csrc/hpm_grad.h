@@ -66,7 +66,7 @@ __device__ __forceinline__ bool tap_frame(const RunConsts& C, int v, int px, int
   f.fv = (float)(vc - f.flv);
   f.lox = (float)(-iu); f.hix = (float)(sw - iu);
   f.loy = (float)(-iv); f.hiy = (float)(sh - iv);
-  const int qp = sw + 1;
+  const int qp = sw + 2;  // elements per row of the pair image
   f.qpf = (float)qp;
   f.base = (unsigned)(iv * qp + iu);
   f.cu = ffma(f.fu, f.fg6, au);
@@ -74,8 +74,8 @@ __device__ __forceinline__ bool tap_frame(const RunConsts& C, int v, int px, int
   f.cv = ffma(f.fv, f.fg6, av);
   f.by = ffma(f.fv, f.fg7, bv);
   f.Zc = Z;
-  f.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(C.quad[v]), (short)0,
-                                             qp * (sh + 1) * (int)sizeof(QuadF32::Q), 0x00020000);
+  f.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(C.pairs[v]), (short)0,
+                                             (qp * (sh + 1)) << PairF32::kShift, 0x00020000);
   return true;
 }
 
@@ -92,11 +92,14 @@ __device__ __forceinline__ void grad_tap(const TapFrame& f, float i, float j, fl
   const float cxl = __builtin_amdgcn_fmed3f(fx, f.lox, f.hix);
   const float cyl = __builtin_amdgcn_fmed3f(fy, f.loy, f.hiy);
   const int rel = (int)ffma(cyl, f.qpf, cxl);
-  const QuadF32::Q q = QuadF32::load(f.rsrc, ((unsigned)rel + f.base) << QuadF32::kShift);
-  // difference form (t00, dx, dy, dxy): s = t00 + ax dx + ay (dy + ax dxy)
-  su = ffma(ay, q.w, q.y);
-  sv = ffma(ax, q.w, q.z);
-  s = ffma(ay, sv, ffma(ax, q.y, q.x));
+  const PairF32::Q q = PairF32::load(f.rsrc, ((unsigned)rel + f.base) << PairF32::kShift);
+  // columns (t0, dv0) and (t1, dv1): l = t0 + ay dv0, r = t1 + ay dv1,
+  // s = l + ax (r - l); ds/dax = r - l, ds/day = dv0 + ax (dv1 - dv0)
+  const float l = ffma(ay, q.y, q.x);
+  const float rt = ffma(ay, q.w, q.z);
+  su = rt - l;
+  sv = ffma(ax, q.w - q.y, q.y);
+  s = ffma(ax, su, l);
 }
 
 // One thread per pixel.  params: (d, t, f, -) per pixel; weights: V per pixel.
@@ -146,7 +149,7 @@ __global__ __launch_bounds__(256) void k_cost_grad(const RunConsts* __restrict__
   const float Ttf[3] = {(float)Tt[0], (float)Tt[1], (float)Tt[2]};
   const float Tff[3] = {(float)Tf[0], (float)Tf[1], (float)Tf[2]};
   FastWin fw;
-  FastEngine<QuadF32>::build(C, px, py, fw);
+  FastEngine<PairF32>::build(C, px, py, fw);
   const float W = 1.0f / fw.inv_w;
   float wsum = 0.0f, csum = 0.0f;
   double g[3] = {0.0, 0.0, 0.0}, Hm[6] = {0, 0, 0, 0, 0, 0};

@@ -16,14 +16,14 @@
 template <class S> using FastFor = hpm::FastEngine<S>;
 
 // Images of a whole scene on one device, shared by the contexts of its
-// reference views: FP32 images plus, per storage precision, the quad images of
+// reference views: FP32 images plus, per storage precision, the column-pair images of
 // the mixed-precision engine (built on first use).
 struct hpm_pool {
   int device = 0;
   std::vector<hpm_camera> cams;
   std::vector<float*> d_img;
-  std::vector<void*> quads[4];  // indexed by HPM_PREC_* (0 unused)
-  std::mutex quads_mu;          // contexts on several host threads may build them first
+  std::vector<void*> pairs[4];  // indexed by HPM_PREC_* (0 unused)
+  std::mutex pairs_mu;          // contexts on several host threads may build them first
 };
 
 struct hpm_ctx {
@@ -76,7 +76,7 @@ struct hpm_ctx {
 };
 
 namespace hpm_detail {
-// Validates the context for a run, builds the quad images of its storage
+// Validates the context for a run, builds the column-pair images of its storage
 // precision when needed and fills the run's constant block (seed included);
 // nothing is launched.  Returns HPM_OK or an error recorded on the context.
 int prepare_run(hpm_ctx* ctx, uint64_t seed, hpm::RunConsts& C);

@@ -105,8 +105,9 @@ int hpm_set_views(hpm_ctx* ctx, int num_images, const hpm_camera* cameras, const
 
 /* Image pool: the images and cameras of a whole scene, uploaded once to
  * `device` and shared by the contexts of its reference views (hpm_set_views
- * would copy the reference and source images into every context).  Quad images
- * of the mixed-precision engine are built once per precision on first use.  No
+ * would copy the reference and source images into every context).  The
+ * column-pair images of the mixed-precision engine are built once per
+ * precision on first use.  No
  * counterpart in the reference, which uploads the images of every problem
  * into every HPM object (HPM.cpp:741-772).  The pool must outlive the
  * contexts that use it; contexts may run concurrently on it. */

@@ -8,10 +8,10 @@ below is run once for every n up to the first that no allocation reaches:
   the flow;
 * the flow's results are bit-identical to those of a run without faults.
 
-Flows: a photometric FP16 run (quad images), a geometric FP32 pass (source
-depths), a pooled FP16 run (the pool's shared quad images), the planar prior
+Flows: a photometric FP16 run (column-pair images), a geometric FP32 pass (source
+depths), a pooled FP16 run (the pool's shared pair images), the planar prior
 at distance 1 (downsampled maps, scratch, prior buffers) and Gauss-Newton
-refinement on smoothed views (smoothed images and quads, gradient buffers)."""
+refinement on smoothed views (smoothed images and pair images, gradient buffers)."""
 import os
 
 import numpy as np
@@ -83,7 +83,7 @@ def test_photometric_fp16_alloc_faults(small_scene):
         return out
 
     hit = _sweep("photometric fp16", flow)
-    assert set(hit) == {"set_views", "run"}  # views and pixel state; the quad images
+    assert set(hit) == {"set_views", "run"}  # views and pixel state; the pair images
 
 
 def test_geometric_fp32_alloc_faults(small_scene):
@@ -126,7 +126,7 @@ def test_pooled_fp16_alloc_faults(small_scene):
         ctx.close()
         return out
 
-    # the pool's quad images are built by the first context that runs at
+    # the pool's pair images are built by the first context that runs at
     # this precision: a fresh pool for each case, so that every case builds
     # them (and can meet the failure there)
     def fresh(ctx, call):

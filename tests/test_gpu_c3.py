@@ -76,7 +76,7 @@ def test_c3_patchmatch_vs_gt_and_fp64_band(c3_scene, ref):
     ctx = H.Context(0)
     ctx.set_params(H.default_params(precision="fp16"))
     ctx.set_views([H.camera_from_scene(c) for c in cams], imgs)
-    ctx.run(SEED)  # first run: quad images, warm-up
+    ctx.run(SEED)  # first run: pair images, warm-up
     t0 = time.perf_counter()
     ctx.run(SEED)
     dt = time.perf_counter() - t0

@@ -1,4 +1,4 @@
-"""Mixed-precision engine (FP16 / BF16 / FP32 quad storage) against the FP64
+"""Mixed-precision engine (FP16 / BF16 / FP32 column-pair storage) against the FP64
 oracle (needs an MI355X).
 
 Tolerances, written per test:
