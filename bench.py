@@ -279,9 +279,11 @@ def uneven_batch_bench(precision="fp16", n_views=64, reps=3):
     same_fused = digest() == ref
     print(f"[bench] fused: {res['fused']:.2f} views/s", file=sys.stderr, flush=True)
     w, st, ms = b.autotune(seeds)
+    nt = b.workgroup()
     res["fused_autotuned"] = timed(lambda: b.run(seeds))
     same_tuned = digest() == ref
-    print(f"[bench] fused, autotuned ({w}, {st}): {res['fused_autotuned']:.2f} views/s", file=sys.stderr, flush=True)
+    print(f"[bench] fused, autotuned ({w}, {st}, {nt} threads): {res['fused_autotuned']:.2f} views/s",
+          file=sys.stderr, flush=True)
     within = []
     for i, c in enumerate(ctxs):
         d = c.planes()[..., 3]
@@ -301,7 +303,8 @@ def uneven_batch_bench(precision="fp16", n_views=64, reps=3):
             "sources": [len(p) for p in sc.pairs],
             "views_s": {k: round(v, 3) for k, v in res.items()},
             "speedup_fused_vs_one_at_a_time": round(best / res["one_at_a_time"], 3),
-            "autotuned_shape": {"colour_tile_w": w, "xcd_strip": st, "schedule_ms": round(ms, 3)},
+            "autotuned_shape": {"colour_tile_w": w, "xcd_strip": st, "threads_per_workgroup": nt,
+                                "schedule_ms": round(ms, 3)},
             "bit_exact_vs_one_at_a_time": {"streams": same_streams, "fused": same_fused, "fused_autotuned": same_tuned},
             "depth_within_1pct_of_gt_mean": round(float(np.mean(within)), 4),
             "data": "synthetic (seeded ray-cast: 40 textured planes + background, 64 cameras on two rings)",
@@ -603,8 +606,10 @@ def main():
     tuned = None
     if args.batch_mode == "fused":  # pick the launch shape (untimed; the runs are ordinary warm-up work)
         tw, ts, tms = fused.autotune([SEED + k for k in range(len(ctxs))])
-        tuned = {"colour_tile_w": tw, "xcd_strip": ts, "schedule_ms": round(tms, 3)}
-        print(f"[bench] autotuned launch shape: tile {tw}, strip {ts}", file=sys.stderr, flush=True)
+        tuned = {"colour_tile_w": tw, "xcd_strip": ts, "threads_per_workgroup": fused.workgroup(),
+                 "schedule_ms": round(tms, 3)}
+        print(f"[bench] autotuned launch shape: tile {tw}, strip {ts}, {tuned['threads_per_workgroup']} threads",
+              file=sys.stderr, flush=True)
     runners = {"fused": lambda seeds: fused.run(seeds), "streams": lambda seeds: Hm.run_batch(ctxs, seeds)}
     batch_rates = {}
     modes = [args.batch_mode] + ([m for m in runners if m != args.batch_mode] if args.compare_batch_modes else [])

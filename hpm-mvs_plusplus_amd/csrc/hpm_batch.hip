@@ -2,8 +2,8 @@
 // contexts (reference views of any size and source count) run as one launch
 // per stage -- init, every colour pass, output -- over the concatenated tile
 // grids of all views, optionally replayed as one hipGraph.  The launch shape
-// (colour tile width, XCD strip height) is a run-time choice that
-// hpm_batch_autotune picks by timing.
+// (colour tile width, XCD strip height, pixels per colour workgroup) is a
+// run-time choice that hpm_batch_autotune picks by timing.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -36,18 +36,19 @@ void free_layout(hpm_batch* b) {
   for (auto& p : b->d_wg) p = nullptr;
 }
 
-int colour_tiles(int tile_w, int W, int H2) {
-  const int ch = 256 / tile_w;
+int colour_tiles(int tile_w, int threads, int W, int H2) {
+  const int ch = threads / tile_w;
   return (((W + 1) / 2 + tile_w - 1) / tile_w) * ((H2 + ch - 1) / ch);
 }
 
 int round8(int n) { return (n + 7) & ~7; }
 
 // (Re)builds the slots and workgroup tables when a view's buffers, size or
-// source count, or the colour tile width, changed since the last run.
+// source count, or the colour tile shape, changed since the last run.
 int ensure_layout(hpm_batch* b) {
   std::vector<unsigned char> key;
   append(key, b->tile_w);
+  append(key, b->threads);
   for (hpm_ctx* c : b->ctxs) {
     append(key, hpm_detail::state_of(c));
     append(key, c->W);
@@ -71,7 +72,7 @@ int ensure_layout(hpm_batch* b) {
     states[i] = hpm_detail::state_of(c);
     m.H2 = (c->H / 2) * 2;
     m.count[kStageInit] = tiles_2d<ExactEngine>(c->W, c->H);  // 16 x 16 tiles for every one-lane engine
-    m.count[kStageColour] = colour_tiles(b->tile_w, c->W, m.H2);
+    m.count[kStageColour] = colour_tiles(b->tile_w, b->threads, c->W, m.H2);
     m.count[kStageOutput] = (int)(((size_t)c->W * c->H + 255) / 256);
     for (int s = 0; s < 3; ++s) {
       m.first[s] = (int)wg[s].size();
@@ -97,8 +98,10 @@ int ensure_layout(hpm_batch* b) {
   return HPM_OK;
 }
 
-bool widths_for(int precision, int tile_w) {
-  return tile_w == 32 || (precision != HPM_PREC_FP32_EXACT && (tile_w == 16 || tile_w == 64));
+bool shape_for(int precision, int tile_w, int threads) {
+  if (tile_w == 32 && threads == 256) return true;
+  return precision != HPM_PREC_FP32_EXACT && (tile_w == 16 || tile_w == 32 || tile_w == 64) &&
+         (threads == 128 || threads == 256);
 }
 
 int run_once(hpm_batch* b, const uint64_t* seeds) {
@@ -116,7 +119,8 @@ int run_once(hpm_batch* b, const uint64_t* seeds) {
   }
   BHIP_OK(hipSetDevice(b->device));
   for (hpm_ctx* c : b->ctxs) BHIP_OK(hipStreamSynchronize(c->stream));  // pair-image builds, uploads
-  if (!widths_for(precision, b->tile_w)) return bfail(b, HPM_E_INVALID, "colour tile width not available for this precision");
+  if (!shape_for(precision, b->tile_w, b->threads))
+    return bfail(b, HPM_E_INVALID, "launch shape not available for this precision");
   int rc = ensure_layout(b);
   if (rc) return rc;
   BHIP_OK(hipMemcpyAsync(b->d_consts, b->consts.data(), sizeof(RunConsts) * n, hipMemcpyHostToDevice, b->stream));
@@ -201,6 +205,20 @@ int hpm_batch_get_shape(const hpm_batch* b, int* tile_w, int* strip) {
   return HPM_OK;
 }
 
+int hpm_batch_set_workgroup(hpm_batch* b, int threads) {
+  if (!b) return HPM_E_INVALID;
+  if (threads == 0) threads = 256;
+  if (threads != 128 && threads != 256) return bfail(b, HPM_E_INVALID, "threads per workgroup must be 128 or 256");
+  b->threads = threads;
+  return HPM_OK;
+}
+
+int hpm_batch_get_workgroup(const hpm_batch* b, int* threads) {
+  if (!b || !threads) return HPM_E_INVALID;
+  *threads = b->threads;
+  return HPM_OK;
+}
+
 int hpm_batch_run(hpm_batch* b, const uint64_t* seeds) {
   if (!b || !seeds) return HPM_E_INVALID;
   return run_once(b, seeds);
@@ -216,21 +234,24 @@ int hpm_batch_autotune(hpm_batch* b, const uint64_t* seeds, float* best_ms) {
       return bfail(b, HPM_E_STATE, "autotuning reruns the schedule: photometric passes only (init overwrites the state)");
   }
   const int precision = b->ctxs[0]->params.precision;
-  const int widths[3] = {16, 32, 64}, strips[3] = {1, 2, 4};
-  int best_w = b->tile_w, best_s = b->strip;
+  const int widths[3] = {16, 32, 64}, strips[3] = {1, 2, 4}, threads[2] = {256, 128};
+  int best_w = b->tile_w, best_s = b->strip, best_t = b->threads;
   float best = -1.0f;
-  for (int w : widths) {
-    if (!widths_for(precision, w)) continue;
-    for (int s : strips) {
-      b->tile_w = w;
-      b->strip = s;
-      const int rc = run_once(b, seeds);
-      if (rc) return rc;
-      if (best < 0.0f || b->last_ms < best) { best = b->last_ms; best_w = w; best_s = s; }
+  for (int t : threads)
+    for (int w : widths) {
+      if (!shape_for(precision, w, t)) continue;
+      for (int s : strips) {
+        b->tile_w = w;
+        b->strip = s;
+        b->threads = t;
+        const int rc = run_once(b, seeds);
+        if (rc) return rc;
+        if (best < 0.0f || b->last_ms < best) { best = b->last_ms; best_w = w; best_s = s; best_t = t; }
+      }
     }
-  }
   b->tile_w = best_w;
   b->strip = best_s;
+  b->threads = best_t;
   // leave the views with the result of the chosen shape
   const int rc = run_once(b, seeds);
   if (rc) return rc;

@@ -33,6 +33,7 @@ struct hpm_batch {
   // launch shape
   int tile_w = HPM_COLOUR_TILE_W;
   int strip = HPM_XCD_STRIP;
+  int threads = 256;  // threads (pixels) per colour-pass workgroup: 256 or 128
   // graph
   bool use_graph = false;
   hipGraphExec_t gexec = nullptr;
@@ -58,29 +59,29 @@ template <class T> void append(std::vector<unsigned char>& k, const T& v) {
 
 using namespace hpm;
 
-inline size_t lds_bytes(const hpm_batch* b) { return (size_t)8 * b->max_v * kRowStride * sizeof(float); }
+inline size_t lds_bytes(const hpm_batch* b) { return (size_t)8 * b->max_v * b->threads * sizeof(float); }
 
-template <class E, int CW> int set_lds(hpm_batch* b) {
+template <class E, int CW, int NT> int set_lds(hpm_batch* b) {
   if (b->max_v <= kLdsMaxSrc)
-    BHIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_propagate_multi<E, true, CW>),
+    BHIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_propagate_multi<E, true, CW, NT>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes(b)));
   return HPM_OK;
 }
 
-template <class E, int CW> int launch_stages(hpm_batch* b, int iters) {
+template <class E, int CW, int NT> int launch_stages(hpm_batch* b, int iters) {
   hipStream_t st = b->stream;
-  const dim3 blk(256);
+  const dim3 blk(256), blk_colour(NT);
   hipLaunchKernelGGL(k_init_multi<E>, dim3(b->grid[kStageInit]), blk, 0, st, b->d_consts, b->d_states, b->d_views,
                      b->d_wg[kStageInit], b->strip);
   BHIP_OK(hipGetLastError());
   for (int it = 0; it < iters; ++it)
     for (int colour = 0; colour < 2; ++colour) {
       if (b->max_v <= kLdsMaxSrc)
-        hipLaunchKernelGGL((k_propagate_multi<E, true, CW>), dim3(b->grid[kStageColour]), blk, lds_bytes(b), st,
-                           b->d_consts, b->d_states, b->d_views, b->d_wg[kStageColour], colour, it, b->strip);
+        hipLaunchKernelGGL((k_propagate_multi<E, true, CW, NT>), dim3(b->grid[kStageColour]), blk_colour, lds_bytes(b),
+                           st, b->d_consts, b->d_states, b->d_views, b->d_wg[kStageColour], colour, it, b->strip);
       else
-        hipLaunchKernelGGL((k_propagate_multi<E, false, CW>), dim3(b->grid[kStageColour]), blk, 0, st, b->d_consts,
-                           b->d_states, b->d_views, b->d_wg[kStageColour], colour, it, b->strip);
+        hipLaunchKernelGGL((k_propagate_multi<E, false, CW, NT>), dim3(b->grid[kStageColour]), blk_colour, 0, st,
+                           b->d_consts, b->d_states, b->d_views, b->d_wg[kStageColour], colour, it, b->strip);
       BHIP_OK(hipGetLastError());
     }
   hipLaunchKernelGGL(k_output_multi, dim3(b->grid[kStageOutput]), blk, 0, st, b->d_consts, b->d_states, b->d_views,
@@ -89,21 +90,22 @@ template <class E, int CW> int launch_stages(hpm_batch* b, int iters) {
   return HPM_OK;
 }
 
-template <class E, int CW> int run_stages(hpm_batch* b, int precision, int iters) {
-  int rc = set_lds<E, CW>(b);
+template <class E, int CW, int NT> int run_stages(hpm_batch* b, int precision, int iters) {
+  int rc = set_lds<E, CW, NT>(b);
   if (rc) return rc;
-  if (!b->use_graph) return launch_stages<E, CW>(b, iters);
+  if (!b->use_graph) return launch_stages<E, CW, NT>(b, iters);
   std::vector<unsigned char> key = b->layout_key;
   append(key, precision);
   append(key, iters);
   append(key, b->strip);
+  append(key, b->threads);
   append(key, b->grid);
   append(key, b->max_v);
   if (!b->gexec || key != b->gkey) {
     if (b->gexec) { BHIP_OK(hipGraphExecDestroy(b->gexec)); b->gexec = nullptr; }
     hipGraph_t g = nullptr;
     BHIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    rc = launch_stages<E, CW>(b, iters);
+    rc = launch_stages<E, CW, NT>(b, iters);
     const hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc) { if (g) hipGraphDestroy(g); return rc; }
     if (e != hipSuccess) return bfail(b, HPM_E_DEVICE, std::string("stream capture: ") + hipGetErrorString(e));
@@ -116,15 +118,22 @@ template <class E, int CW> int run_stages(hpm_batch* b, int precision, int iters
   return HPM_OK;
 }
 
-// Colour tile widths instantiated per engine: 16, 32 and 64 for the
-// mixed-precision engines; the exact engine keeps the default width.
-template <class E, bool kAllWidths> int run_engine(hpm_batch* b, int precision, int iters) {
-  if (b->tile_w == 32) return run_stages<E, 32>(b, precision, iters);
-  if constexpr (kAllWidths) {
-    if (b->tile_w == 16) return run_stages<E, 16>(b, precision, iters);
-    if (b->tile_w == 64) return run_stages<E, 64>(b, precision, iters);
+// Launch shapes instantiated per engine: colour tile widths 16, 32 and 64 and
+// 256 or 128 threads per workgroup for the mixed-precision engines; the exact
+// engine keeps the default width and 256 threads.
+template <class E, bool kAllShapes> int run_engine(hpm_batch* b, int precision, int iters) {
+  if (b->tile_w == 32 && b->threads == 256) return run_stages<E, 32, 256>(b, precision, iters);
+  if constexpr (kAllShapes) {
+    if (b->threads == 256) {
+      if (b->tile_w == 16) return run_stages<E, 16, 256>(b, precision, iters);
+      if (b->tile_w == 64) return run_stages<E, 64, 256>(b, precision, iters);
+    } else if (b->threads == 128) {
+      if (b->tile_w == 16) return run_stages<E, 16, 128>(b, precision, iters);
+      if (b->tile_w == 32) return run_stages<E, 32, 128>(b, precision, iters);
+      if (b->tile_w == 64) return run_stages<E, 64, 128>(b, precision, iters);
+    }
   }
-  return bfail(b, HPM_E_INVALID, "colour tile width not available for this precision");
+  return bfail(b, HPM_E_INVALID, "launch shape not available for this precision");
 }
 
 }  // namespace hpm_batch_detail

@@ -20,32 +20,34 @@
 #ifndef HPM_PROP_WAVES
 #define HPM_PROP_WAVES 2
 #endif
-#define HPM_PROP_ATTR __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_PROP_WAVES)))
+#define HPM_PROP_ATTR_NT(NT) __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(HPM_PROP_WAVES)))
+#define HPM_PROP_ATTR HPM_PROP_ATTR_NT(256)
 
 namespace hpm {
 
 // Cost rows of the 8 sampled directions, direction-major (row d, view v at
 // d * V + v).  They live in LDS, thread-interleaved (element k of a thread at
-// ca[k * kRowStride]) so that a wave's accesses are bank-conflict free; views
-// beyond kLdsMaxSrc fall back to a private array.
-constexpr int kRowStride = 256;  // threads per block
+// ca[k * threads per block]) so that a wave's accesses are bank-conflict
+// free; views beyond kLdsMaxSrc fall back to a private array.
+constexpr int kRowStride = 256;  // threads per block of the single-view colour passes
 constexpr int kLdsMaxSrc = 20;   // 8 * 20 * 256 * 4 B = 160 KiB
 struct Sample {
   float* ca;
-  int cs;  // element stride: kRowStride in LDS, 1 in a private array
+  int cs;  // element stride: threads per block in LDS, 1 in a private array
   int pos[8];
   bool flag[8];
 };
 
-// Pixel tiles of a 256-thread (1D) block.  Engines with one lane per pixel
-// cover 16 x 16 pixels (2D kernels) or CW same-colour columns x 256 / CW rows
-// (colour passes, CW = HPM_COLOUR_TILE_W unless a launch picks another
-// width); four-lane engines cover 8 x 8 and 8 x 8.
+// Pixel tiles of an NT-thread (1D) block, NT = 256 unless a fused launch
+// picks 128.  Engines with one lane per pixel cover 16 x 16 pixels (2D
+// kernels) or CW same-colour columns x NT / CW rows (colour passes, CW =
+// HPM_COLOUR_TILE_W unless a launch picks another width); four-lane engines
+// cover 8 x 8 and 8 x 8.
 #ifndef HPM_COLOUR_TILE_W
 #define HPM_COLOUR_TILE_W 32
 #endif
-template <class E, int CW = (E::kLanes == 1 ? HPM_COLOUR_TILE_W : 8)> struct Tiles {
-  static constexpr int px_per_block = 256 / E::kLanes;
+template <class E, int CW = (E::kLanes == 1 ? HPM_COLOUR_TILE_W : 8), int NT = 256> struct Tiles {
+  static constexpr int px_per_block = NT / E::kLanes;
   static constexpr int sq = E::kLanes == 1 ? 16 : 8;  // 2D tile side
   static constexpr int cw = CW;                       // colour tile width (same-colour columns)
   static constexpr int ch = px_per_block / cw;        // colour tile height
@@ -88,8 +90,8 @@ template <class E> __device__ __forceinline__ void tile_pixel(int id, int strip,
   px = tx * Tiles<E>::sq + pix % Tiles<E>::sq;
   py = ty * Tiles<E>::sq + pix / Tiles<E>::sq;
 }
-template <class E, int CW> __device__ __forceinline__ void colour_pixel(int id, int strip, int& cx, int& py, int W, int H2) {
-  using T = Tiles<E, CW>;
+template <class E, int CW, int NT> __device__ __forceinline__ void colour_pixel(int id, int strip, int& cx, int& py, int W, int H2) {
+  using T = Tiles<E, CW, NT>;
   const int pix = (int)threadIdx.x / E::kLanes;
   int tx, ty;
   xcd_tile(id, ((W + 1) / 2 + T::cw - 1) / T::cw, (H2 + T::ch - 1) / T::ch, strip, tx, ty);
@@ -739,13 +741,13 @@ __device__ __forceinline__ void propagate_pixel_mand(const RunConsts& C, const D
 
 // One colour pass.  Thread (tx, y) of a 32 x 8 block handles the tx-th pixel of
 // the pass's colour in row y: x = 2 tx + ((y + colour) & 1).
-template <class E, bool kLds, int CW>
+template <class E, bool kLds, int CW, int NT = 256>
 __device__ __forceinline__ void propagate_body(const RunConsts& C, const DevState& S, int colour, int iter, int H2, int id,
                                                int strip) {
   extern __shared__ float lds_rows[];
   float private_rows[kLds ? 1 : 8 * kMaxSrc];
   int tx, py;
-  colour_pixel<E, CW>(id, strip, tx, py, C.W, H2);
+  colour_pixel<E, CW, NT>(id, strip, tx, py, C.W, H2);
   const int px = 2 * tx + ((py + colour) & 1);
   unsigned nncc = 0, n_ext = 0;
   bool active = px < C.W && py < H2;
@@ -758,7 +760,7 @@ __device__ __forceinline__ void propagate_body(const RunConsts& C, const DevStat
     E::build(C, px, py, rw);
     Sample s;
     s.ca = kLds ? lds_rows + threadIdx.x : private_rows;
-    s.cs = kLds ? kRowStride : 1;
+    s.cs = kLds ? NT : 1;
     if (C.mand) propagate_pixel_mand<E>(C, S, px, py, iter, rw, s, nncc, n_ext);
     else propagate_pixel<E>(C, S, px, py, iter, rw, s, nncc, n_ext);
   }

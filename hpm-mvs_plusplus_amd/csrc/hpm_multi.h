@@ -38,15 +38,16 @@ __global__ HPM_PROP_ATTR void k_init_multi(const RunConsts* __restrict__ Cs, con
   init_body<E>(Cs[v], Ss[v], id, strip);
 }
 
-template <class E, bool kLds, int CW>
-__global__ HPM_PROP_ATTR void k_propagate_multi(const RunConsts* __restrict__ Cs, const DevState* __restrict__ Ss,
-                                                const MultiView* __restrict__ views, const uint16_t* __restrict__ wg,
-                                                int colour, int iter, int strip) {
+// NT threads per workgroup (256 or 128), CW same-colour columns per tile.
+template <class E, bool kLds, int CW, int NT>
+__global__ HPM_PROP_ATTR_NT(NT) void k_propagate_multi(const RunConsts* __restrict__ Cs, const DevState* __restrict__ Ss,
+                                                       const MultiView* __restrict__ views,
+                                                       const uint16_t* __restrict__ wg, int colour, int iter, int strip) {
   static_assert(E::kLanes == 1, "multi-view launches take one-lane engines");
   const int v = __builtin_amdgcn_readfirstlane((int)wg[blockIdx.x]);
   const int id = (int)blockIdx.x - views[v].first[kStageColour];
   if (id >= views[v].count[kStageColour]) return;
-  propagate_body<E, kLds, CW>(Cs[v], Ss[v], colour, iter, views[v].H2, id, strip);
+  propagate_body<E, kLds, CW, NT>(Cs[v], Ss[v], colour, iter, views[v].H2, id, strip);
 }
 
 // 256 consecutive pixels (row-major) per workgroup

@@ -64,6 +64,7 @@ EXPORTS = [
     "hpm_reload_hypotheses_maps", "hpm_set_canny_u8", "hpm_consistency_check_cb",
     "hpm_batch_create", "hpm_batch_destroy", "hpm_batch_last_error", "hpm_batch_run", "hpm_batch_set_graph",
     "hpm_batch_set_shape", "hpm_batch_get_shape", "hpm_batch_last_ms", "hpm_batch_autotune",
+    "hpm_batch_set_workgroup", "hpm_batch_get_workgroup",
 ]
 
 _lib = None
@@ -130,6 +131,8 @@ def lib():
     L.hpm_batch_set_graph.argtypes = [P, ctypes.c_int]
     L.hpm_batch_set_shape.argtypes = [P, ctypes.c_int, ctypes.c_int]
     L.hpm_batch_get_shape.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.hpm_batch_set_workgroup.argtypes = [P, ctypes.c_int]
+    L.hpm_batch_get_workgroup.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
     L.hpm_batch_last_ms.argtypes = [P]
     L.hpm_batch_last_ms.restype = ctypes.c_float
     L.hpm_batch_autotune.argtypes = [P, ctypes.POINTER(ctypes.c_uint64), fp]
@@ -568,11 +571,21 @@ class Batch:
         self._check(lib().hpm_batch_get_shape(self._h, ctypes.byref(w), ctypes.byref(s)), "hpm_batch_get_shape")
         return w.value, s.value
 
+    def set_workgroup(self, threads=256):
+        """Pixels (threads) per colour-pass workgroup: 256 or 128."""
+        self._check(lib().hpm_batch_set_workgroup(self._h, int(threads)), "hpm_batch_set_workgroup")
+
+    def workgroup(self):
+        t = ctypes.c_int()
+        self._check(lib().hpm_batch_get_workgroup(self._h, ctypes.byref(t)), "hpm_batch_get_workgroup")
+        return t.value
+
     def last_ms(self):
         return float(lib().hpm_batch_last_ms(self._h))
 
     def autotune(self, seeds):
-        """Times every launch shape, keeps the fastest; returns (tile_w, strip, ms)."""
+        """Times every launch shape, keeps the fastest; returns (tile_w, strip, ms)
+        (the chosen pixels per workgroup: workgroup())."""
         ms = ctypes.c_float()
         self._check(lib().hpm_batch_autotune(self._h, self._seeds(seeds), ctypes.byref(ms)), "hpm_batch_autotune")
         return self.shape() + (ms.value,)

@@ -254,12 +254,20 @@ int hpm_batch_set_graph(hpm_batch* batch, int enable);
  * strips each XCD walks (1..8).  Results do not depend on it. */
 int hpm_batch_set_shape(hpm_batch* batch, int tile_w, int strip);
 int hpm_batch_get_shape(const hpm_batch* batch, int* tile_w, int* strip);
+/* Pixels (threads) per colour-pass workgroup: 256 or 128 (0 = 256; the exact
+ * FP32 path takes 256 only).  The colour tile is tile_w same-colour columns
+ * by threads / tile_w rows.  Results do not depend on it.  No counterpart in
+ * the reference, whose checkerboard kernels use BLOCK_W x BLOCK_W / 2 blocks
+ * (HPM.cu:2992-3011). */
+int hpm_batch_set_workgroup(hpm_batch* batch, int threads);
+int hpm_batch_get_workgroup(const hpm_batch* batch, int* threads);
 /* Device time of the last run's schedule, milliseconds (HIP events). */
 float hpm_batch_last_ms(const hpm_batch* batch);
-/* Runs the batch once per launch shape (tile widths x strips 1, 2, 4), keeps
- * the fastest and runs it once more, so the views hold its result; *best_ms =
- * its schedule time.  Photometric passes only (their init overwrites all
- * state, so reruns are identical); otherwise HPM_E_STATE. */
+/* Runs the batch once per launch shape (threads per workgroup 256, 128 x tile
+ * widths x strips 1, 2, 4), keeps the fastest and runs it once more, so the
+ * views hold its result; *best_ms = its schedule time.  Photometric passes
+ * only (their init overwrites all state, so reruns are identical); otherwise
+ * HPM_E_STATE. */
 int hpm_batch_autotune(hpm_batch* batch, const uint64_t* seeds, float* best_ms);
 
 /* Waits for the context stream. */

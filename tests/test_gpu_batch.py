@@ -149,11 +149,15 @@ def test_fused_batch_matches_single_runs(prec):
     for s, sd in zip(single, seeds):
         s.run(sd)
     b = H.Batch(fused)
-    shapes = [(32, 2), (32, 1), (32, 4)] + ([(16, 1), (64, 2), (16, 4)] if prec != "fp32" else [])
+    # (tile width, strip, threads per workgroup)
+    shapes = [(32, 2, 256), (32, 1, 256), (32, 4, 256)]
+    if prec != "fp32":
+        shapes += [(16, 1, 256), (64, 2, 256), (16, 4, 256), (16, 2, 128), (32, 1, 128), (64, 4, 128)]
     for graph in (False, True):
         b.set_graph(graph)
-        for (w, st) in shapes:
+        for (w, st, nt) in shapes:
             b.set_shape(w, st)
+            b.set_workgroup(nt)
             for _ in range(2 if graph else 1):  # capture, then replay
                 b.run(seeds)
                 for f, s in zip(fused, single):
@@ -163,6 +167,12 @@ def test_fused_batch_matches_single_runs(prec):
         with pytest.raises(H.HpmError):
             b.set_shape(16, 2)
             b.run(seeds)
+        b.set_shape(32, 2)
+        with pytest.raises(H.HpmError):
+            b.set_workgroup(128)
+            b.run(seeds)
+    with pytest.raises(H.HpmError):
+        b.set_workgroup(64)
 
 
 def test_fused_batch_geometric_mode():
@@ -211,7 +221,7 @@ def test_fused_batch_private_rows_and_autotune():
     for f, s in zip(fused, single):
         _same(f, s)
     w, st, ms = b.autotune(seeds)
-    assert w in (16, 32, 64) and st in (1, 2, 4) and ms > 0
+    assert w in (16, 32, 64) and st in (1, 2, 4) and b.workgroup() in (128, 256) and ms > 0
     for f, s in zip(fused, single):
         _same(f, s)
     # mixed precisions are refused
