@@ -1,6 +1,8 @@
 // oracle_host.cpp -- CPU restatement of the passes around PatchMatch that the
-// GPU library also implements: joint bilateral upsampling and the multi-view
-// consistency check of fusion / confidence evaluation.
+// GPU library also implements: joint bilateral upsampling, the multi-view
+// consistency check of fusion / confidence evaluation, and the planar prior
+// (support points, triangle planes, raster, depth test, the coarse prior of
+// distances 1 and 2 and its per-pixel reload).
 //
 // TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's CPU
 // baseline leg are the only users.  The product never includes, links or calls
@@ -195,7 +197,10 @@ inline float GetAngle(const float* a, const float* b) {
 // the one minimising cost + 0.2 - confidence (a double expression stored as
 // float), among pixels with cost < 2; each kept when below 0.1 (double).
 // The texture flag the reference computes is never read.
-int SupportPoints(const float* costs, const float* conf, int w, int h, int32_t* out_xy, int cap) {
+// The maps have `stride` columns; the blocks cover the first w x h of them
+// (the coarse prior scans (int)(width * factor) columns of maps resized to
+// round(width * factor), HPM.cpp:1030-1031).
+int SupportPoints(const float* costs, const float* conf, int stride, int w, int h, int32_t* out_xy, int cap) {
   int n = 0;
   for (int col = 0; col < w; col += 5)
     for (int row = 0; row < h; row += 5) {
@@ -203,7 +208,7 @@ int SupportPoints(const float* costs, const float* conf, int w, int h, int32_t* 
       int plain_x = 0, plain_y = 0, shift_x = 0, shift_y = 0;
       for (int c = col; c < std::min(w, col + 5); ++c)
         for (int r = row; r < std::min(h, row + 5); ++r) {
-          const float cost = costs[(size_t)r * w + c], cf = conf[(size_t)r * w + c];
+          const float cost = costs[(size_t)r * stride + c], cf = conf[(size_t)r * stride + c];
           if (!(cost < 2.0f)) continue;
           const float plain = cost - cf;
           const float shifted = (float)(cost + 0.2 - cf);
@@ -221,6 +226,14 @@ int SupportPoints(const float* costs, const float* conf, int w, int h, int32_t* 
 inline void RefCamPoint(const Cam& c, int x, int y, float depth, float X[3]) {
   X[0] = depth * ((float)x - c.K[2]) / c.K[0];
   X[1] = depth * ((float)y - c.K[5]) / c.K[4];
+  X[2] = depth;
+}
+
+// Get3DPointonRefCam_factor (HPM.cpp:247-256): the same with the intrinsics
+// scaled by `factor` inside the expression.
+inline void RefCamPointFactor(const Cam& c, int x, int y, float depth, float factor, float X[3]) {
+  X[0] = depth * ((float)x - c.K[2] * factor) / (c.K[0] * factor);
+  X[1] = depth * ((float)y - c.K[5] * factor) / (c.K[4] * factor);
   X[2] = depth;
 }
 
@@ -247,18 +260,23 @@ inline float PlaneDepth(const Cam& c, const float pl[4], int x, int y) {
   return -pl[3] * c.K[0] / (((float)x - c.K[2]) * pl[0] + (c.K[0] / c.K[4]) * ((float)y - c.K[5]) * pl[1] + c.K[0] * pl[2]);
 }
 
-// The distance-0 prior of ProcessProblem (main.cpp:385-431) and
-// CudaPlanarPriorInitialization (HPM.cpp:964-984): triangles with all three
-// corners inside the image, in order, get index k = 1, 2, ...; each is
-// rasterised by the barycentric walk of main.cpp:396-404 (later triangles
-// overwrite earlier ones) and gets the plane through its back-projected
-// corners; a pixel keeps its triangle only when that plane's depth there is
-// within the pass's depth range.  Prior planes of pixels without a triangle
-// stay 0.
-void PriorAtLevel(const Cam& cam, const float* depths, const int32_t* tris, int ntri, float* prior, uint32_t* masks) {
-  const int w = cam.width, h = cam.height;
-  std::vector<float> mask((size_t)w * h, 0.0f);
-  std::vector<float> planes;
+// GetDepthFromPlaneParam_factor (HPM.cpp:1174-1177).
+inline float PlaneDepthFactor(const Cam& c, const float pl[4], int x, int y, float factor) {
+  return -pl[3] * (c.K[0] * factor) /
+         (((float)x - c.K[2] * factor) * pl[0] + (c.K[0] / c.K[4]) * ((float)y - c.K[5] * factor) * pl[1] +
+          c.K[0] * factor * pl[2]);
+}
+
+// The triangles with all three corners inside the w x h grid, in order, get
+// index k = 1, 2, ...; each is rasterised by the barycentric walk of
+// main.cpp:396-404 / 499-513 (later triangles overwrite earlier ones) into
+// `mask`, and its plane is the plane through its corners back-projected by
+// `point` (x, y, depth at the corner, X).
+template <class PointFn>
+void RasterTrianglePlanes(int w, int h, const float* depths, const int32_t* tris, int ntri, PointFn point,
+                          std::vector<float>& mask, std::vector<float>& planes) {
+  mask.assign((size_t)w * h, 0.0f);
+  planes.clear();
   uint32_t idx = 0;
   for (int t = 0; t < ntri; ++t) {
     const int32_t* T = tris + 6 * t;
@@ -281,13 +299,28 @@ void PriorAtLevel(const Cam& cam, const float* depths, const int32_t* tris, int 
         if (x >= 0 && x < w && y >= 0 && y < h) mask[(size_t)y * w + x] = (float)(idx + 1.0);
       }
     float X1[3], X2[3], X3[3], pl[4];
-    RefCamPoint(cam, x1, y1, depths[(size_t)y1 * w + x1], X1);
-    RefCamPoint(cam, x2, y2, depths[(size_t)y2 * w + x2], X2);
-    RefCamPoint(cam, x3, y3, depths[(size_t)y3 * w + x3], X3);
+    point(x1, y1, depths[(size_t)y1 * w + x1], X1);
+    point(x2, y2, depths[(size_t)y2 * w + x2], X2);
+    point(x3, y3, depths[(size_t)y3 * w + x3], X3);
     PriorPlane(X1, X2, X3, pl);
     planes.insert(planes.end(), pl, pl + 4);
     ++idx;
   }
+}
+
+// The distance-0 prior of ProcessProblem (main.cpp:385-431) and
+// CudaPlanarPriorInitialization (HPM.cpp:964-984): triangles with all three
+// corners inside the image, in order, get index k = 1, 2, ...; each is
+// rasterised by the barycentric walk of main.cpp:396-404 (later triangles
+// overwrite earlier ones) and gets the plane through its back-projected
+// corners; a pixel keeps its triangle only when that plane's depth there is
+// within the pass's depth range.  Prior planes of pixels without a triangle
+// stay 0.
+void PriorAtLevel(const Cam& cam, const float* depths, const int32_t* tris, int ntri, float* prior, uint32_t* masks) {
+  const int w = cam.width, h = cam.height;
+  std::vector<float> mask, planes;
+  RasterTrianglePlanes(w, h, depths, tris, ntri,
+                       [&](int x, int y, float d, float X[3]) { RefCamPoint(cam, x, y, d, X); }, mask, planes);
   const float dmin = cam.depth_min * 0.6f, dmax = cam.depth_max * 1.2f;  // GetMinDepth / GetMaxDepth
   for (int x = 0; x < w; ++x)
     for (int y = 0; y < h; ++y) {
@@ -302,12 +335,96 @@ void PriorAtLevel(const Cam& cam, const float* depths, const int32_t* tris, int 
     }
 }
 
+// The coarse planar prior of ProcessProblem at hpm_scale_distance 1 or 2
+// (main.cpp:443-541), on the maps resized to hw x hh = round(factor x the
+// view's size): triangle planes back-projected with the scaled intrinsics
+// (GetPriorPlaneParams_factor, HPM.cpp:1099-1132), the raster, and per
+// pixel with a triangle the plane's depth (GetDepthFromPlaneParam_factor)
+// when it lies within [GetMinDepth * 0.6, GetMaxDepth * 1.2], with the
+// plane's normal turned to the world frame (TransformNormal,
+// HPM.cpp:1179-1187); other pixels get depth 0 and normal 0.  prior_n holds
+// 4 floats per pixel (normal, 0).
+void PriorCoarse(const Cam& cam, const float* depths, int hw, int hh, const int32_t* tris, int ntri, float factor,
+                 float* prior_d, float* prior_n) {
+  std::vector<float> mask, planes;
+  RasterTrianglePlanes(hw, hh, depths, tris, ntri,
+                       [&](int x, int y, float d, float X[3]) { RefCamPointFactor(cam, x, y, d, factor, X); }, mask,
+                       planes);
+  const float dmin = cam.depth_min * 0.6f, dmax = cam.depth_max * 1.2f;  // GetMinDepth / GetMaxDepth
+  for (int x = 0; x < hw; ++x)
+    for (int y = 0; y < hh; ++y) {
+      const size_t i = (size_t)y * hw + x;
+      prior_d[i] = 0.0f;
+      for (int k = 0; k < 4; ++k) prior_n[4 * i + k] = 0.0f;
+      const uint32_t m = (uint32_t)mask[i];
+      if (m == 0) continue;
+      const float* pl = &planes[4 * (m - 1)];
+      const float d = PlaneDepthFactor(cam, pl, x, y, factor);
+      if (d <= dmax * 1.2f && d >= dmin * 0.6f) {
+        prior_d[i] = d;
+        const float* R = cam.R;
+        prior_n[4 * i + 0] = R[0] * pl[0] + R[3] * pl[1] + R[6] * pl[2];
+        prior_n[4 * i + 1] = R[1] * pl[0] + R[4] * pl[1] + R[7] * pl[2];
+        prior_n[4 * i + 2] = R[2] * pl[0] + R[5] * pl[1] + R[8] * pl[2];
+      }
+    }
+}
+
+// The per-pixel reload of the upsampled coarse prior (main.cpp:553-574):
+// where the upsampled depth lies within [GetMinDepth, GetMaxDepth] the mask
+// is 1 and the prior plane is the upsampled world normal turned to the
+// camera frame (TransformNormal2RefCam, HPM.cpp:1189-1197) with w =
+// GetDistance2Origin (HPM.cpp:1199-1206); elsewhere mask 0, plane 0.
+void PriorReload(const Cam& cam, const float* up_d, const float* up_n, float* prior, uint32_t* masks) {
+  const int w = cam.width, h = cam.height;
+  const float dmin = cam.depth_min * 0.6f, dmax = cam.depth_max * 1.2f;
+  const float* R = cam.R;
+  for (int x = 0; x < w; ++x)
+    for (int y = 0; y < h; ++y) {
+      const size_t c = (size_t)y * w + x;
+      const float d = up_d[c];
+      for (int k = 0; k < 4; ++k) prior[4 * c + k] = 0.0f;
+      masks[c] = 0;
+      if (!(d <= dmax && d >= dmin)) continue;
+      masks[c] = 1;
+      const float n0 = up_n[4 * c], n1 = up_n[4 * c + 1], n2 = up_n[4 * c + 2];
+      const float t0 = R[0] * n0 + R[1] * n1 + R[2] * n2;
+      const float t1 = R[3] * n0 + R[4] * n1 + R[5] * n2;
+      const float t2 = R[6] * n0 + R[7] * n1 + R[8] * n2;
+      const float X0 = d * ((float)x - cam.K[2]) / cam.K[0];
+      const float X1 = d * ((float)y - cam.K[5]) / cam.K[4];
+      const float X2 = d;
+      prior[4 * c + 0] = t0;
+      prior[4 * c + 1] = t1;
+      prior[4 * c + 2] = t2;
+      prior[4 * c + 3] = -(t0 * X0 + t1 * X1 + t2 * X2);
+    }
+}
+
 }  // namespace
 
 extern "C" {
 
 int orc_support_points(const float* costs, const float* conf, int w, int h, int32_t* out_xy, int cap, int* count) {
-  *count = SupportPoints(costs, conf, w, h, out_xy, cap);
+  *count = SupportPoints(costs, conf, w, w, h, out_xy, cap);
+  return 0;
+}
+
+// Support points over the first w x h of maps with `stride` columns.
+int orc_support_points_scan(const float* costs, const float* conf, int stride, int w, int h, int32_t* out_xy, int cap,
+                            int* count) {
+  *count = SupportPoints(costs, conf, stride, w, h, out_xy, cap);
+  return 0;
+}
+
+int orc_prior_coarse(const Cam* cam, const float* depths, int hw, int hh, const int32_t* tris, int ntri, float factor,
+                     float* prior_d, float* prior_n) {
+  PriorCoarse(*cam, depths, hw, hh, tris, ntri, factor, prior_d, prior_n);
+  return 0;
+}
+
+int orc_prior_reload(const Cam* cam, const float* up_d, const float* up_n, float* prior, uint32_t* masks) {
+  PriorReload(*cam, up_d, up_n, prior, masks);
   return 0;
 }
 

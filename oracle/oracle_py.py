@@ -71,6 +71,11 @@ def lib():
                                       ctypes.c_void_p]
         L.orc_prior_plane.argtypes = [ctypes.c_void_p] * 4
         L.orc_prior_plane.restype = None
+        L.orc_support_points_scan.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.orc_prior_coarse.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.c_int, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_prior_reload.argtypes = [ctypes.c_void_p] * 5
         L.orc_cost_grad_f64.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_cost_params_f64.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
