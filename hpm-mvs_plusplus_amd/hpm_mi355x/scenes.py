@@ -216,7 +216,8 @@ def render(cam: Camera, planes, noise_sigma, rng, rows_per_chunk=256):
             best_k = np.where(better, k, best_k)
         # rays_cam has z=1, so t is the depth along the optical axis
         depth[r0 - 0:r1 - 0] = np.where(np.isfinite(best_t), best_t, 0).astype(np.float32)
-        X = C[None, None, :] + best_t[..., None] * rays_w
+        # rays that hit nothing (t = inf) are never read back: give them t = 0
+        X = C[None, None, :] + np.where(np.isfinite(best_t), best_t, 0.0)[..., None] * rays_w
         shade = np.zeros(u.shape)
         for k, pl in enumerate(all_planes):
             m = best_k == k
