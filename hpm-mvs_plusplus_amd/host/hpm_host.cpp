@@ -909,7 +909,10 @@ HPM::HPM() {
   if (hpm_create(&ctx_, 0) != HPM_OK) throw Error("hpm_create failed (no HIP device?)");
 }
 
-HPM::~HPM() { hpm_destroy(ctx_); }
+HPM::~HPM() {
+  hpm_batch_destroy(fused_);
+  hpm_destroy(ctx_);
+}
 
 void HPM::check(int rc, const char* what) const {
   if (rc != HPM_OK) throw Error(std::string(what) + ": " + hpm_last_error(ctx_));
@@ -1091,11 +1094,16 @@ void HPM::RunPatchMatch(uint64_t seed) {
   // gathers in the caches: 7-20 % less time per RunPatchMatch.
   const int64_t px = (int64_t)GetReferenceImageWidth() * GetReferenceImageHeight();
   if (params_.precision != HPM_PREC_FP32_EXACT && px >= g_fused_min_pixels) {
-    hpm_batch* b = nullptr;
-    if (hpm_batch_create(&b, &ctx_, 1) != HPM_OK) throw Error("hpm_batch_create failed");
-    std::unique_ptr<hpm_batch, void (*)(hpm_batch*)> owned(b, hpm_batch_destroy);
-    if (hpm_batch_set_shape(b, 16, 4) != HPM_OK || hpm_batch_run(b, &seed) != HPM_OK)
-      throw Error(std::string("hpm_batch_run: ") + hpm_batch_last_error(b));
+    // one batch per HPM object, made on the first large run and kept: its
+    // stream, events and layout tables serve every later run (the tables are
+    // rebuilt only when the context's buffers or sizes changed)
+    if (!fused_) {
+      if (hpm_batch_create(&fused_, &ctx_, 1) != HPM_OK) throw Error("hpm_batch_create failed");
+      if (hpm_batch_set_shape(fused_, 16, 4) != HPM_OK)
+        throw Error(std::string("hpm_batch_set_shape: ") + hpm_batch_last_error(fused_));
+    }
+    if (hpm_batch_run(fused_, &seed) != HPM_OK)
+      throw Error(std::string("hpm_batch_run: ") + hpm_batch_last_error(fused_));
     return;  // hpm_batch_run returns once its stream has finished
   }
   check(hpm_run_patchmatch(ctx_, seed), "hpm_run_patchmatch");

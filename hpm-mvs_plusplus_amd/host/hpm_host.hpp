@@ -337,6 +337,7 @@ class HPM {
  private:
   void check(int rc, const char* what) const;
   hpm_ctx* ctx_ = nullptr;
+  hpm_batch* fused_ = nullptr;  // one-view fused launch of large views, made on first use
   hpm_params params_{};
   bool multi_geometry_ = false;
   std::vector<hpm_camera> cams_;

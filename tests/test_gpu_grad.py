@@ -115,15 +115,16 @@ def test_gradient_vs_central_differences_of_gpu_cost():
     sc, ids, cams, imgs, params, weights, gt = _problem(noise_seed=1)
     ctx = _ctx(cams, imgs)
     cost, grad = ctx.cost_grad(params, weights)
-    live = (cost > 0.02) & (cost < 1.5)
+    # a zero depth derivative (a window on flat texture) has no relative error
+    live = (cost > 0.02) & (cost < 1.5) & (grad[..., 0] != 0)
     h = 1e-3 * params[..., 0]
     hi, lo = params.copy(), params.copy()
     hi[..., 0] += h
     lo[..., 0] -= h
     fd = (ctx.cost_grad(hi, weights)[0].astype(np.float64) - ctx.cost_grad(lo, weights)[0]) / (2 * h)
-    rel = np.abs(fd - grad[..., 0]) / np.maximum(np.abs(grad[..., 0]), 1e-12)
-    print(f"depth FD (h=1e-3 rel): median rel {np.median(rel[live]):.2e}")
-    assert np.median(rel[live]) <= 5e-2
+    rel = np.abs(fd[live] - grad[..., 0][live]) / np.abs(grad[..., 0][live])
+    print(f"depth FD (h=1e-3 rel): median rel {np.median(rel):.2e}")
+    assert np.median(rel) <= 5e-2
 
 
 def test_torch_autograd_function():
