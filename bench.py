@@ -17,7 +17,10 @@ view at a time, no graph) as `plain_fp32`, and the FP32 mixed-precision
 engine (FP32 texels, the FP16 path's arithmetic) as `fp32_fast`.
 `accuracy_vs_fp64` runs the FP64 oracle's RunPatchMatch of the same view
 with the same seed and gives, per precision, the share of pixels whose depth
-lies within 1 % of the FP64 result and whose normal lies within 5 degrees.
+lies within 1 % of the FP64 result and whose normal lies within 5 degrees;
+`fp64_vs_fp64_source_t_1ulp` is the same agreement between two FP64 oracle
+runs whose source camera translations differ by one float ulp, the floor
+that PatchMatch's chaos puts under any such comparison.
 `cpu_baseline` times the FP32 oracle on bands of the view, with the host's
 threads and with one.  Prints one JSON line.
 
@@ -662,6 +665,16 @@ def main():
             accuracy[prec] = {"depth_within_1pct_of_fp64": dep, "normals_within_5deg_of_fp64": nrm}
         accuracy["fp64_depth_within_1pct_of_gt"] = round(
             float(np.mean(np.abs(r64["planes"][..., 3] - gt) / gt <= 0.01)), 4)
+        # The floor that PatchMatch's chaos sets: the FP64 oracle against
+        # itself with every source camera's translation one float ulp up.
+        print("[bench] FP64 oracle with source translations one ulp up ...", file=sys.stderr, flush=True)
+        ou = [O.camera_from_scene(c) for c in cams]
+        for c in ou[1:]:
+            for k in range(3):
+                c.t[k] = float(np.nextafter(np.float32(c.t[k]), np.float32(np.inf)))
+        r64u = O.run_patchmatch(ou, imgs, O.params(len(ou), ou[0]), last_seed, dtype=np.float64, canny=canny)
+        dep, nrm = agreement(r64u["planes"], r64["planes"])
+        accuracy["fp64_vs_fp64_source_t_1ulp"] = {"depth_within_1pct_of_fp64": dep, "normals_within_5deg_of_fp64": nrm}
     if "fp32" in runs and args.precision != "fp32":
         rp, pe = runs["fp32"]
         pf = runs[args.precision][1]

@@ -25,8 +25,10 @@
 //  * Tap addresses: the clamped cell offset (dy, dx) is combined as the float
 //    dy * (w + 2) + dx, exact below 2^24 elements per view (checked on the host).
 //  * Sums use shifted data: the reference window is centred on its centre
-//    texel, the source window on its first sample, so E[x^2] - E[x]^2 never
-//    cancels large magnitudes.  The reference-side statistics are accumulated
+//    texel, the source window on the bilinear sample at the centre projection
+//    (one more gather per pixel and view), and where that still leaves the
+//    window mean far from zero the window is summed again about its mean, so
+//    E[x^2] - E[x]^2 never cancels large magnitudes.  The reference-side statistics are accumulated
 //    in FP64 once per pixel.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -124,6 +126,14 @@ struct FastWin {
   float var_r;       // weighted variance of the reference window
 };
 
+// Bilateral weight of tap (i, j) with reference texel r around centre texel c
+// (HPM.cu:342-347); one expression for the window build and the second pass
+// below, so both see the same weights bit for bit.
+__device__ __forceinline__ float fast_weight(int i, int j, float r, float c, float den_s, float den_c) {
+  const float spatial = sqrtf((float)(i * i + j * j));
+  return expf_hpm(-spatial / den_s - fabsf(r - c) / den_c);
+}
+
 template <class S>
 struct FastEngine {
   static constexpr int kLanes = 1;
@@ -142,8 +152,7 @@ struct FastEngine {
       for (int jj = 0; jj < kTapsPerAxis; ++jj) {
         const int i = -5 + 2 * ii, j = -5 + 2 * jj;
         const float r = tex_at(ref, W, H, px + i, py + j);
-        const float spatial = sqrtf((float)(i * i + j * j));
-        const float w = expf_hpm(-spatial / den_s - fabsf(r - c) / den_c);
+        const float w = fast_weight(i, j, r, c, den_s, den_c);
         const float rs = r - c;  // exact: integer-valued texels
         fw.w[ii * kTapsPerAxis + jj] = w;
         fw.wr[ii * kTapsPerAxis + jj] = w * rs;
@@ -221,6 +230,12 @@ struct FastEngine {
       const int rel = (int)ffma(cyl, qpf, cxl);  // exact: |rel| < (w + 1) (h + 1) < 2^24
       off = ((unsigned)rel + base) << S::kShift;
     };
+    // The window sums run on shifted samples.  The shift is the window-centre
+    // sample (cell (iu, iv) at fractions (fu, fv), always inside the pair
+    // grid), which lies near the window mean, so S2 / W - m_s^2 does not
+    // cancel: shifted by the first tap, a corner sample 40-70 levels from the
+    // mean, the FP32 sums lost up to 4e-4 of cost in windows of variance 10-80.
+    const typename S::Q qc = S::load(rsrc, base << S::kShift);
     // Software pipeline over the six columns: the loads of columns
     // ii + 1 .. ii + kPipeCols are in flight while column ii is consumed.
     constexpr int L = kPipeCols;
@@ -236,7 +251,8 @@ struct FastEngine {
         q[c % R][jj] = S::load(rsrc, off);
       }
     }
-    float s0 = 0.0f, S1 = 0.0f, S2 = 0.0f, S3 = 0.0f;
+    float s0 = S::bilinear(qc, fu, fv);
+    float S1 = 0.0f, S2 = 0.0f, S3 = 0.0f;
 #pragma unroll
     for (int ii = 0; ii < kTapsPerAxis; ++ii) {
       if (ii + L < kTapsPerAxis) {
@@ -249,17 +265,50 @@ struct FastEngine {
       }
 #pragma unroll
       for (int jj = 0; jj < kTapsPerAxis; ++jj) {
-        const float s = S::bilinear(q[ii % R][jj], axr[ii % R][jj], ayr[ii % R][jj]);
-        if (ii == 0 && jj == 0) s0 = s;
-        const float sp = s - s0;
+        const float sp = S::bilinear(q[ii % R][jj], axr[ii % R][jj], ayr[ii % R][jj]) - s0;
         const float ws = fw.w[ii * kTapsPerAxis + jj] * sp;
         S1 += ws;
         S2 = ffma(ws, sp, S2);
         S3 = ffma(fw.wr[ii * kTapsPerAxis + jj], sp, S3);
       }
     }
-    const float m_s = S1 * fw.inv_w;
-    const float var_s = ffma(-m_s, m_s, S2 * fw.inv_w);
+    float m_s = S1 * fw.inv_w;
+    float var_s = ffma(-m_s, m_s, S2 * fw.inv_w);
+    if (__builtin_expect(m_s * m_s > 16.0f * var_s, 0)) {
+      // The centre sample far from the window mean (windows cut by the source
+      // border, about 5e-5 of the evaluations): the window is summed again
+      // about the first pass's mean, one tap at a time, with the weights
+      // recomputed from the reference image.  Unrolled, or with the weights
+      // kept, this pass took registers from the common path (twice its
+      // spills, 3 % slower); rolled it needs a handful.
+      s0 += m_s;
+      const float* ref = C.img[0];
+      const int W = C.cams[0].width, H = C.cams[0].height;
+      const float c = tex_at(ref, W, H, px, py);
+      const float den_s = 2.0f * C.sigma_spatial * C.sigma_spatial;
+      const float den_c = 2.0f * C.sigma_color * C.sigma_color;
+      S1 = 0.0f;
+      S2 = 0.0f;
+      S3 = 0.0f;
+#pragma nounroll
+      for (int k = 0; k < kTaps; ++k) {
+        const int ii = k / kTapsPerAxis, jj = k - ii * kTapsPerAxis;
+        unsigned off;
+        float ax, ay;
+        tap(ii, jj, off, ax, ay);
+        const typename S::Q qs = S::load(rsrc, off);
+        const int i = -5 + 2 * ii, j = -5 + 2 * jj;
+        const float r = tex_at(ref, W, H, px + i, py + j);
+        const float w = fast_weight(i, j, r, c, den_s, den_c);
+        const float sp = S::bilinear(qs, ax, ay) - s0;
+        const float ws = w * sp;
+        S1 += ws;
+        S2 = ffma(ws, sp, S2);
+        S3 = ffma(w * (r - c), sp, S3);
+      }
+      m_s = S1 * fw.inv_w;
+      var_s = ffma(-m_s, m_s, S2 * fw.inv_w);
+    }
     if (var_s < 1e-5f) return 2.0f;
     const float cov = ffma(-fw.m_r, m_s, S3 * fw.inv_w);
     const float c = 1.0f - cov * __builtin_amdgcn_rsqf(fw.var_r * var_s);

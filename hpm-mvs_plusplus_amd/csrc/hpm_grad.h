@@ -163,13 +163,16 @@ __global__ __launch_bounds__(256) void k_cost_grad(const RunConsts* __restrict__
     float s0 = 0.0f, S1 = 0.0f, S2 = 0.0f, S3 = 0.0f;
     float m_s = 0.0f, var_s = 0.0f, cov = 0.0f, den = 1.0f;
     if (live) {
+      {  // shift: the sample at the window centre, near the window mean
+        float su, sv, x, y, dd;
+        grad_tap(f, 0.0f, 0.0f, s0, su, sv, x, y, dd);
+      }
 #pragma unroll
       for (int ii = 0; ii < kTapsPerAxis; ++ii)
 #pragma unroll
         for (int jj = 0; jj < kTapsPerAxis; ++jj) {
           float s, su, sv, x, y, dd;
           grad_tap(f, (float)(-5 + 2 * ii), (float)(-5 + 2 * jj), s, su, sv, x, y, dd);
-          if (ii == 0 && jj == 0) s0 = s;
           const float spv = s - s0;
           const float ws = fw.w[ii * kTapsPerAxis + jj] * spv;
           S1 += ws;

@@ -32,6 +32,24 @@ def gpu_available():
     return _GPU
 
 
+def nudged_cameras(cams):
+    """Copies of oracle cameras with every source camera's translation one
+    float ulp up (the reference camera unchanged): two FP64 oracle runs on
+    the original and the nudged cameras bound how closely any arithmetic can
+    follow the FP64 result of a chaotic PatchMatch."""
+    import ctypes
+    import oracle_py as O
+    out = []
+    for i, c in enumerate(cams):
+        d = O.orc_camera()
+        ctypes.pointer(d)[0] = c
+        if i > 0:
+            for k in range(3):
+                d.t[k] = float(np.nextafter(np.float32(c.t[k]), np.float32(np.inf)))
+        out.append(d)
+    return out
+
+
 @pytest.fixture(scope="session")
 def small_scene():
     """160x120, 5 cameras, reference view 2 with 4 source views."""

@@ -14,9 +14,16 @@ FP64 oracles on the whole view, same seed.
   decided between refinement candidates whose costs differ by less than that
   noise.  The mixed-precision paths (FP64 view-pair basis, FP32 tap offsets,
   shifted window sums; FP16, BF16 or FP32 texels) reach 0.998 of the depths
-  and 0.984 of the normals.  Tolerances: depth >= 0.99 (north star: 1 % on
-  99 % of pixels), normals >= 0.975 (the north star's 0.99 is not reached;
-  0.984 measured), and never worse than the exact path.
+  and 0.984 of the normals.
+* The floor: the FP64 oracle against itself, with every source camera's
+  translation one float ulp up, agrees on 0.995 of the depths and 0.962 of
+  the normals (seed 109).  A change of the inputs at the last bit moves the
+  FP64 result further than the mixed-precision arithmetic does, so the north
+  star's 99 % of normals within 5 degrees is not a bound any arithmetic can
+  meet on this scene; the test asserts the floor instead.
+Tolerances: depth >= 0.99 (north star: 1 % on 99 % of pixels); normals >=
+0.975 (0.984 measured) and at least the FP64 floor; never worse than the
+exact path.
 """
 import time
 
@@ -24,7 +31,7 @@ import numpy as np
 import pytest
 
 import oracle_py as O
-from conftest import gpu_available
+from conftest import gpu_available, nudged_cameras
 
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="no HIP device")]
 
@@ -54,6 +61,10 @@ def test_headline_view_against_fp32_and_fp64_oracles():
     r64 = O.run_patchmatch(oc, imgs, prm, SEED, dtype=np.float64)
     t2 = time.perf_counter()
     print(f"oracle FP32 {t1 - t0:.1f} s, FP64 {t2 - t1:.1f} s", flush=True)
+    r64u = O.run_patchmatch(nudged_cameras(oc), imgs, prm, SEED, dtype=np.float64)
+    floor64 = agreement(r64u["planes"], r64["planes"])
+    print(f"FP64 vs FP64 with source translations one ulp up: depth {floor64[0]:.4f}, normals {floor64[1]:.4f}",
+          flush=True)
     ctx = H.Context(0)
     ctx.set_views([H.camera_from_scene(c) for c in cams], imgs)
     res = {}
@@ -76,3 +87,4 @@ def test_headline_view_against_fp32_and_fp64_oracles():
     for prec in ("fp16", "bf16", "fp32fast"):
         assert res[prec][0] >= 0.99 and res[prec][1] >= 0.975, prec
         assert res[prec][0] >= floor_d and res[prec][1] >= floor_n, prec
+        assert res[prec][0] >= min(0.99, floor64[0]) and res[prec][1] >= min(0.99, floor64[1]), prec

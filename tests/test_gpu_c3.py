@@ -6,19 +6,24 @@ views, so it runs on a band of 256 rows:
 
 * cost at fixed hypotheses (one random plane per pixel), FP16 storage
   against FP64 on the band (at fixed hypotheses a band holds exactly the
-  full image's rows): |dC| <= 1e-5 at q999, mean <= 2e-6, the same
+  full image's rows): max |dC| <= 1e-5, mean <= 2e-6, the same
   valid / invalid decisions on more than 99.9 % of pairs;
-* a photometric RunPatchMatch in FP16 over the whole view against the
-  ray-cast ground truth, and against the FP64 oracle's RunPatchMatch of the
-  band on the band's central 128 rows.  A band run lacks what propagates in
-  from the rows outside it, and PatchMatch turns one different adoption into
-  a different patch, so that agreement is bounded loosely (depth within 1 %
-  of FP64 on more than 93 % of those pixels; 95-98 % measured on MI355X).
-  The parity bound is the ground-truth accuracy of the two on the same rows:
-  the GPU's within 0.02 of FP64's or better.  View 15, at the end of the high
-  ring with every source to one side, is the hardest of the three: about
-  0.76 of its pixels lie within 1 % of ground truth after one photometric
-  pass (0.90 for view 7), for the FP64 oracle as for the GPU.
+* a photometric RunPatchMatch in FP16 over the whole view (time, accuracy
+  against the ray-cast ground truth);
+* the band as a problem of its own: the reference view cut to rows 896-1151
+  (principal point moved up by 896, so every band pixel keeps its ray; the
+  sources whole), solved by the GPU in FP16 and in plain FP32 and by the FP64
+  oracle, all with the same seed.  On view 7 the plain FP32 planes equal the
+  FP32 oracle's bit for bit.  PatchMatch turns a last-bit difference into a
+  different adoption that then spreads, so the bound against FP64 is the
+  FP64 oracle's agreement with itself when every source camera's translation
+  moves one float ulp: FP16 depth within 1 % and normals within 5 deg at
+  least min(99 %, that floor), and at least as close to FP64 as plain FP32
+  (the reference's own arithmetic).  Against ground truth the FP16 band
+  within 0.01 of the FP64 oracle's.  View 15, at the end
+  of the high ring with every source to one side, is the hardest of the
+  three: about 0.76 of its pixels lie within 1 % of ground truth after one
+  photometric pass (0.90 for view 7), for the FP64 oracle as for the GPU.
 """
 import time
 
@@ -26,7 +31,7 @@ import numpy as np
 import pytest
 
 import oracle_py as O
-from conftest import gpu_available
+from conftest import gpu_available, nudged_cameras
 
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="no HIP device")]
 
@@ -64,7 +69,35 @@ def test_c3_cost_vs_fp64_band(c3_scene, ref):
     print(f"view {ref}: validity agrees {same.mean():.6f}, |dC| max {err.max():.2e} "
           f"q999 {np.quantile(err, 0.999):.2e} mean {err.mean():.2e}")
     assert same.mean() > 0.999
-    assert np.quantile(err, 0.999) <= 1e-5 and err.mean() <= 2e-6
+    assert err.max() <= 1e-5 and err.mean() <= 2e-6
+
+
+def _band_problem(cams, imgs, y0, y1):
+    """The reference view cut to rows y0..y1-1 as an image of its own, for the
+    GPU (hpm cameras) and the oracle (orc cameras)."""
+    import hpm_mi355x as H
+    K = np.array(cams[0].K, np.float64)
+    K[1, 2] -= y0
+    c0 = cams[0]
+    gcams = [H.make_camera(K, c0.R, c0.t, c0.width, y1 - y0, c0.depth_min, c0.depth_max)]
+    gcams += [H.camera_from_scene(c) for c in cams[1:]]
+    ocams = [O.camera(K, c0.R, c0.t, c0.width, y1 - y0, c0.depth_min, c0.depth_max)]
+    ocams += [O.camera_from_scene(c) for c in cams[1:]]
+    return gcams, ocams, [np.ascontiguousarray(imgs[0][y0:y1])] + list(imgs[1:])
+
+
+def _agreement(planes, ref):
+    d, d64 = planes[..., 3].astype(np.float64), ref[..., 3]
+    rel = np.abs(d - d64) / np.abs(d64)
+    n, n64 = planes[..., :3].astype(np.float64), ref[..., :3]
+    cosang = np.sum(n * n64, -1) / (np.linalg.norm(n, axis=-1) * np.linalg.norm(n64, axis=-1))
+    ang = np.degrees(np.arccos(np.clip(cosang, -1, 1)))
+    return float(np.mean(rel <= 0.01)), float(np.mean(ang <= 5.0))
+
+
+def _gt_frac(d, gt):
+    hit = gt > 0
+    return float(np.mean(np.abs(d[hit] - gt[hit]) / gt[hit] <= 0.01))
 
 
 @pytest.mark.parametrize("ref", REFS)
@@ -80,31 +113,38 @@ def test_c3_patchmatch_vs_gt_and_fp64_band(c3_scene, ref):
     t0 = time.perf_counter()
     ctx.run(SEED)
     dt = time.perf_counter() - t0
-    planes = ctx.planes()
-    d = planes[..., 3].astype(np.float64)
     gt = c3_scene.depths[ref].astype(np.float64)
-    hit = gt > 0
-    gt_frac = float(np.mean(np.abs(d[hit] - gt[hit]) / gt[hit] <= 0.01))
-    # the FP64 oracle on rows 896-1151, compared on the central 128 rows
-    y0, y1, m = 896, 1152, 64
-    oc = [O.camera_from_scene(c) for c in cams]
+    gt_frac = _gt_frac(ctx.planes()[..., 3].astype(np.float64), gt)
+    ctx.close()
+    # the band rows 896-1151 as a problem of its own
+    y0, y1 = 896, 1152
+    print(f"case: C3 band {y0}-{y1} of view {ref}: GPU fp16 / fp32, oracle FP64", flush=True)
+    gcams, ocams, bimgs = _band_problem(cams, imgs, y0, y1)
+    band = {}
+    for prec in ("fp16", "fp32"):
+        bctx = H.Context(0)
+        bctx.set_params(H.default_params(precision=prec))
+        bctx.set_views(gcams, bimgs)
+        bctx.run(SEED)
+        band[prec] = bctx.planes()
+        bctx.close()
     t1 = time.perf_counter()
-    r64 = O.run_patchmatch(oc, imgs, O.params(len(oc), oc[0]), SEED, dtype=np.float64, rows=(y0, y1))
+    r64 = O.run_patchmatch(ocams, bimgs, O.params(len(ocams), ocams[0]), SEED, dtype=np.float64)
     t_oracle = time.perf_counter() - t1
-    band = slice(y0 + m, y1 - m)
-    d64 = r64["planes"][band, :, 3]
-    rel = np.abs(d[band] - d64) / np.abs(d64)
-    n, n64 = planes[band, :, :3].astype(np.float64), r64["planes"][band, :, :3]
-    cosang = np.sum(n * n64, -1) / (np.linalg.norm(n, axis=-1) * np.linalg.norm(n64, axis=-1))
-    ang = np.degrees(np.arccos(np.clip(cosang, -1, 1)))
-    gt_band = gt[band]
-    hb = gt_band > 0
-    gt64 = float(np.mean(np.abs(d64[hb] - gt_band[hb]) / gt_band[hb] <= 0.01))
+    r64u = O.run_patchmatch(nudged_cameras(ocams), bimgs, O.params(len(ocams), ocams[0]), SEED, dtype=np.float64)
+    f_d, f_n = _agreement(r64u["planes"], r64["planes"])
+    if ref == REFS[0]:
+        r32 = O.run_patchmatch(ocams, bimgs, O.params(len(ocams), ocams[0]), SEED)
+        assert np.array_equal(band["fp32"].view(np.uint32), r32["planes"].view(np.uint32))
+    d16, n16 = _agreement(band["fp16"], r64["planes"])
+    d32, n32 = _agreement(band["fp32"], r64["planes"])
+    gt_band = gt[y0:y1]
+    g16, g64 = _gt_frac(band["fp16"][..., 3].astype(np.float64), gt_band), _gt_frac(r64["planes"][..., 3], gt_band)
     print(f"view {ref}: GPU RunPatchMatch {1e3 * dt:.1f} ms (oracle FP64 band {t_oracle:.1f} s); "
-          f"within 1 % of GT {gt_frac:.3f} (whole view), band rows: GPU {np.mean(np.abs(d[band][hb] - gt_band[hb]) / gt_band[hb] <= 0.01):.3f}, "
-          f"FP64 {gt64:.3f}; GPU vs FP64: depth within 1 % {np.mean(rel <= 0.01):.3f}, normal within 5 deg "
-          f"{np.mean(ang <= 5.0):.3f}")
-    gpu_band = float(np.mean(np.abs(d[band][hb] - gt_band[hb]) / gt_band[hb] <= 0.01))
+          f"within 1 % of GT {gt_frac:.3f} (whole view); band: GT fp16 {g16:.3f} FP64 {g64:.3f}; "
+          f"vs FP64: fp16 depth {d16:.4f} normals {n16:.4f}, fp32 depth {d32:.4f} normals {n32:.4f}, "
+          f"FP64 with source translations one ulp up: depth {f_d:.4f} normals {f_n:.4f}")
     assert gt_frac > 0.70
-    assert gpu_band >= gt64 - 0.02  # as close to the ground truth as FP64 on the same rows
-    assert np.mean(rel <= 0.01) > 0.93
+    assert g16 >= g64 - 0.01
+    assert d16 >= min(0.99, f_d) and n16 >= min(0.99, f_n)
+    assert d16 >= d32 and n16 >= n32

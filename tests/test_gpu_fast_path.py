@@ -2,8 +2,9 @@
 oracle (needs an MI355X).
 
 Tolerances, written per test:
-  * cost at fixed hypotheses: |cost - cost_fp64| <= 1e-5 on >= 99.9 % of the
-    (pixel, view) pairs whose var >= 1e-5 decision agrees, mean <= 2e-6;
+  * cost at fixed hypotheses: |cost - cost_fp64| <= 1e-5 on every (pixel,
+    view) pair whose var >= 1e-5 decision agrees (the decision agrees on
+    > 99.9 % of pairs), mean <= 2e-6;
   * after a full RunPatchMatch: depth within 1 % of the FP64 oracle on >= 99 %
     of pixels and normals within 5 degrees on >= 99 %.
 """
@@ -28,11 +29,11 @@ def _ctx(cams, imgs, **params):
 
 def test_c2_full_size_cost_vs_fp64():
     """The cost microbenchmark's workload (1600 x 1200, 10 sources, one random
-    plane per pixel): every storage precision against the FP64 oracle on a
-    band of 240 rows (at fixed hypotheses a band is exactly the full image's
-    rows).  Mixed-precision engine: |dC| <= 1e-5 at q999, mean <= 2e-6;
-    exact FP32 engine (the reference's own E[x^2] - E[x]^2 formulation):
-    max <= 1e-2, mean <= 2e-4.  Prints max / mean per precision."""
+    plane per pixel): every storage precision against the FP64 oracle on all
+    19.2 M (pixel, view) pairs.  Mixed-precision engine: max |dC| <= 1e-5,
+    mean <= 2e-6; exact FP32 engine (the reference's own E[x^2] - E[x]^2
+    formulation): max <= 1e-2, mean <= 2e-4.  Prints max / mean per
+    precision."""
     import hpm_mi355x as H
     from hpm_mi355x import scenes
     print("case: cost_fixed 1600x1200 V=10 vs FP64 oracle band", flush=True)
@@ -40,12 +41,11 @@ def test_c2_full_size_cost_vs_fp64():
     sc = scenes.make_plane_scene(seed=1, n_cams=11, width=1600, height=1200, f=f, n_src=10)
     ids, cams, imgs = scenes.reference_view_problem(sc, 5, 10)
     planes = scenes.random_planes_per_pixel(cams[0], seed=4)
-    y0, y1 = 480, 720
     oc = [O.camera_from_scene(c) for c in cams]
-    ref64 = O.cost_fixed(oc, imgs, O.params(len(oc), oc[0]), planes.astype(np.float64), np.float64, rows=(y0, y1))[y0:y1]
+    ref64 = O.cost_fixed(oc, imgs, O.params(len(oc), oc[0]), planes.astype(np.float64), np.float64)
     ctx = _ctx(cams, imgs)
     for prec in ["fp32", "fp32fast", "fp16", "bf16"]:
-        got = ctx.cost_fixed(planes, precision=H.PRECISIONS[prec]).astype(np.float64)[y0:y1]
+        got = ctx.cost_fixed(planes, precision=H.PRECISIONS[prec]).astype(np.float64)
         same = (got == 2.0) == (ref64 == 2.0)
         assert same.mean() > 0.999
         err = np.abs(got - ref64)[same]
@@ -54,7 +54,7 @@ def test_c2_full_size_cost_vs_fp64():
         if prec == "fp32":
             assert err.max() <= 1e-2 and err.mean() <= 2e-4
         else:
-            assert np.quantile(err, 0.999) <= 1e-5 and err.mean() <= 2e-6
+            assert err.max() <= 1e-5 and err.mean() <= 2e-6
 
 
 @pytest.mark.parametrize("prec", PRECISIONS)
@@ -72,7 +72,7 @@ def test_fast_cost_vs_fp64(small_scene, prec):
     assert same.mean() > 0.999
     err = np.abs(got - ref64)[same]
     print(f"{prec}: median {np.median(err):.2e} q999 {np.quantile(err, 0.999):.2e} max {err.max():.2e}")
-    assert np.quantile(err, 0.999) <= 1e-5
+    assert err.max() <= 1e-5
     assert err.mean() <= 2e-6
 
 
