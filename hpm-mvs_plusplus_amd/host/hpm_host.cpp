@@ -1089,6 +1089,7 @@ void SetFusedMinPixels(int64_t n) { g_fused_min_pixels = n; }
 
 void HPM::RunPatchMatch(uint64_t seed) {
   have_results_ = false;
+  ran_ = true;
   check(hpm_set_params(ctx_, &params_), "hpm_set_params");
   // At 3072 x 2048 with 10 sources the narrower tiles keep more of the source
   // gathers in the caches: 7-20 % less time per RunPatchMatch.
@@ -1307,7 +1308,8 @@ void HPM::CudaPlanarPriorRelease() { check(hpm_release_prior(ctx_), "hpm_release
 
 void HPM::CudaSpaceRelease(bool geom_consistency) {
   (void)geom_consistency;  // the source depth maps go with the other buffers
-  FetchResults();
+  if (ran_) FetchResults();  // the getters keep answering from host copies
+  ran_ = false;
   check(hpm_release(ctx_), "hpm_release");
 }
 

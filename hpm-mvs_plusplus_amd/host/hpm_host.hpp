@@ -351,6 +351,7 @@ class HPM {
   // after a run until the next run
   void FetchResults();
   bool have_results_ = false;
+  bool ran_ = false;  // a run's results are on the device (until CudaSpaceRelease)
   std::vector<float> planes_host_, costs_host_, texture_host_;
 };
 
