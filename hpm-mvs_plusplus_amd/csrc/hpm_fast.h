@@ -127,8 +127,7 @@ struct FastWin {
 };
 
 // Bilateral weight of tap (i, j) with reference texel r around centre texel c
-// (HPM.cu:342-347); one expression for the window build and the second pass
-// below, so both see the same weights bit for bit.
+// (HPM.cu:342-347).
 __device__ __forceinline__ float fast_weight(int i, int j, float r, float c, float den_s, float den_c) {
   const float spatial = sqrtf((float)(i * i + j * j));
   return expf_hpm(-spatial / den_s - fabsf(r - c) / den_c);
@@ -276,37 +275,47 @@ struct FastEngine {
     float var_s = ffma(-m_s, m_s, S2 * fw.inv_w);
     if (__builtin_expect(m_s * m_s > 32.0f * var_s, 0)) {
       // The centre sample far from the window mean (m_s^2 > 32 var_s: windows
-      // cut by the source border, a few in a million evaluations): the window
-      // is summed again about the first pass's mean, one tap at a time, with
-      // the weights recomputed from the reference image.  Unrolled, or with
-      // the weights kept, this pass took registers from the common path (twice
-      // its spills, 3 % slower); rolled it needs a handful.  At 16 var_s it
-      // ran ten times as often and cost 2.5 % at 1600 x 1200; at 32 the
-      // largest cost error stays below 6e-6.
+      // cut by the source border and windows whose centre is an outlier, from
+      // a few in a million evaluations at 1600 x 1200 to 1.5e-4 on small views
+      // of wide-baseline scenes): the window is summed again about the first
+      // pass's mean, one column of six taps per iteration, each column's
+      // weights taken from the window's registers through a uniform switch.
+      // Measured alternatives: a loop of single taps that recomputed the
+      // weights from the reference image ran so slowly that the few waves
+      // taking it doubled short launches (768 x 512); unrolled, or with the
+      // column loads and recomputed weights, it doubled the common path's
+      // spills; running the whole first pass again in a loop cost 4.5 % at
+      // 1600 x 1200.  At 32 var_s the largest cost error stays below 6e-6 on
+      // the 1600 x 1200 plane scene and the full-resolution orbit views.
       s0 += m_s;
-      const float* ref = C.img[0];
-      const int W = C.cams[0].width, H = C.cams[0].height;
-      const float c = tex_at(ref, W, H, px, py);
-      const float den_s = 2.0f * C.sigma_spatial * C.sigma_spatial;
-      const float den_c = 2.0f * C.sigma_color * C.sigma_color;
       S1 = 0.0f;
       S2 = 0.0f;
       S3 = 0.0f;
 #pragma nounroll
-      for (int k = 0; k < kTaps; ++k) {
-        const int ii = k / kTapsPerAxis, jj = k - ii * kTapsPerAxis;
-        unsigned off;
-        float ax, ay;
-        tap(ii, jj, off, ax, ay);
-        const typename S::Q qs = S::load(rsrc, off);
-        const int i = -5 + 2 * ii, j = -5 + 2 * jj;
-        const float r = tex_at(ref, W, H, px + i, py + j);
-        const float w = fast_weight(i, j, r, c, den_s, den_c);
-        const float sp = S::bilinear(qs, ax, ay) - s0;
-        const float ws = w * sp;
-        S1 += ws;
-        S2 = ffma(ws, sp, S2);
-        S3 = ffma(w * (r - c), sp, S3);
+      for (int ii = 0; ii < kTapsPerAxis; ++ii) {
+        float wc[kTapsPerAxis], wrc[kTapsPerAxis];
+        switch (ii) {
+// case n copies column n's weights (compile-time register indices)
+#define HPM_COL(n) case n: for (int jj = 0; jj < kTapsPerAxis; ++jj) { wc[jj] = fw.w[n * kTapsPerAxis + jj]; wrc[jj] = fw.wr[n * kTapsPerAxis + jj]; } break;
+          HPM_COL(0) HPM_COL(1) HPM_COL(2) HPM_COL(3) HPM_COL(4) default: HPM_COL(5)
+#undef HPM_COL
+        }
+        typename S::Q qs[kTapsPerAxis];
+        float ax[kTapsPerAxis], ay[kTapsPerAxis];
+#pragma unroll
+        for (int jj = 0; jj < kTapsPerAxis; ++jj) {
+          unsigned off;
+          tap(ii, jj, off, ax[jj], ay[jj]);
+          qs[jj] = S::load(rsrc, off);
+        }
+#pragma unroll
+        for (int jj = 0; jj < kTapsPerAxis; ++jj) {
+          const float sp = S::bilinear(qs[jj], ax[jj], ay[jj]) - s0;
+          const float ws = wc[jj] * sp;
+          S1 += ws;
+          S2 = ffma(ws, sp, S2);
+          S3 = ffma(wrc[jj], sp, S3);
+        }
       }
       m_s = S1 * fw.inv_w;
       var_s = ffma(-m_s, m_s, S2 * fw.inv_w);
