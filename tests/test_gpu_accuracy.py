@@ -18,10 +18,10 @@ FP64 oracles on the whole view, same seed.
 * The floor: the FP64 oracle against itself, with every source camera's
   translation one float ulp up, agrees on 0.995 of the depths and 0.962 of
   the normals (seed 109).  A change of the inputs at the last bit moves the
-  FP64 result further than the mixed-precision arithmetic does, so the north
-  star's 99 % of normals within 5 degrees is not a bound any arithmetic can
-  meet on this scene; the test asserts the floor instead.
-Tolerances: depth >= 0.99 (north star: 1 % on 99 % of pixels); normals >=
+  FP64 result further than the mixed-precision arithmetic does, so 99 % of
+  normals within 5 degrees is not a bound any arithmetic can meet on this
+  scene; the test asserts the floor instead.
+Tolerances: depth >= 0.99 (depth within 1 % on 99 % of pixels); normals >=
 0.975 (0.984 measured) and at least the FP64 floor; never worse than the
 exact path.
 """
