@@ -104,7 +104,7 @@ class MapStoreScope {
   bool owner_ = false;  // false inside another scope, which keeps the store
 };
 // Binary PGM (P5) or PPM (P6), 8 bit.  The reference decodes JPEG with
-// OpenCV; this build reads the uncompressed Netpbm formats.
+// OpenCV; this reader takes the uncompressed Netpbm formats.
 bool ReadNetpbm(const std::string& path, MapU8& out);
 // JPEG (hpm_jpeg.cpp), Huffman-coded sequential or progressive: the pixels
 // libjpeg's default decoding gives (accurate integer IDCT, fancy upsampling,
@@ -270,8 +270,8 @@ class HPM {
   void ReloadPlanarPriorInitialization(const MapF& masks, const std::vector<float>& prior_planes);
   void RunPatchMatch(uint64_t seed);
   // RunPatchMatch() (HPM.cu:2987-3035) with the seed of SetSeed (0 by
-  // default); the reference seeds curand with clock64(), this build's
-  // counter-based generator with the seed.
+  // default); the reference seeds curand with clock64(), this library
+  // seeds its counter-based generator with the seed.
   void RunPatchMatch() { RunPatchMatch(seed_); }
   void SetSeed(uint64_t seed) { seed_ = seed; }
 
