@@ -47,7 +47,11 @@ extern "C" {
 #define HPM_MAX_IMAGES 33       /* reference + 32 sources: view masks are uint32 (HPM.cu:67-75) */
 
 /* Precision of the image storage used by the cost.  All accumulation is FP32.
- * FP32_EXACT reproduces the FP32 oracle bit for bit. */
+ * FP32_EXACT reproduces the FP32 oracle bit for bit.  The mixed-precision
+ * storages (FP16, BF16, FP32_FAST) take views whose (w + 2) x (h + 1)
+ * column-pair grid holds fewer than 2^24 elements (4095 x 4094 holds
+ * 2^24 - 1); a larger view makes the run or cost call return
+ * HPM_E_INVALID before any launch, and FP32_EXACT takes it. */
 #define HPM_PREC_FP32_EXACT 0
 #define HPM_PREC_FP16 1
 #define HPM_PREC_BF16 2
