@@ -114,8 +114,20 @@ int run_once(hpm_batch* b, const uint64_t* seeds) {
     if (i == 0) { precision = c->params.precision; iters = c->params.num_iterations; }
     if (c->params.precision != precision || c->params.num_iterations != iters)
       return bfail(b, HPM_E_INVALID, "a fused batch needs one precision and one iteration count");
-    const int rc = hpm_detail::prepare_run(c, seeds[i], b->consts[i]);
-    if (rc) return bfail(b, rc, "view " + std::to_string(i) + ": " + c->err);
+  }
+  // one storage for the whole launch: 8-bit column pairs only if every view
+  // takes them, else FP16 pairs for all
+  for (bool allow_u8 : {true, false}) {
+    bool all_u8 = true, any_u8 = false;
+    for (int i = 0; i < n; ++i) {
+      hpm_ctx* c = b->ctxs[i];
+      const int rc = hpm_detail::prepare_run(c, seeds[i], b->consts[i], allow_u8);
+      if (rc) return bfail(b, rc, "view " + std::to_string(i) + ": " + c->err);
+      all_u8 = all_u8 && c->lp_kind == kStorageU8;
+      any_u8 = any_u8 || c->lp_kind == kStorageU8;
+    }
+    b->storage = all_u8 ? kStorageU8 : precision;
+    if (all_u8 || !any_u8) break;
   }
   BHIP_OK(hipSetDevice(b->device));
   for (hpm_ctx* c : b->ctxs) BHIP_OK(hipStreamSynchronize(c->stream));  // pair-image builds, uploads
@@ -128,7 +140,10 @@ int run_once(hpm_batch* b, const uint64_t* seeds) {
     BHIP_OK(hipMemsetAsync(b->ctxs[i]->d_counters, 0, 8 * sizeof(unsigned long long), b->stream));
   BHIP_OK(hipEventRecord(b->ev[0], b->stream));
   switch (precision) {
-    case HPM_PREC_FP16: rc = run_engine<FastFor<PairF16>, true>(b, precision, iters); break;
+    case HPM_PREC_FP16:
+      rc = b->storage == kStorageU8 ? run_engine<FastFor<PairU8>, true>(b, precision, iters)
+                                    : run_engine<FastFor<PairF16>, true>(b, precision, iters);
+      break;
     case HPM_PREC_BF16: rc = run_engine<FastFor<PairBF16>, true>(b, precision, iters); break;
     case HPM_PREC_FP32_FAST: rc = run_engine<FastFor<PairF32>, true>(b, precision, iters); break;
     default: rc = run_engine<ExactEngine, false>(b, precision, iters); break;

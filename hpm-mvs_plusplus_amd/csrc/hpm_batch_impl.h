@@ -30,6 +30,7 @@ struct hpm_batch {
   std::vector<uint16_t> h_wg[3];
   int grid[3] = {0, 0, 0};
   int max_v = 0;
+  int storage = -1;  // column-pair storage of the last run (precision, or kStorageU8)
   // launch shape
   int tile_w = HPM_COLOUR_TILE_W;
   int strip = HPM_XCD_STRIP;
@@ -96,6 +97,7 @@ template <class E, int CW, int NT> int run_stages(hpm_batch* b, int precision, i
   if (!b->use_graph) return launch_stages<E, CW, NT>(b, iters);
   std::vector<unsigned char> key = b->layout_key;
   append(key, precision);
+  append(key, b->storage);
   append(key, iters);
   append(key, b->strip);
   append(key, b->threads);
@@ -140,4 +142,5 @@ template <class E, bool kAllShapes> int run_engine(hpm_batch* b, int precision, 
 
 // run_engine of the FP16 and BF16 engines is instantiated in hpm_batch_lp.hip
 extern template int hpm_batch_detail::run_engine<FastFor<hpm::PairF16>, true>(hpm_batch*, int, int);
+extern template int hpm_batch_detail::run_engine<FastFor<hpm::PairU8>, true>(hpm_batch*, int, int);
 extern template int hpm_batch_detail::run_engine<FastFor<hpm::PairBF16>, true>(hpm_batch*, int, int);

@@ -90,7 +90,7 @@ void free_views(hpm_ctx* c) {
   c->pool = nullptr;
   c->pool_ids.clear();
   for (float* p : c->d_dimg) if (p) hipFreeAsync(p, c->stream);
-  c->d_img.clear(); c->d_dimg.clear(); c->d_img_lp.clear(); c->lp_kind = -1;
+  c->d_img.clear(); c->d_dimg.clear(); c->d_img_lp.clear(); c->lp_kind = -1; c->u8_ok = -1;
   void* bufs[] = {c->d_planes, c->d_costs, c->d_pre_costs, c->d_conf, c->d_texture, c->d_sel,
                   c->d_masks,  c->d_canny, c->d_prior,     c->d_scaled};
   for (void* p : bufs) if (p) hipFreeAsync(p, c->stream);
@@ -234,20 +234,74 @@ template <class S> int build_pool_pair_images(hpm_ctx* ctx) {
   return HPM_OK;
 }
 
-int ensure_pair_images(hpm_ctx* ctx, int precision) {
-  if (ctx->lp_kind == precision && (int)ctx->d_img_lp.size() == ctx->nimg) return HPM_OK;
+// Sets *bad to 1 if a texel of the n-texel image is not an integer in [0, 255].
+__global__ __launch_bounds__(256) void k_check_u8(const float* __restrict__ img, int n, int* __restrict__ bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float v = img[i];
+  if (!(v >= 0.0f && v <= 255.0f && v == rintf(v))) atomicOr(bad, 1);
+}
+
+// Whether every image of the context is integral in [0, 255] (8-bit column
+// pairs hold it exactly); checked on the device once per set of views, or once
+// per pool.
+int images_are_u8(hpm_ctx* ctx, bool& ok) {
+  int& known = ctx->pool ? ctx->pool->u8_ok : ctx->u8_ok;
+  if (known < 0) {
+    const std::vector<float*>& imgs = ctx->pool ? ctx->pool->d_img : ctx->d_img;
+    const std::vector<hpm_camera>& cams = ctx->pool ? ctx->pool->cams : ctx->cams;
+    int* d_bad = nullptr;
+    HIP_OK(hipMallocAsync((void**)&d_bad, sizeof(int), ctx->stream));
+    HIP_OK(hipMemsetAsync(d_bad, 0, sizeof(int), ctx->stream));
+    for (size_t i = 0; i < imgs.size(); ++i) {
+      const int n = cams[i].width * cams[i].height;
+      hipLaunchKernelGGL(k_check_u8, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, imgs[i], n, d_bad);
+    }
+    int bad = 1;
+    HIP_OK(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipFreeAsync(d_bad, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    known = bad ? 0 : 1;
+  }
+  ok = known == 1;
+  return HPM_OK;
+}
+
+// Storage of the column-pair images for a mixed precision: the precision
+// itself, except FP16 on 8-bit column pairs (bit-identical costs) when the
+// views' FP16 pairs would take at least u8_min_bytes and the images are 8-bit.
+int pair_storage(hpm_ctx* ctx, int precision, bool allow_u8, int& kind) {
+  kind = precision;
+  if (precision != HPM_PREC_FP16 || !allow_u8 || ctx->u8_min_bytes < 0) return HPM_OK;
+  size_t bytes = 0;
+  for (int i = 0; i < ctx->nimg; ++i) bytes += pair_image_bytes<PairF16>(ctx->cams[i].width, ctx->cams[i].height);
+  if (bytes < (size_t)ctx->u8_min_bytes) return HPM_OK;
+  bool ok = false;
+  const int rc = images_are_u8(ctx, ok);
+  if (rc) return rc;
+  if (ok) kind = kStorageU8;
+  return HPM_OK;
+}
+
+int ensure_pair_images(hpm_ctx* ctx, int precision, bool allow_u8 = true) {
+  int kind;
+  int rc = pair_storage(ctx, precision, allow_u8, kind);
+  if (rc) return rc;
+  if (ctx->lp_kind == kind && (int)ctx->d_img_lp.size() == ctx->nimg) return HPM_OK;
   if (ctx->pool) {
-    switch (precision) {
+    switch (kind) {
       case HPM_PREC_FP16: return build_pool_pair_images<PairF16>(ctx);
       case HPM_PREC_BF16: return build_pool_pair_images<PairBF16>(ctx);
       case HPM_PREC_FP32_FAST: return build_pool_pair_images<PairF32>(ctx);
+      case kStorageU8: return build_pool_pair_images<PairU8>(ctx);
       default: return fail(ctx, HPM_E_INVALID, "unknown precision");
     }
   }
-  switch (precision) {
+  switch (kind) {
     case HPM_PREC_FP16: return build_pair_images<PairF16>(ctx);
     case HPM_PREC_BF16: return build_pair_images<PairBF16>(ctx);
     case HPM_PREC_FP32_FAST: return build_pair_images<PairF32>(ctx);
+    case kStorageU8: return build_pair_images<PairU8>(ctx);
     default: return fail(ctx, HPM_E_INVALID, "unknown precision");
   }
 }
@@ -337,11 +391,11 @@ template <class E> int run_schedule(hpm_ctx* ctx, int n_halfpasses, int do_outpu
   if (!ctx->use_graph || ctx->timing) return launch_schedule<E>(ctx, n_halfpasses, do_output);
   struct Key {
     DevState S;
-    int W, H, nimg, precision, geom, prior, hier, mand, iters, nh, out;
+    int W, H, nimg, precision, storage, geom, prior, hier, mand, iters, nh, out;
   } k;
   std::memset(&k, 0, sizeof(k));
   k.S = make_state(ctx);
-  k.W = ctx->W; k.H = ctx->H; k.nimg = ctx->nimg; k.precision = ctx->params.precision;
+  k.W = ctx->W; k.H = ctx->H; k.nimg = ctx->nimg; k.precision = ctx->params.precision; k.storage = ctx->lp_kind;
   k.geom = ctx->params.geom_consistency; k.prior = ctx->params.prior_consistency;
   k.hier = ctx->params.hierarchy; k.mand = ctx->params.mand_consistency;
   k.iters = ctx->params.num_iterations; k.nh = n_halfpasses; k.out = do_output;
@@ -1006,7 +1060,7 @@ int hpm_set_timing(hpm_ctx* ctx, int enable) {
 
 namespace hpm_detail {
 
-int prepare_run(hpm_ctx* ctx, uint64_t seed, RunConsts& C) {
+int prepare_run(hpm_ctx* ctx, uint64_t seed, RunConsts& C, bool allow_u8) {
   if (ctx->nimg == 0) return fail(ctx, HPM_E_STATE, "hpm_set_views first");
   const hpm_params& P = ctx->params;
   if (P.geom_consistency && (int)ctx->d_dimg.size() != ctx->nimg)
@@ -1016,7 +1070,13 @@ int prepare_run(hpm_ctx* ctx, uint64_t seed, RunConsts& C) {
   if (P.hierarchy && !ctx->d_scaled) return fail(ctx, HPM_E_STATE, "hierarchy needs hpm_set_coarse");
   HIP_OK(hipSetDevice(ctx->device));
   int rc;
-  if (P.precision != HPM_PREC_FP32_EXACT && (rc = ensure_pair_images(ctx, P.precision))) return rc;
+  // 8-bit column pairs pay off only in runs from random planes, whose first
+  // launches scatter their gathers (3072 x 2048, 10 sources: 20.1 ms per launch
+  // against 22.6); runs from converged planes (hierarchy, prior, geometric
+  // passes) are bound by VALU work there and run slower on them.
+  const bool random_init = !P.geom_consistency && !P.prior_consistency && !P.hierarchy && !P.mand_consistency;
+  if (P.precision != HPM_PREC_FP32_EXACT && (rc = ensure_pair_images(ctx, P.precision, allow_u8 && random_init)))
+    return rc;
   build_consts(ctx, C);
   C.seed = seed;
   return HPM_OK;
@@ -1039,11 +1099,19 @@ int hpm_run_patchmatch_partial(hpm_ctx* ctx, uint64_t seed, int n_halfpasses, in
   HIP_OK(hipMemcpyAsync(ctx->d_consts, &C, sizeof(C), hipMemcpyHostToDevice, ctx->stream));
   HIP_OK(hipMemsetAsync(ctx->d_counters, 0, 8 * sizeof(unsigned long long), ctx->stream));
   switch (P.precision) {
-    case HPM_PREC_FP16: return run_schedule<FastFor<PairF16>>(ctx, n_halfpasses, do_output);
+    case HPM_PREC_FP16:
+      if (ctx->lp_kind == kStorageU8) return run_schedule<FastFor<PairU8>>(ctx, n_halfpasses, do_output);
+      return run_schedule<FastFor<PairF16>>(ctx, n_halfpasses, do_output);
     case HPM_PREC_BF16: return run_schedule<FastFor<PairBF16>>(ctx, n_halfpasses, do_output);
     case HPM_PREC_FP32_FAST: return run_schedule<FastFor<PairF32>>(ctx, n_halfpasses, do_output);
     default: return run_schedule<ExactEngine>(ctx, n_halfpasses, do_output);
   }
+}
+
+int hpm_set_u8_pairs_min_bytes(hpm_ctx* ctx, long long min_bytes) {
+  if (!ctx) return HPM_E_INVALID;
+  ctx->u8_min_bytes = min_bytes < 0 ? -1 : min_bytes;
+  return HPM_OK;
 }
 
 int hpm_set_graph(hpm_ctx* ctx, int enable) {
@@ -1181,7 +1249,9 @@ int hpm_cost_fixed_device(hpm_ctx* ctx, const void* planes_dev, void* out_dev, i
   build_consts(ctx, C);
   HIP_OK(hipMemcpyAsync(ctx->d_consts, &C, sizeof(C), hipMemcpyHostToDevice, ctx->stream));
   switch (precision) {
-    case HPM_PREC_FP16: return launch_cost_fixed<FastFor<PairF16>>(ctx, planes_dev, out_dev);
+    case HPM_PREC_FP16:
+      if (ctx->lp_kind == kStorageU8) return launch_cost_fixed<FastFor<PairU8>>(ctx, planes_dev, out_dev);
+      return launch_cost_fixed<FastFor<PairF16>>(ctx, planes_dev, out_dev);
     case HPM_PREC_BF16: return launch_cost_fixed<FastFor<PairBF16>>(ctx, planes_dev, out_dev);
     case HPM_PREC_FP32_FAST: return launch_cost_fixed<FastFor<PairF32>>(ctx, planes_dev, out_dev);
     default: return launch_cost_fixed<ExactEngine>(ctx, planes_dev, out_dev);

@@ -118,6 +118,38 @@ struct PairF32 {
   }
 };
 
+// 8-bit column pairs: the storage of the FP16 precision for view sets whose
+// FP16 column-pair images would not stay in the 256 MB Infinity Cache (the
+// host chooses it; every texel must be an integer in [0, 255]).  Element
+// (x, y) holds the texels (t, t_below) as two bytes, 2 B per element, half the
+// FP16 footprint; a cell is one 4-byte load at element (2-byte) alignment.
+// The sample is l = t0 + ay (b0 - t0), r = t1 + ay (b1 - t1), l + ax (r - l):
+// the same fused multiply-adds of the same exact operands as the FP16 column
+// pair's (t, b - t) halves, so costs are bit-identical to FP16 storage.  It
+// costs six more VALU instructions per tap (byte extraction and conversion):
+// slower where the images stay in the caches, faster where the FP16 images
+// spill from them.
+constexpr int kStorageU8 = 4;  // storage kind beside HPM_PREC_FP16 / BF16 / FP32_FAST
+struct PairU8 {
+  using E = uint16_t;
+  using Q = uint32_t;
+  static constexpr int kind = kStorageU8;
+  static constexpr int kShift = 1;
+  __device__ __forceinline__ static E pack(float top, float bot) {
+    return (E)((uint32_t)top | ((uint32_t)bot << 8));
+  }
+  __device__ __forceinline__ static Q load(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+  }
+  __device__ __forceinline__ static float bilinear(Q q, float ax, float ay) {
+    const float t0 = (float)(q & 0xffu), b0 = (float)((q >> 8) & 0xffu);
+    const float t1 = (float)((q >> 16) & 0xffu), b1 = (float)(q >> 24);
+    const float l = ffma(ay, b0 - t0, t0);
+    const float r = ffma(ay, b1 - t1, t1);
+    return ffma(ax, r - l, l);
+  }
+};
+
 struct FastWin {
   float w[kTaps];    // bilateral weights (HPM.cu:342-347)
   float wr[kTaps];   // w * (r - r_centre)

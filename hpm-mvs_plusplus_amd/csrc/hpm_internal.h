@@ -22,7 +22,8 @@ struct hpm_pool {
   int device = 0;
   std::vector<hpm_camera> cams;
   std::vector<float*> d_img;
-  std::vector<void*> pairs[4];  // indexed by HPM_PREC_* (0 unused)
+  std::vector<void*> pairs[5];  // indexed by storage kind: HPM_PREC_* or kStorageU8 (0 unused)
+  int u8_ok = -1;               // every image integral in [0, 255]: 1 yes, 0 no, -1 not checked
   std::mutex pairs_mu;          // contexts on several host threads may build them first
 };
 
@@ -37,8 +38,13 @@ struct hpm_ctx {
   std::vector<hpm_camera> cams;
   std::vector<float*> d_img;
   std::vector<float*> d_dimg;
-  std::vector<void*> d_img_lp;     // FP16 / BF16 copies (fast paths)
-  int lp_kind = -1;                // precision of d_img_lp
+  std::vector<void*> d_img_lp;     // column-pair images of the mixed-precision engine
+  int lp_kind = -1;                // their storage kind (HPM_PREC_FP16 / BF16 / FP32_FAST, kStorageU8)
+  // FP16 precision: 8-bit column pairs when the views' FP16 column pairs
+  // would take at least this many bytes (-1: never) and every texel is an
+  // integer in [0, 255] (u8_ok, checked once per set of views)
+  long long u8_min_bytes = 192ll << 20;
+  int u8_ok = -1;
   // Allocation fault injection: with HPM_FAULT_ALLOC_AFTER=n in the
   // environment at hpm_create, the context's allocation number n (from 0,
   // after creation) fails as if the device were out of memory, once; -1 = off.
@@ -79,7 +85,9 @@ namespace hpm_detail {
 // Validates the context for a run, builds the column-pair images of its storage
 // precision when needed and fills the run's constant block (seed included);
 // nothing is launched.  Returns HPM_OK or an error recorded on the context.
-int prepare_run(hpm_ctx* ctx, uint64_t seed, hpm::RunConsts& C);
+// allow_u8 = false keeps the FP16 precision on FP16 column pairs (a fused
+// batch whose views do not all qualify for 8-bit pairs).
+int prepare_run(hpm_ctx* ctx, uint64_t seed, hpm::RunConsts& C, bool allow_u8 = true);
 hpm::DevState state_of(hpm_ctx* ctx);
 int record_error(hpm_ctx* ctx, int code, const std::string& msg);
 }  // namespace hpm_detail

@@ -415,6 +415,13 @@ class Context:
         self._check(self._L.hpm_get_selected_views(self._h, _uptr(out)), "hpm_get_selected_views")
         return out
 
+    def set_u8_pairs_min_bytes(self, min_bytes: int):
+        """FP16 storage on 8-bit column pairs from this FP16 pair footprint on
+        (-1 never, 0 always when the images are 8-bit); results are
+        bit-identical either way."""
+        self._check(self._L.hpm_set_u8_pairs_min_bytes(self._h, ctypes.c_longlong(int(min_bytes))),
+                    "hpm_set_u8_pairs_min_bytes")
+
     def cost_fixed(self, planes, precision=PREC_FP32_EXACT):
         p = np.ascontiguousarray(planes, dtype=np.float32)
         out = np.empty((self.height, self.width, self.num_images - 1), np.float32)

@@ -57,6 +57,7 @@ extern "C" {
 #define HPM_PREC_BF16 2
 #define HPM_PREC_FP32_FAST 3
 
+
 /* Mirrors struct Camera (main.h:39-47): x_cam = R * X + t. 100 bytes. */
 typedef struct hpm_camera {
   float K[9];
@@ -98,6 +99,22 @@ const char* hpm_last_error(const hpm_ctx* ctx);
 
 /* Set* mode calls (HPM.cpp:565-587) in one struct. */
 int hpm_set_params(hpm_ctx* ctx, const hpm_params* params);
+
+/* Storage of HPM_PREC_FP16 for large view sets (no counterpart in the
+ * reference, which samples FP32 textures): when the context's views' FP16
+ * column-pair images would take at least min_bytes (default 192 MiB, about
+ * the Infinity Cache's 256 MB with the run's other buffers; -1 = never, 0 =
+ * always) and every texel is an integer in [0, 255] (checked on the device
+ * once per set of views), the images are stored as 8-bit column pairs, half
+ * the bytes.  Costs and planes are bit-identical to FP16 storage.  Runs use
+ * them only when they start from random planes (no geometric, prior,
+ * hierarchy or mandatory-consistency mode): their first launches scatter the
+ * gathers and gain (3072 x 2048 with 10 sources: 12 % less time per launch),
+ * while runs from converged planes and views whose images fit the caches are
+ * bound by VALU work, which the 8-bit pairs raise.  hpm_cost_fixed uses them
+ * by footprint alone.  A fused batch uses 8-bit pairs only if every view
+ * qualifies. */
+int hpm_set_u8_pairs_min_bytes(hpm_ctx* ctx, long long min_bytes);
 
 /* InuputInitialization (HPM.cpp:644-733) + the image/camera half of
  * CudaSpaceInitialization (HPM.cpp:741-772): num_images = 1 reference + sources,
