@@ -55,3 +55,4 @@ def test_error_codes_without_device():
     assert L.hpm_create(ctypes.byref(h), 4096) == -3
     assert L.hpm_last_error(None) == b"null context"
     assert L.hpm_jbu_depth(None, 4, 4, None, 2, 2, None) == -1
+    assert L.hpm_set_u8_pairs_min_bytes(None, ctypes.c_longlong(0)) == -1
