@@ -246,6 +246,9 @@ __global__ __launch_bounds__(256) void k_check_u8(const float* __restrict__ img,
 // pairs hold it exactly); checked on the device once per set of views, or once
 // per pool.
 int images_are_u8(hpm_ctx* ctx, bool& ok) {
+  // a pool's answer is shared by its contexts, which may ask from several host threads
+  std::unique_lock<std::mutex> lock;
+  if (ctx->pool) lock = std::unique_lock<std::mutex>(ctx->pool->pairs_mu);
   int& known = ctx->pool ? ctx->pool->u8_ok : ctx->u8_ok;
   if (known < 0) {
     const std::vector<float*>& imgs = ctx->pool ? ctx->pool->d_img : ctx->d_img;
